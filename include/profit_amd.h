@@ -1,0 +1,193 @@
+/*
+ * profit_amd: C-ABI of the MI355X model-image path (rendering with adaptive
+ * sub-pixel recursion, PSF convolution, fused likelihood and its adjoint).
+ *
+ * Plain pointers and sizes, no C++ types.  Every entry point returns 0 on
+ * success and a non-zero status on failure; the message is available from
+ * profit_amd_last_error() (thread-local).  No exception crosses the ABI.
+ * Images are row-major with x fastest (index i + j*width), exactly as
+ * libprofit's Image and an R matrix with nrow = width.
+ *
+ * Reference interfaces replaced (paths relative to ICRAR/ProFit):
+ *   src/r_profit.cpp:549-681  R_profit_make_model      -> profit_amd_make_model
+ *   src/r_profit.cpp:684-706  R_profit_convolve        -> profit_amd_convolve
+ *   src/r_profit.cpp:504-547  R_profit_make_convolver  -> convolver argument of
+ *                                                          profit_amd_convolve
+ *   src/r_profit.cpp:757-772  R_profit_adjust_mask     -> profit_amd_adjust_mask
+ *   src/r_profit.cpp:483-502  R_profit_convolvers      -> profit_amd_convolvers
+ *   src/r_profit.cpp:708-755  R_profit_downsample/upsample -> profit_amd_downsample/upsample
+ *   src/libprofit/src/profit/model.cpp:271-338  Model::evaluate
+ *   R/profitLikeModel.R:244-332  the likelihood (R code upstream) ->
+ *                                profit_amd_engine_like / _like_grad (new:
+ *                                libprofit has no C/C++ likelihood or gradient)
+ */
+#ifndef PROFIT_AMD_H
+#define PROFIT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* radial profile families (src/libprofit/src/profit/model.cpp:103-135) */
+enum {
+	PROFIT_AMD_SERSIC = 0,
+	PROFIT_AMD_MOFFAT = 1,
+	PROFIT_AMD_FERRER = 2,
+	PROFIT_AMD_KING = 3,
+	PROFIT_AMD_CORESERSIC = 4,
+	PROFIT_AMD_BROKENEXP = 5,
+	PROFIT_AMD_SKY = 100,
+	PROFIT_AMD_NULL = 101
+};
+
+/* convolvers (src/libprofit/src/profit/convolve.h:68-80) */
+enum { PROFIT_AMD_CONV_BRUTE_OLD = 0, PROFIT_AMD_CONV_BRUTE = 1, PROFIT_AMD_CONV_FFT = 3, PROFIT_AMD_CONV_NULL = 4 };
+
+/* likelihood functions (R/profitLikeModel.R:263-329) */
+enum { PROFIT_AMD_LIKE_NORM = 0, PROFIT_AMD_LIKE_CHISQ = 1, PROFIT_AMD_LIKE_T = 2, PROFIT_AMD_LIKE_POIS = 3 };
+
+/* gradient slots per component: d logL / d {xcen, ycen, mag, ang, axrat, box,
+ * shape[0..4]} with shape as in profit_amd_profile */
+#define PROFIT_AMD_NGRAD 11
+
+/*
+ * One profile with libprofit's parameter names (radial.cpp:724-752 and each
+ * profile's constructor).  shape[]:
+ *   sersic re, nser | moffat fwhm, con | ferrer rout, a, b | king rc, rt, a |
+ *   coresersic re, rb, nser, a, b | brokenexp h1, h2, rb, a | sky bg.
+ * profit_amd_profile_defaults() fills libprofit's constructor defaults.
+ */
+typedef struct profit_amd_profile {
+	int32_t type;
+	int32_t convolve, rough, adjust, rescale_flux;
+	int32_t reserved;
+	double xcen, ycen, mag, ang, axrat, box;
+	double acc, rscale_switch, rscale_max;
+	uint32_t resolution, max_recursions;
+	double shape[5];
+} profit_amd_profile;
+
+/* Model settings (src/libprofit/src/profit/model.h:142-343) */
+typedef struct profit_amd_model_desc {
+	uint32_t width, height;
+	double scale_x, scale_y;
+	double magzero;
+	uint32_t finesampling;
+	int32_t return_finesampled, crop, adjust_mask, dry_run;
+	const double *psf; /* psf_h rows of psf_w; normalised as Model::set_psf does; may be NULL */
+	uint32_t psf_w, psf_h;
+	const uint8_t *mask; /* calc region (nonzero = calculate); may be NULL */
+	uint32_t mask_w, mask_h;
+	int32_t convolver;   /* PROFIT_AMD_CONV_* */
+	int32_t device;      /* HIP device ordinal */
+} profit_amd_model_desc;
+
+const char *profit_amd_last_error(void);
+const char *profit_amd_version(void);
+
+/* libprofit constructor defaults for a profile type. */
+int profit_amd_profile_defaults(int32_t type, profit_amd_profile *p);
+
+/* Model::evaluate: writes out_w*out_h doubles to out (capacity out_cap) and
+ * the offset of the returned image.  Replaces R_profit_make_model
+ * (src/r_profit.cpp:549-681) and Model::evaluate (model.cpp:271-338). */
+int profit_amd_make_model(const profit_amd_model_desc *model, const profit_amd_profile *profiles, int32_t nprofiles,
+                          double *out, size_t out_cap, uint32_t *out_w, uint32_t *out_h, int32_t *offset_x,
+                          int32_t *offset_y);
+
+/* Model::get_drawing_dimensions (model.cpp:221-225). */
+int profit_amd_drawing_dims(const profit_amd_model_desc *model, const profit_amd_profile *profiles,
+                            int32_t nprofiles, uint32_t *width, uint32_t *height);
+
+/* Model::adjust(mask, dims, psf, finesampling) (model.cpp:248-257); replaces
+ * R_profit_adjust_mask (src/r_profit.cpp:757-772). */
+int profit_amd_adjust_mask(const uint8_t *mask, uint32_t mask_w, uint32_t mask_h, uint32_t img_w, uint32_t img_h,
+                           const double *psf, uint32_t psf_w, uint32_t psf_h, uint32_t finesampling, uint8_t *out,
+                           size_t out_cap, uint32_t *out_w, uint32_t *out_h);
+
+/* Convolver::convolve(src, krn, mask) for the given convolver; replaces
+ * R_profit_convolve (src/r_profit.cpp:684-706). out has src_w*src_h doubles. */
+int profit_amd_convolve(int32_t convolver, const double *src, uint32_t src_w, uint32_t src_h, const double *krn,
+                        uint32_t krn_w, uint32_t krn_h, const uint8_t *mask, double *out, int32_t device);
+
+/* Names of the available convolvers, comma separated (R_profit_convolvers). */
+const char *profit_amd_convolvers(void);
+
+/* Image::downsample(SUM|SAMPLE|AVERAGE) and Image::upsample (image.cpp:246-367);
+ * replace R_profit_downsample / R_profit_upsample.  mode: 0 sample, 1 sum,
+ * 2 average for downsampling; 0 copy, 1 scale for upsampling. */
+int profit_amd_downsample(const double *img, uint32_t w, uint32_t h, uint32_t factor, int32_t mode, double *out,
+                          size_t out_cap, uint32_t *out_w, uint32_t *out_h);
+int profit_amd_upsample(const double *img, uint32_t w, uint32_t h, uint32_t factor, int32_t mode, double *out,
+                        size_t out_cap, uint32_t *out_w, uint32_t *out_h);
+
+/* ---------------------------------------------------------------- likelihood engine
+ *
+ * A galaxy is what profitSetupData prepares (R/profitSetupData.R:151-344):
+ * image size, PSF, pre-adjusted calc region, likelihood region, data, sigma
+ * and the likelihood function.  A batch is a list of instances, each a
+ * galaxy plus a component count; one call evaluates logL (and its gradient)
+ * of every instance in one launch sequence (a HIP graph after the first call).
+ * Every component of a galaxy with a PSF is convolved.
+ */
+typedef struct profit_amd_engine profit_amd_engine;
+
+typedef struct profit_amd_galaxy_desc {
+	uint32_t width, height;
+	double scale_x, scale_y;
+	double magzero;
+	const double *psf;        /* may be NULL (no convolution) */
+	uint32_t psf_w, psf_h;
+	const uint8_t *calcregion; /* pre-adjusted calc region (drawing size) or NULL */
+	uint32_t calc_w, calc_h;
+	const uint8_t *region;    /* likelihood region (width*height) or NULL = every pixel */
+	const double *data;       /* width*height */
+	const double *sigma;      /* width*height */
+	int32_t likefunc;         /* PROFIT_AMD_LIKE_* */
+} profit_amd_galaxy_desc;
+
+int profit_amd_engine_create(int32_t device, profit_amd_engine **engine);
+void profit_amd_engine_destroy(profit_amd_engine *engine);
+int profit_amd_engine_add_galaxy(profit_amd_engine *engine, const profit_amd_galaxy_desc *galaxy, int32_t *galaxy_id);
+int profit_amd_engine_set_batch(profit_amd_engine *engine, const int32_t *inst_galaxy, const int32_t *inst_ncomp,
+                                int32_t ninst);
+/* Tile height of the convolution / rendering tiles: 4*rows (rows = 2, 4, 8, 16). */
+int profit_amd_engine_set_tile_rows(profit_amd_engine *engine, int32_t rows);
+
+/* logL per instance.  params: one profit_amd_profile per component,
+ * instance-major, in the batch's component order. */
+int profit_amd_engine_like(profit_amd_engine *engine, const profit_amd_profile *params, double *loglike);
+
+/* logL per instance and d logL / d slot (PROFIT_AMD_NGRAD per component): the
+ * gradient callback for R's optimisers (optim(gr=)) and HMC samplers.  The
+ * sub-pixel recursion tree and the adjusted sub-pixel settings of the given
+ * parameters are held fixed (they are piecewise constant in the parameters). */
+int profit_amd_engine_like_grad(profit_amd_engine *engine, const profit_amd_profile *params, double *loglike,
+                                double *grad);
+
+/* The model image of every instance (width*height floats each, instance-major):
+ * the convolved model the likelihood sees. */
+int profit_amd_engine_model_images(profit_amd_engine *engine, const profit_amd_profile *params, float *out);
+
+/* Rendering diagnostics for a batch of one instance: per component the pixel
+ * class on the drawing canvas (0 skipped by the calc region, 1 beyond
+ * rscale_max, 2 direct, 3 sub-pixel integration) and the number of nodes of
+ * each pixel's recursion tree. */
+int profit_amd_engine_diagnostics(profit_amd_engine *engine, const profit_amd_profile *params, int32_t *cls,
+                                  int32_t *nodes);
+
+/* Drawing-canvas dimensions and PSF padding of a registered galaxy. */
+int profit_amd_engine_galaxy_geometry(profit_amd_engine *engine, int32_t galaxy_id, uint32_t *draw_w,
+                                      uint32_t *draw_h, uint32_t *pad_x, uint32_t *pad_y);
+
+/* Milliseconds per stage of the last evaluation when timing is on. */
+int profit_amd_engine_set_timing(profit_amd_engine *engine, int32_t on);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PROFIT_AMD_H */

@@ -1,0 +1,60 @@
+// Device-side data layout shared by the HIP kernels and the host engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "profile_math.h"
+
+namespace profit_amd {
+
+// likelihood kinds (R/profitLikeModel.R:263-329)
+enum LikeKind : int32_t { LIKE_NORM = 0, LIKE_CHISQ = 1, LIKE_T = 2, LIKE_POIS = 3 };
+
+constexpr int kTileW = 64;       // tile width: one wavefront spans a tile row
+constexpr int kRenderRows = 4;   // render tiles are 64 x 4 (256 threads)
+constexpr int kMaxLevels = 32;   // recursion walker depth (max_recursions <= 30)
+
+// One galaxy: image geometry, PSF, masks and data.  Several instances (parameter
+// sets) may share a galaxy.
+struct GalaxyDev {
+	int32_t W, H;          // image (likelihood) dimensions
+	int32_t DW, DH;        // drawing canvas dimensions
+	int32_t padx, pady;    // PSF padding (canvas shift)
+	int32_t Kx, Ky;        // PSF dimensions (0 without convolution)
+	int32_t likefunc;
+	int32_t has_calc;
+	int32_t n_rtiles, n_ctiles, n_atiles;
+	int32_t conv_rows;     // rows per thread of the convolution tiles
+	int32_t reserved;
+	double sx, sy;         // profile pixel scale
+	double img_sx, img_sy; // image pixel scale
+	double magzero;
+	double n_region;       // number of likelihood pixels
+	double pois_scale;     // Poisson rescale factor (1 when not rescaled)
+	const float *psf_fwd;  // flipped PSF, weights of the forward convolution
+	const float *psf_adj;  // PSF as given, weights of the adjoint correlation
+	const uint8_t *calc;   // DW*DH drawing mask (null: every pixel)
+	const uint8_t *region; // W*H likelihood mask (already intersected with calc)
+	const float *data;     // W*H
+	const float *ivar;     // W*H, 1/sigma^2
+	const int2 *rtiles;    // render / adjoint-render tiles (64 x 4) on the canvas
+	const int2 *ctiles;    // forward convolution tiles on the image
+	const int2 *atiles;    // adjoint convolution tiles on the canvas
+};
+
+struct InstDev {
+	int32_t galaxy, comp0, ncomp, reserved;
+	float *canvas;         // DW*DH rendered (to be convolved) image
+	float *w;              // W*H  d logL / d model (up to a per-instance scalar)
+	float *G;              // DW*DH  correlated gradient image / convolved model
+	float *aux;            // DW*DH  profiles drawn without convolution (or null)
+};
+
+struct QEntry {
+	int32_t comp;
+	int32_t pix;
+};
+
+}  // namespace profit_amd

@@ -1,0 +1,628 @@
+// Host engine for the GPU model-image path: galaxy upload, tile lists,
+// batched launch sequences and their hipGraph replay.
+#include "engine.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <sstream>
+
+namespace profit_amd {
+
+void hip_check(hipError_t e, const char *what)
+{
+	if (e != hipSuccess) {
+		std::ostringstream os;
+		os << what << ": " << hipGetErrorString(e) << " (" << (int)e << ")";
+		throw hip_error(os.str());
+	}
+}
+
+void DevBuf::alloc(size_t bytes)
+{
+	release();
+	if (bytes == 0) return;
+	hip_check(hipMalloc(&p_, bytes), "hipMalloc");
+	hip_check(hipMemset(p_, 0, bytes), "hipMemset");
+	n_ = bytes;
+}
+
+void DevBuf::release()
+{
+	if (p_) (void)hipFree(p_);
+	p_ = nullptr;
+	n_ = 0;
+}
+
+void HostBuf::alloc(size_t bytes)
+{
+	release();
+	if (bytes == 0) return;
+	hip_check(hipHostMalloc(&p_, bytes, hipHostMallocDefault), "hipHostMalloc");
+	std::memset(p_, 0, bytes);
+	n_ = bytes;
+}
+
+void HostBuf::release()
+{
+	if (p_) (void)hipHostFree(p_);
+	p_ = nullptr;
+	n_ = 0;
+}
+
+Engine::Engine(int device) : device_(device)
+{
+	hip_check(hipSetDevice(device_), "hipSetDevice");
+	hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+}
+
+Engine::~Engine()
+{
+	for (auto &g : graph_exec_)
+		if (g) (void)hipGraphExecDestroy(g);
+	if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+static void invalidate_graphs(hipGraphExec_t *g)
+{
+	for (int k = 0; k < 2; ++k)
+		if (g[k]) {
+			(void)hipGraphExecDestroy(g[k]);
+			g[k] = nullptr;
+		}
+}
+
+void Engine::set_conv_rows(int r)
+{
+	if (!(r == 2 || r == 4 || r == 8 || r == 16)) throw std::invalid_argument("conv rows must be 2, 4, 8 or 16");
+	if (r == conv_rows_) return;
+	conv_rows_ = r;
+	for (size_t g = 0; g < gal_host_.size(); ++g) build_tiles((int)g);
+	gals_dirty_ = true;
+	invalidate_graphs(graph_exec_);
+	if (!inst_galaxy_.empty()) set_batch(inst_galaxy_, inst_ncomp_);
+}
+
+int Engine::add_galaxy(const GalaxySpec &s)
+{
+	if (s.W <= 0 || s.H <= 0) throw std::invalid_argument("Model's requested dimensions are 0");
+	if (s.scale_x <= 0) throw std::invalid_argument("Model's scale_x cannot be negative or zero");
+	if (s.scale_y <= 0) throw std::invalid_argument("Model's scale_y cannot be negative or zero");
+	if (s.finesampling <= 0) throw std::invalid_argument("finesampling must be > 0");
+	hip_check(hipSetDevice(device_), "hipSetDevice");
+	GalaxyDev g{};
+	GalaxyBuffers bufs;
+	g.W = s.W;
+	g.H = s.H;
+	g.likefunc = s.likefunc;
+	g.magzero = s.magzero;
+	g.img_sx = s.scale_x;
+	g.img_sy = s.scale_y;
+	g.sx = s.scale_x / s.finesampling;
+	g.sy = s.scale_y / s.finesampling;
+	const bool conv = s.psf && s.Kx > 0 && s.Ky > 0;
+	g.Kx = conv ? s.Kx : 0;
+	g.Ky = conv ? s.Ky : 0;
+	// Model::analyze_expansion_requirements with a pre-adjusted mask
+	// (model.cpp:152-184, adjust_mask = false) or without a mask
+	bool pad = conv;
+	if (s.calc) pad = s.cw > s.W && s.ch > s.H;
+	g.padx = pad ? g.Kx / 2 : 0;
+	g.pady = pad ? g.Ky / 2 : 0;
+	g.DW = s.W + 2 * g.padx;
+	g.DH = s.H + 2 * g.pady;
+	if (s.calc && (s.cw != g.DW || s.ch != g.DH)) {
+		std::ostringstream os;
+		os << "Mask dimensions != drawing dimensions: [" << s.cw << ", " << s.ch << "] != [" << g.DW << ", " << g.DH
+		   << "]";
+		throw std::invalid_argument(os.str());
+	}
+	g.has_calc = s.calc != nullptr;
+	const size_t npix = (size_t)s.W * s.H, ndraw = (size_t)g.DW * g.DH;
+	if (conv) {
+		std::vector<double> p(s.psf, s.psf + (size_t)s.Kx * s.Ky);
+		double sum = 0;
+		for (double v : p) sum += v;
+		if (sum > 0)
+			for (double &v : p) v /= sum;  // Model::set_psf normalises (model.h:160-170)
+		std::vector<float> fwd(p.size()), adj(p.size());
+		for (int ky = 0; ky < s.Ky; ++ky)
+			for (int kx = 0; kx < s.Kx; ++kx) {
+				fwd[(size_t)ky * s.Kx + kx] = (float)p[(size_t)(s.Ky - 1 - ky) * s.Kx + (s.Kx - 1 - kx)];
+				adj[(size_t)ky * s.Kx + kx] = (float)p[(size_t)ky * s.Kx + kx];
+			}
+		bufs.psf_fwd.alloc(fwd.size() * sizeof(float));
+		bufs.psf_adj.alloc(adj.size() * sizeof(float));
+		hip_check(hipMemcpy(bufs.psf_fwd.as<float>(), fwd.data(), fwd.size() * 4, hipMemcpyHostToDevice), "psf");
+		hip_check(hipMemcpy(bufs.psf_adj.as<float>(), adj.data(), adj.size() * 4, hipMemcpyHostToDevice), "psf");
+	}
+	std::vector<uint8_t> calc(ndraw, 1);
+	if (s.calc)
+		for (size_t k = 0; k < ndraw; ++k) calc[k] = s.calc[k] ? 1 : 0;
+	bufs.calc.alloc(ndraw);
+	hip_check(hipMemcpy(bufs.calc.as<uint8_t>(), calc.data(), ndraw, hipMemcpyHostToDevice), "calc");
+	std::vector<uint8_t> region(npix, 1);
+	double nreg = 0;
+	for (int j = 0; j < s.H; ++j)
+		for (int i = 0; i < s.W; ++i) {
+			const size_t o = (size_t)j * s.W + i;
+			const uint8_t r = s.region ? (s.region[o] ? 1 : 0) : 1;
+			region[o] = r;
+			nreg += r;
+		}
+	g.n_region = nreg;
+	bufs.region.alloc(npix);
+	hip_check(hipMemcpy(bufs.region.as<uint8_t>(), region.data(), npix, hipMemcpyHostToDevice), "region");
+	g.pois_scale = 1;
+	if (s.data && s.sigma) {
+		std::vector<float> d(npix), iv(npix);
+		double smax = 0;
+		for (size_t k = 0; k < npix; ++k) {
+			d[k] = (float)s.data[k];
+			const double sg = s.sigma[k];
+			iv[k] = (float)(1.0 / (sg * sg));
+			if (region[k]) smax = std::max(smax, std::fabs(s.data[k]) / (std::fabs(sg) * std::fabs(sg)));
+		}
+		// R/profitLikeModel.R:324-328
+		if (smax < 0.1 || smax > 10) g.pois_scale = smax;
+		bufs.data.alloc(npix * 4);
+		bufs.ivar.alloc(npix * 4);
+		hip_check(hipMemcpy(bufs.data.as<float>(), d.data(), npix * 4, hipMemcpyHostToDevice), "data");
+		hip_check(hipMemcpy(bufs.ivar.as<float>(), iv.data(), npix * 4, hipMemcpyHostToDevice), "ivar");
+	}
+	g.psf_fwd = bufs.psf_fwd.as<float>();
+	g.psf_adj = bufs.psf_adj.as<float>();
+	g.calc = bufs.calc.as<uint8_t>();
+	g.region = bufs.region.as<uint8_t>();
+	g.data = bufs.data.as<float>();
+	g.ivar = bufs.ivar.as<float>();
+	gal_host_.push_back(g);
+	gal_bufs_.push_back(std::move(bufs));
+	build_tiles((int)gal_host_.size() - 1);
+	gals_dirty_ = true;
+	invalidate_graphs(graph_exec_);
+	return (int)gal_host_.size() - 1;
+}
+
+// Tiles of 64 x (4*conv_rows) pixels: on the image where the likelihood region
+// has a pixel (forward convolution), and on the canvas where the calc region
+// has one (rendering, adjoint convolution and adjoint rendering).
+void Engine::build_tiles(int gi)
+{
+	GalaxyDev &g = gal_host_[gi];
+	GalaxyBuffers &b = gal_bufs_[gi];
+	const int th = 4 * conv_rows_;
+	g.conv_rows = conv_rows_;
+	std::vector<uint8_t> calc((size_t)g.DW * g.DH), region((size_t)g.W * g.H);
+	hip_check(hipMemcpy(calc.data(), g.calc, calc.size(), hipMemcpyDeviceToHost), "calc");
+	hip_check(hipMemcpy(region.data(), g.region, region.size(), hipMemcpyDeviceToHost), "region");
+	auto tiles = [&](const std::vector<uint8_t> &m, int w, int h) {
+		std::vector<int2> out;
+		for (int ty = 0; ty * th < h; ++ty)
+			for (int tx = 0; tx * kTileW < w; ++tx) {
+				bool any = false;
+				for (int j = ty * th; j < std::min(h, (ty + 1) * th) && !any; ++j)
+					for (int i = tx * kTileW; i < std::min(w, (tx + 1) * kTileW); ++i)
+						if (m[(size_t)j * w + i]) {
+							any = true;
+							break;
+						}
+				if (any) out.push_back(make_int2(tx, ty));
+			}
+		return out;
+	};
+	b.ct_host = tiles(region, g.W, g.H);
+	b.at_host = tiles(calc, g.DW, g.DH);
+	b.ctiles.alloc(std::max<size_t>(1, b.ct_host.size()) * sizeof(int2));
+	b.atiles.alloc(std::max<size_t>(1, b.at_host.size()) * sizeof(int2));
+	if (!b.ct_host.empty())
+		hip_check(hipMemcpy(b.ctiles.as<int2>(), b.ct_host.data(), b.ct_host.size() * sizeof(int2), hipMemcpyHostToDevice),
+		          "tiles");
+	if (!b.at_host.empty())
+		hip_check(hipMemcpy(b.atiles.as<int2>(), b.at_host.data(), b.at_host.size() * sizeof(int2), hipMemcpyHostToDevice),
+		          "tiles");
+	g.ctiles = b.ctiles.as<int2>();
+	g.atiles = b.atiles.as<int2>();
+	g.n_ctiles = (int)b.ct_host.size();
+	g.n_atiles = (int)b.at_host.size();
+}
+
+void Engine::upload_galaxies()
+{
+	if (!gals_dirty_) return;
+	d_gals_.alloc(std::max<size_t>(1, gal_host_.size()) * sizeof(GalaxyDev));
+	hip_check(hipMemcpy(d_gals_.as<GalaxyDev>(), gal_host_.data(), gal_host_.size() * sizeof(GalaxyDev),
+	                    hipMemcpyHostToDevice),
+	          "galaxies");
+	gals_dirty_ = false;
+}
+
+void Engine::ensure_queue(long need)
+{
+	if (need <= qcap_) return;
+	qcap_ = need;
+	d_queue_.alloc((size_t)qcap_ * sizeof(QEntry));
+	invalidate_graphs(graph_exec_);
+}
+
+void Engine::set_batch(const std::vector<int> &inst_galaxy, const std::vector<int> &inst_ncomp)
+{
+	if (inst_galaxy.size() != inst_ncomp.size()) throw std::invalid_argument("instance arrays differ in length");
+	hip_check(hipSetDevice(device_), "hipSetDevice");
+	invalidate_graphs(graph_exec_);
+	upload_galaxies();
+	const int n = (int)inst_galaxy.size();
+	inst_galaxy_ = inst_galaxy;
+	inst_ncomp_ = inst_ncomp;
+	inst_comp0_.assign(n, 0);
+	ct_off_.assign(n + 1, 0);
+	at_off_.assign(n + 1, 0);
+	ncomp_total_ = 0;
+	kx_max_ = 0;
+	long qneed = 1024;
+	inst_bufs_.clear();
+	std::vector<InstDev> insts(n);
+	std::vector<int> comp_inst;
+	for (int i = 0; i < n; ++i) {
+		const int gi = inst_galaxy[i];
+		if (gi < 0 || gi >= (int)gal_host_.size()) throw std::invalid_argument("unknown galaxy id");
+		if (inst_ncomp[i] < 0) throw std::invalid_argument("negative component count");
+		const GalaxyDev &g = gal_host_[gi];
+		inst_comp0_[i] = ncomp_total_;
+		ncomp_total_ += inst_ncomp[i];
+		for (int k = 0; k < inst_ncomp[i]; ++k) comp_inst.push_back(i);
+		ct_off_[i + 1] = ct_off_[i] + g.n_ctiles;
+		at_off_[i + 1] = at_off_[i] + g.n_atiles;
+		kx_max_ = std::max(kx_max_, g.Kx);
+		const size_t ndraw = (size_t)g.DW * g.DH, npix = (size_t)g.W * g.H;
+		DevBuf canvas(ndraw * 4), w(npix * 4), G(ndraw * 4), aux(ndraw * 4);
+		insts[i] = InstDev{gi, inst_comp0_[i], inst_ncomp[i], 0, canvas.as<float>(), w.as<float>(), G.as<float>(),
+		                   aux.as<float>()};
+		inst_bufs_.push_back(std::move(canvas));
+		inst_bufs_.push_back(std::move(w));
+		inst_bufs_.push_back(std::move(G));
+		inst_bufs_.push_back(std::move(aux));
+		qneed += (long)inst_ncomp[i] * (long)std::min<size_t>(ndraw, 4096);
+	}
+	n_ct_blocks_ = ct_off_[n];
+	n_at_blocks_ = at_off_[n];
+	d_insts_.alloc(std::max(1, n) * sizeof(InstDev));
+	if (n) hip_check(hipMemcpy(d_insts_.as<InstDev>(), insts.data(), n * sizeof(InstDev), hipMemcpyHostToDevice), "insts");
+	d_ct_off_.alloc((n + 1) * sizeof(int));
+	d_at_off_.alloc((n + 1) * sizeof(int));
+	hip_check(hipMemcpy(d_ct_off_.as<int>(), ct_off_.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice), "off");
+	hip_check(hipMemcpy(d_at_off_.as<int>(), at_off_.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice), "off");
+	const int nc = std::max(1, ncomp_total_);
+	d_comp_inst_.alloc(nc * sizeof(int));
+	if (ncomp_total_)
+		hip_check(hipMemcpy(d_comp_inst_.as<int>(), comp_inst.data(), ncomp_total_ * sizeof(int), hipMemcpyHostToDevice),
+		          "comp_inst");
+	d_params_.alloc(nc * sizeof(ProfileParams));
+	d_comps_.alloc(nc * sizeof(Comp));
+	d_err_.alloc(nc * sizeof(int));
+	d_partial_.alloc(std::max(1, n_ct_blocks_) * sizeof(double));
+	d_ll_.alloc(std::max(1, n) * sizeof(double));
+	d_gscale_.alloc(std::max(1, n) * sizeof(double));
+	d_grad_acc_.alloc((size_t)nc * NGRAD * sizeof(double));
+	d_grad_out_.alloc((size_t)nc * NGRAD * sizeof(double));
+	d_qcount_.alloc(sizeof(int));
+	h_params_.alloc(nc * sizeof(ProfileParams));
+	h_ll_.alloc(std::max(1, n) * sizeof(double));
+	h_grad_.alloc((size_t)nc * NGRAD * sizeof(double));
+	h_err_.alloc(nc * sizeof(int));
+	h_qcount_.alloc(sizeof(int));
+	qcap_ = 0;
+	ensure_queue(qneed);
+	int ncu = 256;
+	hipDeviceProp_t prop;
+	if (hipGetDeviceProperties(&prop, device_) == hipSuccess) ncu = prop.multiProcessorCount;
+	subsample_blocks_ = ncu * 8;
+}
+
+void Engine::record(bool grad)
+{
+	hipStream_t st = stream_;
+	const int n = ninst();
+	const int nc = ncomp_total_;
+	const GalaxyDev *gals = d_gals_.as<GalaxyDev>();
+	const InstDev *insts = d_insts_.as<InstDev>();
+	const int kc = kc_max_;
+	hip_check(hipMemcpyAsync(d_params_.as<ProfileParams>(), h_params_.as<ProfileParams>(), nc * sizeof(ProfileParams),
+	                         hipMemcpyHostToDevice, st),
+	          "params");
+	hip_check(launch_setup(d_params_.as<ProfileParams>(), d_comp_inst_.as<int>(), insts, gals, d_comps_.as<Comp>(),
+	                       d_err_.as<int>(), nc, st),
+	          "setup");
+	hip_check(launch_zero_i(d_qcount_.as<int>(), 1, st), "zero");
+	if (grad) hip_check(launch_zero_d(d_grad_acc_.as<double>(), (long)nc * NGRAD, st), "zero");
+	hip_check(launch_render(gals, insts, d_at_off_.as<int>(), n, n_at_blocks_, d_comps_.as<Comp>(),
+	                        d_queue_.as<QEntry>(), d_qcount_.as<int>(), (int)qcap_, -1, 0, nullptr, st),
+	          "render");
+	hip_check(launch_subsample(false, subsample_blocks_, gals, insts, d_comps_.as<Comp>(), d_queue_.as<QEntry>(),
+	                           d_qcount_.as<int>(), (int)qcap_, 0, nullptr, st),
+	          "subsample");
+	hip_check(launch_conv(0, conv_rows_, n_ct_blocks_, kx_max_, kc, gals, insts, d_ct_off_.as<int>(), n,
+	                      d_partial_.as<double>(), st),
+	          "conv");
+	hip_check(launch_like_finalize(gals, insts, d_ct_off_.as<int>(), n, d_partial_.as<double>(), d_ll_.as<double>(),
+	                               d_gscale_.as<double>(), st),
+	          "like");
+	if (grad) {
+		hip_check(launch_conv(2, conv_rows_, n_at_blocks_, kx_max_, kc, gals, insts, d_at_off_.as<int>(), n, nullptr, st),
+		          "adjoint conv");
+		hip_check(launch_adj_render(gals, insts, d_at_off_.as<int>(), n, n_at_blocks_, d_comps_.as<Comp>(),
+		                            d_grad_acc_.as<double>(), st),
+		          "adjoint render");
+		hip_check(launch_subsample(true, subsample_blocks_, gals, insts, d_comps_.as<Comp>(), d_queue_.as<QEntry>(),
+		                           d_qcount_.as<int>(), (int)qcap_, 0, d_grad_acc_.as<double>(), st),
+		          "adjoint subsample");
+		hip_check(launch_grad_finalize(d_comps_.as<Comp>(), d_gscale_.as<double>(), d_grad_acc_.as<double>(),
+		                               d_grad_out_.as<double>(), nc, st),
+		          "grad");
+		hip_check(hipMemcpyAsync(h_grad_.as<double>(), d_grad_out_.as<double>(), (size_t)nc * NGRAD * sizeof(double),
+		                         hipMemcpyDeviceToHost, st),
+		          "grad copy");
+	}
+	hip_check(hipMemcpyAsync(h_ll_.as<double>(), d_ll_.as<double>(), n * sizeof(double), hipMemcpyDeviceToHost, st),
+	          "ll copy");
+	hip_check(hipMemcpyAsync(h_err_.as<int>(), d_err_.as<int>(), nc * sizeof(int), hipMemcpyDeviceToHost, st), "err");
+	hip_check(hipMemcpyAsync(h_qcount_.as<int>(), d_qcount_.as<int>(), sizeof(int), hipMemcpyDeviceToHost, st), "qcount");
+}
+
+void Engine::check_setup_errors()
+{
+	const int *err = h_err_.as<int>();
+	for (int c = 0; c < ncomp_total_; ++c)
+		if (err[c] != SE_OK) {
+			std::ostringstream os;
+			os << "component " << c << ": " << setup_error_message(err[c]);
+			throw std::invalid_argument(os.str());
+		}
+}
+
+void Engine::run(bool grad, bool use_graph)
+{
+	hip_check(hipSetDevice(device_), "hipSetDevice");
+	for (int attempt = 0; attempt < 4; ++attempt) {
+		const int gi = grad ? 1 : 0;
+		if (use_graph) {
+			if (!graph_exec_[gi]) {
+				// first call runs eagerly (sets kernel attributes), later calls replay a graph
+				record(grad);
+				hip_check(hipStreamSynchronize(stream_), "sync");
+				if (*h_qcount_.as<int>() <= qcap_) {
+					hipGraph_t graph;
+					hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "capture");
+					record(grad);
+					hip_check(hipStreamEndCapture(stream_, &graph), "capture end");
+					hip_check(hipGraphInstantiate(&graph_exec_[gi], graph, nullptr, nullptr, 0), "instantiate");
+					(void)hipGraphDestroy(graph);
+				}
+			}
+			else {
+				hip_check(hipGraphLaunch(graph_exec_[gi], stream_), "graph launch");
+				hip_check(hipStreamSynchronize(stream_), "sync");
+			}
+		}
+		else {
+			record(grad);
+			hip_check(hipStreamSynchronize(stream_), "sync");
+		}
+		const int qc = *h_qcount_.as<int>();
+		stats_.queue_count = qc;
+		if (qc <= qcap_) return;
+		ensure_queue((long)qc + qc / 4 + 1024);  // drops the graphs; run again
+	}
+	throw std::runtime_error("sub-pixel work queue kept overflowing");
+}
+
+void Engine::evaluate(const ProfileParams *params, double *ll, double *grad, bool use_graph)
+{
+	if (ninst() == 0) return;
+	std::memcpy(h_params_.as<ProfileParams>(), params, ncomp_total_ * sizeof(ProfileParams));
+	run(grad != nullptr, use_graph);
+	check_setup_errors();
+	if (ll) std::memcpy(ll, h_ll_.as<double>(), ninst() * sizeof(double));
+	if (grad) std::memcpy(grad, h_grad_.as<double>(), (size_t)ncomp_total_ * NGRAD * sizeof(double));
+}
+
+void Engine::diagnostics(const ProfileParams *params, int *cls, int *nodes)
+{
+	if (ninst() != 1) throw std::invalid_argument("diagnostics need a batch of one instance");
+	hip_check(hipSetDevice(device_), "hipSetDevice");
+	const GalaxyDev &g = gal_host_[inst_galaxy_[0]];
+	const size_t plane = (size_t)g.DW * g.DH;
+	const int nc = ncomp_total_;
+	DevBuf dcls(plane * nc * sizeof(int)), dnodes(plane * nc * sizeof(int));
+	std::memcpy(h_params_.as<ProfileParams>(), params, nc * sizeof(ProfileParams));
+	hipStream_t st = stream_;
+	const GalaxyDev *gals = d_gals_.as<GalaxyDev>();
+	const InstDev *insts = d_insts_.as<InstDev>();
+	for (int attempt = 0; attempt < 4; ++attempt) {
+		hip_check(hipMemcpyAsync(d_params_.as<ProfileParams>(), h_params_.as<ProfileParams>(), nc * sizeof(ProfileParams),
+		                         hipMemcpyHostToDevice, st),
+		          "params");
+		hip_check(launch_setup(d_params_.as<ProfileParams>(), d_comp_inst_.as<int>(), insts, gals, d_comps_.as<Comp>(),
+		                       d_err_.as<int>(), nc, st),
+		          "setup");
+		hip_check(launch_zero_i(d_qcount_.as<int>(), 1, st), "zero");
+		hip_check(launch_render(gals, insts, d_at_off_.as<int>(), 1, n_at_blocks_, d_comps_.as<Comp>(),
+		                        d_queue_.as<QEntry>(), d_qcount_.as<int>(), (int)qcap_, -1, 0, dcls.as<int>(), st),
+		          "render");
+		hip_check(launch_subsample_nodes(subsample_blocks_, gals, insts, d_comps_.as<Comp>(), d_queue_.as<QEntry>(),
+		                                 d_qcount_.as<int>(), (int)qcap_, dnodes.as<int>(), st),
+		          "nodes");
+		hip_check(hipMemcpyAsync(h_err_.as<int>(), d_err_.as<int>(), nc * sizeof(int), hipMemcpyDeviceToHost, st), "err");
+		hip_check(hipMemcpyAsync(h_qcount_.as<int>(), d_qcount_.as<int>(), sizeof(int), hipMemcpyDeviceToHost, st),
+		          "qcount");
+		hip_check(hipStreamSynchronize(st), "sync");
+		check_setup_errors();
+		const int qc = *h_qcount_.as<int>();
+		if (qc <= qcap_) break;
+		ensure_queue((long)qc + 1024);
+	}
+	hip_check(hipMemcpy(cls, dcls.as<int>(), plane * nc * sizeof(int), hipMemcpyDeviceToHost), "cls");
+	hip_check(hipMemcpy(nodes, dnodes.as<int>(), plane * nc * sizeof(int), hipMemcpyDeviceToHost), "nodes");
+}
+
+void Engine::model_images(const ProfileParams *params, float *out)
+{
+	hip_check(hipSetDevice(device_), "hipSetDevice");
+	const int n = ninst(), nc = ncomp_total_;
+	std::memcpy(h_params_.as<ProfileParams>(), params, nc * sizeof(ProfileParams));
+	hipStream_t st = stream_;
+	const GalaxyDev *gals = d_gals_.as<GalaxyDev>();
+	const InstDev *insts = d_insts_.as<InstDev>();
+	for (int attempt = 0; attempt < 4; ++attempt) {
+		hip_check(hipMemcpyAsync(d_params_.as<ProfileParams>(), h_params_.as<ProfileParams>(), nc * sizeof(ProfileParams),
+		                         hipMemcpyHostToDevice, st),
+		          "params");
+		hip_check(launch_setup(d_params_.as<ProfileParams>(), d_comp_inst_.as<int>(), insts, gals, d_comps_.as<Comp>(),
+		                       d_err_.as<int>(), nc, st),
+		          "setup");
+		hip_check(launch_zero_i(d_qcount_.as<int>(), 1, st), "zero");
+		hip_check(launch_render(gals, insts, d_at_off_.as<int>(), n, n_at_blocks_, d_comps_.as<Comp>(),
+		                        d_queue_.as<QEntry>(), d_qcount_.as<int>(), (int)qcap_, -1, 0, nullptr, st),
+		          "render");
+		hip_check(launch_subsample(false, subsample_blocks_, gals, insts, d_comps_.as<Comp>(), d_queue_.as<QEntry>(),
+		                           d_qcount_.as<int>(), (int)qcap_, 0, nullptr, st),
+		          "subsample");
+		if (kx_max_ > 0)
+			hip_check(launch_conv(1, conv_rows_, n_at_blocks_, kx_max_, kc_max_, gals, insts, d_at_off_.as<int>(), n,
+			                      nullptr, st),
+			          "conv");
+		hip_check(hipMemcpyAsync(h_err_.as<int>(), d_err_.as<int>(), nc * sizeof(int), hipMemcpyDeviceToHost, st), "err");
+		hip_check(hipMemcpyAsync(h_qcount_.as<int>(), d_qcount_.as<int>(), sizeof(int), hipMemcpyDeviceToHost, st),
+		          "qcount");
+		hip_check(hipStreamSynchronize(st), "sync");
+		check_setup_errors();
+		const int qc = *h_qcount_.as<int>();
+		if (qc <= qcap_) break;
+		ensure_queue((long)qc + 1024);
+	}
+	size_t off = 0;
+	for (int i = 0; i < n; ++i) {
+		const GalaxyDev &g = gal_host_[inst_galaxy_[i]];
+		std::vector<float> canvas((size_t)g.DW * g.DH);
+		const float *src = g.Kx > 0 ? inst_bufs_[4 * i + 2].as<float>() : inst_bufs_[4 * i + 0].as<float>();
+		hip_check(hipMemcpy(canvas.data(), src, canvas.size() * 4, hipMemcpyDeviceToHost), "model");
+		for (int j = 0; j < g.H; ++j)
+			for (int k = 0; k < g.W; ++k) out[off + (size_t)j * g.W + k] = canvas[(size_t)(j + g.pady) * g.DW + k + g.padx];
+		off += (size_t)g.W * g.H;
+	}
+}
+
+void Engine::model_canvas(const ProfileParams *params, float *out)
+{
+	if (ninst() != 1) throw std::invalid_argument("model_canvas needs a batch of one instance");
+	hip_check(hipSetDevice(device_), "hipSetDevice");
+	const int nc = ncomp_total_;
+	std::memcpy(h_params_.as<ProfileParams>(), params, nc * sizeof(ProfileParams));
+	hipStream_t st = stream_;
+	const GalaxyDev *gals = d_gals_.as<GalaxyDev>();
+	const InstDev *insts = d_insts_.as<InstDev>();
+	const GalaxyDev &g = gal_host_[inst_galaxy_[0]];
+	const bool conv = g.Kx > 0;
+	for (int attempt = 0; attempt < 4; ++attempt) {
+		hip_check(hipMemcpyAsync(d_params_.as<ProfileParams>(), h_params_.as<ProfileParams>(), nc * sizeof(ProfileParams),
+		                         hipMemcpyHostToDevice, st),
+		          "params");
+		hip_check(launch_setup(d_params_.as<ProfileParams>(), d_comp_inst_.as<int>(), insts, gals, d_comps_.as<Comp>(),
+		                       d_err_.as<int>(), nc, st),
+		          "setup");
+		hip_check(launch_zero_i(d_qcount_.as<int>(), 1, st), "zero");
+		// convolved profiles into the canvas, the others into the aux canvas
+		hip_check(launch_render(gals, insts, d_at_off_.as<int>(), 1, n_at_blocks_, d_comps_.as<Comp>(),
+		                        d_queue_.as<QEntry>(), d_qcount_.as<int>(), (int)qcap_, conv ? 1 : -1, 0, nullptr, st),
+		          "render");
+		if (conv)
+			hip_check(launch_render(gals, insts, d_at_off_.as<int>(), 1, n_at_blocks_, d_comps_.as<Comp>(),
+			                        d_queue_.as<QEntry>(), d_qcount_.as<int>(), (int)qcap_, 0, 1, nullptr, st),
+			          "render");
+		hip_check(launch_subsample(false, subsample_blocks_, gals, insts, d_comps_.as<Comp>(), d_queue_.as<QEntry>(),
+		                           d_qcount_.as<int>(), (int)qcap_, conv ? 1 : 0, nullptr, st),
+		          "subsample");
+		if (conv)
+			hip_check(launch_conv(1, conv_rows_, n_at_blocks_, kx_max_, kc_max_, gals, insts, d_at_off_.as<int>(), 1,
+			                      nullptr, st),
+			          "conv");
+		hip_check(hipMemcpyAsync(h_err_.as<int>(), d_err_.as<int>(), nc * sizeof(int), hipMemcpyDeviceToHost, st), "err");
+		hip_check(hipMemcpyAsync(h_qcount_.as<int>(), d_qcount_.as<int>(), sizeof(int), hipMemcpyDeviceToHost, st),
+		          "qcount");
+		hip_check(hipStreamSynchronize(st), "sync");
+		check_setup_errors();
+		const int qc = *h_qcount_.as<int>();
+		if (qc <= qcap_) break;
+		ensure_queue((long)qc + 1024);
+	}
+	const size_t nd = (size_t)g.DW * g.DH;
+	if (conv) {
+		std::vector<float> aux(nd);
+		hip_check(hipMemcpy(out, inst_bufs_[2].as<float>(), nd * 4, hipMemcpyDeviceToHost), "model");
+		hip_check(hipMemcpy(aux.data(), inst_bufs_[3].as<float>(), nd * 4, hipMemcpyDeviceToHost), "model");
+		for (size_t k = 0; k < nd; ++k) out[k] += aux[k];
+	}
+	else {
+		hip_check(hipMemcpy(out, inst_bufs_[0].as<float>(), nd * 4, hipMemcpyDeviceToHost), "model");
+	}
+}
+
+void Engine::dilate(const uint8_t *in, uint8_t *out, int w, int h, int padx, int pady, int device)
+{
+	hip_check(hipSetDevice(device), "hipSetDevice");
+	const size_t n = (size_t)w * h;
+	DevBuf a(n), b(n), c(n);
+	hip_check(hipMemcpy(a.as<uint8_t>(), in, n, hipMemcpyHostToDevice), "mask");
+	hip_check(launch_dilate(a.as<uint8_t>(), b.as<uint8_t>(), c.as<uint8_t>(), w, h, padx, pady, nullptr), "dilate");
+	hip_check(hipDeviceSynchronize(), "sync");
+	hip_check(hipMemcpy(out, c.as<uint8_t>(), n, hipMemcpyDeviceToHost), "mask");
+}
+
+void convolve_image(const double *src, int w, int h, const double *krn, int kw, int kh, const uint8_t *mask,
+                    double *out, int device)
+{
+	hip_check(hipSetDevice(device), "hipSetDevice");
+	const int rows = 4;
+	const size_t n = (size_t)w * h;
+	std::vector<float> s(n), f((size_t)kw * kh);
+	for (size_t k = 0; k < n; ++k) s[k] = (float)src[k];
+	for (int ky = 0; ky < kh; ++ky)
+		for (int kx = 0; kx < kw; ++kx) f[(size_t)ky * kw + kx] = (float)krn[(size_t)(kh - 1 - ky) * kw + (kw - 1 - kx)];
+	std::vector<int2> tiles;
+	const int th = 4 * rows;
+	for (int ty = 0; ty * th < h; ++ty)
+		for (int tx = 0; tx * kTileW < w; ++tx) tiles.push_back(make_int2(tx, ty));
+	DevBuf d_src(n * 4), d_out(n * 4), d_krn(f.size() * 4), d_mask(mask ? n : 1), d_tiles(tiles.size() * sizeof(int2));
+	DevBuf d_gal(sizeof(GalaxyDev)), d_inst(sizeof(InstDev)), d_off(2 * sizeof(int));
+	hip_check(hipMemcpy(d_src.as<float>(), s.data(), n * 4, hipMemcpyHostToDevice), "src");
+	hip_check(hipMemcpy(d_krn.as<float>(), f.data(), f.size() * 4, hipMemcpyHostToDevice), "krn");
+	if (mask) hip_check(hipMemcpy(d_mask.as<uint8_t>(), mask, n, hipMemcpyHostToDevice), "mask");
+	hip_check(hipMemcpy(d_tiles.as<int2>(), tiles.data(), tiles.size() * sizeof(int2), hipMemcpyHostToDevice), "tiles");
+	GalaxyDev g{};
+	g.W = g.DW = w;
+	g.H = g.DH = h;
+	g.Kx = kw;
+	g.Ky = kh;
+	g.has_calc = mask != nullptr;
+	g.calc = mask ? d_mask.as<uint8_t>() : nullptr;
+	g.psf_fwd = d_krn.as<float>();
+	g.atiles = d_tiles.as<int2>();
+	g.n_atiles = (int)tiles.size();
+	g.conv_rows = rows;
+	InstDev I{0, 0, 0, 0, d_src.as<float>(), nullptr, d_out.as<float>(), nullptr};
+	const int off[2] = {0, (int)tiles.size()};
+	hip_check(hipMemcpy(d_gal.as<GalaxyDev>(), &g, sizeof(g), hipMemcpyHostToDevice), "gal");
+	hip_check(hipMemcpy(d_inst.as<InstDev>(), &I, sizeof(I), hipMemcpyHostToDevice), "inst");
+	hip_check(hipMemcpy(d_off.as<int>(), off, sizeof(off), hipMemcpyHostToDevice), "off");
+	const int kc = std::min(kh, 64);
+	hip_check(launch_conv(1, rows, (int)tiles.size(), kw, kc, d_gal.as<GalaxyDev>(), d_inst.as<InstDev>(), d_off.as<int>(),
+	                      1, nullptr, nullptr),
+	          "conv");
+	hip_check(hipDeviceSynchronize(), "sync");
+	std::vector<float> o(n);
+	hip_check(hipMemcpy(o.data(), d_out.as<float>(), n * 4, hipMemcpyDeviceToHost), "out");
+	for (size_t k = 0; k < n; ++k) out[k] = o[k];
+}
+
+}  // namespace profit_amd

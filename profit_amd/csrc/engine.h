@@ -1,0 +1,190 @@
+// Host engine: device-resident galaxies (PSF, masks, data) and batches of
+// instances (parameter sets) evaluated by one launch sequence per batch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "device.h"
+
+namespace profit_amd {
+
+// kernel launchers (kernels.hip)
+int launch_setup(const ProfileParams *params, const int *comp_inst, const InstDev *insts, const GalaxyDev *gals,
+                 Comp *comps, int *errors, int ncomp, hipStream_t st);
+int launch_zero_d(double *p, long n, hipStream_t st);
+int launch_zero_i(int *p, long n, hipStream_t st);
+int launch_render(const GalaxyDev *gals, const InstDev *insts, const int *tile_off, int ninst, int nblocks,
+                  const Comp *comps, QEntry *queue, int *qcount, int qcap, int filter, int target, int *diag_cls,
+                  hipStream_t st);
+int launch_subsample(bool grad, int nblocks, const GalaxyDev *gals, const InstDev *insts, const Comp *comps,
+                     const QEntry *queue, const int *qcount, int qcap, int split, double *grad_acc, hipStream_t st);
+int launch_subsample_nodes(int nblocks, const GalaxyDev *gals, const InstDev *insts, const Comp *comps,
+                           const QEntry *queue, const int *qcount, int qcap, int *nodes, hipStream_t st);
+int launch_conv(int mode, int rows, int nblocks, int kx_max, int kc, const GalaxyDev *gals, const InstDev *insts,
+                const int *tile_off, int ninst, double *partial, hipStream_t st);
+int launch_like_finalize(const GalaxyDev *gals, const InstDev *insts, const int *tile_off, int ninst,
+                         const double *partial, double *ll, double *gscale, hipStream_t st);
+int launch_adj_render(const GalaxyDev *gals, const InstDev *insts, const int *tile_off, int ninst, int nblocks,
+                      const Comp *comps, double *grad_acc, hipStream_t st);
+int launch_grad_finalize(const Comp *comps, const double *gscale, const double *grad_acc, double *grad_out,
+                         int ncomp, hipStream_t st);
+int launch_dilate(const uint8_t *in, uint8_t *tmp, uint8_t *out, int w, int h, int padx, int pady, hipStream_t st);
+
+// Convolver::convolve_impl on the GPU: brute force, masked output pixels are 0.
+// The kernel is used as given (libprofit normalises only the Model's PSF).
+void convolve_image(const double *src, int w, int h, const double *krn, int kw, int kh, const uint8_t *mask,
+                    double *out, int device);
+
+class hip_error : public std::runtime_error {
+  public:
+	using std::runtime_error::runtime_error;
+};
+
+void hip_check(hipError_t e, const char *what);
+inline void hip_check(int e, const char *what) { hip_check((hipError_t)e, what); }
+
+// owning device allocation
+class DevBuf {
+  public:
+	DevBuf() = default;
+	explicit DevBuf(size_t bytes) { alloc(bytes); }
+	DevBuf(const DevBuf &) = delete;
+	DevBuf &operator=(const DevBuf &) = delete;
+	DevBuf(DevBuf &&o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+	DevBuf &operator=(DevBuf &&o) noexcept
+	{
+		if (this != &o) {
+			release();
+			p_ = o.p_;
+			n_ = o.n_;
+			o.p_ = nullptr;
+			o.n_ = 0;
+		}
+		return *this;
+	}
+	~DevBuf() { release(); }
+	void alloc(size_t bytes);
+	void release();
+	template <class T> T *as() const { return static_cast<T *>(p_); }
+	size_t size() const { return n_; }
+
+  private:
+	void *p_ = nullptr;
+	size_t n_ = 0;
+};
+
+// pinned host staging
+class HostBuf {
+  public:
+	HostBuf() = default;
+	HostBuf(const HostBuf &) = delete;
+	HostBuf &operator=(const HostBuf &) = delete;
+	~HostBuf() { release(); }
+	void alloc(size_t bytes);
+	void release();
+	template <class T> T *as() const { return static_cast<T *>(p_); }
+	size_t size() const { return n_; }
+
+  private:
+	void *p_ = nullptr;
+	size_t n_ = 0;
+};
+
+// A galaxy as the likelihood engine sees it: ProFit's profitSetupData output.
+struct GalaxySpec {
+	int W = 0, H = 0;                    // image dimensions
+	double scale_x = 1, scale_y = 1;     // image pixel scale
+	double magzero = 0;
+	const double *psf = nullptr;         // PSF (normalised here), Ky rows of Kx
+	int Kx = 0, Ky = 0;
+	const uint8_t *calc = nullptr;       // pre-adjusted calc region (cw x ch) or null
+	int cw = 0, ch = 0;
+	const uint8_t *region = nullptr;     // W*H likelihood region, null = every pixel
+	const double *data = nullptr;        // W*H
+	const double *sigma = nullptr;       // W*H
+	int likefunc = LIKE_CHISQ;
+	int finesampling = 1;                // W, H are then the fine grid; profiles use scale/finesampling
+};
+
+struct EngineStats {
+	double setup_ms = 0, render_ms = 0, subsample_ms = 0, conv_ms = 0, adj_conv_ms = 0, adj_render_ms = 0,
+	       adj_sub_ms = 0, total_ms = 0;
+	int queue_count = 0;
+};
+
+class Engine {
+  public:
+	explicit Engine(int device = 0);
+	~Engine();
+	Engine(const Engine &) = delete;
+	Engine &operator=(const Engine &) = delete;
+
+	int add_galaxy(const GalaxySpec &spec);
+	void set_batch(const std::vector<int> &inst_galaxy, const std::vector<int> &inst_ncomp);
+	// params: one ProfileParams per component, instance-major.  ll: one per
+	// instance; grad (nullable): NGRAD per component.
+	void evaluate(const ProfileParams *params, double *ll, double *grad, bool use_graph = true);
+	// rendering diagnostics for parity tests: per-component class and node maps
+	void diagnostics(const ProfileParams *params, int *cls, int *nodes);
+	// the convolved model of every instance (W*H floats each, region-independent)
+	void model_images(const ProfileParams *params, float *out);
+	// Model::evaluate's drawing canvas for a single instance: convolved profiles
+	// (masked by the calc region) plus the profiles drawn without convolution.
+	void model_canvas(const ProfileParams *params, float *out);
+	// Mask::expand_by on the device
+	static void dilate(const uint8_t *in, uint8_t *out, int w, int h, int padx, int pady, int device = 0);
+
+	int ninst() const { return (int)inst_galaxy_.size(); }
+	int ncomp() const { return ncomp_total_; }
+	int conv_rows() const { return conv_rows_; }
+	void set_conv_rows(int r);
+	void set_timing(bool on) { timing_ = on; }
+	const EngineStats &stats() const { return stats_; }
+	const GalaxyDev &galaxy(int g) const { return gal_host_.at(g); }
+	hipStream_t stream() const { return stream_; }
+
+  private:
+	struct GalaxyBuffers {
+		DevBuf psf_fwd, psf_adj, calc, region, data, ivar, rtiles, ctiles, atiles;
+		std::vector<int2> ct_host, at_host;
+	};
+	void build_tiles(int g);
+	void upload_galaxies();
+	void run(bool grad, bool use_graph);
+	void record(bool grad);
+	void ensure_queue(long need);
+	void check_setup_errors();
+
+	int device_ = 0;
+	hipStream_t stream_ = nullptr;
+	int conv_rows_ = 8;
+	int kc_max_ = 64;
+	bool timing_ = false;
+	EngineStats stats_;
+
+	std::vector<GalaxyDev> gal_host_;
+	std::vector<GalaxyBuffers> gal_bufs_;
+	DevBuf d_gals_;
+	bool gals_dirty_ = true;
+
+	std::vector<int> inst_galaxy_, inst_ncomp_, inst_comp0_;
+	int ncomp_total_ = 0;
+	std::vector<int> ct_off_, at_off_;
+	int n_ct_blocks_ = 0, n_at_blocks_ = 0;
+	int kx_max_ = 0;
+	DevBuf d_insts_, d_ct_off_, d_at_off_, d_comp_inst_;
+	std::vector<DevBuf> inst_bufs_;
+	DevBuf d_params_, d_comps_, d_err_, d_queue_, d_qcount_, d_partial_, d_ll_, d_gscale_, d_grad_acc_, d_grad_out_;
+	long qcap_ = 0;
+	HostBuf h_params_, h_ll_, h_grad_, h_err_, h_qcount_;
+
+	hipGraphExec_t graph_exec_[2] = {nullptr, nullptr};
+	int subsample_blocks_ = 2048;
+};
+
+}  // namespace profit_amd

@@ -1,0 +1,881 @@
+// HIP kernels for the model-image hot path on CDNA4 (gfx950).
+//
+//   k_setup          per-component FP64 initial_calculations (radial.cpp:136-199)
+//   k_render         pixel classification (FP64, bit-exact with libprofit) and
+//                    direct FP32 evaluation (radial.cpp:291-344); pixels that need
+//                    sub-pixel integration are appended to a work queue
+//   k_subsample      one wavefront per queued pixel walks libprofit's adaptive
+//                    recursion (radial.cpp:54-134) depth-first in FP64; the walk
+//                    keeps one node and a 256-bit "cells to refine" mask per level
+//   k_conv           LDS-tiled brute-force convolution (convolve.cpp:158-231) with
+//                    three epilogues: fused masked likelihood, model image, and
+//                    the adjoint correlation
+//   k_like_finalize  compensated FP64 reduction of the likelihood partials
+//   k_adj_render / k_subsample<true>
+//                    fixed-tree adjoint of the rendering: sum_q G(q) d pixel_q / d theta
+//   k_dilate_rows/cols  Mask::expand_by (image.cpp:113-162) as a separable dilation
+//
+// Compiled with -ffp-contract=off: every FP64 operation of the classification
+// and of the recursion rounds exactly as libprofit's own build does.
+#include <hip/hip_runtime.h>
+
+#include "device.h"
+
+namespace profit_amd {
+
+__device__ inline int owner_of(const int *__restrict__ off, int n, int b)
+{
+	int lo = 0, hi = n - 1;
+	while (lo < hi) {
+		const int mid = (lo + hi + 1) >> 1;
+		if (off[mid] <= b) lo = mid;
+		else hi = mid - 1;
+	}
+	return lo;
+}
+
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+	return v;
+}
+
+// ------------------------------------------------------------------ set-up
+
+__global__ void k_setup(const ProfileParams *__restrict__ params, const int *__restrict__ comp_inst,
+                        const InstDev *__restrict__ insts, const GalaxyDev *__restrict__ gals, Comp *comps,
+                        int *errors, int ncomp)
+{
+	const int c = blockIdx.x * blockDim.x + threadIdx.x;
+	if (c >= ncomp) return;
+	const int inst = comp_inst[c];
+	const GalaxyDev &g = gals[insts[inst].galaxy];
+	sf::QuadState qs;
+	Comp out;
+	const int e = setup_component(params[c], g.magzero, g.sx, g.sy, g.img_sx, g.img_sy, g.padx, g.pady, &qs, &out);
+	out.instance = inst;
+	comps[c] = out;
+	errors[c] = e;
+}
+
+__global__ void k_fill_zero_d(double *p, long n)
+{
+	const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) p[i] = 0;
+}
+
+__global__ void k_fill_zero_i(int *p, long n)
+{
+	const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) p[i] = 0;
+}
+
+// ------------------------------------------------------------------ classification
+
+struct PixelClass {
+	double x, y, xp, yp;
+	int cls;  // 1 zero (beyond rscale_max), 2 direct, 3 subsample
+};
+
+__device__ inline PixelClass classify(const Comp &c, double x, double y)
+{
+	PixelClass pc;
+	pc.x = x;
+	pc.y = y;
+	const double xr = x - c.xc, yr = y - c.yc;
+	pc.xp = xr * c.cos_ang + yr * c.sin_ang;
+	pc.yp = -xr * c.sin_over + yr * c.cos_over;
+	const double r = sqrt(pc.xp * pc.xp + pc.yp * pc.yp);
+	if (c.has_rmax && r > c.rmax_r) pc.cls = 1;
+	else if (c.rough || r > c.rsw_r) pc.cls = 2;
+	else pc.cls = 3;
+	return pc;
+}
+
+// ------------------------------------------------------------------ forward render
+
+// Tiles are 64 x (4*rows) pixels; thread (tx, ty) covers column tx and rows
+// ty*rows .. ty*rows+rows-1.  filter: -1 every component, 0/1 only those whose
+// convolve flag equals it.  target: 0 canvas, 1 the aux canvas.
+__global__ __launch_bounds__(256) void k_render(const GalaxyDev *__restrict__ gals, const InstDev *__restrict__ insts,
+                                                const int *__restrict__ tile_off, int ninst,
+                                                const Comp *__restrict__ comps, QEntry *queue, int *qcount,
+                                                int qcap, int filter, int target, int *diag_cls)
+{
+	const int b = blockIdx.x;
+	const int inst = owner_of(tile_off, ninst, b);
+	const InstDev I = insts[inst];
+	const GalaxyDev &g = gals[I.galaxy];
+	const int rows = g.conv_rows;
+	const int2 t = g.atiles[b - tile_off[inst]];
+	const int lane = threadIdx.x & 63;
+	const int i = t.x * kTileW + lane;
+	const int j0 = t.y * 4 * rows + (threadIdx.x >> 6) * rows;
+	const double hx = g.sx / 2., hy = g.sy / 2.;
+	float *out = target ? I.aux : I.canvas;
+	for (int r = 0; r < rows; ++r) {
+		const int j = j0 + r;
+		const bool inb = i < g.DW && j < g.DH;
+		const int pix = i + j * g.DW;
+		const bool active = inb && (!g.has_calc || g.calc[pix]);
+		// radial.cpp:313-314: the x centre uses scale.y and the y centre scale.x
+		const double y = hy + j * g.sx;
+		const double x = hx + i * g.sy;
+		float acc = 0.f;
+		for (int k = 0; k < I.ncomp; ++k) {
+			const Comp &c = comps[I.comp0 + k];
+			if (filter >= 0 && c.convolve != filter) continue;
+			bool sub = false;
+			int cls = 0;
+			if (active) {
+				const PixelClass pc = classify(c, x, y);
+				cls = pc.cls;
+				if (pc.cls == 2) acc += (float)c.flux_scale * eval_at<float>(c, (float)pc.xp, (float)pc.yp);
+				else if (pc.cls == 3) sub = true;
+			}
+			if (diag_cls && inb) diag_cls[(long)(I.comp0 + k) * g.DW * g.DH + pix] = cls;
+			const unsigned long long m = __ballot(sub);
+			if (m) {
+				const int leader = __ffsll((long long)m) - 1;
+				int base = 0;
+				if (lane == leader) base = atomicAdd(qcount, __popcll(m));
+				base = __shfl(base, leader);
+				if (sub) {
+					const int slot = base + __popcll(m & ((1ull << lane) - 1ull));
+					if (slot < qcap) queue[slot] = QEntry{I.comp0 + k, pix};
+				}
+			}
+		}
+		if (inb) out[pix] = acc;
+	}
+}
+
+// ------------------------------------------------------------------ recursion walker
+
+struct WalkLevel {
+	double x0, x1, y0, y1, weight;
+	unsigned long long mask[4];
+};
+
+__device__ inline double cell_centre(double lo, double half, int idx)
+{
+	// the same sequence of additions as libprofit's loops (radial.cpp:86-106)
+	double v = lo;
+	for (int t = 0; t < idx; ++t) {
+		v += half;
+		v += half;
+	}
+	return v + half;
+}
+
+template <bool kGrad>
+__device__ inline void sample_accumulate(const Comp &c, double x, double y, double xp, double yp, double wgt,
+                                         double *gs)
+{
+	double o[9];
+	sample_grad<double>(c, xp, yp, o);
+	const double q = c.axrat;
+	const double fx = o[1], fy = o[2];
+	gs[G_XCEN] += wgt * (fx * -c.cos_ang + fy * (c.sin_ang / q));
+	gs[G_YCEN] += wgt * (fx * -c.sin_ang + fy * (-c.cos_ang / q));
+	gs[G_ANG] += wgt * (fx * (q * yp) + fy * (-xp / q)) * (3.14159265358979323846 / 180.);
+	gs[G_AXRAT] += wgt * (fy * (-yp / q));
+	gs[G_BOX] += wgt * o[3];
+#pragma unroll
+	for (int s = 0; s < 5; ++s) gs[G_S0 + s] += wgt * o[4 + s];
+}
+
+// Walks the recursion tree of one pixel.  Every lane returns its share of
+// sum_s weight_s * f(s) (and of the derivative sums when kGrad); `scale`
+// multiplies the weights (G(q) * flux_scale in the adjoint, 1 otherwise).
+template <bool kGrad>
+__device__ void walk_pixel(const Comp &c, WalkLevel *L, int lane, double x0, double x1, double y0, double y1, int res,
+                           int maxrec, double scale, double &sum, double *gs)
+{
+	const int ncell = res * res;
+	const int nchunk = (ncell + 63) >> 6;
+	const double inv_cells = 1.0 / (double)ncell;
+	int level = 0;
+	double nx0 = x0, nx1 = x1, ny0 = y0, ny1 = y1;
+	double weight = inv_cells;
+	for (;;) {
+		const double xbin = (nx1 - nx0) / res;
+		const double ybin = (ny1 - ny0) / res;
+		const double hx = xbin / 2., hy = ybin / 2.;
+		const bool recurse = res > 1 && level < maxrec;
+		unsigned long long m[4] = {0ull, 0ull, 0ull, 0ull};
+		for (int k = 0; k < nchunk; ++k) {
+			const int cell = k * 64 + lane;
+			bool fail = false;
+			if (cell < ncell) {
+				const int ci = cell / res, cj = cell - ci * res;
+				const double x = cell_centre(nx0, hx, ci);
+				const double y = cell_centre(ny0, hy, cj);
+				const double xr = x - c.xc, yr = y - c.yc;
+				const double xp = xr * c.cos_ang + yr * c.sin_ang;
+				const double yp = -xr * c.sin_over + yr * c.cos_over;
+				const double val = eval_at<double>(c, xp, yp);
+				if (recurse) {
+					const double dyp = -xbin * c.sin_over + ybin * c.cos_over;
+					const double test = eval_at<double>(c, fabs(xp), fabs(yp) + fabs(dyp));
+					fail = fabs(test - val) > c.acc * val;
+				}
+				if (!fail) {
+					sum += val * (weight * scale);
+					if (kGrad) sample_accumulate<kGrad>(c, x, y, xp, yp, weight * scale, gs);
+				}
+			}
+			m[k] = __ballot(fail);
+		}
+		if (lane == 0) {
+			L[level].x0 = nx0;
+			L[level].x1 = nx1;
+			L[level].y0 = ny0;
+			L[level].y1 = ny1;
+			L[level].weight = weight;
+#pragma unroll
+			for (int k = 0; k < 4; ++k) L[level].mask[k] = m[k];
+		}
+		__builtin_amdgcn_wave_barrier();
+		// next node: lowest pending cell of the deepest level that has one
+		int lv = level;
+		int cell = -1;
+		for (;;) {
+			int found = -1;
+#pragma unroll
+			for (int k = 0; k < 4; ++k) {
+				const unsigned long long mk = L[lv].mask[k];
+				if (found < 0 && mk) {
+					found = k * 64 + (__ffsll((long long)mk) - 1);
+				}
+			}
+			if (found >= 0) {
+				cell = found;
+				break;
+			}
+			if (lv == 0) break;
+			--lv;
+		}
+		if (cell < 0) break;
+		__builtin_amdgcn_wave_barrier();
+		if (lane == 0) L[lv].mask[cell >> 6] &= ~(1ull << (cell & 63));
+		const double px0 = L[lv].x0, px1 = L[lv].x1, py0 = L[lv].y0, py1 = L[lv].y1;
+		const double pxbin = (px1 - px0) / res, pybin = (py1 - py0) / res;
+		const double phx = pxbin / 2., phy = pybin / 2.;
+		const int ci = cell / res, cj = cell - ci * res;
+		const double cx = cell_centre(px0, phx, ci);
+		const double cy = cell_centre(py0, phy, cj);
+		nx0 = cx - phx;
+		nx1 = cx + phx;
+		ny0 = cy - phy;
+		ny1 = cy + phy;
+		weight = L[lv].weight * inv_cells;
+		level = lv + 1;
+		__builtin_amdgcn_wave_barrier();
+	}
+}
+
+// One wavefront per queued pixel.  kGrad=false: adds flux_scale * value to the
+// canvas.  kGrad=true: accumulates G(q) * d(flux_scale * value)/d slot; per-lane
+// sums are carried across consecutive entries of the same component and flushed
+// with one set of atomics per run.
+template <bool kGrad>
+__global__ __launch_bounds__(256) void k_subsample(const GalaxyDev *__restrict__ gals,
+                                                   const InstDev *__restrict__ insts,
+                                                   const Comp *__restrict__ comps, const QEntry *__restrict__ queue,
+                                                   const int *__restrict__ qcount, int qcap, int chunk,
+                                                   int split, double *grad_acc)
+{
+	__shared__ WalkLevel levels[4][kMaxLevels];
+	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+	WalkLevel *L = levels[wv];
+	const int gw = blockIdx.x * 4 + wv, nw = gridDim.x * 4;
+	int count = *qcount;
+	if (count > qcap) count = qcap;
+	double gs[NGRAD];
+#pragma unroll
+	for (int s = 0; s < NGRAD; ++s) gs[s] = 0;
+	double vsum = 0;
+	int cur = -1;
+	for (int base = gw * chunk; base < count; base += nw * chunk) {
+		const int end = min(base + chunk, count);
+		for (int e = base; e < end; ++e) {
+			const QEntry q = queue[e];
+			const Comp &c = comps[q.comp];
+			if (kGrad && q.comp != cur) {
+				if (cur >= 0) {
+					const Comp &pc = comps[cur];
+					const double tv = wave_sum(vsum);
+#pragma unroll
+					for (int s = 0; s < NGRAD; ++s) {
+						const double t = wave_sum(gs[s]) + pc.dlnfs[s] * tv;
+						if (lane == 0 && t != 0) atomicAdd(&grad_acc[(long)cur * NGRAD + s], t);
+						gs[s] = 0;
+					}
+					vsum = 0;
+				}
+				cur = q.comp;
+			}
+			const InstDev &I = insts[c.instance];
+			const GalaxyDev &g = gals[I.galaxy];
+			double scale = 1.0;
+			if (kGrad) {
+				const float Gq = I.G[q.pix];
+				if (Gq == 0.f) continue;
+				scale = (double)Gq * c.flux_scale;
+			}
+			const int i = q.pix % g.DW, j = q.pix / g.DW;
+			const double hx = g.sx / 2., hy = g.sy / 2.;
+			const double y = hy + j * g.sx;
+			const double x = hx + i * g.sy;
+			int res = c.res, maxrec = c.maxrec;
+			// sersic.cpp:331-347: centre pixels of nser > 1 profiles
+			if (c.centre_rule && fabs(x - c.xc) < c.img_sx && fabs(y - c.yc) < c.img_sy) {
+				res = 8;
+				maxrec = 10;
+			}
+			double s_local = 0;
+			walk_pixel<kGrad>(c, L, lane, x - hx, x + hx, y - hy, y + hy, res, maxrec, scale, s_local, gs);
+			if (kGrad) {
+				vsum += s_local;
+			}
+			else {
+				const double tot = wave_sum(s_local);
+				float *dst = (split && !c.convolve) ? I.aux : I.canvas;
+				if (lane == 0) atomicAdd(&dst[q.pix], (float)(c.flux_scale * tot));
+			}
+		}
+	}
+	if (kGrad && cur >= 0) {
+		const Comp &pc = comps[cur];
+		const double tv = wave_sum(vsum);
+#pragma unroll
+		for (int s = 0; s < NGRAD; ++s) {
+			const double t = wave_sum(gs[s]) + pc.dlnfs[s] * tv;
+			if (lane == 0 && t != 0) atomicAdd(&grad_acc[(long)cur * NGRAD + s], t);
+		}
+	}
+}
+
+// Counts the nodes visited by the recursion of each queued pixel (diagnostics
+// for decision parity with libprofit).
+__global__ __launch_bounds__(256) void k_subsample_nodes(const GalaxyDev *__restrict__ gals,
+                                                         const InstDev *__restrict__ insts,
+                                                         const Comp *__restrict__ comps,
+                                                         const QEntry *__restrict__ queue,
+                                                         const int *__restrict__ qcount, int qcap, int *nodes_out)
+{
+	__shared__ WalkLevel levels[4][kMaxLevels];
+	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+	WalkLevel *L = levels[wv];
+	const int gw = blockIdx.x * 4 + wv, nw = gridDim.x * 4;
+	int count = *qcount;
+	if (count > qcap) count = qcap;
+	for (int e = gw; e < count; e += nw) {
+		const QEntry q = queue[e];
+		const Comp &c = comps[q.comp];
+		const InstDev &I = insts[c.instance];
+		const GalaxyDev &g = gals[I.galaxy];
+		const int i = q.pix % g.DW, j = q.pix / g.DW;
+		const double hx = g.sx / 2., hy = g.sy / 2.;
+		const double y = hy + j * g.sx, x = hx + i * g.sy;
+		int res = c.res, maxrec = c.maxrec;
+		if (c.centre_rule && fabs(x - c.xc) < c.img_sx && fabs(y - c.yc) < c.img_sy) {
+			res = 8;
+			maxrec = 10;
+		}
+		// count nodes: one per processed node, i.e. 1 + number of refined cells
+		const int ncell = res * res, nchunk = (ncell + 63) >> 6;
+		int nodes = 0;
+		int level = 0;
+		double nx0 = x - hx, nx1 = x + hx, ny0 = y - hy, ny1 = y + hy;
+		for (;;) {
+			++nodes;
+			const double xbin = (nx1 - nx0) / res, ybin = (ny1 - ny0) / res;
+			const double chx = xbin / 2., chy = ybin / 2.;
+			const bool recurse = res > 1 && level < maxrec;
+			unsigned long long m[4] = {0ull, 0ull, 0ull, 0ull};
+			for (int k = 0; k < nchunk; ++k) {
+				const int cell = k * 64 + lane;
+				bool fail = false;
+				if (cell < ncell && recurse) {
+					const int ci = cell / res, cj = cell - ci * res;
+					const double cx = cell_centre(nx0, chx, ci), cy = cell_centre(ny0, chy, cj);
+					const double xr = cx - c.xc, yr = cy - c.yc;
+					const double xp = xr * c.cos_ang + yr * c.sin_ang;
+					const double yp = -xr * c.sin_over + yr * c.cos_over;
+					const double val = eval_at<double>(c, xp, yp);
+					const double dyp = -xbin * c.sin_over + ybin * c.cos_over;
+					const double test = eval_at<double>(c, fabs(xp), fabs(yp) + fabs(dyp));
+					fail = fabs(test - val) > c.acc * val;
+				}
+				m[k] = __ballot(fail);
+			}
+			if (lane == 0) {
+				L[level].x0 = nx0;
+				L[level].x1 = nx1;
+				L[level].y0 = ny0;
+				L[level].y1 = ny1;
+#pragma unroll
+				for (int k = 0; k < 4; ++k) L[level].mask[k] = m[k];
+			}
+			__builtin_amdgcn_wave_barrier();
+			int lv = level, cell = -1;
+			for (;;) {
+				int found = -1;
+#pragma unroll
+				for (int k = 0; k < 4; ++k) {
+					const unsigned long long mk = L[lv].mask[k];
+					if (found < 0 && mk) found = k * 64 + (__ffsll((long long)mk) - 1);
+				}
+				if (found >= 0) {
+					cell = found;
+					break;
+				}
+				if (lv == 0) break;
+				--lv;
+			}
+			if (cell < 0) break;
+			__builtin_amdgcn_wave_barrier();
+			if (lane == 0) L[lv].mask[cell >> 6] &= ~(1ull << (cell & 63));
+			const double px0 = L[lv].x0, px1 = L[lv].x1, py0 = L[lv].y0, py1 = L[lv].y1;
+			const double phx = ((px1 - px0) / res) / 2., phy = ((py1 - py0) / res) / 2.;
+			const int ci = cell / res, cj = cell - ci * res;
+			const double cx = cell_centre(px0, phx, ci), cy = cell_centre(py0, phy, cj);
+			nx0 = cx - phx;
+			nx1 = cx + phx;
+			ny0 = cy - phy;
+			ny1 = cy + phy;
+			level = lv + 1;
+			__builtin_amdgcn_wave_barrier();
+		}
+		if (lane == 0) nodes_out[(long)q.comp * g.DW * g.DH + q.pix] = nodes;
+	}
+}
+
+// ------------------------------------------------------------------ convolution
+
+enum ConvMode : int { CONV_LIKE = 0, CONV_MODEL = 1, CONV_ADJ = 2 };
+
+// out(p) = sum_{ky,kx} in(p.y + oy + ky, p.x + ox + kx) * wt[ky*Kx + kx],
+// in() is zero outside its image.  Block = 64 columns x 4 row groups; each
+// thread produces R consecutive output rows of one column with a sliding
+// register window along ky and PSF weights read through the scalar cache.
+// The kernel rows are processed in chunks of at most KC rows held in LDS.
+template <int R, int MODE>
+__global__ __launch_bounds__(256) void k_conv(const GalaxyDev *__restrict__ gals, const InstDev *__restrict__ insts,
+                                              const int *__restrict__ tile_off, int ninst, int kc_max,
+                                              double *partial)
+{
+	extern __shared__ float lds[];
+	const int b = blockIdx.x;
+	const int inst = owner_of(tile_off, ninst, b);
+	const InstDev I = insts[inst];
+	const GalaxyDev &g = gals[I.galaxy];
+	const int2 t = (MODE == CONV_LIKE) ? g.ctiles[b - tile_off[inst]] : g.atiles[b - tile_off[inst]];
+	const int Kx = g.Kx, Ky = g.Ky;
+	const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+	const int x0 = t.x * kTileW, y0 = t.y * 4 * R;
+
+	const float *__restrict__ in;
+	const float *__restrict__ wt;
+	int IW, IH, ox, oy, OW, OH;
+	if (MODE == CONV_ADJ) {
+		in = I.w;
+		IW = g.W;
+		IH = g.H;
+		OW = g.DW;
+		OH = g.DH;
+		ox = -g.padx + Kx / 2 - Kx + 1;
+		oy = -g.pady + Ky / 2 - Ky + 1;
+		wt = g.psf_adj;
+	}
+	else {
+		in = I.canvas;
+		IW = g.DW;
+		IH = g.DH;
+		if (MODE == CONV_LIKE) {
+			OW = g.W;
+			OH = g.H;
+			ox = g.padx - Kx / 2;
+			oy = g.pady - Ky / 2;
+		}
+		else {
+			OW = g.DW;
+			OH = g.DH;
+			ox = -(Kx / 2);
+			oy = -(Ky / 2);
+		}
+		wt = g.psf_fwd;
+	}
+	const int LW = kTileW + Kx - 1;
+
+	float acc[R];
+#pragma unroll
+	for (int r = 0; r < R; ++r) acc[r] = 0.f;
+
+	for (int ky0 = 0; ky0 < Ky; ky0 += kc_max) {
+		const int kc = min(kc_max, Ky - ky0);
+		const int LH = 4 * R + kc - 1;
+		__syncthreads();
+		const int gy0 = y0 + oy + ky0, gx0 = x0 + ox;
+		for (int idx = threadIdx.x; idx < LH * LW; idx += 256) {
+			const int rr = idx / LW, cc = idx - rr * LW;
+			const int gy = gy0 + rr, gx = gx0 + cc;
+			float v = 0.f;
+			if (gy >= 0 && gy < IH && gx >= 0 && gx < IW) v = in[(long)gy * IW + gx];
+			lds[idx] = v;
+		}
+		__syncthreads();
+		const float *colbase = lds + ty * R * LW + tx;
+		for (int kx = 0; kx < Kx; ++kx) {
+			const float *col = colbase + kx;
+			const float *wcol = wt + (long)ky0 * Kx + kx;
+			float win[R];
+#pragma unroll
+			for (int r = 0; r < R - 1; ++r) win[r] = col[r * LW];
+			int ky = 0;
+			for (; ky + R <= kc; ky += R) {
+#pragma unroll
+				for (int u = 0; u < R; ++u) {
+					win[(u + R - 1) % R] = col[(ky + u + R - 1) * LW];
+					const float w = wcol[(ky + u) * Kx];
+#pragma unroll
+					for (int r = 0; r < R; ++r) acc[r] = fmaf(win[(u + r) % R], w, acc[r]);
+				}
+			}
+			for (; ky < kc; ++ky) {
+				const float w = wcol[ky * Kx];
+#pragma unroll
+				for (int r = 0; r < R; ++r) acc[r] = fmaf(col[(ky + r) * LW], w, acc[r]);
+			}
+		}
+	}
+
+	const int oxx = x0 + tx;
+	if (MODE == CONV_LIKE) {
+		double part = 0;
+		const bool pois = g.likefunc == LIKE_POIS;
+		const double s = g.pois_scale;
+#pragma unroll
+		for (int r = 0; r < R; ++r) {
+			const int oyy = y0 + ty * R + r;
+			if (oxx < OW && oyy < OH) {
+				const long o = (long)oyy * OW + oxx;
+				float wv = 0.f;
+				if (g.region[o]) {
+					// the model is zero outside the calc region (libprofit masks the
+					// convolution output there); such pixels do not feed the adjoint
+					const bool in_calc = !g.has_calc || g.calc[(long)(oyy + g.pady) * g.DW + oxx + g.padx];
+					const float m = in_calc ? acc[r] : 0.f;
+					const float d = g.data[o];
+					if (pois) {
+						const double mm = s * (double)m, dd = s * (double)d;
+						part += mm - dd * log(mm);
+						wv = (float)(-s * (1.0 - (double)d / (double)m));
+					}
+					else {
+						const float diff = d - m;
+						const float iv = g.ivar[o];
+						part += (double)(diff * diff * iv);
+						wv = diff * iv;
+					}
+					if (!in_calc) wv = 0.f;
+				}
+				I.w[o] = wv;
+			}
+		}
+		__shared__ double red[4];
+		part = wave_sum(part);
+		if (tx == 0) red[ty] = part;
+		__syncthreads();
+		if (threadIdx.x == 0) partial[b] = (red[0] + red[1]) + (red[2] + red[3]);
+	}
+	else {
+		float *out = I.G;
+#pragma unroll
+		for (int r = 0; r < R; ++r) {
+			const int oyy = y0 + ty * R + r;
+			if (oxx < OW && oyy < OH) {
+				const long o = (long)oyy * OW + oxx;
+				float v = acc[r];
+				if (MODE == CONV_MODEL && g.has_calc && !g.calc[o]) v = 0.f;
+				out[o] = v;
+			}
+		}
+	}
+}
+
+// ------------------------------------------------------------------ likelihood finalisation
+
+__global__ void k_like_finalize(const GalaxyDev *__restrict__ gals, const InstDev *__restrict__ insts,
+                                const int *__restrict__ tile_off, int ninst, const double *__restrict__ partial,
+                                double *ll, double *gscale)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= ninst) return;
+	const GalaxyDev &g = gals[insts[i].galaxy];
+	// Neumaier-compensated sum of the block partials
+	double s = 0, comp = 0;
+	for (int b = tile_off[i]; b < tile_off[i + 1]; ++b) {
+		const double v = partial[b];
+		const double tsum = s + v;
+		if (fabs(s) >= fabs(v)) comp += (s - tsum) + v;
+		else comp += (v - tsum) + s;
+		s = tsum;
+	}
+	const double X = s + comp;
+	const double N = g.n_region;
+	double L = 0, gsc = 1;
+	switch (g.likefunc) {
+	case LIKE_CHISQ: {
+		// dchisq(X, N, log=TRUE) = dgamma(X, N/2, scale=2, log=TRUE)
+		const double k = 0.5 * N;
+		if (k >= 1) L = (k - 1) * log(X / 2) - X / 2 - lgamma(k) - log(2.0);
+		else L = k * log(X / 2) - X / 2 - lgamma(k) - log(X);
+		gsc = -2 * ((k - 1) / X - 0.5);
+		break;
+	}
+	case LIKE_NORM:
+		L = -N * 0.918938533204672741780329736406 - 0.5 * X;
+		gsc = 1;
+		break;
+	case LIKE_POIS:
+		L = -X;
+		gsc = 1;
+		break;
+	default: L = NAN; break;
+	}
+	ll[i] = L;
+	gscale[i] = gsc;
+}
+
+// ------------------------------------------------------------------ adjoint render
+
+__global__ __launch_bounds__(256) void k_adj_render(const GalaxyDev *__restrict__ gals,
+                                                    const InstDev *__restrict__ insts,
+                                                    const int *__restrict__ tile_off, int ninst,
+                                                    const Comp *__restrict__ comps, double *grad_acc)
+{
+	__shared__ double red[4][NGRAD];
+	const int b = blockIdx.x;
+	const int inst = owner_of(tile_off, ninst, b);
+	const InstDev I = insts[inst];
+	const GalaxyDev &g = gals[I.galaxy];
+	const int rows = g.conv_rows;
+	const int2 t = g.atiles[b - tile_off[inst]];
+	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+	const int i = t.x * kTileW + lane;
+	const int j0 = t.y * 4 * rows + wv * rows;
+	const double hx = g.sx / 2., hy = g.sy / 2.;
+	for (int k = 0; k < I.ncomp; ++k) {
+		const Comp &c = comps[I.comp0 + k];
+		double gl[NGRAD];
+#pragma unroll
+		for (int s = 0; s < NGRAD; ++s) gl[s] = 0;
+		for (int r = 0; r < rows; ++r) {
+			const int j = j0 + r;
+			if (i >= g.DW || j >= g.DH) continue;
+			const int pix = i + j * g.DW;
+			if (g.has_calc && !g.calc[pix]) continue;
+			const float Gq = I.G[pix];
+			if (Gq == 0.f) continue;
+			const double y = hy + j * g.sx, x = hx + i * g.sy;
+			const PixelClass pc = classify(c, x, y);
+			if (pc.cls != 2) continue;
+			float o[9];
+			const float xp = (float)pc.xp, yp = (float)pc.yp;
+			sample_grad<float>(c, xp, yp, o);
+			const float q = (float)c.axrat;
+			const float ca = (float)c.cos_ang, sa = (float)c.sin_ang;
+			const float fx = o[1], fy = o[2];
+			const double wq = (double)Gq * c.flux_scale;
+			const double f = o[0];
+			gl[G_XCEN] += wq * (double)(fx * -ca + fy * (sa / q));
+			gl[G_YCEN] += wq * (double)(fx * -sa + fy * (-ca / q));
+			gl[G_ANG] += wq * (double)((fx * (q * yp) + fy * (-xp / q)) * (3.14159265358979323846f / 180.f));
+			gl[G_AXRAT] += wq * (double)(fy * (-yp / q));
+			gl[G_BOX] += wq * (double)o[3];
+#pragma unroll
+			for (int s = 0; s < 5; ++s) gl[G_S0 + s] += wq * (double)o[4 + s];
+			// flux-scale dependence: d fs / d slot = fs * dlnfs
+#pragma unroll
+			for (int s = 0; s < NGRAD; ++s) gl[s] += wq * f * c.dlnfs[s];
+		}
+#pragma unroll
+		for (int s = 0; s < NGRAD; ++s) {
+			const double v = wave_sum(gl[s]);
+			if (lane == 0) red[wv][s] = v;
+		}
+		__syncthreads();
+		if (threadIdx.x < NGRAD) {
+			const int s = threadIdx.x;
+			const double v = (red[0][s] + red[1][s]) + (red[2][s] + red[3][s]);
+			if (v != 0) atomicAdd(&grad_acc[(long)(I.comp0 + k) * NGRAD + s], v);
+		}
+		__syncthreads();
+	}
+}
+
+__global__ void k_grad_finalize(const Comp *__restrict__ comps, const double *__restrict__ gscale,
+                                const double *__restrict__ grad_acc, double *grad_out, int ncomp)
+{
+	const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+	if (idx >= ncomp * NGRAD) return;
+	const int c = idx / NGRAD;
+	grad_out[idx] = gscale[comps[c].instance] * grad_acc[idx];
+}
+
+// ------------------------------------------------------------------ masks
+
+// Mask::expand_by: true where any input pixel within +-pad is true.  Done as a
+// row pass followed by a column pass (box dilation is separable).
+__global__ void k_dilate_rows(const uint8_t *__restrict__ in, uint8_t *out, int w, int h, int pad)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	const int j = blockIdx.y;
+	if (i >= w || j >= h) return;
+	const int lo = max(0, i - pad), hi = min(w - 1, i + pad);
+	uint8_t v = 0;
+	for (int k = lo; k <= hi && !v; ++k) v = in[(long)j * w + k] ? 1 : 0;
+	out[(long)j * w + i] = v;
+}
+
+__global__ void k_dilate_cols(const uint8_t *__restrict__ in, uint8_t *out, int w, int h, int pad)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	const int j = blockIdx.y;
+	if (i >= w || j >= h) return;
+	const int lo = max(0, j - pad), hi = min(h - 1, j + pad);
+	uint8_t v = 0;
+	for (int k = lo; k <= hi && !v; ++k) v = in[(long)k * w + i] ? 1 : 0;
+	out[(long)j * w + i] = v;
+}
+
+// ------------------------------------------------------------------ launch wrappers
+
+#define PA_CHECK(x)                                  \
+	do {                                             \
+		hipError_t e_ = (x);                         \
+		if (e_ != hipSuccess) return (int)e_;        \
+	} while (0)
+
+int launch_setup(const ProfileParams *params, const int *comp_inst, const InstDev *insts, const GalaxyDev *gals,
+                 Comp *comps, int *errors, int ncomp, hipStream_t st)
+{
+	if (ncomp <= 0) return 0;
+	k_setup<<<(ncomp + 63) / 64, 64, 0, st>>>(params, comp_inst, insts, gals, comps, errors, ncomp);
+	return (int)hipGetLastError();
+}
+
+int launch_zero_d(double *p, long n, hipStream_t st)
+{
+	if (n <= 0) return 0;
+	k_fill_zero_d<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(p, n);
+	return (int)hipGetLastError();
+}
+
+int launch_zero_i(int *p, long n, hipStream_t st)
+{
+	if (n <= 0) return 0;
+	k_fill_zero_i<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(p, n);
+	return (int)hipGetLastError();
+}
+
+int launch_render(const GalaxyDev *gals, const InstDev *insts, const int *tile_off, int ninst, int nblocks,
+                  const Comp *comps, QEntry *queue, int *qcount, int qcap, int filter, int target, int *diag_cls,
+                  hipStream_t st)
+{
+	if (nblocks <= 0) return 0;
+	k_render<<<nblocks, 256, 0, st>>>(gals, insts, tile_off, ninst, comps, queue, qcount, qcap, filter, target,
+	                                  diag_cls);
+	return (int)hipGetLastError();
+}
+
+int launch_subsample(bool grad, int nblocks, const GalaxyDev *gals, const InstDev *insts, const Comp *comps,
+                     const QEntry *queue, const int *qcount, int qcap, int split, double *grad_acc, hipStream_t st)
+{
+	if (nblocks <= 0) return 0;
+	if (grad)
+		k_subsample<true><<<nblocks, 256, 0, st>>>(gals, insts, comps, queue, qcount, qcap, 4, split, grad_acc);
+	else
+		k_subsample<false><<<nblocks, 256, 0, st>>>(gals, insts, comps, queue, qcount, qcap, 1, split, grad_acc);
+	return (int)hipGetLastError();
+}
+
+int launch_subsample_nodes(int nblocks, const GalaxyDev *gals, const InstDev *insts, const Comp *comps,
+                           const QEntry *queue, const int *qcount, int qcap, int *nodes, hipStream_t st)
+{
+	if (nblocks <= 0) return 0;
+	k_subsample_nodes<<<nblocks, 256, 0, st>>>(gals, insts, comps, queue, qcount, qcap, nodes);
+	return (int)hipGetLastError();
+}
+
+template <int R, int MODE>
+static int conv_launch(int nblocks, size_t lds, const GalaxyDev *gals, const InstDev *insts, const int *tile_off,
+                       int ninst, int kc, double *partial, hipStream_t st)
+{
+	if (lds > 64 * 1024) {
+		PA_CHECK(hipFuncSetAttribute((const void *)k_conv<R, MODE>, hipFuncAttributeMaxDynamicSharedMemorySize,
+		                             (int)lds));
+	}
+	k_conv<R, MODE><<<nblocks, 256, lds, st>>>(gals, insts, tile_off, ninst, kc, partial);
+	return (int)hipGetLastError();
+}
+
+int launch_conv(int mode, int rows, int nblocks, int kx_max, int kc, const GalaxyDev *gals, const InstDev *insts,
+                const int *tile_off, int ninst, double *partial, hipStream_t st)
+{
+	if (nblocks <= 0) return 0;
+	const size_t lds = sizeof(float) * (size_t)(4 * rows + kc - 1) * (size_t)(kTileW + kx_max - 1);
+	if (lds > 160 * 1024) return (int)hipErrorInvalidValue;
+#define PA_CONV_CASE(RR)                                                                                  \
+	case RR:                                                                                              \
+		if (mode == CONV_LIKE) return conv_launch<RR, CONV_LIKE>(nblocks, lds, gals, insts, tile_off, ninst, kc, partial, st); \
+		if (mode == CONV_MODEL) return conv_launch<RR, CONV_MODEL>(nblocks, lds, gals, insts, tile_off, ninst, kc, partial, st); \
+		return conv_launch<RR, CONV_ADJ>(nblocks, lds, gals, insts, tile_off, ninst, kc, partial, st);
+	switch (rows) {
+		PA_CONV_CASE(2)
+		PA_CONV_CASE(4)
+		PA_CONV_CASE(8)
+		PA_CONV_CASE(16)
+	default: return (int)hipErrorInvalidValue;
+	}
+#undef PA_CONV_CASE
+}
+
+int launch_like_finalize(const GalaxyDev *gals, const InstDev *insts, const int *tile_off, int ninst,
+                         const double *partial, double *ll, double *gscale, hipStream_t st)
+{
+	if (ninst <= 0) return 0;
+	k_like_finalize<<<(ninst + 63) / 64, 64, 0, st>>>(gals, insts, tile_off, ninst, partial, ll, gscale);
+	return (int)hipGetLastError();
+}
+
+int launch_adj_render(const GalaxyDev *gals, const InstDev *insts, const int *tile_off, int ninst, int nblocks,
+                      const Comp *comps, double *grad_acc, hipStream_t st)
+{
+	if (nblocks <= 0) return 0;
+	k_adj_render<<<nblocks, 256, 0, st>>>(gals, insts, tile_off, ninst, comps, grad_acc);
+	return (int)hipGetLastError();
+}
+
+int launch_grad_finalize(const Comp *comps, const double *gscale, const double *grad_acc, double *grad_out,
+                         int ncomp, hipStream_t st)
+{
+	if (ncomp <= 0) return 0;
+	const int n = ncomp * NGRAD;
+	k_grad_finalize<<<(n + 255) / 256, 256, 0, st>>>(comps, gscale, grad_acc, grad_out, ncomp);
+	return (int)hipGetLastError();
+}
+
+int launch_dilate(const uint8_t *in, uint8_t *tmp, uint8_t *out, int w, int h, int padx, int pady, hipStream_t st)
+{
+	dim3 grid((w + 255) / 256, h);
+	k_dilate_rows<<<grid, 256, 0, st>>>(in, tmp, w, h, padx);
+	k_dilate_cols<<<grid, 256, 0, st>>>(tmp, out, w, h, pady);
+	return (int)hipGetLastError();
+}
+
+}  // namespace profit_amd

@@ -1,0 +1,610 @@
+// Radial-profile maths shared by host set-up and the GPU kernels.
+//
+// Reference (paths relative to src/libprofit/src/profit/):
+//   radial.cpp:46-52    image -> profile coordinates
+//   radial.cpp:136-199  RadialProfile::initial_calculations (adjust=true rewrites
+//                       rscale_switch, resolution, rscale_max and acc)
+//   radial.cpp:204-214  RadialProfile::validate
+//   radial.h:93-102     boxy_r
+//   sersic.cpp:79-360   Sersic: pow-free r_factor templates, b_n = qgamma(0.5, 2n),
+//                       lumtot, fluxfrac-based adjustments, centre-pixel rule
+//   moffat.cpp:52-107, ferrer.cpp:50-125, king.cpp:51-118,
+//   coresersic.cpp:51-140, brokenexponential.cpp:38-149
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "special.h"
+
+namespace profit_amd {
+
+#define PA_HD __host__ __device__ inline
+
+enum ProfileType : int32_t { SERSIC = 0, MOFFAT = 1, FERRER = 2, KING = 3, CORESERSIC = 4, BROKENEXP = 5 };
+
+// gradient slots per component
+enum GradSlot : int { G_XCEN = 0, G_YCEN, G_MAG, G_ANG, G_AXRAT, G_BOX, G_S0, G_S1, G_S2, G_S3, G_S4, NGRAD };
+
+// The libprofit parameters of one radial profile; the C-ABI struct
+// profit_amd_profile has the identical layout.
+struct ProfileParams {
+	int32_t type;
+	int32_t convolve, rough, adjust, rescale_flux;
+	int32_t reserved;
+	double xcen, ycen, mag, ang, axrat, box;
+	double acc, rscale_switch, rscale_max;
+	uint32_t resolution, max_recursions;
+	double shape[5];
+};
+
+PA_HD void default_params(int type, ProfileParams *p)
+{
+	p->type = type;
+	p->convolve = 0;
+	p->rough = 0;
+	p->adjust = 1;
+	p->rescale_flux = 0;
+	p->reserved = 0;
+	p->xcen = 0;
+	p->ycen = 0;
+	p->mag = 15;
+	p->ang = 0;
+	p->axrat = 1;
+	p->box = 0;
+	p->acc = 0.1;
+	p->rscale_switch = 1;
+	p->rscale_max = 0;
+	p->resolution = 9;
+	p->max_recursions = 2;
+	for (int i = 0; i < 5; ++i) p->shape[i] = 0;
+	switch (type) {
+	case SERSIC: p->shape[0] = 1; p->shape[1] = 1; break;
+	case MOFFAT: p->shape[0] = 3; p->shape[1] = 2; break;
+	case FERRER: p->shape[0] = 3; p->shape[1] = 1; p->shape[2] = 1; p->acc = 1; break;
+	case KING: p->shape[0] = 1; p->shape[1] = 3; p->shape[2] = 2; break;
+	case CORESERSIC: p->shape[0] = 1; p->shape[1] = 1; p->shape[2] = 4; p->shape[3] = 1; p->shape[4] = 1; break;
+	case BROKENEXP: p->shape[0] = 1; p->shape[1] = 1; p->shape[2] = 1; p->shape[3] = 1; break;
+	default: break;
+	}
+}
+
+// Sersic r_factor specialisations (sersic.cpp:79-163)
+enum SersicKind : int32_t { K_HALF = 0, K_ONE, K_TWO, K_THREE, K_FOUR, K_EIGHT, K_SIXTEEN, K_GENERAL };
+
+// Device descriptor of one component after initial_calculations.
+struct Comp {
+	int32_t type, kind, res, maxrec;
+	int32_t rough, centre_rule, boxy, has_rmax;
+	int32_t convolve, instance, grad_ok, adjust;
+	double xc, yc;                    // centre after the PSF-padding shift
+	double cos_ang, sin_ang, cos_over, sin_over;
+	double rmax_r, rsw_r;             // rscale_max*rscale, rscale_switch*rscale
+	double acc, flux_scale;
+	double img_sx, img_sy;            // Model image pixel scale (centre rule)
+	double box, rscale, bn, axrat;
+	double s[5];
+	double dbn;                       // d b_n / d nser (Sersic, core-Sersic)
+	double dlnfs[NGRAD];              // d ln(flux_scale) / d slot
+	double lumtot, rbox, ie;
+	double rscale_switch, rscale_max; // adjusted values
+};
+
+// validation error codes (radial.cpp:204-214 and each profile's validate())
+enum SetupError : int32_t {
+	SE_OK = 0,
+	SE_AXRAT_LE0, SE_AXRAT_GT1, SE_BOX, SE_RE, SE_NSER, SE_FWHM, SE_CON, SE_ROUT, SE_FA, SE_FB,
+	SE_RC, SE_RT, SE_KA, SE_CS_RE, SE_CS_RB, SE_CS_NSER, SE_CS_A, SE_CS_B, SE_H1, SE_H2, SE_BE_RB,
+	SE_TYPE, SE_MAXREC, SE_RESOLUTION
+};
+
+inline const char *setup_error_message(int e)
+{
+	switch (e) {
+	case SE_AXRAT_LE0: return "axrat <= 0, must have axrat > 0";
+	case SE_AXRAT_GT1: return "axrat > 1, must have axrat <= 1";
+	case SE_BOX: return "box <= -2, must have box > -2";
+	case SE_RE: return "re <= 0, must have re > 0";
+	case SE_NSER: return "nser <= 0, must have nser > 0";
+	case SE_FWHM: return "fwhm <= 0, must have fwhm > 0";
+	case SE_CON: return "con < 0, must have con >= 0";
+	case SE_ROUT: return "rout <= 0, must have rout >= 0";
+	case SE_FA: return "a < 0, must have a >= 0";
+	case SE_FB: return "b > 2, must have b <= 2";
+	case SE_RC: return "rc <= 0, must have rc > 0";
+	case SE_RT: return "rt <= 0, must have rt > 0";
+	case SE_KA: return "a < 0, must have a >=0";
+	case SE_CS_RE: return "re <= 0, must have re > 0";
+	case SE_CS_RB: return "rb <= 0, must have rb > 0";
+	case SE_CS_NSER: return "nser <= 0, must have nser > 0";
+	case SE_CS_A: return "a <= 0, must have a > 0";
+	case SE_CS_B: return "b > 1.999, must have b < 1.999";
+	case SE_H1: return "h1 <= 0, must have h1 > 0";
+	case SE_H2: return "h2 <= 0, must have h2 > 0";
+	case SE_BE_RB: return "rb <= 0, must have rb > 0";
+	case SE_TYPE: return "unknown radial profile type";
+	case SE_MAXREC: return "max_recursions above 30 is not supported by the GPU recursion walker";
+	case SE_RESOLUTION: return "resolution above 16 is not supported by the GPU recursion walker";
+	default: return "ok";
+	}
+}
+
+PA_HD int validate_params(const ProfileParams &p)
+{
+	const double *s = p.shape;
+	if (p.axrat <= 0) return SE_AXRAT_LE0;
+	if (p.axrat > 1) return SE_AXRAT_GT1;
+	if (p.box <= -2) return SE_BOX;
+	switch (p.type) {
+	case SERSIC:
+		if (s[0] <= 0) return SE_RE;
+		if (s[1] <= 0) return SE_NSER;
+		break;
+	case MOFFAT:
+		if (s[0] <= 0) return SE_FWHM;
+		if (s[1] < 0) return SE_CON;
+		break;
+	case FERRER:
+		if (s[0] <= 0) return SE_ROUT;
+		if (s[1] < 0) return SE_FA;
+		if (s[2] > 2) return SE_FB;
+		break;
+	case KING:
+		if (s[0] <= 0) return SE_RC;
+		if (s[1] <= 0) return SE_RT;
+		if (s[2] < 0) return SE_KA;
+		break;
+	case CORESERSIC:
+		if (s[0] <= 0) return SE_CS_RE;
+		if (s[1] <= 0) return SE_CS_RB;
+		if (s[2] <= 0) return SE_CS_NSER;
+		if (s[3] <= 0) return SE_CS_A;
+		if (s[4] > 1.999) return SE_CS_B;
+		break;
+	case BROKENEXP:
+		if (s[0] <= 0) return SE_H1;
+		if (s[1] <= 0) return SE_H2;
+		if (s[2] <= 0) return SE_BE_RB;
+		break;
+	default: return SE_TYPE;
+	}
+	return SE_OK;
+}
+
+// ------------------------------------------------------------------ evaluate_at
+
+template <class T>
+PA_HD T boxy_r(const Comp &c, T x, T y)
+{
+	if (!c.boxy) return ::sqrt(x * x + y * y);
+	const T b2 = T(c.box) + T(2);
+	return ::pow(::pow(::fabs(x), b2) + ::pow(::fabs(y), b2), T(1) / b2);
+}
+
+template <class T>
+PA_HD T sersic_at(const Comp &c, T x, T y)
+{
+	const T re = T(c.s[0]), nser = T(c.s[1]), bn = T(c.bn);
+	T base, rf;
+	if (c.boxy) {
+		const T B = T(c.box) + T(2);
+		base = ::pow(::fabs(x / re), B) + ::pow(::fabs(y / re), B);
+		switch (c.kind) {
+		case K_HALF: rf = base * base; break;
+		case K_ONE: rf = base; break;
+		case K_TWO: rf = ::sqrt(base); break;
+		case K_THREE: rf = ::cbrt(base); break;
+		case K_FOUR: rf = ::sqrt(::sqrt(base)); break;
+		case K_EIGHT: rf = ::sqrt(::sqrt(::sqrt(base))); break;
+		case K_SIXTEEN: rf = ::sqrt(::sqrt(::sqrt(::sqrt(base)))); break;
+		default: rf = ::pow(base, T(1) / (nser * B)); break;
+		}
+	}
+	else {
+		base = (x * x + y * y) / (re * re);
+		switch (c.kind) {
+		case K_HALF: rf = base; break;
+		case K_ONE: rf = ::sqrt(base); break;
+		case K_TWO: rf = ::sqrt(::sqrt(base)); break;
+		case K_THREE: rf = ::cbrt(::sqrt(base)); break;
+		case K_FOUR: rf = ::sqrt(::sqrt(::sqrt(base))); break;
+		case K_EIGHT: rf = ::sqrt(::sqrt(::sqrt(::sqrt(base)))); break;
+		case K_SIXTEEN: rf = ::sqrt(::sqrt(::sqrt(::sqrt(::sqrt(base))))); break;
+		default: rf = ::pow(::sqrt(base), T(1) / nser); break;
+		}
+	}
+	return ::exp(-bn * (rf - T(1)));
+}
+
+template <class T>
+PA_HD T king_core(T r, T rc, T rt, T a)
+{
+	return ::pow(T(1) / ::pow(T(1) + ::pow(r / rc, T(2)), T(1) / a) - T(1) / ::pow(T(1) + ::pow(rt / rc, T(2)), T(1) / a), a);
+}
+
+template <class T>
+PA_HD T coresersic_core(const Comp &c, T r)
+{
+	const T re = T(c.s[0]), rb = T(c.s[1]), nser = T(c.s[2]), a = T(c.s[3]), b = T(c.s[4]);
+	return ::pow(T(1) + ::pow(r / rb, -a), b / a) *
+	       ::exp(-T(c.bn) * ::pow((::pow(r, a) + ::pow(rb, a)) / ::pow(re, a), T(1) / (nser * a)));
+}
+
+template <class T>
+PA_HD T brokenexp_core(T r, T h1, T h2, T rb, T a)
+{
+	T base = r - rb;
+	const T expo = T(1) / h1 - T(1) / h2;
+	if (a * base < T(40)) base = ::log(T(1) + ::exp(a * base)) / a;
+	return ::exp(-r / h1 + expo * base);
+}
+
+template <class T>
+PA_HD T eval_at(const Comp &c, T x, T y)
+{
+	switch (c.type) {
+	case SERSIC: return sersic_at<T>(c, x, y);
+	case MOFFAT: {
+		const T rf = boxy_r<T>(c, x, y) / T(c.rscale);
+		return ::pow(T(1) + rf * rf, -T(c.s[1]));
+	}
+	case FERRER: {
+		const T rf = boxy_r<T>(c, x, y) / T(c.rscale);
+		if (rf < T(1)) return ::pow(T(1) - ::pow(rf, T(2) - T(c.s[2])), T(c.s[1]));
+		return T(0);
+	}
+	case KING: {
+		const T r = boxy_r<T>(c, x, y);
+		if (r < T(c.s[1])) return king_core<T>(r, T(c.s[0]), T(c.s[1]), T(c.s[2]));
+		return T(0);
+	}
+	case CORESERSIC: return coresersic_core<T>(c, boxy_r<T>(c, x, y));
+	case BROKENEXP: return brokenexp_core<T>(boxy_r<T>(c, x, y), T(c.s[0]), T(c.s[1]), T(c.s[2]), T(c.s[3]));
+	default: return T(0);
+	}
+}
+
+// ------------------------------------------------------------------ derivatives
+
+template <class T>
+PA_HD void dr_terms(const Comp &c, T x, T y, T r, T *drx, T *dry, T *drB)
+{
+	*drx = *dry = *drB = T(0);
+	if (!(r > T(0))) return;
+	const T B = T(c.box) + T(2);
+	const T ax = ::fabs(x), ay = ::fabs(y);
+	if (!c.boxy) {
+		*drx = x / r;
+		*dry = y / r;
+	}
+	else {
+		const T rp = ::pow(r, T(1) - B);
+		*drx = ax > T(0) ? rp * ::pow(ax, B - T(1)) * (x > T(0) ? T(1) : T(-1)) : T(0);
+		*dry = ay > T(0) ? rp * ::pow(ay, B - T(1)) * (y > T(0) ? T(1) : T(-1)) : T(0);
+	}
+	const T px = ::pow(ax, B), py = ::pow(ay, B);
+	const T S = px + py;
+	const T Sl = (ax > T(0) ? px * ::log(ax) : T(0)) + (ay > T(0) ? py * ::log(ay) : T(0));
+	*drB = r * (Sl / (B * S) - ::log(S) / (B * B));
+}
+
+template <class T>
+PA_HD T king_dcore(T r, T rc, T rt, T a, int which)
+{
+	const T u2 = (r / rc) * (r / rc), w2 = (rt / rc) * (rt / rc);
+	const T A = ::pow(T(1) + u2, -T(1) / a), C = ::pow(T(1) + w2, -T(1) / a), D = A - C;
+	if (!(D > T(0))) return T(0);
+	const T Da1 = a * ::pow(D, a - T(1));
+	switch (which) {
+	case -1: return Da1 * (-(T(2) * r / (a * rc * rc)) * A / (T(1) + u2));
+	case 0: return Da1 * ((T(2) * u2 / (a * rc)) * A / (T(1) + u2) - (T(2) * w2 / (a * rc)) * C / (T(1) + w2));
+	case 1: return Da1 * ((T(2) * rt / (a * rc * rc)) * C / (T(1) + w2));
+	default: return ::pow(D, a) * (::log(D) + (A * ::log(T(1) + u2) - C * ::log(T(1) + w2)) / (a * D));
+	}
+}
+
+// o[0] = f, o[1] = df/dx_prof, o[2] = df/dy_prof, o[3] = df/dbox, o[4..8] = df/dshape
+template <class T>
+PA_HD void sample_grad(const Comp &c, T x, T y, T *o)
+{
+	for (int i = 0; i < 9; ++i) o[i] = T(0);
+	switch (c.type) {
+	case SERSIC: {
+		const T re = T(c.s[0]), n = T(c.s[1]), bn = T(c.bn), B = T(c.box) + T(2);
+		const T u = ::fabs(x / re), v = ::fabs(y / re);
+		const T pu = ::pow(u, B), pv = ::pow(v, B);
+		const T base = pu + pv;
+		const T f = sersic_at<T>(c, x, y);
+		o[0] = f;
+		if (!(base > T(0))) return;
+		const T lb = ::log(base);
+		const T rf = ::exp(lb / (n * B));
+		const T k = -bn * f * rf / (n * B * base);  // df/dbase
+		o[1] = u > T(0) ? k * B * pu / u / re * (x > T(0) ? T(1) : T(-1)) : T(0);
+		o[2] = v > T(0) ? k * B * pv / v / re * (y > T(0) ? T(1) : T(-1)) : T(0);
+		const T dbase_dB = (u > T(0) ? pu * ::log(u) : T(0)) + (v > T(0) ? pv * ::log(v) : T(0));
+		o[3] = -bn * f * rf * (dbase_dB / (n * B * base) - lb / (n * B * B));
+		o[4] = bn * f * rf / (n * re);
+		o[5] = f * (-T(c.dbn) * (rf - T(1)) + bn * rf * lb / (n * n * B));
+		return;
+	}
+	case MOFFAT: {
+		const T r = boxy_r<T>(c, x, y), rs = T(c.rscale), con = T(c.s[1]);
+		const T t = r / rs, q = T(1) + t * t;
+		const T f = ::pow(q, -con);
+		o[0] = f;
+		const T dfdr = -con * f / q * T(2) * t / rs;
+		T drx, dry, drB;
+		dr_terms<T>(c, x, y, r, &drx, &dry, &drB);
+		o[1] = dfdr * drx;
+		o[2] = dfdr * dry;
+		o[3] = dfdr * drB;
+		const T dfdrs = -dfdr * r / rs;
+		const T tt = ::pow(T(2), T(1) / con);
+		const T dlnrs_dcon = T(0.5) * tt * T(0.69314718055994530942) / (con * con * (tt - T(1)));
+		o[4] = dfdrs * rs / T(c.s[0]);
+		o[5] = -f * ::log(q) + dfdrs * rs * dlnrs_dcon;
+		return;
+	}
+	case FERRER: {
+		const T r = boxy_r<T>(c, x, y), rout = T(c.s[0]), a = T(c.s[1]), b = T(c.s[2]);
+		const T t = r / rout;
+		if (!(t < T(1))) return;
+		const T g = T(1) - ::pow(t, T(2) - b);
+		const T f = ::pow(g, a);
+		o[0] = f;
+		T drx, dry, drB;
+		dr_terms<T>(c, x, y, r, &drx, &dry, &drB);
+		const T dfdr = r > T(0) ? a * ::pow(g, a - T(1)) * (-(T(2) - b) * ::pow(t, T(1) - b)) / rout : T(0);
+		o[1] = dfdr * drx;
+		o[2] = dfdr * dry;
+		o[3] = dfdr * drB;
+		o[4] = -dfdr * r / rout;
+		o[5] = g > T(0) ? f * ::log(g) : T(0);
+		o[6] = t > T(0) ? a * ::pow(g, a - T(1)) * ::pow(t, T(2) - b) * ::log(t) : T(0);
+		return;
+	}
+	case KING: {
+		const T r = boxy_r<T>(c, x, y);
+		const T rc = T(c.s[0]), rt = T(c.s[1]), a = T(c.s[2]);
+		if (!(r < rt)) return;
+		o[0] = king_core<T>(r, rc, rt, a);
+		T drx, dry, drB;
+		dr_terms<T>(c, x, y, r, &drx, &dry, &drB);
+		const T dfdr = king_dcore<T>(r, rc, rt, a, -1);
+		o[1] = dfdr * drx;
+		o[2] = dfdr * dry;
+		o[3] = dfdr * drB;
+		for (int w = 0; w < 3; ++w) o[4 + w] = king_dcore<T>(r, rc, rt, a, w);
+		return;
+	}
+	default: return;
+	}
+}
+
+// ------------------------------------------------------------------ set-up
+
+struct KingIntegrand {
+	double rc, rt, a;
+	int which;  // -2: r f(r); 0..2: r df/dshape
+	PA_HD double operator()(double r) const
+	{
+		if (!(r < rt)) return 0;
+		if (which == -2) return r * king_core<double>(r, rc, rt, a);
+		return r * king_dcore<double>(r, rc, rt, a, which);
+	}
+};
+
+struct CoreSersicIntegrand {
+	const Comp *c;
+	PA_HD double operator()(double r) const { return r * coresersic_core<double>(*c, r); }
+};
+
+struct BrokenExpIntegrand {
+	double h1, h2, rb, a;
+	PA_HD double operator()(double r) const { return r * brokenexp_core<double>(r, h1, h2, rb, a); }
+};
+
+PA_HD bool almost_equal(double x, double y) { return ::fabs(x - y) < 1e-10; }
+
+// initial_calculations() + the evaluate()-time centre shift, in FP64.
+// offx/offy: PSF padding (drawing-canvas shift); sx/sy: profile pixel scale
+// (image scale / finesampling); img_sx/img_sy: image pixel scale.
+PA_HD int setup_component(const ProfileParams &p, double magzero, double sx, double sy, double img_sx,
+                          double img_sy, unsigned offx, unsigned offy, sf::QuadState *qs, Comp *c)
+{
+	const int verr = validate_params(p);
+	if (verr != SE_OK) return verr;
+	const double *s = p.shape;
+	const double pi = 3.14159265358979323846;
+	c->type = p.type;
+	c->rough = p.rough;
+	c->adjust = p.adjust;
+	c->convolve = p.convolve;
+	c->box = p.box;
+	c->boxy = p.box != 0;
+	c->axrat = p.axrat;
+	c->img_sx = img_sx;
+	c->img_sy = img_sy;
+	c->grad_ok = (p.type == SERSIC && !p.rescale_flux) || p.type == MOFFAT || p.type == FERRER || p.type == KING;
+	for (int i = 0; i < 5; ++i) c->s[i] = s[i];
+	for (int i = 0; i < NGRAD; ++i) c->dlnfs[i] = 0;
+	c->bn = 0;
+	c->dbn = 0;
+
+	if (p.type == SERSIC || p.type == CORESERSIC) c->bn = sf::qgamma(0.5, 2 * (p.type == SERSIC ? s[1] : s[2]));
+	switch (p.type) {
+	case SERSIC: c->rscale = s[0]; break;
+	case MOFFAT: c->rscale = s[0] / (2 * ::sqrt(::pow(2.0, 1 / s[1]) - 1)); break;
+	case FERRER: c->rscale = s[0]; break;
+	case KING: c->rscale = s[1]; break;
+	case CORESERSIC: c->rscale = s[0]; break;
+	default: c->rscale = s[0] * 4; break;
+	}
+
+	double lumtot = 0;
+	switch (p.type) {
+	case SERSIC:
+		lumtot = ::pow(s[0], 2.0) * 2 * pi * s[1] * sf::gam(2 * s[1]) * ::exp(c->bn) / ::pow(c->bn, 2 * s[1]);
+		break;
+	case MOFFAT: lumtot = ::pow(c->rscale, 2.0) * pi / (s[1] - 1); break;
+	case FERRER: lumtot = ::pow(s[0], 2.0) * pi * (s[1] * sf::beta(s[1], 1 + 2 / (2 - s[2]))); break;
+	case KING: {
+		KingIntegrand f{s[0], s[1], s[2], -2};
+		lumtot = 2 * pi * sf::integrate(f, 0.0, s[1], false, *qs);
+		break;
+	}
+	case CORESERSIC: {
+		CoreSersicIntegrand f{c};
+		lumtot = 2 * pi * sf::integrate(f, 0.0, 0.0, true, *qs);
+		break;
+	}
+	default: {
+		BrokenExpIntegrand f{s[0], s[1], s[2], s[3]};
+		lumtot = 2 * pi * sf::integrate(f, 0.0, 0.0, true, *qs);
+		break;
+	}
+	}
+	c->lumtot = lumtot;
+	const double b2 = p.box + 2;
+	const double r_box = pi * b2 / (2 * sf::beta(1 / b2, 1 / b2));
+	c->rbox = r_box;
+	const double ie = ::pow(10.0, -0.4 * (p.mag - magzero)) / (lumtot * p.axrat / r_box);
+	c->ie = ie;
+
+	double acc = p.acc, rsw = p.rscale_switch, rmax = p.rscale_max;
+	unsigned res = p.resolution;
+	if (p.adjust) {
+		switch (p.type) {
+		case SERSIC: {
+			const double n = s[1];
+			const double ff = s[0] * ::pow(sf::qgamma(1. - n * n / 2e3, 2 * n) / c->bn, n);
+			double v = ::ceil(ff);
+			v = ::fmax(::fmin(v, 20.), 2.);
+			rsw = v / s[0];
+			break;
+		}
+		case MOFFAT: {
+			const double ff = c->rscale * ::sqrt(::pow(1 - 0.9999, 1 / (1 - s[1])) - 1);
+			const double v = ::fmax(ff, 1.823 * s[0]);
+			rsw = ::fmax(::fmin(v, 20.), 2.) / c->rscale;
+			break;
+		}
+		case FERRER:
+		case KING: rsw = 0.5; break;
+		default: rsw = 1; break;
+		}
+		unsigned r = (unsigned)::ceil(160 / (rsw * c->rscale));
+		r += r % 2;
+		res = r < 4U ? 4U : (r > 16U ? 16U : r);
+		if (rmax == 0) {
+			switch (p.type) {
+			case SERSIC: {
+				const double n = s[1];
+				const double ff = s[0] * ::pow(sf::qgamma(0.9999, 2 * n) / c->bn, n);
+				rmax = ::ceil(::fmax(ff, 2.) / s[0]);
+				break;
+			}
+			case MOFFAT: {
+				const double ff = c->rscale * ::sqrt(::pow(1 - 0.9999, 1 / (1 - s[1])) - 1);
+				rmax = ::ceil(::fmax(ff, 2.) / c->rscale);
+				break;
+			}
+			case FERRER: rmax = 1; break;
+			case KING: rmax = 1.5; break;
+			default: rmax = 100; break;
+			}
+		}
+		if (p.type == SERSIC) acc = ::fmax(0.1, acc / ::sqrt(s[1])) / p.axrat;
+		else if (p.type == MOFFAT) acc = 0.1 / p.axrat;
+	}
+	c->acc = acc;
+	c->rscale_switch = rsw;
+	c->rscale_max = rmax;
+	c->res = (int)res;
+	c->maxrec = (int)p.max_recursions;
+	if (p.max_recursions > 30) return SE_MAXREC;
+	if (res > 16 || res == 0) return SE_RESOLUTION;
+	c->has_rmax = rmax > 0;
+	c->rmax_r = rmax * c->rscale;
+	c->rsw_r = rsw * c->rscale;
+
+	const double angrad = ::fmod(p.ang + 90, 360.) * pi / 180.;
+	c->cos_ang = ::cos(angrad);
+	c->sin_ang = ::sin(angrad);
+	c->cos_over = c->cos_ang / p.axrat;
+	c->sin_over = c->sin_ang / p.axrat;
+
+	double flux_scale = sx * sy * ie;
+	if (p.type == SERSIC && p.rescale_flux) {
+		double rescale = 1;
+		if (p.adjust) rescale = 1 / sf::pgamma(c->bn * ::pow(rmax / s[0], 1 / s[1]), 2 * s[1]);
+		flux_scale *= rescale;
+	}
+	c->flux_scale = flux_scale;
+	c->xc = p.xcen + offx * sx;
+	c->yc = p.ycen + offy * sy;
+	c->centre_rule = p.type == SERSIC && p.adjust && s[1] > 1;
+
+	c->kind = K_GENERAL;
+	if (p.type == SERSIC) {
+		double ie_ = s[1];
+		if (p.box != 0) ie_ *= (p.box + 2);
+		if (almost_equal(ie_, 0.5)) c->kind = K_HALF;
+		else if (almost_equal(ie_, 1)) c->kind = K_ONE;
+		else if (almost_equal(ie_, 2)) c->kind = K_TWO;
+		else if (almost_equal(ie_, 3)) c->kind = K_THREE;
+		else if (almost_equal(ie_, 4)) c->kind = K_FOUR;
+		else if (almost_equal(ie_, 8)) c->kind = K_EIGHT;
+		else if (almost_equal(ie_, 16)) c->kind = K_SIXTEEN;
+	}
+
+	// d ln(flux_scale)/d slot for the adjoint
+	c->dlnfs[G_MAG] = -0.4 * 2.302585092994045684;
+	c->dlnfs[G_AXRAT] = -1 / p.axrat;
+	{
+		const double a = 1 / b2;
+		c->dlnfs[G_BOX] = 1 / b2 + 2 * (sf::digamma(a) - sf::digamma(2 * a)) / (b2 * b2);
+	}
+	switch (p.type) {
+	case SERSIC: {
+		const double n = s[1], bn = c->bn, a = 2 * n;
+		const double dens = ::exp((a - 1) * ::log(bn) - bn - sf::lgam(a));
+		c->dbn = -2 * sf::dpgamma_da(bn, a) / dens;
+		const double dlnL = 1 / n + 2 * sf::digamma(2 * n) + c->dbn - 2 * ::log(bn) - 2 * n * c->dbn / bn;
+		c->dlnfs[G_S0] = -2 / s[0];
+		c->dlnfs[G_S1] = -dlnL;
+		break;
+	}
+	case MOFFAT: {
+		const double con = s[1], t = ::pow(2.0, 1 / con);
+		const double dlnrs = 0.5 * t * 0.69314718055994530942 / (con * con * (t - 1));
+		c->dlnfs[G_S0] = -2 / s[0];
+		c->dlnfs[G_S1] = -(2 * dlnrs - 1 / (con - 1));
+		break;
+	}
+	case FERRER: {
+		const double a = s[1], b = s[2], cc = 1 + 2 / (2 - b);
+		c->dlnfs[G_S0] = -2 / s[0];
+		c->dlnfs[G_S1] = -(1 / a + sf::digamma(a) - sf::digamma(a + cc));
+		c->dlnfs[G_S2] = -((sf::digamma(cc) - sf::digamma(a + cc)) * 2 / ((2 - b) * (2 - b)));
+		break;
+	}
+	case KING: {
+		for (int w = 0; w < 3; ++w) {
+			KingIntegrand f{s[0], s[1], s[2], w};
+			const double d = 2 * pi * sf::integrate(f, 0.0, s[1], false, *qs);
+			c->dlnfs[G_S0 + w] = -d / lumtot;
+		}
+		break;
+	}
+	default: break;
+	}
+	return SE_OK;
+}
+
+#undef PA_HD
+
+}  // namespace profit_amd

@@ -1,0 +1,107 @@
+"""Synthetic workloads of the model-image path.
+
+Each builder returns the inputs ProFit's profitSetupData would hold: image
+size, PSF, likelihood region, component parameters and (given a function that
+renders the noiseless model) data and sigma maps drawn from a seeded
+counter-based generator, so any host regenerates identical inputs.
+
+  config1()  100x100, one Sersic (n=2.5, re=8), 25x25 Gaussian PSF (FWHM 3),
+             sigma = sqrt(model + 10^2), all-true mask, chi^2.
+  config2()  400x400 bulge (n=4, re=3) + disk (n=1, re=15), 51x51 Moffat PSF
+             (FWHM 4, con 3), elliptical region of ~70% of the pixels,
+             sigma = sqrt(max(model, 0)/gain + 5^2), gain 1.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+GAUSS_FWHM_TO_SIGMA = 1.0 / (2.0 * np.sqrt(2.0 * np.log(2.0)))
+
+
+def splitmix64(seed: int, n: int) -> np.ndarray:
+    """n uint64 draws of SplitMix64 seeded with ``seed``."""
+    with np.errstate(over="ignore"):
+        idx = np.arange(1, n + 1, dtype=np.uint64)
+        z = np.uint64(seed) + idx * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def normal_draws(seed: int, n: int) -> np.ndarray:
+    """Standard normal deviates by Box-Muller over SplitMix64 uniforms."""
+    m = (n + 1) // 2
+    u = splitmix64(seed, 2 * m)
+    u1 = ((u[0::2] >> np.uint64(11)).astype(np.float64) + 0.5) / 9007199254740992.0
+    u2 = ((u[1::2] >> np.uint64(11)).astype(np.float64) + 0.5) / 9007199254740992.0
+    r = np.sqrt(-2.0 * np.log(u1))
+    z = np.concatenate([r * np.cos(2 * np.pi * u2), r * np.sin(2 * np.pi * u2)])
+    return z[:n]
+
+
+def gaussian_psf(size: int, fwhm: float) -> np.ndarray:
+    s = fwhm * GAUSS_FWHM_TO_SIGMA
+    k = np.arange(size) - size // 2
+    p = np.exp(-(k[None, :] ** 2 + k[:, None] ** 2) / (2 * s * s))
+    return p / p.sum()
+
+
+def moffat_psf(size: int, fwhm: float, con: float) -> np.ndarray:
+    rs = fwhm / (2 * np.sqrt(2 ** (1 / con) - 1))
+    k = np.arange(size) - size // 2
+    r2 = (k[None, :] ** 2 + k[:, None] ** 2) / (rs * rs)
+    p = (1 + r2) ** (-con)
+    return p / p.sum()
+
+
+def ellipse_mask(width: int, height: int, cx: float, cy: float, a: float, b: float) -> np.ndarray:
+    x = np.arange(width) + 0.5
+    y = np.arange(height) + 0.5
+    return ((((x[None, :] - cx) / a) ** 2 + ((y[:, None] - cy) / b) ** 2) <= 1).astype(np.uint8)
+
+
+def config1_components():
+    return [dict(type="sersic", xcen=50.3, ycen=49.7, mag=15.0, re=8.0, nser=2.5, ang=30.0, axrat=0.6,
+                 box=0.0)]
+
+
+def config2_components():
+    return [dict(type="sersic", xcen=200.4, ycen=199.8, mag=17.0, re=3.0, nser=4.0, ang=0.0, axrat=0.9, box=0.0),
+            dict(type="sersic", xcen=200.4, ycen=199.8, mag=16.0, re=15.0, nser=1.0, ang=60.0, axrat=0.4,
+                 box=0.0)]
+
+
+def config1(model_fn=None, magzero=0.0, seed=0):
+    """Config 1.  model_fn(width, height, psf, components, magzero) -> noiseless model."""
+    w = h = 100
+    psf = gaussian_psf(25, 3.0)
+    comps = config1_components()
+    region = np.ones((h, w), dtype=np.uint8)
+    out = dict(width=w, height=h, psf=psf, components=comps, region=region, magzero=magzero,
+               likefunc="chisq")
+    if model_fn is not None:
+        model = model_fn(w, h, psf, comps, magzero)
+        sigma = np.sqrt(model + 10.0 ** 2)
+        data = model + normal_draws(seed, w * h).reshape(h, w) * sigma
+        out.update(model=model, data=data, sigma=sigma)
+    return out
+
+
+def config2(model_fn=None, magzero=30.0, seed=2, gain=1.0, read_noise=5.0):
+    """Config 2 (magzero 30 by default so that the galaxy rises above the noise)."""
+    w = h = 400
+    psf = moffat_psf(51, 4.0, 3.0)
+    comps = config2_components()
+    region = ellipse_mask(w, h, 200.0, 200.0, 199.5, 178.7)
+    out = dict(width=w, height=h, psf=psf, components=comps, region=region, magzero=magzero,
+               likefunc="chisq")
+    if model_fn is not None:
+        model = model_fn(w, h, psf, comps, magzero)
+        sigma = np.sqrt(np.maximum(model, 0) / gain + read_noise ** 2)
+        data = model + normal_draws(seed, w * h).reshape(h, w) * sigma
+        out.update(model=model, data=data, sigma=sigma)
+    return out
+
+
+GRAD_FREE = ("xcen", "ycen", "mag", "re", "nser", "ang", "axrat")

@@ -1,0 +1,123 @@
+"""GPU path vs the FP64 oracle on the same inputs, through the C-ABI.
+
+Gates: pixel classes and recursion trees bit-exact; per-pixel relative error
+<= 1e-5 above 1e-6 of the peak; |logL difference| <= 1e-3.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import profit_amd as P
+from profit_amd import workloads as W
+from tests.helpers import gpu_profiles, oracle_model_fn, oracle_profiles, rel_err
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 1e-5
+
+
+def _report(name, err, n):
+    print(f"[parity] {name}: max rel err {err:.3e} over {n} pixels", flush=True)
+
+
+def test_config1_model_image():
+    wl = W.config1()
+    oimg = O.model_evaluate(O.ModelSpec(100, 100, psf=wl["psf"], threads=8),
+                            oracle_profiles(wl["components"]))["image"]
+    gimg = P.make_model(100, 100, gpu_profiles(wl["components"]), psf=wl["psf"])["image"]
+    assert gimg.shape == oimg.shape == (100, 100)
+    err, n = rel_err(gimg, oimg)
+    _report("config1 model", err, n)
+    assert err <= REL_TOL
+
+
+def test_config1_classes_and_recursion_trees_bit_exact():
+    wl = W.config1(oracle_model_fn())
+    ref = O.model_evaluate(O.ModelSpec(100, 100, psf=wl["psf"], threads=8),
+                           oracle_profiles(wl["components"]), diag=True)
+    eng = P.Engine()
+    g = eng.add_galaxy(100, 100, wl["data"], wl["sigma"], psf=wl["psf"])
+    eng.set_batch([g], [1])
+    cls, nodes = eng.diagnostics(gpu_profiles(wl["components"]))
+    assert cls.shape == ref["cls"].shape
+    assert np.array_equal(cls, ref["cls"].astype(np.int32))
+    assert np.array_equal(nodes, ref["nodes"].astype(np.int32))
+    print(f"[parity] config1 classes: {np.bincount(cls.ravel(), minlength=4)}, nodes {nodes.sum()}", flush=True)
+
+
+def test_config1_likelihood():
+    wl = W.config1(oracle_model_fn())
+    fit = O.FitSpec(O.ModelSpec(100, 100, psf=wl["psf"], threads=8), wl["data"], wl["sigma"], wl["region"])
+    oll = O.loglike(fit, oracle_profiles(wl["components"]))
+    eng = P.Engine()
+    g = eng.add_galaxy(100, 100, wl["data"], wl["sigma"], psf=wl["psf"], region=wl["region"])
+    eng.set_batch([g], [1])
+    gll = eng.like(gpu_profiles(wl["components"]))[0]
+    print(f"[parity] config1 logL gpu {gll:.9f} oracle {oll:.9f} diff {gll - oll:.3e}", flush=True)
+    assert abs(gll - oll) <= 1e-3
+
+
+def _config2_fit(magzero=30.0):
+    wl = W.config2(oracle_model_fn(), magzero=magzero)
+    calc = O.adjust_mask(wl["region"], (400, 400), wl["psf"])
+    spec = O.ModelSpec(400, 400, psf=wl["psf"], mask=calc, adjust_mask=False, magzero=magzero, threads=8)
+    fit = O.FitSpec(spec, wl["data"], wl["sigma"], wl["region"])
+    return wl, calc, fit
+
+
+def test_config2_model_and_likelihood():
+    wl, calc, fit = _config2_fit()
+    assert calc.shape == (450, 450)
+    oll, omodel = O.loglike(fit, oracle_profiles(wl["components"]), want_model=True)
+    eng = P.Engine()
+    g = eng.add_galaxy(400, 400, wl["data"], wl["sigma"], psf=wl["psf"], calcregion=calc, region=wl["region"],
+                       magzero=30.0)
+    assert eng.geometry(g) == (450, 450, 25, 25)
+    eng.set_batch([g], [2])
+    params = gpu_profiles(wl["components"])
+    gmodel = eng.model_images(params)[0]
+    err, n = rel_err(gmodel, omodel)
+    _report("config2 model", err, n)
+    assert err <= REL_TOL
+    gll = eng.like(params)[0]
+    print(f"[parity] config2 logL gpu {gll:.6f} oracle {oll:.6f} diff {gll - oll:.3e}", flush=True)
+    assert abs(gll - oll) <= 1e-3
+
+
+def test_config2_gradient_against_oracle():
+    wl, calc, fit = _config2_fit()
+    ocomps = oracle_profiles(wl["components"])
+    oll, ograd = O.loglike_grad(fit, ocomps)
+    eng = P.Engine()
+    g = eng.add_galaxy(400, 400, wl["data"], wl["sigma"], psf=wl["psf"], calcregion=calc, region=wl["region"],
+                       magzero=30.0)
+    eng.set_batch([g], [2])
+    gll, ggrad = eng.like_grad(gpu_profiles(wl["components"]))
+    slots = [0, 1, 2, 3, 4, 6, 7]  # xcen ycen mag ang axrat re nser
+    for k in range(2):
+        for s in slots:
+            o, gv = ograd[k, s], ggrad[k, s]
+            scale = max(abs(o), 1e-3 * np.abs(ograd[k, slots]).max())
+            print(f"[parity] grad comp {k} slot {O.GRAD_SLOTS[s]}: gpu {gv:.6e} oracle {o:.6e}", flush=True)
+            assert abs(gv - o) <= 2e-3 * scale + 1e-6
+
+
+def test_adjust_mask_bit_exact():
+    wl = W.config2()
+    ref = O.adjust_mask(wl["region"], (400, 400), wl["psf"])
+    got = P.adjust_mask(wl["region"], (400, 400), wl["psf"])
+    assert got.shape == ref.shape and np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("ksize", [5, 25, 51])
+def test_brute_convolution(ksize):
+    rng = np.random.default_rng(ksize)
+    src = rng.random((90, 70))
+    krn = rng.random((ksize, ksize))
+    mask = (rng.random((90, 70)) > 0.3).astype(np.uint8)
+    ref = O.convolve(src, krn, mask)
+    got = P.convolve(src, krn, mask)
+    err, n = rel_err(got, ref)
+    _report(f"brute conv k={ksize}", err, n)
+    assert err <= REL_TOL
+    assert np.all(got[mask == 0] == 0)
