@@ -183,8 +183,20 @@ int profit_amd_engine_diagnostics(profit_amd_engine *engine, const profit_amd_pr
 int profit_amd_engine_galaxy_geometry(profit_amd_engine *engine, int32_t galaxy_id, uint32_t *draw_w,
                                       uint32_t *draw_h, uint32_t *pad_x, uint32_t *pad_y);
 
-/* Milliseconds per stage of the last evaluation when timing is on. */
+/* Per-stage device timing with HIP events on the engine's stream (evaluations
+ * then run as an eager launch sequence rather than a graph).  Stages: 0 set-up,
+ * 1 render, 2 sub-pixel walk, 3 convolution + likelihood, 4 likelihood
+ * finalisation, 5 adjoint correlation, 6 adjoint render, 7 adjoint sub-pixel
+ * walk, 8 gradient finalisation. */
 int profit_amd_engine_set_timing(profit_amd_engine *engine, int32_t on);
+
+/* Accumulated milliseconds and launch counts per stage (9 each), the number of
+ * evaluations, the last sub-pixel queue length, and the algorithmic FLOPs of
+ * the forward convolution and of the adjoint correlation of one evaluation of
+ * the batch (2 * psf_w * psf_h * output pixels summed over instances). */
+int profit_amd_engine_stats(profit_amd_engine *engine, double *stage_ms, int64_t *stage_calls, int64_t *evaluations,
+                            int32_t *queue_length, double *conv_flops, double *adjoint_flops);
+int profit_amd_engine_reset_stats(profit_amd_engine *engine);
 
 #ifdef __cplusplus
 }

@@ -1,7 +1,7 @@
 """profit_amd: MI355X-native model-image path of libprofit (ProFit's C++ core).
 
 Python binding of the C-ABI in ``include/profit_amd.h`` (``libprofit_amd.so``,
-built in-tree by ``profit_amd/build.py``).  Every compute call runs the HIP
+built in-tree by ``profit_amd/hipbuild.py``).  Every compute call runs the HIP
 kernels; there is no CPU fallback: without the built library, or without a
 GPU, calls raise.
 
@@ -105,14 +105,19 @@ SYMBOLS = [
     ("profit_amd_engine_galaxy_geometry", C.c_int, [C.c_void_p, C.c_int32, _P(C.c_uint32), _P(C.c_uint32),
                                                      _P(C.c_uint32), _P(C.c_uint32)]),
     ("profit_amd_engine_set_timing", C.c_int, [C.c_void_p, C.c_int32]),
+    ("profit_amd_engine_stats", C.c_int, [C.c_void_p, _P(C.c_double), _P(C.c_int64), _P(C.c_int64),
+                                          _P(C.c_int32), _P(C.c_double), _P(C.c_double)]),
+    ("profit_amd_engine_reset_stats", C.c_int, [C.c_void_p]),
 ]
+STAGES = ("setup", "render", "subsample", "conv_like", "like_finalize", "adjoint_conv", "adjoint_render",
+          "adjoint_subsample", "grad_finalize")
 
 _lib = None
 
 
 def build():
-    from . import build as _b
-    return _b.build()
+    from . import hipbuild
+    return hipbuild.build()
 
 
 def lib():
@@ -371,6 +376,23 @@ class Engine:
             imgs.append(out[off:off + n].reshape(h, w))
             off += n
         return imgs
+
+    def set_timing(self, on=True):
+        _check(lib().profit_amd_engine_set_timing(self._h, int(on)))
+
+    def reset_stats(self):
+        _check(lib().profit_amd_engine_reset_stats(self._h))
+
+    def stats(self):
+        ms = np.zeros(len(STAGES))
+        calls = np.zeros(len(STAGES), dtype=np.int64)
+        ev, q = C.c_int64(), C.c_int32()
+        cf, af = C.c_double(), C.c_double()
+        _check(lib().profit_amd_engine_stats(self._h, _d(ms), calls.ctypes.data_as(C.POINTER(C.c_int64)),
+                                             C.byref(ev), C.byref(q), C.byref(cf), C.byref(af)))
+        return {"stage_ms": dict(zip(STAGES, ms.tolist())), "stage_calls": dict(zip(STAGES, calls.tolist())),
+                "evaluations": ev.value, "queue_length": q.value, "conv_flops": cf.value,
+                "adjoint_flops": af.value}
 
     def diagnostics(self, params):
         arr = self._params(params)

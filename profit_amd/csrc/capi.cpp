@@ -336,4 +336,25 @@ int profit_amd_engine_set_timing(profit_amd_engine *engine, int32_t on)
 	return guarded([&] { engine->eng.set_timing(on != 0); });
 }
 
+int profit_amd_engine_stats(profit_amd_engine *engine, double *stage_ms, int64_t *stage_calls, int64_t *evaluations,
+                            int32_t *queue_length, double *conv_flops, double *adjoint_flops)
+{
+	return guarded([&] {
+		const profit_amd::EngineStats &s = engine->eng.stats();
+		for (int k = 0; k < profit_amd::ST_COUNT; ++k) {
+			if (stage_ms) stage_ms[k] = s.ms[k];
+			if (stage_calls) stage_calls[k] = s.calls[k];
+		}
+		if (evaluations) *evaluations = s.evaluations;
+		if (queue_length) *queue_length = s.queue_count;
+		if (conv_flops) *conv_flops = engine->eng.conv_flops_forward();
+		if (adjoint_flops) *adjoint_flops = engine->eng.conv_flops_adjoint();
+	});
+}
+
+int profit_amd_engine_reset_stats(profit_amd_engine *engine)
+{
+	return guarded([&] { engine->eng.reset_stats(); });
+}
+
 }  // extern "C"

@@ -32,6 +32,7 @@ struct GalaxyDev {
 	double img_sx, img_sy; // image pixel scale
 	double magzero;
 	double n_region;       // number of likelihood pixels
+	double n_calc;         // number of calc-region pixels of the canvas
 	double pois_scale;     // Poisson rescale factor (1 when not rescaled)
 	const float *psf_fwd;  // flipped PSF, weights of the forward convolution
 	const float *psf_adj;  // PSF as given, weights of the adjoint correlation

@@ -61,7 +61,41 @@ Engine::~Engine()
 {
 	for (auto &g : graph_exec_)
 		if (g) (void)hipGraphExecDestroy(g);
+	for (auto &e : events_)
+		if (e) (void)hipEventDestroy(e);
 	if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Engine::set_timing(bool on)
+{
+	timing_ = on;
+	if (on && !events_[0])
+		for (auto &e : events_) hip_check(hipEventCreate(&e), "hipEventCreate");
+}
+
+void Engine::mark(int k)
+{
+	if (timing_) hip_check(hipEventRecord(events_[k], stream_), "hipEventRecord");
+}
+
+double Engine::conv_flops_forward() const
+{
+	double f = 0;
+	for (int i = 0; i < ninst(); ++i) {
+		const GalaxyDev &g = gal_host_[inst_galaxy_[i]];
+		f += 2.0 * g.Kx * g.Ky * g.n_region;
+	}
+	return f;
+}
+
+double Engine::conv_flops_adjoint() const
+{
+	double f = 0;
+	for (int i = 0; i < ninst(); ++i) {
+		const GalaxyDev &g = gal_host_[inst_galaxy_[i]];
+		f += 2.0 * g.Kx * g.Ky * g.n_calc;
+	}
+	return f;
 }
 
 static void invalidate_graphs(hipGraphExec_t *g)
@@ -140,6 +174,8 @@ int Engine::add_galaxy(const GalaxySpec &s)
 	std::vector<uint8_t> calc(ndraw, 1);
 	if (s.calc)
 		for (size_t k = 0; k < ndraw; ++k) calc[k] = s.calc[k] ? 1 : 0;
+	g.n_calc = 0;
+	for (size_t k = 0; k < ndraw; ++k) g.n_calc += calc[k];
 	bufs.calc.alloc(ndraw);
 	hip_check(hipMemcpy(bufs.calc.as<uint8_t>(), calc.data(), ndraw, hipMemcpyHostToDevice), "calc");
 	std::vector<uint8_t> region(npix, 1);
@@ -331,35 +367,45 @@ void Engine::record(bool grad)
 	hip_check(hipMemcpyAsync(d_params_.as<ProfileParams>(), h_params_.as<ProfileParams>(), nc * sizeof(ProfileParams),
 	                         hipMemcpyHostToDevice, st),
 	          "params");
+	mark(0);
 	hip_check(launch_setup(d_params_.as<ProfileParams>(), d_comp_inst_.as<int>(), insts, gals, d_comps_.as<Comp>(),
 	                       d_err_.as<int>(), nc, st),
 	          "setup");
 	hip_check(launch_zero_i(d_qcount_.as<int>(), 1, st), "zero");
 	if (grad) hip_check(launch_zero_d(d_grad_acc_.as<double>(), (long)nc * NGRAD, st), "zero");
+	mark(1);
 	hip_check(launch_render(gals, insts, d_at_off_.as<int>(), n, n_at_blocks_, d_comps_.as<Comp>(),
 	                        d_queue_.as<QEntry>(), d_qcount_.as<int>(), (int)qcap_, -1, 0, nullptr, st),
 	          "render");
+	mark(2);
 	hip_check(launch_subsample(false, subsample_blocks_, gals, insts, d_comps_.as<Comp>(), d_queue_.as<QEntry>(),
 	                           d_qcount_.as<int>(), (int)qcap_, 0, nullptr, st),
 	          "subsample");
+	mark(3);
 	hip_check(launch_conv(0, conv_rows_, n_ct_blocks_, kx_max_, kc, gals, insts, d_ct_off_.as<int>(), n,
 	                      d_partial_.as<double>(), st),
 	          "conv");
+	mark(4);
 	hip_check(launch_like_finalize(gals, insts, d_ct_off_.as<int>(), n, d_partial_.as<double>(), d_ll_.as<double>(),
 	                               d_gscale_.as<double>(), st),
 	          "like");
+	mark(5);
 	if (grad) {
 		hip_check(launch_conv(2, conv_rows_, n_at_blocks_, kx_max_, kc, gals, insts, d_at_off_.as<int>(), n, nullptr, st),
 		          "adjoint conv");
+		mark(6);
 		hip_check(launch_adj_render(gals, insts, d_at_off_.as<int>(), n, n_at_blocks_, d_comps_.as<Comp>(),
 		                            d_grad_acc_.as<double>(), st),
 		          "adjoint render");
+		mark(7);
 		hip_check(launch_subsample(true, subsample_blocks_, gals, insts, d_comps_.as<Comp>(), d_queue_.as<QEntry>(),
 		                           d_qcount_.as<int>(), (int)qcap_, 0, d_grad_acc_.as<double>(), st),
 		          "adjoint subsample");
+		mark(8);
 		hip_check(launch_grad_finalize(d_comps_.as<Comp>(), d_gscale_.as<double>(), d_grad_acc_.as<double>(),
 		                               d_grad_out_.as<double>(), nc, st),
 		          "grad");
+		mark(9);
 		hip_check(hipMemcpyAsync(h_grad_.as<double>(), d_grad_out_.as<double>(), (size_t)nc * NGRAD * sizeof(double),
 		                         hipMemcpyDeviceToHost, st),
 		          "grad copy");
@@ -384,6 +430,7 @@ void Engine::check_setup_errors()
 void Engine::run(bool grad, bool use_graph)
 {
 	hip_check(hipSetDevice(device_), "hipSetDevice");
+	if (timing_) use_graph = false;  // events between kernels need the eager sequence
 	for (int attempt = 0; attempt < 4; ++attempt) {
 		const int gi = grad ? 1 : 0;
 		if (use_graph) {
@@ -411,7 +458,19 @@ void Engine::run(bool grad, bool use_graph)
 		}
 		const int qc = *h_qcount_.as<int>();
 		stats_.queue_count = qc;
-		if (qc <= qcap_) return;
+		if (qc <= qcap_) {
+			if (timing_) {
+				const int last = grad ? ST_COUNT : ST_ADJ_CONV;
+				for (int k = 0; k < last; ++k) {
+					float ms = 0;
+					hip_check(hipEventElapsedTime(&ms, events_[k], events_[k + 1]), "elapsed");
+					stats_.ms[k] += ms;
+					stats_.calls[k] += 1;
+				}
+			}
+			stats_.evaluations += 1;
+			return;
+		}
 		ensure_queue((long)qc + qc / 4 + 1024);  // drops the graphs; run again
 	}
 	throw std::runtime_error("sub-pixel work queue kept overflowing");

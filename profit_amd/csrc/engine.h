@@ -111,10 +111,15 @@ struct GalaxySpec {
 	int finesampling = 1;                // W, H are then the fine grid; profiles use scale/finesampling
 };
 
+// Accumulated per-stage device time (HIP events on the engine stream) over the
+// evaluations run while timing is on.
+enum Stage { ST_SETUP = 0, ST_RENDER, ST_SUBSAMPLE, ST_CONV, ST_LIKE, ST_ADJ_CONV, ST_ADJ_RENDER, ST_ADJ_SUB, ST_GRAD,
+             ST_COUNT };
 struct EngineStats {
-	double setup_ms = 0, render_ms = 0, subsample_ms = 0, conv_ms = 0, adj_conv_ms = 0, adj_render_ms = 0,
-	       adj_sub_ms = 0, total_ms = 0;
+	double ms[ST_COUNT] = {0};
+	long calls[ST_COUNT] = {0};
 	int queue_count = 0;
+	long evaluations = 0;
 };
 
 class Engine {
@@ -143,7 +148,12 @@ class Engine {
 	int ncomp() const { return ncomp_total_; }
 	int conv_rows() const { return conv_rows_; }
 	void set_conv_rows(int r);
-	void set_timing(bool on) { timing_ = on; }
+	void set_timing(bool on);
+	void reset_stats() { stats_ = EngineStats{}; }
+	// algorithmic work of the batch: pixels of the likelihood region and of the
+	// calc region summed over instances, and the PSF taps of each
+	double conv_flops_forward() const;
+	double conv_flops_adjoint() const;
 	const EngineStats &stats() const { return stats_; }
 	const GalaxyDev &galaxy(int g) const { return gal_host_.at(g); }
 	hipStream_t stream() const { return stream_; }
@@ -184,6 +194,8 @@ class Engine {
 	HostBuf h_params_, h_ll_, h_grad_, h_err_, h_qcount_;
 
 	hipGraphExec_t graph_exec_[2] = {nullptr, nullptr};
+	hipEvent_t events_[ST_COUNT + 1] = {};
+	void mark(int k);
 	int subsample_blocks_ = 2048;
 };
 
