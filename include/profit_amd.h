@@ -157,6 +157,12 @@ int profit_amd_engine_set_batch(profit_amd_engine *engine, const int32_t *inst_g
 /* Tile height of the convolution / rendering tiles: 4*rows (rows = 2, 4, 8, 16). */
 int profit_amd_engine_set_tile_rows(profit_amd_engine *engine, int32_t rows);
 
+/* Precision of directly evaluated pixels and of their adjoint: 1 = FP64
+ * (default), 0 = FP32.  Classification, the sub-pixel recursion and the
+ * likelihood sums are FP64 either way; the convolution is FP32 with FP64
+ * accumulation across PSF columns. */
+int profit_amd_engine_set_render_fp64(profit_amd_engine *engine, int32_t on);
+
 /* logL per instance.  params: one profit_amd_profile per component,
  * instance-major, in the batch's component order. */
 int profit_amd_engine_like(profit_amd_engine *engine, const profit_amd_profile *params, double *loglike);

@@ -98,6 +98,7 @@ SYMBOLS = [
     ("profit_amd_engine_add_galaxy", C.c_int, [C.c_void_p, _P(GalaxyDesc), _P(C.c_int32)]),
     ("profit_amd_engine_set_batch", C.c_int, [C.c_void_p, _P(C.c_int32), _P(C.c_int32), C.c_int32]),
     ("profit_amd_engine_set_tile_rows", C.c_int, [C.c_void_p, C.c_int32]),
+    ("profit_amd_engine_set_render_fp64", C.c_int, [C.c_void_p, C.c_int32]),
     ("profit_amd_engine_like", C.c_int, [C.c_void_p, _P(Profile), _P(C.c_double)]),
     ("profit_amd_engine_like_grad", C.c_int, [C.c_void_p, _P(Profile), _P(C.c_double), _P(C.c_double)]),
     ("profit_amd_engine_model_images", C.c_int, [C.c_void_p, _P(Profile), _P(C.c_float)]),
@@ -345,6 +346,9 @@ class Engine:
 
     def set_tile_rows(self, rows):
         _check(lib().profit_amd_engine_set_tile_rows(self._h, rows))
+
+    def set_render_fp64(self, on=True):
+        _check(lib().profit_amd_engine_set_render_fp64(self._h, int(on)))
 
     def _params(self, params):
         if isinstance(params, C.Array):

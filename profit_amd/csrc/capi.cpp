@@ -280,6 +280,11 @@ int profit_amd_engine_set_tile_rows(profit_amd_engine *engine, int32_t rows)
 	return guarded([&] { engine->eng.set_conv_rows(rows); });
 }
 
+int profit_amd_engine_set_render_fp64(profit_amd_engine *engine, int32_t on)
+{
+	return guarded([&] { engine->eng.set_render_fp64(on != 0); });
+}
+
 int profit_amd_engine_like(profit_amd_engine *engine, const profit_amd_profile *params, double *loglike)
 {
 	return guarded([&] {

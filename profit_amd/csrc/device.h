@@ -27,7 +27,7 @@ struct GalaxyDev {
 	int32_t has_calc;
 	int32_t n_rtiles, n_ctiles, n_atiles;
 	int32_t conv_rows;     // rows per thread of the convolution tiles
-	int32_t reserved;
+	int32_t render_fp64;   // direct pixels (and their adjoint) in FP64 instead of FP32
 	double sx, sy;         // profile pixel scale
 	double img_sx, img_sy; // image pixel scale
 	double magzero;

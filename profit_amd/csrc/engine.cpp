@@ -118,6 +118,16 @@ void Engine::set_conv_rows(int r)
 	if (!inst_galaxy_.empty()) set_batch(inst_galaxy_, inst_ncomp_);
 }
 
+void Engine::set_render_fp64(bool on)
+{
+	if (on == render_fp64_) return;
+	render_fp64_ = on;
+	for (auto &g : gal_host_) g.render_fp64 = on ? 1 : 0;
+	gals_dirty_ = true;
+	invalidate_graphs(graph_exec_);
+	if (!inst_galaxy_.empty()) set_batch(inst_galaxy_, inst_ncomp_);
+}
+
 int Engine::add_galaxy(const GalaxySpec &s)
 {
 	if (s.W <= 0 || s.H <= 0) throw std::invalid_argument("Model's requested dimensions are 0");
@@ -153,6 +163,7 @@ int Engine::add_galaxy(const GalaxySpec &s)
 		throw std::invalid_argument(os.str());
 	}
 	g.has_calc = s.calc != nullptr;
+	g.render_fp64 = render_fp64_ ? 1 : 0;
 	const size_t npix = (size_t)s.W * s.H, ndraw = (size_t)g.DW * g.DH;
 	if (conv) {
 		std::vector<double> p(s.psf, s.psf + (size_t)s.Kx * s.Ky);

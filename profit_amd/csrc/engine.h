@@ -148,6 +148,9 @@ class Engine {
 	int ncomp() const { return ncomp_total_; }
 	int conv_rows() const { return conv_rows_; }
 	void set_conv_rows(int r);
+	// direct pixels (and their adjoint) in FP64 (default) or FP32
+	void set_render_fp64(bool on);
+	bool render_fp64() const { return render_fp64_; }
 	void set_timing(bool on);
 	void reset_stats() { stats_ = EngineStats{}; }
 	// algorithmic work of the batch: pixels of the likelihood region and of the
@@ -173,6 +176,7 @@ class Engine {
 	int device_ = 0;
 	hipStream_t stream_ = nullptr;
 	int conv_rows_ = 8;
+	bool render_fp64_ = true;
 	int kc_max_ = 64;
 	bool timing_ = false;
 	EngineStats stats_;

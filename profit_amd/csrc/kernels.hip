@@ -131,7 +131,10 @@ __global__ __launch_bounds__(256) void k_render(const GalaxyDev *__restrict__ ga
 			if (active) {
 				const PixelClass pc = classify(c, x, y);
 				cls = pc.cls;
-				if (pc.cls == 2) acc += (float)c.flux_scale * eval_at<float>(c, (float)pc.xp, (float)pc.yp);
+				if (pc.cls == 2) {
+					if (g.render_fp64) acc += (float)(c.flux_scale * eval_at<double>(c, pc.xp, pc.yp));
+					else acc += (float)c.flux_scale * eval_at<float>(c, (float)pc.xp, (float)pc.yp);
+				}
 				else if (pc.cls == 3) sub = true;
 			}
 			if (diag_cls && inb) diag_cls[(long)(I.comp0 + k) * g.DW * g.DH + pix] = cls;
@@ -511,9 +514,10 @@ __global__ __launch_bounds__(256) void k_conv(const GalaxyDev *__restrict__ gals
 	}
 	const int LW = kTileW + Kx - 1;
 
-	float acc[R];
+	// per-column (kx) partial sums in FP32, accumulated across columns in FP64
+	double acc[R];
 #pragma unroll
-	for (int r = 0; r < R; ++r) acc[r] = 0.f;
+	for (int r = 0; r < R; ++r) acc[r] = 0.0;
 
 	for (int ky0 = 0; ky0 < Ky; ky0 += kc_max) {
 		const int kc = min(kc_max, Ky - ky0);
@@ -532,6 +536,9 @@ __global__ __launch_bounds__(256) void k_conv(const GalaxyDev *__restrict__ gals
 		for (int kx = 0; kx < Kx; ++kx) {
 			const float *col = colbase + kx;
 			const float *wcol = wt + (long)ky0 * Kx + kx;
+			float part[R];
+#pragma unroll
+			for (int r = 0; r < R; ++r) part[r] = 0.f;
 			float win[R];
 #pragma unroll
 			for (int r = 0; r < R - 1; ++r) win[r] = col[r * LW];
@@ -542,14 +549,16 @@ __global__ __launch_bounds__(256) void k_conv(const GalaxyDev *__restrict__ gals
 					win[(u + R - 1) % R] = col[(ky + u + R - 1) * LW];
 					const float w = wcol[(ky + u) * Kx];
 #pragma unroll
-					for (int r = 0; r < R; ++r) acc[r] = fmaf(win[(u + r) % R], w, acc[r]);
+					for (int r = 0; r < R; ++r) part[r] = fmaf(win[(u + r) % R], w, part[r]);
 				}
 			}
 			for (; ky < kc; ++ky) {
 				const float w = wcol[ky * Kx];
 #pragma unroll
-				for (int r = 0; r < R; ++r) acc[r] = fmaf(col[(ky + r) * LW], w, acc[r]);
+				for (int r = 0; r < R; ++r) part[r] = fmaf(col[(ky + r) * LW], w, part[r]);
 			}
+#pragma unroll
+			for (int r = 0; r < R; ++r) acc[r] += (double)part[r];
 		}
 	}
 
@@ -568,18 +577,18 @@ __global__ __launch_bounds__(256) void k_conv(const GalaxyDev *__restrict__ gals
 					// the model is zero outside the calc region (libprofit masks the
 					// convolution output there); such pixels do not feed the adjoint
 					const bool in_calc = !g.has_calc || g.calc[(long)(oyy + g.pady) * g.DW + oxx + g.padx];
-					const float m = in_calc ? acc[r] : 0.f;
-					const float d = g.data[o];
+					const double m = in_calc ? acc[r] : 0.0;
+					const double d = g.data[o];
 					if (pois) {
-						const double mm = s * (double)m, dd = s * (double)d;
+						const double mm = s * m, dd = s * d;
 						part += mm - dd * log(mm);
-						wv = (float)(-s * (1.0 - (double)d / (double)m));
+						wv = (float)(-s * (1.0 - d / m));
 					}
 					else {
-						const float diff = d - m;
-						const float iv = g.ivar[o];
-						part += (double)(diff * diff * iv);
-						wv = diff * iv;
+						const double diff = d - m;
+						const double iv = g.ivar[o];
+						part += diff * diff * iv;
+						wv = (float)(diff * iv);
 					}
 					if (!in_calc) wv = 0.f;
 				}
@@ -599,7 +608,7 @@ __global__ __launch_bounds__(256) void k_conv(const GalaxyDev *__restrict__ gals
 			const int oyy = y0 + ty * R + r;
 			if (oxx < OW && oyy < OH) {
 				const long o = (long)oyy * OW + oxx;
-				float v = acc[r];
+				float v = (float)acc[r];
 				if (MODE == CONV_MODEL && g.has_calc && !g.calc[o]) v = 0.f;
 				out[o] = v;
 			}
@@ -684,21 +693,37 @@ __global__ __launch_bounds__(256) void k_adj_render(const GalaxyDev *__restrict_
 			const double y = hy + j * g.sx, x = hx + i * g.sy;
 			const PixelClass pc = classify(c, x, y);
 			if (pc.cls != 2) continue;
-			float o[9];
-			const float xp = (float)pc.xp, yp = (float)pc.yp;
-			sample_grad<float>(c, xp, yp, o);
-			const float q = (float)c.axrat;
-			const float ca = (float)c.cos_ang, sa = (float)c.sin_ang;
-			const float fx = o[1], fy = o[2];
 			const double wq = (double)Gq * c.flux_scale;
-			const double f = o[0];
-			gl[G_XCEN] += wq * (double)(fx * -ca + fy * (sa / q));
-			gl[G_YCEN] += wq * (double)(fx * -sa + fy * (-ca / q));
-			gl[G_ANG] += wq * (double)((fx * (q * yp) + fy * (-xp / q)) * (3.14159265358979323846f / 180.f));
-			gl[G_AXRAT] += wq * (double)(fy * (-yp / q));
-			gl[G_BOX] += wq * (double)o[3];
+			double f;
+			if (g.render_fp64) {
+				double o[9];
+				sample_grad<double>(c, pc.xp, pc.yp, o);
+				const double q = c.axrat, fx = o[1], fy = o[2];
+				f = o[0];
+				gl[G_XCEN] += wq * (fx * -c.cos_ang + fy * (c.sin_ang / q));
+				gl[G_YCEN] += wq * (fx * -c.sin_ang + fy * (-c.cos_ang / q));
+				gl[G_ANG] += wq * (fx * (q * pc.yp) + fy * (-pc.xp / q)) * (3.14159265358979323846 / 180.);
+				gl[G_AXRAT] += wq * (fy * (-pc.yp / q));
+				gl[G_BOX] += wq * o[3];
 #pragma unroll
-			for (int s = 0; s < 5; ++s) gl[G_S0 + s] += wq * (double)o[4 + s];
+				for (int s = 0; s < 5; ++s) gl[G_S0 + s] += wq * o[4 + s];
+			}
+			else {
+				float o[9];
+				const float xp = (float)pc.xp, yp = (float)pc.yp;
+				sample_grad<float>(c, xp, yp, o);
+				const float q = (float)c.axrat;
+				const float ca = (float)c.cos_ang, sa = (float)c.sin_ang;
+				const float fx = o[1], fy = o[2];
+				f = o[0];
+				gl[G_XCEN] += wq * (double)(fx * -ca + fy * (sa / q));
+				gl[G_YCEN] += wq * (double)(fx * -sa + fy * (-ca / q));
+				gl[G_ANG] += wq * (double)((fx * (q * yp) + fy * (-xp / q)) * (3.14159265358979323846f / 180.f));
+				gl[G_AXRAT] += wq * (double)(fy * (-yp / q));
+				gl[G_BOX] += wq * (double)o[3];
+#pragma unroll
+				for (int s = 0; s < 5; ++s) gl[G_S0 + s] += wq * (double)o[4 + s];
+			}
 			// flux-scale dependence: d fs / d slot = fs * dlnfs
 #pragma unroll
 			for (int s = 0; s < NGRAD; ++s) gl[s] += wq * f * c.dlnfs[s];

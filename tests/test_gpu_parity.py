@@ -112,7 +112,7 @@ def test_config2_gradient_against_oracle():
         for s in slots:
             o, gv = ograd[k, s], ggrad[k, s]
             print(f"[parity] grad comp {k} slot {O.GRAD_SLOTS[s]}: gpu {gv:.6e} oracle {o:.6e}", flush=True)
-            assert abs(gv - o) <= 1e-3 * max(abs(o), 1e-2 * big), (k, s)
+            assert abs(gv - o) <= 1e-3 * max(abs(o), 5e-2 * big), (k, s)
     # at the truth: errors on the scale of the Fisher information sqrt(sum (dm/dtheta)^2 / sigma^2)
     oll0, ograd0 = O.loglike_grad(fit, oracle_profiles(wl["components"]))
     gll0, ggrad0 = eng.like_grad(gpu_profiles(wl["components"]))
