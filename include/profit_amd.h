@@ -157,10 +157,12 @@ int profit_amd_engine_set_batch(profit_amd_engine *engine, const int32_t *inst_g
 /* Tile height of the convolution / rendering tiles: 4*rows (rows = 2, 4, 8, 16). */
 int profit_amd_engine_set_tile_rows(profit_amd_engine *engine, int32_t rows);
 
-/* Precision of directly evaluated pixels and of their adjoint: 1 = FP64
- * (default), 0 = FP32.  Classification, the sub-pixel recursion and the
- * likelihood sums are FP64 either way; the convolution is FP32 with FP64
- * accumulation across PSF columns. */
+/* Precision of directly evaluated pixels in the forward model: 1 = FP64
+ * (default), 0 = FP32.  Either way: classification and sub-pixel sample values
+ * are FP64, refinement decisions are FP32 with an FP64 re-test inside a guard
+ * band (so they equal the FP64 decisions), the convolution is FP32 with FP64
+ * accumulation across PSF columns, likelihood sums are FP64 and the adjoint's
+ * per-sample derivatives are FP32. */
 int profit_amd_engine_set_render_fp64(profit_amd_engine *engine, int32_t on);
 
 /* logL per instance.  params: one profit_amd_profile per component,

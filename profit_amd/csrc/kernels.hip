@@ -172,26 +172,50 @@ __device__ inline double cell_centre(double lo, double half, int idx)
 	return v + half;
 }
 
-template <bool kGrad>
-__device__ inline void sample_accumulate(const Comp &c, double x, double y, double xp, double yp, double wgt,
-                                         double *gs)
+// d(f)/d slot of one sample in FP32, chain rule to the component slots,
+// accumulated with weight wgt (FP64) into gs.
+__device__ inline float sample_accumulate_f32(const Comp &c, double xp, double yp, double wgt, double *gs)
 {
-	double o[9];
-	sample_grad<double>(c, xp, yp, o);
-	const double q = c.axrat;
-	const double fx = o[1], fy = o[2];
-	gs[G_XCEN] += wgt * (fx * -c.cos_ang + fy * (c.sin_ang / q));
-	gs[G_YCEN] += wgt * (fx * -c.sin_ang + fy * (-c.cos_ang / q));
-	gs[G_ANG] += wgt * (fx * (q * yp) + fy * (-xp / q)) * (3.14159265358979323846 / 180.);
-	gs[G_AXRAT] += wgt * (fy * (-yp / q));
-	gs[G_BOX] += wgt * o[3];
+	float o[9];
+	const float fxp = (float)xp, fyp = (float)yp;
+	sample_grad<float>(c, fxp, fyp, o);
+	const float q = (float)c.axrat;
+	const float ca = (float)c.cos_ang, sa = (float)c.sin_ang;
+	const float fx = o[1], fy = o[2];
+	gs[G_XCEN] += wgt * (double)(fx * -ca + fy * (sa / q));
+	gs[G_YCEN] += wgt * (double)(fx * -sa + fy * (-ca / q));
+	gs[G_ANG] += wgt * (double)((fx * (q * fyp) + fy * (-fxp / q)) * (3.14159265358979323846f / 180.f));
+	gs[G_AXRAT] += wgt * (double)(fy * (-fyp / q));
+	gs[G_BOX] += wgt * (double)o[3];
 #pragma unroll
-	for (int s = 0; s < 5; ++s) gs[G_S0 + s] += wgt * o[4 + s];
+	for (int s = 0; s < 5; ++s) gs[G_S0 + s] += wgt * (double)o[4 + s];
+	return o[0];
+}
+
+// libprofit's refinement test |test - val| > acc * val (radial.cpp:92-100).
+// Evaluated in FP32 first; only when the FP32 margin lies within a guard band
+// of 1e-3 * val around the threshold (far wider than the FP32 evaluation
+// error) is it re-evaluated in FP64, so the decision equals the FP64 one.
+// Returns the decision; *v64 receives the FP64 value when it was computed.
+__device__ inline bool refine_cell(const Comp &c, double xp, double yp, double dyp, bool recurse, float v32,
+                                   double *v64, bool *have64)
+{
+	*have64 = false;
+	if (!recurse) return false;
+	const float t32 = eval_at<float>(c, (float)fabs(xp), (float)(fabs(yp) + fabs(dyp)));
+	const float margin = fabsf(t32 - v32) - (float)c.acc * v32;
+	if (v32 > 1e-30f && isfinite(margin) && fabsf(margin) > 1e-3f * v32) return margin > 0.f;
+	const double v = eval_at<double>(c, xp, yp);
+	const double t = eval_at<double>(c, fabs(xp), fabs(yp) + fabs(dyp));
+	*v64 = v;
+	*have64 = true;
+	return fabs(t - v) > c.acc * v;
 }
 
 // Walks the recursion tree of one pixel.  Every lane returns its share of
-// sum_s weight_s * f(s) (and of the derivative sums when kGrad); `scale`
-// multiplies the weights (G(q) * flux_scale in the adjoint, 1 otherwise).
+// sum_s weight_s * f(s) (FP64 sample values) or, when kGrad, of the
+// derivative sums (FP32 per-sample derivatives); `scale` multiplies the
+// weights (G(q) * flux_scale in the adjoint, 1 otherwise).
 template <bool kGrad>
 __device__ void walk_pixel(const Comp &c, WalkLevel *L, int lane, double x0, double x1, double y0, double y1, int res,
                            int maxrec, double scale, double &sum, double *gs)
@@ -207,6 +231,7 @@ __device__ void walk_pixel(const Comp &c, WalkLevel *L, int lane, double x0, dou
 		const double ybin = (ny1 - ny0) / res;
 		const double hx = xbin / 2., hy = ybin / 2.;
 		const bool recurse = res > 1 && level < maxrec;
+		const double dyp = -xbin * c.sin_over + ybin * c.cos_over;
 		unsigned long long m[4] = {0ull, 0ull, 0ull, 0ull};
 		for (int k = 0; k < nchunk; ++k) {
 			const int cell = k * 64 + lane;
@@ -218,15 +243,20 @@ __device__ void walk_pixel(const Comp &c, WalkLevel *L, int lane, double x0, dou
 				const double xr = x - c.xc, yr = y - c.yc;
 				const double xp = xr * c.cos_ang + yr * c.sin_ang;
 				const double yp = -xr * c.sin_over + yr * c.cos_over;
-				const double val = eval_at<double>(c, xp, yp);
-				if (recurse) {
-					const double dyp = -xbin * c.sin_over + ybin * c.cos_over;
-					const double test = eval_at<double>(c, fabs(xp), fabs(yp) + fabs(dyp));
-					fail = fabs(test - val) > c.acc * val;
-				}
+				const float v32 = eval_at<float>(c, (float)xp, (float)yp);
+				double v64 = 0;
+				bool have64 = false;
+				fail = refine_cell(c, xp, yp, dyp, recurse, v32, &v64, &have64);
 				if (!fail) {
-					sum += val * (weight * scale);
-					if (kGrad) sample_accumulate<kGrad>(c, x, y, xp, yp, weight * scale, gs);
+					if (kGrad) {
+						// value (for the flux-scale derivatives) and derivatives in FP32
+						const float f = sample_accumulate_f32(c, xp, yp, weight * scale, gs);
+						sum += (double)f * (weight * scale);
+					}
+					else {
+						if (!have64) v64 = eval_at<double>(c, xp, yp);
+						sum += v64 * (weight * scale);
+					}
 				}
 			}
 			m[k] = __ballot(fail);
@@ -403,15 +433,17 @@ __global__ __launch_bounds__(256) void k_subsample_nodes(const GalaxyDev *__rest
 				const int cell = k * 64 + lane;
 				bool fail = false;
 				if (cell < ncell && recurse) {
+					// the production decision path (FP32 test with the FP64 guard band)
 					const int ci = cell / res, cj = cell - ci * res;
 					const double cx = cell_centre(nx0, chx, ci), cy = cell_centre(ny0, chy, cj);
 					const double xr = cx - c.xc, yr = cy - c.yc;
 					const double xp = xr * c.cos_ang + yr * c.sin_ang;
 					const double yp = -xr * c.sin_over + yr * c.cos_over;
-					const double val = eval_at<double>(c, xp, yp);
 					const double dyp = -xbin * c.sin_over + ybin * c.cos_over;
-					const double test = eval_at<double>(c, fabs(xp), fabs(yp) + fabs(dyp));
-					fail = fabs(test - val) > c.acc * val;
+					const float v32 = eval_at<float>(c, (float)xp, (float)yp);
+					double v64;
+					bool have64;
+					fail = refine_cell(c, xp, yp, dyp, true, v32, &v64, &have64);
 				}
 				m[k] = __ballot(fail);
 			}
@@ -694,36 +726,21 @@ __global__ __launch_bounds__(256) void k_adj_render(const GalaxyDev *__restrict_
 			const PixelClass pc = classify(c, x, y);
 			if (pc.cls != 2) continue;
 			const double wq = (double)Gq * c.flux_scale;
-			double f;
-			if (g.render_fp64) {
-				double o[9];
-				sample_grad<double>(c, pc.xp, pc.yp, o);
-				const double q = c.axrat, fx = o[1], fy = o[2];
-				f = o[0];
-				gl[G_XCEN] += wq * (fx * -c.cos_ang + fy * (c.sin_ang / q));
-				gl[G_YCEN] += wq * (fx * -c.sin_ang + fy * (-c.cos_ang / q));
-				gl[G_ANG] += wq * (fx * (q * pc.yp) + fy * (-pc.xp / q)) * (3.14159265358979323846 / 180.);
-				gl[G_AXRAT] += wq * (fy * (-pc.yp / q));
-				gl[G_BOX] += wq * o[3];
+			// per-sample derivatives in FP32 (the forward pixels may be FP64)
+			float o[9];
+			const float xp = (float)pc.xp, yp = (float)pc.yp;
+			sample_grad<float>(c, xp, yp, o);
+			const float q = (float)c.axrat;
+			const float ca = (float)c.cos_ang, sa = (float)c.sin_ang;
+			const float fx = o[1], fy = o[2];
+			const double f = o[0];
+			gl[G_XCEN] += wq * (double)(fx * -ca + fy * (sa / q));
+			gl[G_YCEN] += wq * (double)(fx * -sa + fy * (-ca / q));
+			gl[G_ANG] += wq * (double)((fx * (q * yp) + fy * (-xp / q)) * (3.14159265358979323846f / 180.f));
+			gl[G_AXRAT] += wq * (double)(fy * (-yp / q));
+			gl[G_BOX] += wq * (double)o[3];
 #pragma unroll
-				for (int s = 0; s < 5; ++s) gl[G_S0 + s] += wq * o[4 + s];
-			}
-			else {
-				float o[9];
-				const float xp = (float)pc.xp, yp = (float)pc.yp;
-				sample_grad<float>(c, xp, yp, o);
-				const float q = (float)c.axrat;
-				const float ca = (float)c.cos_ang, sa = (float)c.sin_ang;
-				const float fx = o[1], fy = o[2];
-				f = o[0];
-				gl[G_XCEN] += wq * (double)(fx * -ca + fy * (sa / q));
-				gl[G_YCEN] += wq * (double)(fx * -sa + fy * (-ca / q));
-				gl[G_ANG] += wq * (double)((fx * (q * yp) + fy * (-xp / q)) * (3.14159265358979323846f / 180.f));
-				gl[G_AXRAT] += wq * (double)(fy * (-yp / q));
-				gl[G_BOX] += wq * (double)o[3];
-#pragma unroll
-				for (int s = 0; s < 5; ++s) gl[G_S0 + s] += wq * (double)o[4 + s];
-			}
+			for (int s = 0; s < 5; ++s) gl[G_S0 + s] += wq * (double)o[4 + s];
 			// flux-scale dependence: d fs / d slot = fs * dlnfs
 #pragma unroll
 			for (int s = 0; s < NGRAD; ++s) gl[s] += wq * f * c.dlnfs[s];
