@@ -165,6 +165,14 @@ int profit_amd_engine_set_tile_rows(profit_amd_engine *engine, int32_t rows);
  * per-sample derivatives are FP32. */
 int profit_amd_engine_set_render_fp64(profit_amd_engine *engine, int32_t on);
 
+/* Convolution method of the engine's galaxies with a PSF, the choice that
+ * Model::set_convolver (src/libprofit/src/profit/model.h:173-176) makes with a
+ * BruteForceConvolver or an FFTConvolver (convolve.h:120-209):
+ * PROFIT_AMD_CONV_BRUTE (default; tiled FP32 with FP64 accumulation) or
+ * PROFIT_AMD_CONV_FFT (FP64 2-D FFTs over a power-of-two frame).  Rebuilds the
+ * current batch. */
+int profit_amd_engine_set_conv_method(profit_amd_engine *engine, int32_t convolver);
+
 /* logL per instance.  params: one profit_amd_profile per component,
  * instance-major, in the batch's component order. */
 int profit_amd_engine_like(profit_amd_engine *engine, const profit_amd_profile *params, double *loglike);

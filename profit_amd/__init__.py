@@ -99,6 +99,7 @@ SYMBOLS = [
     ("profit_amd_engine_set_batch", C.c_int, [C.c_void_p, _P(C.c_int32), _P(C.c_int32), C.c_int32]),
     ("profit_amd_engine_set_tile_rows", C.c_int, [C.c_void_p, C.c_int32]),
     ("profit_amd_engine_set_render_fp64", C.c_int, [C.c_void_p, C.c_int32]),
+    ("profit_amd_engine_set_conv_method", C.c_int, [C.c_void_p, C.c_int32]),
     ("profit_amd_engine_like", C.c_int, [C.c_void_p, _P(Profile), _P(C.c_double)]),
     ("profit_amd_engine_like_grad", C.c_int, [C.c_void_p, _P(Profile), _P(C.c_double), _P(C.c_double)]),
     ("profit_amd_engine_model_images", C.c_int, [C.c_void_p, _P(Profile), _P(C.c_float)]),
@@ -349,6 +350,10 @@ class Engine:
 
     def set_render_fp64(self, on=True):
         _check(lib().profit_amd_engine_set_render_fp64(self._h, int(on)))
+
+    def set_conv_method(self, method="brute"):
+        """'brute' (tiled brute force) or 'fft' (FP64 FFT) for galaxies with a PSF."""
+        _check(lib().profit_amd_engine_set_conv_method(self._h, CONVOLVERS[method]))
 
     def _params(self, params):
         if isinstance(params, C.Array):

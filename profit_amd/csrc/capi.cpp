@@ -285,6 +285,15 @@ int profit_amd_engine_set_render_fp64(profit_amd_engine *engine, int32_t on)
 	return guarded([&] { engine->eng.set_render_fp64(on != 0); });
 }
 
+int profit_amd_engine_set_conv_method(profit_amd_engine *engine, int32_t convolver)
+{
+	return guarded([&] {
+		if (convolver == PROFIT_AMD_CONV_BRUTE) engine->eng.set_conv_method(0);
+		else if (convolver == PROFIT_AMD_CONV_FFT) engine->eng.set_conv_method(1);
+		else throw std::invalid_argument("the engine convolves with PROFIT_AMD_CONV_BRUTE or PROFIT_AMD_CONV_FFT");
+	});
+}
+
 int profit_amd_engine_like(profit_amd_engine *engine, const profit_amd_profile *params, double *loglike)
 {
 	return guarded([&] {

@@ -22,7 +22,9 @@ struct GalaxyDev {
 	int32_t W, H;          // image (likelihood) dimensions
 	int32_t DW, DH;        // drawing canvas dimensions
 	int32_t padx, pady;    // PSF padding (canvas shift)
-	int32_t Kx, Ky;        // PSF dimensions (0 without convolution)
+	int32_t Kx, Ky;        // PSF dimensions (1 x 1 unit kernel without a PSF)
+	int32_t has_psf;       // 0: no PSF (the unit kernel stands in)
+	int32_t use_fft;       // convolved through the FFT path
 	int32_t likefunc;
 	int32_t has_calc;
 	int32_t n_rtiles, n_ctiles, n_atiles;
@@ -40,6 +42,9 @@ struct GalaxyDev {
 	const uint8_t *region; // W*H likelihood mask (already intersected with calc)
 	const float *data;     // W*H
 	const float *ivar;     // W*H, 1/sigma^2
+	int32_t fft_n1, fft_n2;   // FFT frame (powers of two) when the FFT path is on
+	int32_t fft_l1, fft_l2;   // their base-2 logarithms
+	const double2 *fft_spec;  // N2 x N1 spectrum of the PSF, wrapped about the origin
 	const int2 *rtiles;    // render / adjoint-render tiles (64 x 4) on the canvas
 	const int2 *ctiles;    // forward convolution tiles on the image
 	const int2 *atiles;    // adjoint convolution tiles on the canvas
@@ -51,6 +56,7 @@ struct InstDev {
 	float *w;              // W*H  d logL / d model (up to a per-instance scalar)
 	float *G;              // DW*DH  correlated gradient image / convolved model
 	float *aux;            // DW*DH  profiles drawn without convolution (or null)
+	double2 *fbuf;         // N2 x N1 FFT frame (FFT path only)
 };
 
 struct QEntry {

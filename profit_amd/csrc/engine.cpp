@@ -118,6 +118,149 @@ void Engine::set_conv_rows(int r)
 	if (!inst_galaxy_.empty()) set_batch(inst_galaxy_, inst_ncomp_);
 }
 
+void Engine::set_conv_method(int m)
+{
+	if (m != 0 && m != 1) throw std::invalid_argument("convolution method must be 0 (brute) or 1 (fft)");
+	if (m == conv_method_) return;
+	conv_method_ = m;
+	invalidate_graphs(graph_exec_);
+	if (!inst_galaxy_.empty()) set_batch(inst_galaxy_, inst_ncomp_);
+}
+
+static int next_pow2(int v)
+{
+	int n = 1;
+	while (n < v) n <<= 1;
+	return n;
+}
+
+static int ilog2(int n)
+{
+	int l = 0;
+	while ((1 << l) < n) ++l;
+	return l;
+}
+
+// PSF spectra for the FFT path: g(d) = psf[K-1-K/2+d] about the origin (the
+// brute-force kernel's weights as a function of the offset), wrapped into an
+// N1 x N2 frame and transformed with the same FFT kernels.
+void Engine::ensure_fft_spectra()
+{
+	int twn = 2;
+	for (const auto &g : gal_host_)
+		if (g.use_fft) {
+			const int n1 = next_pow2(g.DW + g.Kx), n2 = next_pow2(g.DH + g.Ky);
+			twn = std::max(twn, std::max(n1, n2));
+		}
+	if (twn > 4096) throw std::invalid_argument("FFT frame above 4096 points per axis is not supported");
+	if (twn != twn_) {
+		std::vector<double2> tw(twn / 2);
+		const long double pi = 3.141592653589793238462643383279502884L;
+		for (int t = 0; t < twn / 2; ++t) {
+			const long double a = 2 * pi * (long double)t / (long double)twn;
+			tw[t] = make_double2((double)cosl(a), (double)(-sinl(a)));
+		}
+		d_tw_.alloc(tw.size() * sizeof(double2));
+		hip_check(hipMemcpy(d_tw_.as<double2>(), tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice), "tw");
+		twn_ = twn;
+		for (auto &b : gal_bufs_) b.fft_spec.release();
+	}
+	for (size_t gi = 0; gi < gal_host_.size(); ++gi) {
+		GalaxyDev &g = gal_host_[gi];
+		GalaxyBuffers &b = gal_bufs_[gi];
+		if (!g.use_fft || b.fft_spec.size()) continue;
+		const int n1 = next_pow2(g.DW + g.Kx), n2 = next_pow2(g.DH + g.Ky);
+		g.fft_n1 = n1;
+		g.fft_n2 = n2;
+		g.fft_l1 = ilog2(n1);
+		g.fft_l2 = ilog2(n2);
+		std::vector<double> frame((size_t)n1 * n2, 0.0);
+		const int ax = g.Kx - 1 - g.Kx / 2, ay = g.Ky - 1 - g.Ky / 2;
+		for (int ky = 0; ky < g.Ky; ++ky)
+			for (int kx = 0; kx < g.Kx; ++kx) {
+				const int dx = kx - ax, dy = ky - ay;
+				frame[(size_t)((dy + n2) % n2) * n1 + (dx + n1) % n1] = b.psf64[(size_t)ky * g.Kx + kx];
+			}
+		b.fft_spec.alloc(frame.size() * sizeof(double2));
+		DevBuf src(frame.size() * sizeof(double));
+		hip_check(hipMemcpy(src.as<double>(), frame.data(), frame.size() * sizeof(double), hipMemcpyHostToDevice), "psf");
+		g.fft_spec = b.fft_spec.as<double2>();
+		// one-instance context pointing at this galaxy
+		DevBuf dg(sizeof(GalaxyDev)), di(sizeof(InstDev)), doff(2 * sizeof(int));
+		InstDev I{};
+		I.galaxy = 0;
+		hip_check(hipMemcpy(dg.as<GalaxyDev>(), &g, sizeof(g), hipMemcpyHostToDevice), "gal");
+		hip_check(hipMemcpy(di.as<InstDev>(), &I, sizeof(I), hipMemcpyHostToDevice), "inst");
+		int off[2] = {0, n2};
+		hip_check(hipMemcpy(doff.as<int>(), off, sizeof(off), hipMemcpyHostToDevice), "off");
+		hip_check(launch_fft_rows_fwd(n2, n1, dg.as<GalaxyDev>(), di.as<InstDev>(), doff.as<int>(), 1, 2,
+		                              d_tw_.as<double2>(), twn_, src.as<double>(), b.fft_spec.as<double2>(), stream_),
+		          "psf rows");
+		const int cb = fft_cols_per_block(n2);
+		off[1] = (n1 + cb - 1) / cb;
+		hip_check(hipMemcpy(doff.as<int>(), off, sizeof(off), hipMemcpyHostToDevice), "off");
+		hip_check(launch_fft_cols(off[1], n2, cb, dg.as<GalaxyDev>(), di.as<InstDev>(), doff.as<int>(), 1, 0, 2, 0,
+		                          d_tw_.as<double2>(), twn_, b.fft_spec.as<double2>(), stream_),
+		          "psf cols");
+		hip_check(hipStreamSynchronize(stream_), "sync");
+		gals_dirty_ = true;
+	}
+}
+
+// forward convolution of every instance's canvas: mode 0 likelihood epilogue,
+// mode 1 model image into G (masked by the calc region).  Instances of
+// galaxies on the FFT path go through the three FFT passes, the others through
+// the tiled brute-force kernel; the FFT partials follow the brute-force ones.
+void Engine::conv_forward(int mode, hipStream_t st)
+{
+	const int n = ninst();
+	const GalaxyDev *gals = d_gals_.as<GalaxyDev>();
+	const InstDev *insts = d_insts_.as<InstDev>();
+	if (mode == 0)
+		hip_check(launch_conv(0, conv_rows_, n_ct_blocks_, kx_max_, kc_max_, gals, insts, d_ct_off_.as<int>(), n,
+		                      d_partial_.as<double>(), st),
+		          "conv");
+	else
+		hip_check(launch_conv(1, conv_rows_, n_bt_blocks_, kx_max_, kc_max_, gals, insts, d_bt_off_.as<int>(), n,
+		                      nullptr, st),
+		          "conv");
+	if (rf0_off_.empty() || rf0_off_[n] == 0) return;
+	hip_check(launch_fft_rows_fwd(rf0_off_[n], fft_nmax1_, gals, insts, d_rf0_off_.as<int>(), n, 0, d_tw_.as<double2>(),
+	                              twn_, nullptr, nullptr, st),
+	          "fft rows");
+	hip_check(launch_fft_cols(col_off_[n], fft_nmax2_, fft_cb_, gals, insts, d_col_off_.as<int>(), n, 1, 0,
+	                          mode == 0 ? 0 : 1, d_tw_.as<double2>(), twn_, nullptr, st),
+	          "fft cols");
+	if (mode == 0)
+		hip_check(launch_fft_rows_inv(ri0_off_[n], fft_nmax1_, gals, insts, d_ri0_off_.as<int>(), n, 0,
+		                              d_tw_.as<double2>(), twn_, d_partial_.as<double>() + n_ct_blocks_, st),
+		          "fft rows inv");
+	else
+		hip_check(launch_fft_rows_inv(ri1_off_[n], fft_nmax1_, gals, insts, d_ri1_off_.as<int>(), n, 1,
+		                              d_tw_.as<double2>(), twn_, nullptr, st),
+		          "fft rows inv");
+}
+
+void Engine::conv_adjoint(hipStream_t st)
+{
+	const int n = ninst();
+	const GalaxyDev *gals = d_gals_.as<GalaxyDev>();
+	const InstDev *insts = d_insts_.as<InstDev>();
+	hip_check(launch_conv(2, conv_rows_, n_bt_blocks_, kx_max_, kc_max_, gals, insts, d_bt_off_.as<int>(), n, nullptr,
+	                      st),
+	          "adjoint conv");
+	if (rf1_off_.empty() || rf1_off_[n] == 0) return;
+	hip_check(launch_fft_rows_fwd(rf1_off_[n], fft_nmax1_, gals, insts, d_rf1_off_.as<int>(), n, 1, d_tw_.as<double2>(),
+	                              twn_, nullptr, nullptr, st),
+	          "fft rows");
+	hip_check(launch_fft_cols(col_off_[n], fft_nmax2_, fft_cb_, gals, insts, d_col_off_.as<int>(), n, 2, 1, 1,
+	                          d_tw_.as<double2>(), twn_, nullptr, st),
+	          "fft cols");
+	hip_check(launch_fft_rows_inv(ri1_off_[n], fft_nmax1_, gals, insts, d_ri1_off_.as<int>(), n, 2, d_tw_.as<double2>(),
+	                              twn_, nullptr, st),
+	          "fft rows inv");
+}
+
 void Engine::set_render_fp64(bool on)
 {
 	if (on == render_fp64_) return;
@@ -146,8 +289,12 @@ int Engine::add_galaxy(const GalaxySpec &s)
 	g.sx = s.scale_x / s.finesampling;
 	g.sy = s.scale_y / s.finesampling;
 	const bool conv = s.psf && s.Kx > 0 && s.Ky > 0;
-	g.Kx = conv ? s.Kx : 0;
-	g.Ky = conv ? s.Ky : 0;
+	// without a PSF the likelihood path convolves with a 1 x 1 unit kernel,
+	// which copies the canvas exactly
+	g.has_psf = conv ? 1 : 0;
+	g.use_fft = 0;
+	g.Kx = conv ? s.Kx : 1;
+	g.Ky = conv ? s.Ky : 1;
 	// Model::analyze_expansion_requirements with a pre-adjusted mask
 	// (model.cpp:152-184, adjust_mask = false) or without a mask
 	bool pad = conv;
@@ -165,17 +312,21 @@ int Engine::add_galaxy(const GalaxySpec &s)
 	g.has_calc = s.calc != nullptr;
 	g.render_fp64 = render_fp64_ ? 1 : 0;
 	const size_t npix = (size_t)s.W * s.H, ndraw = (size_t)g.DW * g.DH;
-	if (conv) {
-		std::vector<double> p(s.psf, s.psf + (size_t)s.Kx * s.Ky);
-		double sum = 0;
-		for (double v : p) sum += v;
-		if (sum > 0)
-			for (double &v : p) v /= sum;  // Model::set_psf normalises (model.h:160-170)
+	{
+		std::vector<double> p(1, 1.0);
+		if (conv) {
+			p.assign(s.psf, s.psf + (size_t)s.Kx * s.Ky);
+			double sum = 0;
+			for (double v : p) sum += v;
+			if (s.normalise_psf && sum > 0)
+				for (double &v : p) v /= sum;  // Model::set_psf normalises (model.h:160-170)
+		}
+		bufs.psf64 = p;
 		std::vector<float> fwd(p.size()), adj(p.size());
-		for (int ky = 0; ky < s.Ky; ++ky)
-			for (int kx = 0; kx < s.Kx; ++kx) {
-				fwd[(size_t)ky * s.Kx + kx] = (float)p[(size_t)(s.Ky - 1 - ky) * s.Kx + (s.Kx - 1 - kx)];
-				adj[(size_t)ky * s.Kx + kx] = (float)p[(size_t)ky * s.Kx + kx];
+		for (int ky = 0; ky < g.Ky; ++ky)
+			for (int kx = 0; kx < g.Kx; ++kx) {
+				fwd[(size_t)ky * g.Kx + kx] = (float)p[(size_t)(g.Ky - 1 - ky) * g.Kx + (g.Kx - 1 - kx)];
+				adj[(size_t)ky * g.Kx + kx] = (float)p[(size_t)ky * g.Kx + kx];
 			}
 		bufs.psf_fwd.alloc(fwd.size() * sizeof(float));
 		bufs.psf_adj.alloc(adj.size() * sizeof(float));
@@ -298,6 +449,16 @@ void Engine::set_batch(const std::vector<int> &inst_galaxy, const std::vector<in
 	if (inst_galaxy.size() != inst_ncomp.size()) throw std::invalid_argument("instance arrays differ in length");
 	hip_check(hipSetDevice(device_), "hipSetDevice");
 	invalidate_graphs(graph_exec_);
+	bool any_fft = false;
+	for (auto &g : gal_host_) {
+		const int u = (conv_method_ == 1 && g.has_psf) ? 1 : 0;
+		if (u != g.use_fft) {
+			g.use_fft = u;
+			gals_dirty_ = true;
+		}
+		any_fft |= u != 0;
+	}
+	if (any_fft) ensure_fft_spectra();
 	upload_galaxies();
 	const int n = (int)inst_galaxy.size();
 	inst_galaxy_ = inst_galaxy;
@@ -319,21 +480,59 @@ void Engine::set_batch(const std::vector<int> &inst_galaxy, const std::vector<in
 		inst_comp0_[i] = ncomp_total_;
 		ncomp_total_ += inst_ncomp[i];
 		for (int k = 0; k < inst_ncomp[i]; ++k) comp_inst.push_back(i);
-		ct_off_[i + 1] = ct_off_[i] + g.n_ctiles;
+		ct_off_[i + 1] = ct_off_[i] + (g.use_fft ? 0 : g.n_ctiles);
 		at_off_[i + 1] = at_off_[i] + g.n_atiles;
-		kx_max_ = std::max(kx_max_, g.Kx);
+		if (!g.use_fft) kx_max_ = std::max(kx_max_, g.Kx);
 		const size_t ndraw = (size_t)g.DW * g.DH, npix = (size_t)g.W * g.H;
-		DevBuf canvas(ndraw * 4), w(npix * 4), G(ndraw * 4), aux(ndraw * 4);
-		insts[i] = InstDev{gi, inst_comp0_[i], inst_ncomp[i], 0, canvas.as<float>(), w.as<float>(), G.as<float>(),
-		                   aux.as<float>()};
+		DevBuf canvas(ndraw * 4), w(npix * 4), G(ndraw * 4), aux(ndraw * 4), fbuf;
+		if (g.use_fft) fbuf.alloc((size_t)g.fft_n1 * g.fft_n2 * sizeof(double2));
+		insts[i] = InstDev{gi,          inst_comp0_[i], inst_ncomp[i],   0,
+		                   canvas.as<float>(), w.as<float>(), G.as<float>(), aux.as<float>(), fbuf.as<double2>()};
 		inst_bufs_.push_back(std::move(canvas));
 		inst_bufs_.push_back(std::move(w));
 		inst_bufs_.push_back(std::move(G));
 		inst_bufs_.push_back(std::move(aux));
+		inst_bufs_.push_back(std::move(fbuf));
 		qneed += (long)inst_ncomp[i] * (long)std::min<size_t>(ndraw, 4096);
 	}
 	n_ct_blocks_ = ct_off_[n];
 	n_at_blocks_ = at_off_[n];
+	// brute-force model / adjoint tiles and the FFT passes, per instance
+	bt_off_.assign(n + 1, 0);
+	rf0_off_.assign(n + 1, 0);
+	rf1_off_.assign(n + 1, 0);
+	col_off_.assign(n + 1, 0);
+	ri0_off_.assign(n + 1, 0);
+	ri1_off_.assign(n + 1, 0);
+	fft_nmax1_ = fft_nmax2_ = 0;
+	for (int i = 0; i < n; ++i) {
+		const GalaxyDev &g = gal_host_[inst_galaxy[i]];
+		if (!g.use_fft) continue;
+		fft_nmax1_ = std::max(fft_nmax1_, g.fft_n1);
+		fft_nmax2_ = std::max(fft_nmax2_, g.fft_n2);
+	}
+	fft_cb_ = fft_cols_per_block(std::max(fft_nmax2_, 2));
+	for (int i = 0; i < n; ++i) {
+		const GalaxyDev &g = gal_host_[inst_galaxy[i]];
+		const bool f = g.use_fft != 0;
+		bt_off_[i + 1] = bt_off_[i] + (f ? 0 : g.n_atiles);
+		rf0_off_[i + 1] = rf0_off_[i] + (f ? g.DH : 0);
+		rf1_off_[i + 1] = rf1_off_[i] + (f ? g.H : 0);
+		col_off_[i + 1] = col_off_[i] + (f ? (g.fft_n1 + fft_cb_ - 1) / fft_cb_ : 0);
+		ri0_off_[i + 1] = ri0_off_[i] + (f ? g.H : 0);
+		ri1_off_[i + 1] = ri1_off_[i] + (f ? g.DH : 0);
+	}
+	n_bt_blocks_ = bt_off_[n];
+	auto upload_off = [&](DevBuf &d, const std::vector<int> &h) {
+		d.alloc((n + 1) * sizeof(int));
+		hip_check(hipMemcpy(d.as<int>(), h.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice), "off");
+	};
+	upload_off(d_bt_off_, bt_off_);
+	upload_off(d_rf0_off_, rf0_off_);
+	upload_off(d_rf1_off_, rf1_off_);
+	upload_off(d_col_off_, col_off_);
+	upload_off(d_ri0_off_, ri0_off_);
+	upload_off(d_ri1_off_, ri1_off_);
 	d_insts_.alloc(std::max(1, n) * sizeof(InstDev));
 	if (n) hip_check(hipMemcpy(d_insts_.as<InstDev>(), insts.data(), n * sizeof(InstDev), hipMemcpyHostToDevice), "insts");
 	d_ct_off_.alloc((n + 1) * sizeof(int));
@@ -348,7 +547,7 @@ void Engine::set_batch(const std::vector<int> &inst_galaxy, const std::vector<in
 	d_params_.alloc(nc * sizeof(ProfileParams));
 	d_comps_.alloc(nc * sizeof(Comp));
 	d_err_.alloc(nc * sizeof(int));
-	d_partial_.alloc(std::max(1, n_ct_blocks_) * sizeof(double));
+	d_partial_.alloc(std::max(1, n_ct_blocks_ + ri0_off_[n]) * sizeof(double));
 	d_ll_.alloc(std::max(1, n) * sizeof(double));
 	d_gscale_.alloc(std::max(1, n) * sizeof(double));
 	d_grad_acc_.alloc((size_t)nc * NGRAD * sizeof(double));
@@ -374,7 +573,6 @@ void Engine::record(bool grad)
 	const int nc = ncomp_total_;
 	const GalaxyDev *gals = d_gals_.as<GalaxyDev>();
 	const InstDev *insts = d_insts_.as<InstDev>();
-	const int kc = kc_max_;
 	hip_check(hipMemcpyAsync(d_params_.as<ProfileParams>(), h_params_.as<ProfileParams>(), nc * sizeof(ProfileParams),
 	                         hipMemcpyHostToDevice, st),
 	          "params");
@@ -393,17 +591,14 @@ void Engine::record(bool grad)
 	                           d_qcount_.as<int>(), (int)qcap_, 0, nullptr, st),
 	          "subsample");
 	mark(3);
-	hip_check(launch_conv(0, conv_rows_, n_ct_blocks_, kx_max_, kc, gals, insts, d_ct_off_.as<int>(), n,
-	                      d_partial_.as<double>(), st),
-	          "conv");
+	conv_forward(0, st);
 	mark(4);
-	hip_check(launch_like_finalize(gals, insts, d_ct_off_.as<int>(), n, d_partial_.as<double>(), d_ll_.as<double>(),
-	                               d_gscale_.as<double>(), st),
+	hip_check(launch_like_finalize(gals, insts, d_ct_off_.as<int>(), d_ri0_off_.as<int>(), n_ct_blocks_, n,
+	                               d_partial_.as<double>(), d_ll_.as<double>(), d_gscale_.as<double>(), st),
 	          "like");
 	mark(5);
 	if (grad) {
-		hip_check(launch_conv(2, conv_rows_, n_at_blocks_, kx_max_, kc, gals, insts, d_at_off_.as<int>(), n, nullptr, st),
-		          "adjoint conv");
+		conv_adjoint(st);
 		mark(6);
 		hip_check(launch_adj_render(gals, insts, d_at_off_.as<int>(), n, n_at_blocks_, d_comps_.as<Comp>(),
 		                            d_grad_acc_.as<double>(), st),
@@ -558,10 +753,7 @@ void Engine::model_images(const ProfileParams *params, float *out)
 		hip_check(launch_subsample(false, subsample_blocks_, gals, insts, d_comps_.as<Comp>(), d_queue_.as<QEntry>(),
 		                           d_qcount_.as<int>(), (int)qcap_, 0, nullptr, st),
 		          "subsample");
-		if (kx_max_ > 0)
-			hip_check(launch_conv(1, conv_rows_, n_at_blocks_, kx_max_, kc_max_, gals, insts, d_at_off_.as<int>(), n,
-			                      nullptr, st),
-			          "conv");
+		conv_forward(1, st);
 		hip_check(hipMemcpyAsync(h_err_.as<int>(), d_err_.as<int>(), nc * sizeof(int), hipMemcpyDeviceToHost, st), "err");
 		hip_check(hipMemcpyAsync(h_qcount_.as<int>(), d_qcount_.as<int>(), sizeof(int), hipMemcpyDeviceToHost, st),
 		          "qcount");
@@ -575,7 +767,7 @@ void Engine::model_images(const ProfileParams *params, float *out)
 	for (int i = 0; i < n; ++i) {
 		const GalaxyDev &g = gal_host_[inst_galaxy_[i]];
 		std::vector<float> canvas((size_t)g.DW * g.DH);
-		const float *src = g.Kx > 0 ? inst_bufs_[4 * i + 2].as<float>() : inst_bufs_[4 * i + 0].as<float>();
+		const float *src = inst_bufs_[5 * i + 2].as<float>();
 		hip_check(hipMemcpy(canvas.data(), src, canvas.size() * 4, hipMemcpyDeviceToHost), "model");
 		for (int j = 0; j < g.H; ++j)
 			for (int k = 0; k < g.W; ++k) out[off + (size_t)j * g.W + k] = canvas[(size_t)(j + g.pady) * g.DW + k + g.padx];
@@ -593,7 +785,7 @@ void Engine::model_canvas(const ProfileParams *params, float *out)
 	const GalaxyDev *gals = d_gals_.as<GalaxyDev>();
 	const InstDev *insts = d_insts_.as<InstDev>();
 	const GalaxyDev &g = gal_host_[inst_galaxy_[0]];
-	const bool conv = g.Kx > 0;
+	const bool conv = g.has_psf != 0;
 	for (int attempt = 0; attempt < 4; ++attempt) {
 		hip_check(hipMemcpyAsync(d_params_.as<ProfileParams>(), h_params_.as<ProfileParams>(), nc * sizeof(ProfileParams),
 		                         hipMemcpyHostToDevice, st),
@@ -613,10 +805,7 @@ void Engine::model_canvas(const ProfileParams *params, float *out)
 		hip_check(launch_subsample(false, subsample_blocks_, gals, insts, d_comps_.as<Comp>(), d_queue_.as<QEntry>(),
 		                           d_qcount_.as<int>(), (int)qcap_, conv ? 1 : 0, nullptr, st),
 		          "subsample");
-		if (conv)
-			hip_check(launch_conv(1, conv_rows_, n_at_blocks_, kx_max_, kc_max_, gals, insts, d_at_off_.as<int>(), 1,
-			                      nullptr, st),
-			          "conv");
+		if (conv) conv_forward(1, st);
 		hip_check(hipMemcpyAsync(h_err_.as<int>(), d_err_.as<int>(), nc * sizeof(int), hipMemcpyDeviceToHost, st), "err");
 		hip_check(hipMemcpyAsync(h_qcount_.as<int>(), d_qcount_.as<int>(), sizeof(int), hipMemcpyDeviceToHost, st),
 		          "qcount");
@@ -638,6 +827,23 @@ void Engine::model_canvas(const ProfileParams *params, float *out)
 	}
 }
 
+void Engine::convolve_canvas(const float *canvas, float *out)
+{
+	if (ninst() != 1) throw std::invalid_argument("convolve_canvas needs a batch of one instance");
+	hip_check(hipSetDevice(device_), "hipSetDevice");
+	const GalaxyDev &g = gal_host_[inst_galaxy_[0]];
+	const size_t nd = (size_t)g.DW * g.DH;
+	hip_check(hipMemcpy(inst_bufs_[0].as<float>(), canvas, nd * 4, hipMemcpyHostToDevice), "canvas");
+	if (g.has_psf) {
+		conv_forward(1, stream_);
+		hip_check(hipStreamSynchronize(stream_), "sync");
+		hip_check(hipMemcpy(out, inst_bufs_[2].as<float>(), nd * 4, hipMemcpyDeviceToHost), "out");
+	}
+	else {
+		std::memcpy(out, canvas, nd * 4);
+	}
+}
+
 void Engine::dilate(const uint8_t *in, uint8_t *out, int w, int h, int padx, int pady, int device)
 {
 	hip_check(hipSetDevice(device), "hipSetDevice");
@@ -650,9 +856,36 @@ void Engine::dilate(const uint8_t *in, uint8_t *out, int w, int h, int padx, int
 }
 
 void convolve_image(const double *src, int w, int h, const double *krn, int kw, int kh, const uint8_t *mask,
-                    double *out, int device)
+                    double *out, int device, int method)
 {
 	hip_check(hipSetDevice(device), "hipSetDevice");
+	if (method == 1) {
+		// FFT path: a one-instance engine whose canvas is the source image
+		// (no padding, the kernel as given, the mask as its calc region)
+		const size_t n = (size_t)w * h;
+		std::vector<double> zeros(n, 0.0), ones(n, 1.0);
+		std::vector<float> s(n), r(n);
+		for (size_t k = 0; k < n; ++k) s[k] = (float)src[k];
+		GalaxySpec gs;
+		gs.W = w;
+		gs.H = h;
+		gs.psf = krn;
+		gs.Kx = kw;
+		gs.Ky = kh;
+		gs.normalise_psf = false;
+		gs.calc = mask;
+		gs.cw = mask ? w : 0;
+		gs.ch = mask ? h : 0;
+		gs.data = zeros.data();
+		gs.sigma = ones.data();
+		Engine eng(device);
+		eng.set_conv_method(1);
+		eng.add_galaxy(gs);
+		eng.set_batch({0}, {0});
+		eng.convolve_canvas(s.data(), r.data());
+		for (size_t k = 0; k < n; ++k) out[k] = r[k];
+		return;
+	}
 	const int rows = 4;
 	const size_t n = (size_t)w * h;
 	std::vector<float> s(n), f((size_t)kw * kh);

@@ -27,18 +27,29 @@ int launch_subsample_nodes(int nblocks, const GalaxyDev *gals, const InstDev *in
                            const QEntry *queue, const int *qcount, int qcap, int *nodes, hipStream_t st);
 int launch_conv(int mode, int rows, int nblocks, int kx_max, int kc, const GalaxyDev *gals, const InstDev *insts,
                 const int *tile_off, int ninst, double *partial, hipStream_t st);
-int launch_like_finalize(const GalaxyDev *gals, const InstDev *insts, const int *tile_off, int ninst,
-                         const double *partial, double *ll, double *gscale, hipStream_t st);
+int launch_like_finalize(const GalaxyDev *gals, const InstDev *insts, const int *off_a, const int *off_b, int base_b,
+                         int ninst, const double *partial, double *ll, double *gscale, hipStream_t st);
 int launch_adj_render(const GalaxyDev *gals, const InstDev *insts, const int *tile_off, int ninst, int nblocks,
                       const Comp *comps, double *grad_acc, hipStream_t st);
 int launch_grad_finalize(const Comp *comps, const double *gscale, const double *grad_acc, double *grad_out,
                          int ncomp, hipStream_t st);
 int launch_dilate(const uint8_t *in, uint8_t *tmp, uint8_t *out, int w, int h, int padx, int pady, hipStream_t st);
+// FFT path (fft.hip)
+int launch_fft_rows_fwd(int nblocks, int nmax, const GalaxyDev *gals, const InstDev *insts, const int *row_off,
+                        int ninst, int mode, const double2 *tw, int twn, const double *src64, double2 *dst64,
+                        hipStream_t st);
+int launch_fft_cols(int nblocks, int n2max, int cb, const GalaxyDev *gals, const InstDev *insts, const int *col_off,
+                    int ninst, int op, int in_mode, int out_mode, const double2 *tw, int twn, double2 *dst64,
+                    hipStream_t st);
+int launch_fft_rows_inv(int nblocks, int nmax, const GalaxyDev *gals, const InstDev *insts, const int *row_off,
+                        int ninst, int mode, const double2 *tw, int twn, double *partial, hipStream_t st);
+int fft_cols_per_block(int n2);
 
-// Convolver::convolve_impl on the GPU: brute force, masked output pixels are 0.
-// The kernel is used as given (libprofit normalises only the Model's PSF).
+// Convolver::convolve_impl on the GPU, masked output pixels are 0: method 0
+// tiled brute force, 1 FFT.  The kernel is used as given (libprofit
+// normalises only the Model's PSF).
 void convolve_image(const double *src, int w, int h, const double *krn, int kw, int kh, const uint8_t *mask,
-                    double *out, int device);
+                    double *out, int device, int method = 0);
 
 class hip_error : public std::runtime_error {
   public:
@@ -109,6 +120,7 @@ struct GalaxySpec {
 	const double *sigma = nullptr;       // W*H
 	int likefunc = LIKE_CHISQ;
 	int finesampling = 1;                // W, H are then the fine grid; profiles use scale/finesampling
+	bool normalise_psf = true;           // Model::set_psf normalises; a bare Convolver does not
 };
 
 // Accumulated per-stage device time (HIP events on the engine stream) over the
@@ -130,6 +142,13 @@ class Engine {
 	Engine &operator=(const Engine &) = delete;
 
 	int add_galaxy(const GalaxySpec &spec);
+	// convolution method for every galaxy: 0 brute force (LDS-tiled FP32 with
+	// FP64 column sums), 1 FFT (FP64, power-of-two frames up to 4096)
+	void set_conv_method(int m);
+	int conv_method() const { return conv_method_; }
+	// a single instance's canvas supplied directly (no profiles): the model-mode
+	// convolution of `canvas` (DW*DH floats) into out (DW*DH floats)
+	void convolve_canvas(const float *canvas, float *out);
 	void set_batch(const std::vector<int> &inst_galaxy, const std::vector<int> &inst_ncomp);
 	// params: one ProfileParams per component, instance-major.  ll: one per
 	// instance; grad (nullable): NGRAD per component.
@@ -163,9 +182,13 @@ class Engine {
 
   private:
 	struct GalaxyBuffers {
-		DevBuf psf_fwd, psf_adj, calc, region, data, ivar, rtiles, ctiles, atiles;
+		DevBuf psf_fwd, psf_adj, calc, region, data, ivar, rtiles, ctiles, atiles, fft_spec;
 		std::vector<int2> ct_host, at_host;
+		std::vector<double> psf64;  // normalised (or raw) PSF, Ky rows of Kx
 	};
+	void ensure_fft_spectra();
+	void conv_forward(int mode, hipStream_t st);  // 0 likelihood, 1 model image
+	void conv_adjoint(hipStream_t st);
 	void build_tiles(int g);
 	void upload_galaxies();
 	void run(bool grad, bool use_graph);
@@ -176,6 +199,17 @@ class Engine {
 	int device_ = 0;
 	hipStream_t stream_ = nullptr;
 	int conv_rows_ = 8;
+	int conv_method_ = 0;
+	DevBuf d_tw_;
+	int twn_ = 0;
+	int fft_nmax1_ = 0, fft_nmax2_ = 0;
+	// per-instance launch tables (prefix sums of blocks): brute-force model and
+	// adjoint tiles, and the FFT passes (row forward on the canvas / image,
+	// columns, row inverse on the image / canvas)
+	std::vector<int> bt_off_, rf0_off_, rf1_off_, col_off_, ri0_off_, ri1_off_;
+	DevBuf d_bt_off_, d_rf0_off_, d_rf1_off_, d_col_off_, d_ri0_off_, d_ri1_off_;
+	int n_bt_blocks_ = 0;
+	int fft_cb_ = 8;
 	bool render_fp64_ = true;
 	int kc_max_ = 64;
 	bool timing_ = false;

@@ -321,7 +321,9 @@ __global__ __launch_bounds__(256) void k_subsample(const GalaxyDev *__restrict__
                                                    int split, double *grad_acc)
 {
 	__shared__ WalkLevel levels[4][kMaxLevels];
-	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+	// the wavefront index is uniform: make that visible to the compiler so queue
+	// entries and component fields are read through the scalar cache
+	const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 	WalkLevel *L = levels[wv];
 	const int gw = blockIdx.x * 4 + wv, nw = gridDim.x * 4;
 	int count = *qcount;
@@ -400,7 +402,9 @@ __global__ __launch_bounds__(256) void k_subsample_nodes(const GalaxyDev *__rest
                                                          const int *__restrict__ qcount, int qcap, int *nodes_out)
 {
 	__shared__ WalkLevel levels[4][kMaxLevels];
-	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+	// the wavefront index is uniform: make that visible to the compiler so queue
+	// entries and component fields are read through the scalar cache
+	const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 	WalkLevel *L = levels[wv];
 	const int gw = blockIdx.x * 4 + wv, nw = gridDim.x * 4;
 	int count = *qcount;
@@ -650,22 +654,28 @@ __global__ __launch_bounds__(256) void k_conv(const GalaxyDev *__restrict__ gals
 
 // ------------------------------------------------------------------ likelihood finalisation
 
+// The partials of instance i sit at [off_a[i], off_a[i+1]) (brute-force tiles)
+// and at base_b + [off_b[i], off_b[i+1]) (FFT rows); one of the two is empty.
 __global__ void k_like_finalize(const GalaxyDev *__restrict__ gals, const InstDev *__restrict__ insts,
-                                const int *__restrict__ tile_off, int ninst, const double *__restrict__ partial,
-                                double *ll, double *gscale)
+                                const int *__restrict__ off_a, const int *__restrict__ off_b, int base_b, int ninst,
+                                const double *__restrict__ partial, double *ll, double *gscale)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= ninst) return;
 	const GalaxyDev &g = gals[insts[i].galaxy];
 	// Neumaier-compensated sum of the block partials
 	double s = 0, comp = 0;
-	for (int b = tile_off[i]; b < tile_off[i + 1]; ++b) {
-		const double v = partial[b];
-		const double tsum = s + v;
-		if (fabs(s) >= fabs(v)) comp += (s - tsum) + v;
-		else comp += (v - tsum) + s;
-		s = tsum;
-	}
+	auto add = [&](int lo, int hi) {
+		for (int b = lo; b < hi; ++b) {
+			const double v = partial[b];
+			const double tsum = s + v;
+			if (fabs(s) >= fabs(v)) comp += (s - tsum) + v;
+			else comp += (v - tsum) + s;
+			s = tsum;
+		}
+	};
+	add(off_a[i], off_a[i + 1]);
+	add(base_b + off_b[i], base_b + off_b[i + 1]);
 	const double X = s + comp;
 	const double N = g.n_region;
 	double L = 0, gsc = 1;
@@ -887,11 +897,11 @@ int launch_conv(int mode, int rows, int nblocks, int kx_max, int kc, const Galax
 #undef PA_CONV_CASE
 }
 
-int launch_like_finalize(const GalaxyDev *gals, const InstDev *insts, const int *tile_off, int ninst,
-                         const double *partial, double *ll, double *gscale, hipStream_t st)
+int launch_like_finalize(const GalaxyDev *gals, const InstDev *insts, const int *off_a, const int *off_b, int base_b,
+                         int ninst, const double *partial, double *ll, double *gscale, hipStream_t st)
 {
 	if (ninst <= 0) return 0;
-	k_like_finalize<<<(ninst + 63) / 64, 64, 0, st>>>(gals, insts, tile_off, ninst, partial, ll, gscale);
+	k_like_finalize<<<(ninst + 63) / 64, 64, 0, st>>>(gals, insts, off_a, off_b, base_b, ninst, partial, ll, gscale);
 	return (int)hipGetLastError();
 }
 

@@ -294,7 +294,8 @@ Image Convolver::convolve_impl(const Image &src, const Image &krn, const Mask &m
 {
 	Image out(src.getDimensions());
 	profit_amd::convolve_image(src.data(), (int)src.getWidth(), (int)src.getHeight(), krn.data(), (int)krn.getWidth(),
-	                           (int)krn.getHeight(), mask ? mask.bytes() : nullptr, out.data(), device_);
+	                           (int)krn.getHeight(), mask ? mask.bytes() : nullptr, out.data(), device_,
+	                           type_ == FFT ? 1 : 0);
 	return out;
 }
 
@@ -467,6 +468,7 @@ void Model::evaluate(Image &image, Point &offset_out)
 			gs.ch = (int)draw_mask->getHeight();
 		}
 		profit_amd::Engine eng(device_);
+		if (convolver_ && convolver_->type() == FFT) eng.set_conv_method(1);
 		eng.add_galaxy(gs);
 		eng.set_batch({0}, {(int)radial.size()});
 		const profit_amd::GalaxyDev &g = eng.galaxy(0);
