@@ -115,7 +115,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile-rows", type=int, default=8)
     ap.add_argument("--render-fp32", action="store_true", help="direct pixels in FP32 instead of FP64")
-    ap.add_argument("--conv", choices=("brute", "fft"), default="brute", help="PSF convolution method")
+    ap.add_argument("--conv", choices=("brute", "fft", "brute-bf16x3"), default="brute",
+                    help="PSF convolution method")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU figures are not produced",

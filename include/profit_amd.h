@@ -45,6 +45,10 @@ enum {
 
 /* convolvers (src/libprofit/src/profit/convolve.h:68-80) */
 enum { PROFIT_AMD_CONV_BRUTE_OLD = 0, PROFIT_AMD_CONV_BRUTE = 1, PROFIT_AMD_CONV_FFT = 3, PROFIT_AMD_CONV_NULL = 4 };
+/* profit_amd's own: the brute-force sum on the matrix cores with split-BF16x3
+ * products (bf16 hi + lo operands, hi*hi + hi*lo + lo*hi, FP32 accumulation
+ * flushed to FP64 every 8 PSF rows); it has no libprofit counterpart. */
+enum { PROFIT_AMD_CONV_BRUTE_BF16X3 = 16 };
 
 /* likelihood functions (R/profitLikeModel.R:263-329) */
 enum { PROFIT_AMD_LIKE_NORM = 0, PROFIT_AMD_LIKE_CHISQ = 1, PROFIT_AMD_LIKE_T = 2, PROFIT_AMD_LIKE_POIS = 3 };
@@ -168,8 +172,9 @@ int profit_amd_engine_set_render_fp64(profit_amd_engine *engine, int32_t on);
 /* Convolution method of the engine's galaxies with a PSF, the choice that
  * Model::set_convolver (src/libprofit/src/profit/model.h:173-176) makes with a
  * BruteForceConvolver or an FFTConvolver (convolve.h:120-209):
- * PROFIT_AMD_CONV_BRUTE (default; tiled FP32 with FP64 accumulation) or
- * PROFIT_AMD_CONV_FFT (FP64 2-D FFTs over a power-of-two frame).  Rebuilds the
+ * PROFIT_AMD_CONV_BRUTE (default; tiled FP32 with FP64 accumulation),
+ * PROFIT_AMD_CONV_FFT (FP64 2-D FFTs over a power-of-two frame) or
+ * PROFIT_AMD_CONV_BRUTE_BF16X3 (matrix cores, split-BF16x3).  Rebuilds the
  * current batch. */
 int profit_amd_engine_set_conv_method(profit_amd_engine *engine, int32_t convolver);
 

@@ -34,7 +34,7 @@ SHAPE_NAMES = {
     SKY: ("bg",),
     NULL: (),
 }
-CONVOLVERS = {"brute-old": 0, "brute": 1, "fft": 3, "null": 4}
+CONVOLVERS = {"brute-old": 0, "brute": 1, "fft": 3, "null": 4, "brute-bf16x3": 16}
 LIKE = {"norm": 0, "chisq": 1, "t": 2, "pois": 3}
 NGRAD = 11
 GRAD_SLOTS = ("xcen", "ycen", "mag", "ang", "axrat", "box", "s0", "s1", "s2", "s3", "s4")
@@ -352,7 +352,8 @@ class Engine:
         _check(lib().profit_amd_engine_set_render_fp64(self._h, int(on)))
 
     def set_conv_method(self, method="brute"):
-        """'brute' (tiled brute force) or 'fft' (FP64 FFT) for galaxies with a PSF."""
+        """'brute' (tiled FP32 brute force), 'fft' (FP64 FFT) or 'brute-bf16x3' (matrix
+        cores, split-BF16x3) for the galaxies with a PSF."""
         _check(lib().profit_amd_engine_set_conv_method(self._h, CONVOLVERS[method]))
 
     def _params(self, params):

@@ -53,6 +53,7 @@ void build_model(profit::Model &m, const profit_amd_model_desc *d, const profit_
 	case PROFIT_AMD_CONV_BRUTE: m.set_convolver(profit::create_convolver(profit::BRUTE)); break;
 	case PROFIT_AMD_CONV_FFT: m.set_convolver(profit::create_convolver(profit::FFT)); break;
 	case PROFIT_AMD_CONV_NULL: m.set_convolver(profit::create_convolver(profit::NO_OP)); break;
+	case PROFIT_AMD_CONV_BRUTE_BF16X3: m.set_convolver(profit::create_convolver(profit::BRUTE_BF16X3)); break;
 	default: throw profit::invalid_parameter("Unsupported convolver type: " + std::to_string(d->convolver));
 	}
 	for (int k = 0; k < n; ++k) {
@@ -164,6 +165,7 @@ int profit_amd_convolve(int32_t convolver, const double *src, uint32_t src_w, ui
 		case PROFIT_AMD_CONV_BRUTE: c = profit::create_convolver(profit::BRUTE, prefs); break;
 		case PROFIT_AMD_CONV_FFT: c = profit::create_convolver(profit::FFT, prefs); break;
 		case PROFIT_AMD_CONV_NULL: c = profit::create_convolver(profit::NO_OP, prefs); break;
+		case PROFIT_AMD_CONV_BRUTE_BF16X3: c = profit::create_convolver(profit::BRUTE_BF16X3, prefs); break;
 		default: throw profit::invalid_parameter("Unsupported convolver type: " + std::to_string(convolver));
 		}
 		profit::Image s(std::vector<double>(src, src + (size_t)src_w * src_h), src_w, src_h);
@@ -175,7 +177,7 @@ int profit_amd_convolve(int32_t convolver, const double *src, uint32_t src_w, ui
 	});
 }
 
-const char *profit_amd_convolvers(void) { return "brute,brute-old,fft,null"; }
+const char *profit_amd_convolvers(void) { return "brute,brute-old,fft,null,brute-bf16x3"; }
 
 // image.cpp:246-367
 int profit_amd_downsample(const double *img, uint32_t w, uint32_t h, uint32_t f, int32_t mode, double *out,
@@ -288,9 +290,12 @@ int profit_amd_engine_set_render_fp64(profit_amd_engine *engine, int32_t on)
 int profit_amd_engine_set_conv_method(profit_amd_engine *engine, int32_t convolver)
 {
 	return guarded([&] {
-		if (convolver == PROFIT_AMD_CONV_BRUTE) engine->eng.set_conv_method(0);
-		else if (convolver == PROFIT_AMD_CONV_FFT) engine->eng.set_conv_method(1);
-		else throw std::invalid_argument("the engine convolves with PROFIT_AMD_CONV_BRUTE or PROFIT_AMD_CONV_FFT");
+		if (convolver == PROFIT_AMD_CONV_BRUTE) engine->eng.set_conv_method(profit_amd::CK_VALU);
+		else if (convolver == PROFIT_AMD_CONV_FFT) engine->eng.set_conv_method(profit_amd::CK_FFT);
+		else if (convolver == PROFIT_AMD_CONV_BRUTE_BF16X3) engine->eng.set_conv_method(profit_amd::CK_MFMA);
+		else
+			throw std::invalid_argument(
+			    "the engine convolves with PROFIT_AMD_CONV_BRUTE, PROFIT_AMD_CONV_FFT or PROFIT_AMD_CONV_BRUTE_BF16X3");
 	});
 }
 

@@ -15,6 +15,13 @@ enum LikeKind : int32_t { LIKE_NORM = 0, LIKE_CHISQ = 1, LIKE_T = 2, LIKE_POIS =
 constexpr int kTileW = 64;       // tile width: one wavefront spans a tile row
 constexpr int kRenderRows = 4;   // render tiles are 64 x 4 (256 threads)
 constexpr int kMaxLevels = 32;   // recursion walker depth (max_recursions <= 30)
+constexpr int kMfmaTile = 128;   // output tile side of the split-BF16x3 convolution
+
+// output of a convolution: fused likelihood, model image, adjoint correlation
+enum ConvMode : int { CONV_LIKE = 0, CONV_MODEL = 1, CONV_ADJ = 2 };
+
+// convolution kernel of a galaxy
+enum ConvKind { CK_VALU = 0, CK_FFT = 1, CK_MFMA = 2 };
 
 // One galaxy: image geometry, PSF, masks and data.  Several instances (parameter
 // sets) may share a galaxy.
@@ -24,7 +31,7 @@ struct GalaxyDev {
 	int32_t padx, pady;    // PSF padding (canvas shift)
 	int32_t Kx, Ky;        // PSF dimensions (1 x 1 unit kernel without a PSF)
 	int32_t has_psf;       // 0: no PSF (the unit kernel stands in)
-	int32_t use_fft;       // convolved through the FFT path
+	int32_t conv_kind;     // CK_VALU, CK_FFT or CK_MFMA
 	int32_t likefunc;
 	int32_t has_calc;
 	int32_t n_rtiles, n_ctiles, n_atiles;
@@ -45,6 +52,12 @@ struct GalaxyDev {
 	int32_t fft_n1, fft_n2;   // FFT frame (powers of two) when the FFT path is on
 	int32_t fft_l1, fft_l2;   // their base-2 logarithms
 	const double2 *fft_spec;  // N2 x N1 spectrum of the PSF, wrapped about the origin
+	int32_t mfma_kc;          // 16-wide K chunks of one Toeplitz row block (split-BF16x3 path)
+	int32_t n_mtl, n_mtc;     // 128 x 128 tiles: likelihood (image) and canvas
+	const uint4 *mfrag_fwd;   // B fragments of the forward weights, [ky][kc][hi|lo][lane]
+	const uint4 *mfrag_adj;   // and of the adjoint weights
+	const int2 *mtiles_like;  // 128 x 128 tiles on the image holding region pixels
+	const int2 *mtiles_canvas;// 128 x 128 tiles on the canvas holding calc pixels
 	const int2 *rtiles;    // render / adjoint-render tiles (64 x 4) on the canvas
 	const int2 *ctiles;    // forward convolution tiles on the image
 	const int2 *atiles;    // adjoint convolution tiles on the canvas

@@ -120,7 +120,8 @@ void Engine::set_conv_rows(int r)
 
 void Engine::set_conv_method(int m)
 {
-	if (m != 0 && m != 1) throw std::invalid_argument("convolution method must be 0 (brute) or 1 (fft)");
+	if (m < CK_VALU || m > CK_MFMA)
+		throw std::invalid_argument("convolution method must be 0 (brute FP32), 1 (fft) or 2 (brute split-BF16x3)");
 	if (m == conv_method_) return;
 	conv_method_ = m;
 	invalidate_graphs(graph_exec_);
@@ -148,7 +149,7 @@ void Engine::ensure_fft_spectra()
 {
 	int twn = 2;
 	for (const auto &g : gal_host_)
-		if (g.use_fft) {
+		if (g.conv_kind == CK_FFT) {
 			const int n1 = next_pow2(g.DW + g.Kx), n2 = next_pow2(g.DH + g.Ky);
 			twn = std::max(twn, std::max(n1, n2));
 		}
@@ -168,7 +169,7 @@ void Engine::ensure_fft_spectra()
 	for (size_t gi = 0; gi < gal_host_.size(); ++gi) {
 		GalaxyDev &g = gal_host_[gi];
 		GalaxyBuffers &b = gal_bufs_[gi];
-		if (!g.use_fft || b.fft_spec.size()) continue;
+		if (g.conv_kind != CK_FFT || b.fft_spec.size()) continue;
 		const int n1 = next_pow2(g.DW + g.Kx), n2 = next_pow2(g.DH + g.Ky);
 		g.fft_n1 = n1;
 		g.fft_n2 = n2;
@@ -208,23 +209,34 @@ void Engine::ensure_fft_spectra()
 }
 
 // forward convolution of every instance's canvas: mode 0 likelihood epilogue,
-// mode 1 model image into G (masked by the calc region).  Instances of
-// galaxies on the FFT path go through the three FFT passes, the others through
-// the tiled brute-force kernel; the FFT partials follow the brute-force ones.
+// mode 1 model image into G (masked by the calc region).  Each galaxy's
+// instances go through its kernel: tiled FP32 brute force, the three FFT
+// passes, or the split-BF16x3 matrix-core brute force; likelihood partials
+// land at the instance's base in the shared partials array.
 void Engine::conv_forward(int mode, hipStream_t st)
 {
 	const int n = ninst();
 	const GalaxyDev *gals = d_gals_.as<GalaxyDev>();
 	const InstDev *insts = d_insts_.as<InstDev>();
-	if (mode == 0)
-		hip_check(launch_conv(0, conv_rows_, n_ct_blocks_, kx_max_, kc_max_, gals, insts, d_ct_off_.as<int>(), n,
-		                      d_partial_.as<double>(), st),
+	double *partial = d_partial_.as<double>();
+	const int *lk = d_lk_off_.as<int>();
+	if (mode == 0) {
+		hip_check(launch_conv(CONV_LIKE, conv_rows_, n_ct_blocks_, kx_max_, kc_max_, gals, insts, d_ct_off_.as<int>(),
+		                      n, lk, partial, st),
 		          "conv");
-	else
-		hip_check(launch_conv(1, conv_rows_, n_bt_blocks_, kx_max_, kc_max_, gals, insts, d_bt_off_.as<int>(), n,
-		                      nullptr, st),
+		hip_check(launch_conv_mfma(CONV_LIKE, ml_off_[n], mfma_pitch_, mfma_kch_, gals, insts, d_ml_off_.as<int>(), n,
+		                           lk, partial, st),
+		          "conv mfma");
+	}
+	else {
+		hip_check(launch_conv(CONV_MODEL, conv_rows_, n_bt_blocks_, kx_max_, kc_max_, gals, insts, d_bt_off_.as<int>(),
+		                      n, lk, partial, st),
 		          "conv");
-	if (rf0_off_.empty() || rf0_off_[n] == 0) return;
+		hip_check(launch_conv_mfma(CONV_MODEL, mc_off_[n], mfma_pitch_, mfma_kch_, gals, insts, d_mc_off_.as<int>(), n,
+		                           lk, partial, st),
+		          "conv mfma");
+	}
+	if (rf0_off_[n] == 0) return;
 	hip_check(launch_fft_rows_fwd(rf0_off_[n], fft_nmax1_, gals, insts, d_rf0_off_.as<int>(), n, 0, d_tw_.as<double2>(),
 	                              twn_, nullptr, nullptr, st),
 	          "fft rows");
@@ -233,11 +245,11 @@ void Engine::conv_forward(int mode, hipStream_t st)
 	          "fft cols");
 	if (mode == 0)
 		hip_check(launch_fft_rows_inv(ri0_off_[n], fft_nmax1_, gals, insts, d_ri0_off_.as<int>(), n, 0,
-		                              d_tw_.as<double2>(), twn_, d_partial_.as<double>() + n_ct_blocks_, st),
+		                              d_tw_.as<double2>(), twn_, lk, partial, st),
 		          "fft rows inv");
 	else
 		hip_check(launch_fft_rows_inv(ri1_off_[n], fft_nmax1_, gals, insts, d_ri1_off_.as<int>(), n, 1,
-		                              d_tw_.as<double2>(), twn_, nullptr, st),
+		                              d_tw_.as<double2>(), twn_, lk, partial, st),
 		          "fft rows inv");
 }
 
@@ -246,10 +258,14 @@ void Engine::conv_adjoint(hipStream_t st)
 	const int n = ninst();
 	const GalaxyDev *gals = d_gals_.as<GalaxyDev>();
 	const InstDev *insts = d_insts_.as<InstDev>();
-	hip_check(launch_conv(2, conv_rows_, n_bt_blocks_, kx_max_, kc_max_, gals, insts, d_bt_off_.as<int>(), n, nullptr,
-	                      st),
+	const int *lk = d_lk_off_.as<int>();
+	hip_check(launch_conv(CONV_ADJ, conv_rows_, n_bt_blocks_, kx_max_, kc_max_, gals, insts, d_bt_off_.as<int>(), n, lk,
+	                      nullptr, st),
 	          "adjoint conv");
-	if (rf1_off_.empty() || rf1_off_[n] == 0) return;
+	hip_check(launch_conv_mfma(CONV_ADJ, mc_off_[n], mfma_pitch_, mfma_kch_, gals, insts, d_mc_off_.as<int>(), n, lk,
+	                           nullptr, st),
+	          "adjoint conv mfma");
+	if (rf1_off_[n] == 0) return;
 	hip_check(launch_fft_rows_fwd(rf1_off_[n], fft_nmax1_, gals, insts, d_rf1_off_.as<int>(), n, 1, d_tw_.as<double2>(),
 	                              twn_, nullptr, nullptr, st),
 	          "fft rows");
@@ -257,7 +273,7 @@ void Engine::conv_adjoint(hipStream_t st)
 	                          d_tw_.as<double2>(), twn_, nullptr, st),
 	          "fft cols");
 	hip_check(launch_fft_rows_inv(ri1_off_[n], fft_nmax1_, gals, insts, d_ri1_off_.as<int>(), n, 2, d_tw_.as<double2>(),
-	                              twn_, nullptr, st),
+	                              twn_, lk, nullptr, st),
 	          "fft rows inv");
 }
 
@@ -292,7 +308,7 @@ int Engine::add_galaxy(const GalaxySpec &s)
 	// without a PSF the likelihood path convolves with a 1 x 1 unit kernel,
 	// which copies the canvas exactly
 	g.has_psf = conv ? 1 : 0;
-	g.use_fft = 0;
+	g.conv_kind = CK_VALU;
 	g.Kx = conv ? s.Kx : 1;
 	g.Ky = conv ? s.Ky : 1;
 	// Model::analyze_expansion_requirements with a pre-adjusted mask
@@ -332,6 +348,15 @@ int Engine::add_galaxy(const GalaxySpec &s)
 		bufs.psf_adj.alloc(adj.size() * sizeof(float));
 		hip_check(hipMemcpy(bufs.psf_fwd.as<float>(), fwd.data(), fwd.size() * 4, hipMemcpyHostToDevice), "psf");
 		hip_check(hipMemcpy(bufs.psf_adj.as<float>(), adj.data(), adj.size() * 4, hipMemcpyHostToDevice), "psf");
+		// Toeplitz B fragments of both weight arrays for the matrix-core path
+		g.mfma_kc = mfma_kc(g.Kx);
+		std::vector<uint16_t> fr;
+		mfma_fragments(fwd.data(), g.Kx, g.Ky, fr);
+		bufs.mfrag_fwd.alloc(fr.size() * 2);
+		hip_check(hipMemcpy(bufs.mfrag_fwd.as<void>(), fr.data(), fr.size() * 2, hipMemcpyHostToDevice), "frag");
+		mfma_fragments(adj.data(), g.Kx, g.Ky, fr);
+		bufs.mfrag_adj.alloc(fr.size() * 2);
+		hip_check(hipMemcpy(bufs.mfrag_adj.as<void>(), fr.data(), fr.size() * 2, hipMemcpyHostToDevice), "frag");
 	}
 	std::vector<uint8_t> calc(ndraw, 1);
 	if (s.calc)
@@ -371,6 +396,8 @@ int Engine::add_galaxy(const GalaxySpec &s)
 	}
 	g.psf_fwd = bufs.psf_fwd.as<float>();
 	g.psf_adj = bufs.psf_adj.as<float>();
+	g.mfrag_fwd = bufs.mfrag_fwd.as<uint4>();
+	g.mfrag_adj = bufs.mfrag_adj.as<uint4>();
 	g.calc = bufs.calc.as<uint8_t>();
 	g.region = bufs.region.as<uint8_t>();
 	g.data = bufs.data.as<float>();
@@ -412,6 +439,35 @@ void Engine::build_tiles(int gi)
 	};
 	b.ct_host = tiles(region, g.W, g.H);
 	b.at_host = tiles(calc, g.DW, g.DH);
+	// 128 x 128 tiles of the matrix-core path
+	auto big_tiles = [&](const std::vector<uint8_t> &m, int w, int h) {
+		std::vector<int2> out;
+		for (int ty = 0; ty * kMfmaTile < h; ++ty)
+			for (int tx = 0; tx * kMfmaTile < w; ++tx) {
+				bool any = false;
+				for (int j = ty * kMfmaTile; j < std::min(h, (ty + 1) * kMfmaTile) && !any; ++j)
+					for (int i = tx * kMfmaTile; i < std::min(w, (tx + 1) * kMfmaTile); ++i)
+						if (m[(size_t)j * w + i]) {
+							any = true;
+							break;
+						}
+				if (any) out.push_back(make_int2(tx, ty));
+			}
+		return out;
+	};
+	const std::vector<int2> ml = big_tiles(region, g.W, g.H), mc = big_tiles(calc, g.DW, g.DH);
+	b.mtiles_like.alloc(std::max<size_t>(1, ml.size()) * sizeof(int2));
+	b.mtiles_canvas.alloc(std::max<size_t>(1, mc.size()) * sizeof(int2));
+	if (!ml.empty())
+		hip_check(hipMemcpy(b.mtiles_like.as<int2>(), ml.data(), ml.size() * sizeof(int2), hipMemcpyHostToDevice),
+		          "tiles");
+	if (!mc.empty())
+		hip_check(hipMemcpy(b.mtiles_canvas.as<int2>(), mc.data(), mc.size() * sizeof(int2), hipMemcpyHostToDevice),
+		          "tiles");
+	g.mtiles_like = b.mtiles_like.as<int2>();
+	g.mtiles_canvas = b.mtiles_canvas.as<int2>();
+	g.n_mtl = (int)ml.size();
+	g.n_mtc = (int)mc.size();
 	b.ctiles.alloc(std::max<size_t>(1, b.ct_host.size()) * sizeof(int2));
 	b.atiles.alloc(std::max<size_t>(1, b.at_host.size()) * sizeof(int2));
 	if (!b.ct_host.empty())
@@ -451,12 +507,12 @@ void Engine::set_batch(const std::vector<int> &inst_galaxy, const std::vector<in
 	invalidate_graphs(graph_exec_);
 	bool any_fft = false;
 	for (auto &g : gal_host_) {
-		const int u = (conv_method_ == 1 && g.has_psf) ? 1 : 0;
-		if (u != g.use_fft) {
-			g.use_fft = u;
+		const int k = g.has_psf ? conv_method_ : CK_VALU;
+		if (k != g.conv_kind) {
+			g.conv_kind = k;
 			gals_dirty_ = true;
 		}
-		any_fft |= u != 0;
+		any_fft |= k == CK_FFT;
 	}
 	if (any_fft) ensure_fft_spectra();
 	upload_galaxies();
@@ -480,12 +536,12 @@ void Engine::set_batch(const std::vector<int> &inst_galaxy, const std::vector<in
 		inst_comp0_[i] = ncomp_total_;
 		ncomp_total_ += inst_ncomp[i];
 		for (int k = 0; k < inst_ncomp[i]; ++k) comp_inst.push_back(i);
-		ct_off_[i + 1] = ct_off_[i] + (g.use_fft ? 0 : g.n_ctiles);
+		ct_off_[i + 1] = ct_off_[i] + (g.conv_kind == CK_VALU ? g.n_ctiles : 0);
 		at_off_[i + 1] = at_off_[i] + g.n_atiles;
-		if (!g.use_fft) kx_max_ = std::max(kx_max_, g.Kx);
+		if (g.conv_kind == CK_VALU) kx_max_ = std::max(kx_max_, g.Kx);
 		const size_t ndraw = (size_t)g.DW * g.DH, npix = (size_t)g.W * g.H;
 		DevBuf canvas(ndraw * 4), w(npix * 4), G(ndraw * 4), aux(ndraw * 4), fbuf;
-		if (g.use_fft) fbuf.alloc((size_t)g.fft_n1 * g.fft_n2 * sizeof(double2));
+		if (g.conv_kind == CK_FFT) fbuf.alloc((size_t)g.fft_n1 * g.fft_n2 * sizeof(double2));
 		insts[i] = InstDev{gi,          inst_comp0_[i], inst_ncomp[i],   0,
 		                   canvas.as<float>(), w.as<float>(), G.as<float>(), aux.as<float>(), fbuf.as<double2>()};
 		inst_bufs_.push_back(std::move(canvas));
@@ -499,23 +555,40 @@ void Engine::set_batch(const std::vector<int> &inst_galaxy, const std::vector<in
 	n_at_blocks_ = at_off_[n];
 	// brute-force model / adjoint tiles and the FFT passes, per instance
 	bt_off_.assign(n + 1, 0);
+	ml_off_.assign(n + 1, 0);
+	mc_off_.assign(n + 1, 0);
+	lk_off_.assign(n + 1, 0);
 	rf0_off_.assign(n + 1, 0);
 	rf1_off_.assign(n + 1, 0);
 	col_off_.assign(n + 1, 0);
 	ri0_off_.assign(n + 1, 0);
 	ri1_off_.assign(n + 1, 0);
 	fft_nmax1_ = fft_nmax2_ = 0;
+	int kc_m = 0, ky_m = 0;
 	for (int i = 0; i < n; ++i) {
 		const GalaxyDev &g = gal_host_[inst_galaxy[i]];
-		if (!g.use_fft) continue;
+		if (g.conv_kind == CK_MFMA) {
+			kc_m = std::max(kc_m, g.mfma_kc);
+			ky_m = std::max(ky_m, g.Ky);
+		}
+		if (g.conv_kind != CK_FFT) continue;
 		fft_nmax1_ = std::max(fft_nmax1_, g.fft_n1);
 		fft_nmax2_ = std::max(fft_nmax2_, g.fft_n2);
+	}
+	mfma_pitch_ = mfma_kch_ = 0;
+	if (kc_m) {
+		mfma_pitch_ = mfma_pitch(kc_m);
+		mfma_kch_ = mfma_rows_per_chunk(mfma_pitch_, ky_m, 150 * 1024);
+		if (mfma_kch_ < 1) throw std::invalid_argument("PSF too wide for the split-BF16x3 convolution");
 	}
 	fft_cb_ = fft_cols_per_block(std::max(fft_nmax2_, 2));
 	for (int i = 0; i < n; ++i) {
 		const GalaxyDev &g = gal_host_[inst_galaxy[i]];
-		const bool f = g.use_fft != 0;
-		bt_off_[i + 1] = bt_off_[i] + (f ? 0 : g.n_atiles);
+		const bool f = g.conv_kind == CK_FFT;
+		bt_off_[i + 1] = bt_off_[i] + (g.conv_kind == CK_VALU ? g.n_atiles : 0);
+		ml_off_[i + 1] = ml_off_[i] + (g.conv_kind == CK_MFMA ? g.n_mtl : 0);
+		mc_off_[i + 1] = mc_off_[i] + (g.conv_kind == CK_MFMA ? g.n_mtc : 0);
+		lk_off_[i + 1] = lk_off_[i] + (g.conv_kind == CK_VALU ? g.n_ctiles : g.conv_kind == CK_MFMA ? g.n_mtl : g.H);
 		rf0_off_[i + 1] = rf0_off_[i] + (f ? g.DH : 0);
 		rf1_off_[i + 1] = rf1_off_[i] + (f ? g.H : 0);
 		col_off_[i + 1] = col_off_[i] + (f ? (g.fft_n1 + fft_cb_ - 1) / fft_cb_ : 0);
@@ -528,6 +601,9 @@ void Engine::set_batch(const std::vector<int> &inst_galaxy, const std::vector<in
 		hip_check(hipMemcpy(d.as<int>(), h.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice), "off");
 	};
 	upload_off(d_bt_off_, bt_off_);
+	upload_off(d_ml_off_, ml_off_);
+	upload_off(d_mc_off_, mc_off_);
+	upload_off(d_lk_off_, lk_off_);
 	upload_off(d_rf0_off_, rf0_off_);
 	upload_off(d_rf1_off_, rf1_off_);
 	upload_off(d_col_off_, col_off_);
@@ -547,7 +623,7 @@ void Engine::set_batch(const std::vector<int> &inst_galaxy, const std::vector<in
 	d_params_.alloc(nc * sizeof(ProfileParams));
 	d_comps_.alloc(nc * sizeof(Comp));
 	d_err_.alloc(nc * sizeof(int));
-	d_partial_.alloc(std::max(1, n_ct_blocks_ + ri0_off_[n]) * sizeof(double));
+	d_partial_.alloc(std::max(1, lk_off_[n]) * sizeof(double));
 	d_ll_.alloc(std::max(1, n) * sizeof(double));
 	d_gscale_.alloc(std::max(1, n) * sizeof(double));
 	d_grad_acc_.alloc((size_t)nc * NGRAD * sizeof(double));
@@ -593,8 +669,8 @@ void Engine::record(bool grad)
 	mark(3);
 	conv_forward(0, st);
 	mark(4);
-	hip_check(launch_like_finalize(gals, insts, d_ct_off_.as<int>(), d_ri0_off_.as<int>(), n_ct_blocks_, n,
-	                               d_partial_.as<double>(), d_ll_.as<double>(), d_gscale_.as<double>(), st),
+	hip_check(launch_like_finalize(gals, insts, d_lk_off_.as<int>(), n, d_partial_.as<double>(), d_ll_.as<double>(),
+	                               d_gscale_.as<double>(), st),
 	          "like");
 	mark(5);
 	if (grad) {
@@ -859,12 +935,18 @@ void convolve_image(const double *src, int w, int h, const double *krn, int kw, 
                     double *out, int device, int method)
 {
 	hip_check(hipSetDevice(device), "hipSetDevice");
-	if (method == 1) {
-		// FFT path: a one-instance engine whose canvas is the source image
+	if (method == CK_FFT || method == CK_MFMA) {
+		// FFT or matrix-core path: a one-instance engine whose canvas is the source image
 		// (no padding, the kernel as given, the mask as its calc region)
 		const size_t n = (size_t)w * h;
 		std::vector<double> zeros(n, 0.0), ones(n, 1.0);
 		std::vector<float> s(n), r(n);
+		// an all-true mask when none is given keeps the canvas unpadded
+		std::vector<uint8_t> all;
+		if (!mask) {
+			all.assign(n, 1);
+			mask = all.data();
+		}
 		for (size_t k = 0; k < n; ++k) s[k] = (float)src[k];
 		GalaxySpec gs;
 		gs.W = w;
@@ -874,14 +956,15 @@ void convolve_image(const double *src, int w, int h, const double *krn, int kw, 
 		gs.Ky = kh;
 		gs.normalise_psf = false;
 		gs.calc = mask;
-		gs.cw = mask ? w : 0;
-		gs.ch = mask ? h : 0;
+		gs.cw = w;
+		gs.ch = h;
 		gs.data = zeros.data();
 		gs.sigma = ones.data();
 		Engine eng(device);
-		eng.set_conv_method(1);
+		eng.set_conv_method(method);
 		eng.add_galaxy(gs);
 		eng.set_batch({0}, {0});
+		if (eng.galaxy(0).DW != w || eng.galaxy(0).DH != h) throw std::logic_error("convolution canvas was padded");
 		eng.convolve_canvas(s.data(), r.data());
 		for (size_t k = 0; k < n; ++k) out[k] = r[k];
 		return;
@@ -920,7 +1003,7 @@ void convolve_image(const double *src, int w, int h, const double *krn, int kw, 
 	hip_check(hipMemcpy(d_off.as<int>(), off, sizeof(off), hipMemcpyHostToDevice), "off");
 	const int kc = std::min(kh, 64);
 	hip_check(launch_conv(1, rows, (int)tiles.size(), kw, kc, d_gal.as<GalaxyDev>(), d_inst.as<InstDev>(), d_off.as<int>(),
-	                      1, nullptr, nullptr),
+	                      1, nullptr, nullptr, nullptr),
 	          "conv");
 	hip_check(hipDeviceSynchronize(), "sync");
 	std::vector<float> o(n);

@@ -26,9 +26,9 @@ int launch_subsample(bool grad, int nblocks, const GalaxyDev *gals, const InstDe
 int launch_subsample_nodes(int nblocks, const GalaxyDev *gals, const InstDev *insts, const Comp *comps,
                            const QEntry *queue, const int *qcount, int qcap, int *nodes, hipStream_t st);
 int launch_conv(int mode, int rows, int nblocks, int kx_max, int kc, const GalaxyDev *gals, const InstDev *insts,
-                const int *tile_off, int ninst, double *partial, hipStream_t st);
-int launch_like_finalize(const GalaxyDev *gals, const InstDev *insts, const int *off_a, const int *off_b, int base_b,
-                         int ninst, const double *partial, double *ll, double *gscale, hipStream_t st);
+                const int *tile_off, int ninst, const int *lk_base, double *partial, hipStream_t st);
+int launch_like_finalize(const GalaxyDev *gals, const InstDev *insts, const int *lk_off, int ninst,
+                         const double *partial, double *ll, double *gscale, hipStream_t st);
 int launch_adj_render(const GalaxyDev *gals, const InstDev *insts, const int *tile_off, int ninst, int nblocks,
                       const Comp *comps, double *grad_acc, hipStream_t st);
 int launch_grad_finalize(const Comp *comps, const double *gscale, const double *grad_acc, double *grad_out,
@@ -42,8 +42,16 @@ int launch_fft_cols(int nblocks, int n2max, int cb, const GalaxyDev *gals, const
                     int ninst, int op, int in_mode, int out_mode, const double2 *tw, int twn, double2 *dst64,
                     hipStream_t st);
 int launch_fft_rows_inv(int nblocks, int nmax, const GalaxyDev *gals, const InstDev *insts, const int *row_off,
-                        int ninst, int mode, const double2 *tw, int twn, double *partial, hipStream_t st);
+                        int ninst, int mode, const double2 *tw, int twn, const int *lk_base, double *partial,
+                        hipStream_t st);
 int fft_cols_per_block(int n2);
+// split-BF16x3 matrix-core path (conv_mfma.hip)
+int mfma_kc(int kx);
+int mfma_pitch(int kc);
+int mfma_rows_per_chunk(int pitch, int ky, size_t budget);
+void mfma_fragments(const float *wt, int kx, int ky, std::vector<uint16_t> &out);
+int launch_conv_mfma(int mode, int nblocks, int pitch, int kch, const GalaxyDev *gals, const InstDev *insts,
+                     const int *tile_off, int ninst, const int *lk_base, double *partial, hipStream_t st);
 
 // Convolver::convolve_impl on the GPU, masked output pixels are 0: method 0
 // tiled brute force, 1 FFT.  The kernel is used as given (libprofit
@@ -183,6 +191,7 @@ class Engine {
   private:
 	struct GalaxyBuffers {
 		DevBuf psf_fwd, psf_adj, calc, region, data, ivar, rtiles, ctiles, atiles, fft_spec;
+		DevBuf mfrag_fwd, mfrag_adj, mtiles_like, mtiles_canvas;
 		std::vector<int2> ct_host, at_host;
 		std::vector<double> psf64;  // normalised (or raw) PSF, Ky rows of Kx
 	};
@@ -206,9 +215,12 @@ class Engine {
 	// per-instance launch tables (prefix sums of blocks): brute-force model and
 	// adjoint tiles, and the FFT passes (row forward on the canvas / image,
 	// columns, row inverse on the image / canvas)
-	std::vector<int> bt_off_, rf0_off_, rf1_off_, col_off_, ri0_off_, ri1_off_;
-	DevBuf d_bt_off_, d_rf0_off_, d_rf1_off_, d_col_off_, d_ri0_off_, d_ri1_off_;
+	// split-BF16x3 tiles (likelihood / canvas), and the likelihood partials
+	// base of each instance
+	std::vector<int> bt_off_, rf0_off_, rf1_off_, col_off_, ri0_off_, ri1_off_, ml_off_, mc_off_, lk_off_;
+	DevBuf d_bt_off_, d_rf0_off_, d_rf1_off_, d_col_off_, d_ri0_off_, d_ri1_off_, d_ml_off_, d_mc_off_, d_lk_off_;
 	int n_bt_blocks_ = 0;
+	int mfma_pitch_ = 0, mfma_kch_ = 0;
 	int fft_cb_ = 8;
 	bool render_fp64_ = true;
 	int kc_max_ = 64;

@@ -184,7 +184,8 @@ __global__ __launch_bounds__(256) void k_fft_cols(const GalaxyDev *__restrict__ 
 // mode 2 (adjoint): one block per canvas row; G = Re / (N1 N2).
 __global__ __launch_bounds__(256) void k_fft_rows_inv(const GalaxyDev *__restrict__ gals, const InstDev *__restrict__ insts,
                                                       const int *__restrict__ row_off, int ninst, int mode,
-                                                      const double2 *__restrict__ tw, int twn, double *partial)
+                                                      const double2 *__restrict__ tw, int twn,
+                                                      const int *__restrict__ lk_base, double *partial)
 {
 	extern __shared__ double2 sh[];
 	const int b = blockIdx.x;
@@ -233,7 +234,7 @@ __global__ __launch_bounds__(256) void k_fft_rows_inv(const GalaxyDev *__restric
 		if (threadIdx.x == 0) {
 			double t = 0;
 			for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t += red[k];
-			partial[b] = t;
+			partial[lk_base[inst] + r] = t;
 		}
 		return;
 	}
@@ -290,12 +291,13 @@ int launch_fft_cols(int nblocks, int n2max, int cb, const GalaxyDev *gals, const
 }
 
 int launch_fft_rows_inv(int nblocks, int nmax, const GalaxyDev *gals, const InstDev *insts, const int *row_off,
-                        int ninst, int mode, const double2 *tw, int twn, double *partial, hipStream_t st)
+                        int ninst, int mode, const double2 *tw, int twn, const int *lk_base, double *partial,
+                        hipStream_t st)
 {
 	if (nblocks <= 0) return 0;
 	const size_t lds = sizeof(double2) * (size_t)nmax;
 	if (int e = ensure_lds((const void *)k_fft_rows_inv, lds, lds_rows_inv)) return e;
-	k_fft_rows_inv<<<nblocks, 256, lds, st>>>(gals, insts, row_off, ninst, mode, tw, twn, partial);
+	k_fft_rows_inv<<<nblocks, 256, lds, st>>>(gals, insts, row_off, ninst, mode, tw, twn, lk_base, partial);
 	return (int)hipGetLastError();
 }
 

@@ -495,7 +495,6 @@ __global__ __launch_bounds__(256) void k_subsample_nodes(const GalaxyDev *__rest
 
 // ------------------------------------------------------------------ convolution
 
-enum ConvMode : int { CONV_LIKE = 0, CONV_MODEL = 1, CONV_ADJ = 2 };
 
 // out(p) = sum_{ky,kx} in(p.y + oy + ky, p.x + ox + kx) * wt[ky*Kx + kx],
 // in() is zero outside its image.  Block = 64 columns x 4 row groups; each
@@ -505,7 +504,7 @@ enum ConvMode : int { CONV_LIKE = 0, CONV_MODEL = 1, CONV_ADJ = 2 };
 template <int R, int MODE>
 __global__ __launch_bounds__(256) void k_conv(const GalaxyDev *__restrict__ gals, const InstDev *__restrict__ insts,
                                               const int *__restrict__ tile_off, int ninst, int kc_max,
-                                              double *partial)
+                                              const int *__restrict__ lk_base, double *partial)
 {
 	extern __shared__ float lds[];
 	const int b = blockIdx.x;
@@ -635,7 +634,7 @@ __global__ __launch_bounds__(256) void k_conv(const GalaxyDev *__restrict__ gals
 		part = wave_sum(part);
 		if (tx == 0) red[ty] = part;
 		__syncthreads();
-		if (threadIdx.x == 0) partial[b] = (red[0] + red[1]) + (red[2] + red[3]);
+		if (threadIdx.x == 0) partial[lk_base[inst] + (b - tile_off[inst])] = (red[0] + red[1]) + (red[2] + red[3]);
 	}
 	else {
 		float *out = I.G;
@@ -654,28 +653,24 @@ __global__ __launch_bounds__(256) void k_conv(const GalaxyDev *__restrict__ gals
 
 // ------------------------------------------------------------------ likelihood finalisation
 
-// The partials of instance i sit at [off_a[i], off_a[i+1]) (brute-force tiles)
-// and at base_b + [off_b[i], off_b[i+1]) (FFT rows); one of the two is empty.
+// The likelihood partials of instance i sit at [lk_off[i], lk_off[i+1]),
+// written by whichever convolution kernel its galaxy uses.
 __global__ void k_like_finalize(const GalaxyDev *__restrict__ gals, const InstDev *__restrict__ insts,
-                                const int *__restrict__ off_a, const int *__restrict__ off_b, int base_b, int ninst,
-                                const double *__restrict__ partial, double *ll, double *gscale)
+                                const int *__restrict__ lk_off, int ninst, const double *__restrict__ partial,
+                                double *ll, double *gscale)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= ninst) return;
 	const GalaxyDev &g = gals[insts[i].galaxy];
 	// Neumaier-compensated sum of the block partials
 	double s = 0, comp = 0;
-	auto add = [&](int lo, int hi) {
-		for (int b = lo; b < hi; ++b) {
-			const double v = partial[b];
-			const double tsum = s + v;
-			if (fabs(s) >= fabs(v)) comp += (s - tsum) + v;
-			else comp += (v - tsum) + s;
-			s = tsum;
-		}
-	};
-	add(off_a[i], off_a[i + 1]);
-	add(base_b + off_b[i], base_b + off_b[i + 1]);
+	for (int b = lk_off[i]; b < lk_off[i + 1]; ++b) {
+		const double v = partial[b];
+		const double tsum = s + v;
+		if (fabs(s) >= fabs(v)) comp += (s - tsum) + v;
+		else comp += (v - tsum) + s;
+		s = tsum;
+	}
 	const double X = s + comp;
 	const double N = g.n_region;
 	double L = 0, gsc = 1;
@@ -866,27 +861,27 @@ int launch_subsample_nodes(int nblocks, const GalaxyDev *gals, const InstDev *in
 
 template <int R, int MODE>
 static int conv_launch(int nblocks, size_t lds, const GalaxyDev *gals, const InstDev *insts, const int *tile_off,
-                       int ninst, int kc, double *partial, hipStream_t st)
+                       int ninst, int kc, const int *lk_base, double *partial, hipStream_t st)
 {
 	if (lds > 64 * 1024) {
 		PA_CHECK(hipFuncSetAttribute((const void *)k_conv<R, MODE>, hipFuncAttributeMaxDynamicSharedMemorySize,
 		                             (int)lds));
 	}
-	k_conv<R, MODE><<<nblocks, 256, lds, st>>>(gals, insts, tile_off, ninst, kc, partial);
+	k_conv<R, MODE><<<nblocks, 256, lds, st>>>(gals, insts, tile_off, ninst, kc, lk_base, partial);
 	return (int)hipGetLastError();
 }
 
 int launch_conv(int mode, int rows, int nblocks, int kx_max, int kc, const GalaxyDev *gals, const InstDev *insts,
-                const int *tile_off, int ninst, double *partial, hipStream_t st)
+                const int *tile_off, int ninst, const int *lk_base, double *partial, hipStream_t st)
 {
 	if (nblocks <= 0) return 0;
 	const size_t lds = sizeof(float) * (size_t)(4 * rows + kc - 1) * (size_t)(kTileW + kx_max - 1);
 	if (lds > 160 * 1024) return (int)hipErrorInvalidValue;
 #define PA_CONV_CASE(RR)                                                                                  \
 	case RR:                                                                                              \
-		if (mode == CONV_LIKE) return conv_launch<RR, CONV_LIKE>(nblocks, lds, gals, insts, tile_off, ninst, kc, partial, st); \
-		if (mode == CONV_MODEL) return conv_launch<RR, CONV_MODEL>(nblocks, lds, gals, insts, tile_off, ninst, kc, partial, st); \
-		return conv_launch<RR, CONV_ADJ>(nblocks, lds, gals, insts, tile_off, ninst, kc, partial, st);
+		if (mode == CONV_LIKE) return conv_launch<RR, CONV_LIKE>(nblocks, lds, gals, insts, tile_off, ninst, kc, lk_base, partial, st); \
+		if (mode == CONV_MODEL) return conv_launch<RR, CONV_MODEL>(nblocks, lds, gals, insts, tile_off, ninst, kc, lk_base, partial, st); \
+		return conv_launch<RR, CONV_ADJ>(nblocks, lds, gals, insts, tile_off, ninst, kc, lk_base, partial, st);
 	switch (rows) {
 		PA_CONV_CASE(2)
 		PA_CONV_CASE(4)
@@ -897,11 +892,11 @@ int launch_conv(int mode, int rows, int nblocks, int kx_max, int kc, const Galax
 #undef PA_CONV_CASE
 }
 
-int launch_like_finalize(const GalaxyDev *gals, const InstDev *insts, const int *off_a, const int *off_b, int base_b,
-                         int ninst, const double *partial, double *ll, double *gscale, hipStream_t st)
+int launch_like_finalize(const GalaxyDev *gals, const InstDev *insts, const int *lk_off, int ninst,
+                         const double *partial, double *ll, double *gscale, hipStream_t st)
 {
 	if (ninst <= 0) return 0;
-	k_like_finalize<<<(ninst + 63) / 64, 64, 0, st>>>(gals, insts, off_a, off_b, base_b, ninst, partial, ll, gscale);
+	k_like_finalize<<<(ninst + 63) / 64, 64, 0, st>>>(gals, insts, lk_off, ninst, partial, ll, gscale);
 	return (int)hipGetLastError();
 }
 

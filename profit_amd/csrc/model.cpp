@@ -295,7 +295,7 @@ Image Convolver::convolve_impl(const Image &src, const Image &krn, const Mask &m
 	Image out(src.getDimensions());
 	profit_amd::convolve_image(src.data(), (int)src.getWidth(), (int)src.getHeight(), krn.data(), (int)krn.getWidth(),
 	                           (int)krn.getHeight(), mask ? mask.bytes() : nullptr, out.data(), device_,
-	                           type_ == FFT ? 1 : 0);
+	                           type_ == FFT ? profit_amd::CK_FFT : type_ == BRUTE_BF16X3 ? profit_amd::CK_MFMA : profit_amd::CK_VALU);
 	return out;
 }
 
@@ -324,6 +324,7 @@ ConvolverPtr create_convolver(ConvolverType type, const ConvolverCreationPrefere
 	case BRUTE_OLD:
 	case BRUTE:
 	case FFT:
+	case BRUTE_BF16X3:
 	case NO_OP: return std::make_shared<Convolver>(type, prefs.device);
 	default: throw invalid_parameter("Unsupported convolver type: " + std::to_string((int)type));
 	}
@@ -335,6 +336,7 @@ ConvolverPtr create_convolver(const std::string &type, const ConvolverCreationPr
 	if (type == "brute") return create_convolver(BRUTE, prefs);
 	if (type == "fft") return create_convolver(FFT, prefs);
 	if (type == "null") return create_convolver(NO_OP, prefs);
+	if (type == "brute-bf16x3") return create_convolver(BRUTE_BF16X3, prefs);
 	throw invalid_parameter("Convolver of type " + type + " is not supported");
 }
 
@@ -468,7 +470,8 @@ void Model::evaluate(Image &image, Point &offset_out)
 			gs.ch = (int)draw_mask->getHeight();
 		}
 		profit_amd::Engine eng(device_);
-		if (convolver_ && convolver_->type() == FFT) eng.set_conv_method(1);
+		if (convolver_ && convolver_->type() == FFT) eng.set_conv_method(profit_amd::CK_FFT);
+		if (convolver_ && convolver_->type() == BRUTE_BF16X3) eng.set_conv_method(profit_amd::CK_MFMA);
 		eng.add_galaxy(gs);
 		eng.set_batch({0}, {(int)radial.size()});
 		const profit_amd::GalaxyDev &g = eng.galaxy(0);

@@ -152,7 +152,7 @@ class Profile {
 };
 using ProfilePtr = std::shared_ptr<Profile>;
 
-enum ConvolverType { BRUTE_OLD = 0, BRUTE, OPENCL, FFT, NO_OP };
+enum ConvolverType { BRUTE_OLD = 0, BRUTE, OPENCL, FFT, NO_OP, BRUTE_BF16X3 = 16 };
 
 class ConvolverCreationPreferences {
   public:

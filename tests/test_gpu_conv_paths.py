@@ -1,9 +1,11 @@
-"""FFT convolution path and PSF-less galaxies vs the FP64 oracle, through the C-ABI.
+"""The FFT and split-BF16x3 convolution paths, and PSF-less galaxies, vs the FP64
+oracle through the C-ABI.
 
 libprofit's FFTConvolver (convolve.cpp:233-312) computes the same linear
-convolution as the brute-force one, so the FFT path is held to the brute-force
-gates: per-pixel relative error <= 1e-5 above 1e-6 of the peak, |logL
-difference| <= 1e-3, gradients as in the brute-force parity tests.
+convolution as the brute-force one (convolve.cpp:158-231); the matrix-core
+path is that brute-force sum with split-BF16x3 products.  Both are held to the
+brute-force gates: per-pixel relative error <= 1e-5 above 1e-6 of the peak,
+|logL difference| <= 1e-3, gradients as in the brute-force parity tests.
 """
 import numpy as np
 import pytest
@@ -18,34 +20,40 @@ pytestmark = pytest.mark.gpu
 REL_TOL = 1e-5
 
 
+METHODS = ["fft", "brute-bf16x3"]
+
+
 def _report(name, err, n):
-    print(f"[fft] {name}: max rel err {err:.3e} over {n} pixels", flush=True)
+    print(f"[conv] {name}: max rel err {err:.3e} over {n} pixels", flush=True)
 
 
-@pytest.mark.parametrize("shape,ksize", [((90, 70), 5), ((90, 70), 25), ((64, 64), 51), ((33, 130), 9)])
-def test_fft_convolution(shape, ksize):
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("shape,ksize", [((90, 70), 5), ((90, 70), 25), ((64, 64), 51), ((33, 130), 9),
+                                         ((300, 260), 41)])
+def test_convolution(method, shape, ksize):
     rng = np.random.default_rng(ksize)
     src = rng.random(shape)
     krn = rng.random((ksize, ksize))
     mask = (rng.random(shape) > 0.3).astype(np.uint8)
     ref = O.convolve(src, krn, mask)
-    got = P.convolve(src, krn, mask, convolver="fft")
+    got = P.convolve(src, krn, mask, convolver=method)
     err, n = rel_err(got, ref)
-    _report(f"conv {shape} k={ksize}", err, n)
+    _report(f"{method} {shape} k={ksize}", err, n)
     assert err <= REL_TOL
     assert np.all(got[mask == 0] == 0)
-    nomask = P.convolve(src, krn, None, convolver="fft")
+    nomask = P.convolve(src, krn, None, convolver=method)
     err, n = rel_err(nomask, O.convolve(src, krn, None))
     assert err <= REL_TOL
 
 
-def test_fft_config1_make_model():
+@pytest.mark.parametrize("method", METHODS)
+def test_config1_make_model(method):
     wl = W.config1()
     oimg = O.model_evaluate(O.ModelSpec(100, 100, psf=wl["psf"], threads=8),
                             oracle_profiles(wl["components"]))["image"]
-    gimg = P.make_model(100, 100, gpu_profiles(wl["components"]), psf=wl["psf"], convolver="fft")["image"]
+    gimg = P.make_model(100, 100, gpu_profiles(wl["components"]), psf=wl["psf"], convolver=method)["image"]
     err, n = rel_err(gimg, oimg)
-    _report("config1 make_model", err, n)
+    _report(f"{method} config1 make_model", err, n)
     assert err <= REL_TOL
 
 
@@ -70,16 +78,17 @@ def _walker(comps):
     return out
 
 
-def test_fft_config2_model_likelihood_gradient():
-    wl, fit, eng = _config2_engine("fft")
+@pytest.mark.parametrize("method", METHODS)
+def test_config2_model_likelihood_gradient(method):
+    wl, fit, eng = _config2_engine(method)
     params = gpu_profiles(wl["components"])
     oll, omodel = O.loglike(fit, oracle_profiles(wl["components"]), want_model=True)
     gmodel = eng.model_images(params)[0]
     err, n = rel_err(gmodel, omodel)
-    _report("config2 model", err, n)
+    _report(f"{method} config2 model", err, n)
     assert err <= REL_TOL
     gll = eng.like(params)[0]
-    print(f"[fft] config2 logL gpu {gll:.6f} oracle {oll:.6f} diff {gll - oll:.3e}", flush=True)
+    print(f"[conv] {method} config2 logL gpu {gll:.6f} oracle {oll:.6f} diff {gll - oll:.3e}", flush=True)
     assert abs(gll - oll) <= 1e-3
     comps = _walker(wl["components"])
     oll, ograd = O.loglike_grad(fit, oracle_profiles(comps))
@@ -90,13 +99,14 @@ def test_fft_config2_model_likelihood_gradient():
         big = np.abs(ograd[k, slots]).max()
         for s in slots:
             o, gv = ograd[k, s], ggrad[k, s]
-            print(f"[fft] grad comp {k} slot {O.GRAD_SLOTS[s]}: gpu {gv:.6e} oracle {o:.6e}", flush=True)
+            print(f"[conv] {method} grad comp {k} slot {O.GRAD_SLOTS[s]}: gpu {gv:.6e} oracle {o:.6e}", flush=True)
             assert abs(gv - o) <= 1e-3 * max(abs(o), 5e-2 * big), (k, s)
 
 
-def test_fft_and_brute_agree_on_a_batch():
-    """A batch of perturbed walkers: FFT and brute-force logL and gradients agree."""
-    wl, _, eng_f = _config2_engine("fft")
+@pytest.mark.parametrize("method", METHODS)
+def test_agrees_with_brute_on_a_batch(method):
+    """A batch of perturbed walkers: logL and gradients agree with the FP32 brute-force path."""
+    wl, _, eng_f = _config2_engine(method)
     _, _, eng_b = _config2_engine("brute")
     rng = np.random.default_rng(7)
     walkers = []
@@ -110,7 +120,7 @@ def test_fft_and_brute_agree_on_a_batch():
     params = [p for c in walkers for p in gpu_profiles(c)]
     llf, gf = eng_f.like_grad(params)
     llb, gb = eng_b.like_grad(params)
-    print(f"[fft] batch logL max diff {np.max(np.abs(llf - llb)):.3e}", flush=True)
+    print(f"[conv] {method} batch logL max diff {np.max(np.abs(llf - llb)):.3e}", flush=True)
     assert np.max(np.abs(llf - llb)) <= 1e-3
     scale = np.abs(gb).max()
     assert np.max(np.abs(gf - gb)) <= 1e-3 * scale
@@ -126,14 +136,14 @@ def test_psf_less_galaxy_likelihood_and_gradient():
     fit_p = O.FitSpec(O.ModelSpec(100, 100, psf=wl["psf"], magzero=30.0, threads=8), wl["data"], wl["sigma"],
                       wl["region"])
     oll_p = O.loglike(fit_p, oracle_profiles(comps))
-    for method in ("brute", "fft"):
+    for method in ["brute"] + METHODS:
         eng = P.Engine()
         eng.set_conv_method(method)
         g0 = eng.add_galaxy(100, 100, wl["data"], wl["sigma"], magzero=30.0)
         g1 = eng.add_galaxy(100, 100, wl["data"], wl["sigma"], psf=wl["psf"], magzero=30.0)
         eng.set_batch([g0, g1], [1, 1])
         ll, grad = eng.like_grad(gpu_profiles(comps) + gpu_profiles(comps))
-        print(f"[fft] {method} no-PSF logL gpu {ll[0]:.6f} oracle {oll_np:.6f}; PSF logL gpu {ll[1]:.6f} "
+        print(f"[conv] {method} no-PSF logL gpu {ll[0]:.6f} oracle {oll_np:.6f}; PSF logL gpu {ll[1]:.6f} "
               f"oracle {oll_p:.6f}", flush=True)
         assert abs(ll[0] - oll_np) <= 1e-3
         assert abs(ll[1] - oll_p) <= 1e-3
