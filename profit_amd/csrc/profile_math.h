@@ -241,6 +241,54 @@ PA_HD T brokenexp_core(T r, T h1, T h2, T rb, T a)
 	return ::exp(-r / h1 + expo * base);
 }
 
+// d ln f / dr and d ln f / d shape of the core-Sersic profile
+// f = (1 + (r/rb)^-a)^(b/a) exp(-bn ((r^a + rb^a)/re^a)^(1/(n a))),
+// shape = (re, rb, nser, a, b); bn = qgamma(0.5, 2 nser), dbn its n-derivative.
+template <class T>
+PA_HD T coresersic_dlog(const Comp &c, T r, T *ds)
+{
+	const T re = T(c.s[0]), rb = T(c.s[1]), n = T(c.s[2]), a = T(c.s[3]), b = T(c.s[4]), bn = T(c.bn);
+	const T t = ::pow(r / rb, -a), Q = T(1) + t;
+	const T ra = ::pow(r, a), rba = ::pow(rb, a), sum = ra + rba;
+	const T S = sum / ::pow(re, a);
+	const T lS = ::log(S);
+	const T W = ::exp(lS / (n * a));
+	ds[0] = bn * W / (n * re);
+	ds[1] = b * t / (rb * Q) - bn * W * rba / (rb * n * sum);
+	ds[2] = -T(c.dbn) * W + bn * W * lS / (n * n * a);
+	const T dlnS_da = (ra * ::log(r) + rba * ::log(rb)) / sum - ::log(re);
+	ds[3] = -(b / (a * a)) * ::log(Q) - (b / a) * t * ::log(r / rb) / Q - bn * W * (dlnS_da / (n * a) - lS / (n * a * a));
+	ds[4] = ::log(Q) / a;
+	return -b * t / (r * Q) - bn * W * ra / (r * n * sum);
+}
+
+// d ln f / dr and d ln f / d shape of the broken exponential,
+// shape = (h1, h2, rb, a); the soft break switches to its asymptote when
+// a (r - rb) >= 40 as the profile itself does.
+template <class T>
+PA_HD T brokenexp_dlog(T r, T h1, T h2, T rb, T a, T *ds)
+{
+	const T z = a * (r - rb);
+	const T expo = T(1) / h1 - T(1) / h2;
+	T B, sig, dBda;
+	if (z < T(40)) {
+		const T ez = ::exp(z);
+		B = ::log(T(1) + ez) / a;
+		sig = ez / (T(1) + ez);
+		dBda = (r - rb) * sig / a - B / a;
+	}
+	else {
+		B = r - rb;
+		sig = T(1);
+		dBda = T(0);
+	}
+	ds[0] = (r - B) / (h1 * h1);
+	ds[1] = B / (h2 * h2);
+	ds[2] = -expo * sig;
+	ds[3] = expo * dBda;
+	return -T(1) / h1 + expo * sig;
+}
+
 template <class T>
 PA_HD T eval_at(const Comp &c, T x, T y)
 {
@@ -380,6 +428,36 @@ PA_HD void sample_grad(const Comp &c, T x, T y, T *o)
 		for (int w = 0; w < 3; ++w) o[4 + w] = king_dcore<T>(r, rc, rt, a, w);
 		return;
 	}
+	case CORESERSIC: {
+		const T r = boxy_r<T>(c, x, y);
+		const T f = coresersic_core<T>(c, r);
+		o[0] = f;
+		if (!(r > T(0))) return;
+		T ds[5];
+		const T dfdr = f * coresersic_dlog<T>(c, r, ds);
+		T drx, dry, drB;
+		dr_terms<T>(c, x, y, r, &drx, &dry, &drB);
+		o[1] = dfdr * drx;
+		o[2] = dfdr * dry;
+		o[3] = dfdr * drB;
+		for (int w = 0; w < 5; ++w) o[4 + w] = f * ds[w];
+		return;
+	}
+	case BROKENEXP: {
+		const T r = boxy_r<T>(c, x, y);
+		const T h1 = T(c.s[0]), h2 = T(c.s[1]), rb = T(c.s[2]), a = T(c.s[3]);
+		const T f = brokenexp_core<T>(r, h1, h2, rb, a);
+		o[0] = f;
+		T ds[4];
+		const T dfdr = f * brokenexp_dlog<T>(r, h1, h2, rb, a, ds);
+		T drx, dry, drB;
+		dr_terms<T>(c, x, y, r, &drx, &dry, &drB);
+		o[1] = dfdr * drx;
+		o[2] = dfdr * dry;
+		o[3] = dfdr * drB;
+		for (int w = 0; w < 4; ++w) o[4 + w] = f * ds[w];
+		return;
+	}
 	default: return;
 	}
 }
@@ -399,12 +477,29 @@ struct KingIntegrand {
 
 struct CoreSersicIntegrand {
 	const Comp *c;
-	PA_HD double operator()(double r) const { return r * coresersic_core<double>(*c, r); }
+	int which = -1;  // -1: r f(r); 0..4: r df/dshape
+	PA_HD double operator()(double r) const
+	{
+		const double f = coresersic_core<double>(*c, r);
+		if (which < 0) return r * f;
+		if (!(r > 0)) return 0;
+		double ds[5];
+		coresersic_dlog<double>(*c, r, ds);
+		return r * f * ds[which];
+	}
 };
 
 struct BrokenExpIntegrand {
 	double h1, h2, rb, a;
-	PA_HD double operator()(double r) const { return r * brokenexp_core<double>(r, h1, h2, rb, a); }
+	int which = -1;  // -1: r f(r); 0..3: r df/dshape
+	PA_HD double operator()(double r) const
+	{
+		const double f = brokenexp_core<double>(r, h1, h2, rb, a);
+		if (which < 0) return r * f;
+		double ds[4];
+		brokenexp_dlog<double>(r, h1, h2, rb, a, ds);
+		return r * f * ds[which];
+	}
 };
 
 PA_HD bool almost_equal(double x, double y) { return ::fabs(x - y) < 1e-10; }
@@ -428,7 +523,7 @@ PA_HD int setup_component(const ProfileParams &p, double magzero, double sx, dou
 	c->axrat = p.axrat;
 	c->img_sx = img_sx;
 	c->img_sy = img_sy;
-	c->grad_ok = (p.type == SERSIC && !p.rescale_flux) || p.type == MOFFAT || p.type == FERRER || p.type == KING;
+	c->grad_ok = !(p.type == SERSIC && p.rescale_flux);
 	for (int i = 0; i < 5; ++i) c->s[i] = s[i];
 	for (int i = 0; i < NGRAD; ++i) c->dlnfs[i] = 0;
 	c->bn = 0;
@@ -596,6 +691,25 @@ PA_HD int setup_component(const ProfileParams &p, double magzero, double sx, dou
 		for (int w = 0; w < 3; ++w) {
 			KingIntegrand f{s[0], s[1], s[2], w};
 			const double d = 2 * pi * sf::integrate(f, 0.0, s[1], false, *qs);
+			c->dlnfs[G_S0 + w] = -d / lumtot;
+		}
+		break;
+	}
+	case CORESERSIC: {
+		const double n = s[2], bn = c->bn, a = 2 * n;
+		const double dens = ::exp((a - 1) * ::log(bn) - bn - sf::lgam(a));
+		c->dbn = -2 * sf::dpgamma_da(bn, a) / dens;
+		for (int w = 0; w < 5; ++w) {
+			CoreSersicIntegrand f{c, w};
+			const double d = 2 * pi * sf::integrate(f, 0.0, 0.0, true, *qs);
+			c->dlnfs[G_S0 + w] = -d / lumtot;
+		}
+		break;
+	}
+	case BROKENEXP: {
+		for (int w = 0; w < 4; ++w) {
+			BrokenExpIntegrand f{s[0], s[1], s[2], s[3], w};
+			const double d = 2 * pi * sf::integrate(f, 0.0, 0.0, true, *qs);
 			c->dlnfs[G_S0 + w] = -d / lumtot;
 		}
 		break;

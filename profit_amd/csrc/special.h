@@ -25,6 +25,7 @@ namespace sf {
 
 #define PA_HD __host__ __device__ inline
 
+PA_HD bool is_inf(double x) { return ::fabs(x) == HUGE_VAL; }
 PA_HD double lgam(double x) { return ::lgamma(x); }
 PA_HD double gam(double x) { return ::tgamma(x); }
 
@@ -91,7 +92,7 @@ PA_HD double q_cfrac(double a, double x)
 PA_HD double pgamma(double x, double a)
 {
 	if (!(x > 0)) return 0;
-	if (::isinf(x)) return 1;
+	if (is_inf(x)) return 1;
 	return x < a + 1 ? p_series(a, x) : 1 - q_cfrac(a, x);
 }
 
@@ -135,7 +136,7 @@ PA_HD double qgamma(double p, double a)
 		double h = 1 - 0.5 * newton * curv;
 		if (!(h > 0.5)) h = 0.5;
 		double xn = x - newton / h;
-		if (!(xn > lo && xn < hi)) xn = ::isinf(hi) ? 2 * x : 0.5 * (lo + hi);
+		if (!(xn > lo && xn < hi)) xn = is_inf(hi) ? 2 * x : 0.5 * (lo + hi);
 		const double dx = ::fabs(xn - x);
 		x = xn;
 		if (dx <= 2e-16 * x) break;

@@ -105,3 +105,68 @@ def config2(model_fn=None, magzero=30.0, seed=2, gain=1.0, read_noise=5.0):
 
 
 GRAD_FREE = ("xcen", "ycen", "mag", "re", "nser", "ang", "axrat")
+
+
+def uniforms(seed: int, n: int) -> np.ndarray:
+    """n uniform deviates in (0, 1) from SplitMix64."""
+    return ((splitmix64(seed, n) >> np.uint64(11)).astype(np.float64) + 0.5) / 9007199254740992.0
+
+
+CONFIG4_TYPES = ("sersic", "moffat", "ferrer", "king", "coresersic")
+
+
+def _config4_component(kind, u, w, h):
+    """One component from 8 uniforms: centre within +-2 px of the image centre,
+    mag U(14, 20), ang U(0, 180), axrat U(0.2, 1), re U(1, 30), nser
+    log-uniform in (0.5, 8).  Non-Sersic profiles take their scale radius from
+    re and their shape from nser."""
+    re = 1.0 + 29.0 * u[4]
+    nser = float(np.exp(np.log(0.5) + (np.log(8.0) - np.log(0.5)) * u[5]))
+    c = dict(type=kind, xcen=w / 2 + 4 * u[0] - 2, ycen=h / 2 + 4 * u[1] - 2, mag=14.0 + 6.0 * u[2],
+             ang=180.0 * u[3], axrat=0.2 + 0.8 * u[6], box=0.0)
+    if kind == "sersic":
+        c.update(re=re, nser=nser)
+    elif kind == "moffat":
+        c.update(fwhm=re, con=1.5 + 0.5 * nser)
+    elif kind == "ferrer":
+        c.update(rout=2.0 * re, a=0.5 + nser / 4, b=0.5)
+    elif kind == "king":
+        c.update(rc=re / 4, rt=4.0 * re, a=2.0)
+    else:
+        c.update(re=re, rb=re / 4, nser=nser, a=2.0, b=0.5)
+    return c
+
+
+def config4_galaxies(n=4096, seed=4, types=CONFIG4_TYPES):
+    """Config 4 geometry and parameters: n galaxies with sides uniform in
+    32..256 px (width and height independent), 1-3 components of the given
+    types, each galaxy with its own Gaussian or Moffat PSF of odd side 15..41
+    px (FWHM U(2, 5), Moffat con U(2, 5)) and an elliptical likelihood region
+    with semi-axes 0.48 of the sides.  No data: see config4_attach_data."""
+    out = []
+    for i in range(n):
+        u = uniforms(seed * 1000003 + i, 64)
+        w = 32 + int(u[0] * 225)
+        h = 32 + int(u[1] * 225)
+        ncomp = 1 + int(u[2] * 3)
+        k = 15 + 2 * int(u[3] * 14)
+        fwhm = 2.0 + 3.0 * u[4]
+        psf = gaussian_psf(k, fwhm) if u[5] < 0.5 else moffat_psf(k, fwhm, 2.0 + 3.0 * u[6])
+        comps = []
+        for j in range(ncomp):
+            v = u[8 + 8 * j: 16 + 8 * j]
+            kind = types[int(v[7] * len(types)) % len(types)]
+            comps.append(_config4_component(kind, v, w, h))
+        region = ellipse_mask(w, h, w / 2, h / 2, 0.48 * w, 0.48 * h)
+        out.append(dict(width=w, height=h, psf=psf, components=comps, region=region, magzero=30.0,
+                        likefunc="chisq"))
+    return out
+
+
+def config4_attach_data(gals, models, seed=40, read_noise=5.0):
+    """Adds sigma = sqrt(max(model, 0) + read_noise^2) and seeded noisy data."""
+    for i, (g, m) in enumerate(zip(gals, models)):
+        sigma = np.sqrt(np.maximum(m, 0) + read_noise ** 2)
+        g.update(model=m, sigma=sigma,
+                 data=m + normal_draws(seed * 1000003 + i, m.size).reshape(m.shape) * sigma)
+    return gals
