@@ -178,6 +178,16 @@ int profit_amd_engine_set_render_fp64(profit_amd_engine *engine, int32_t on);
  * current batch. */
 int profit_amd_engine_set_conv_method(profit_amd_engine *engine, int32_t convolver);
 
+/* Autotuner (no libprofit counterpart; libprofit's convolver choice is the
+ * caller's): times the FP32 brute force at 4, 8 and 16 rows per thread, the
+ * FFT and the split-BF16x3 kernels on the current batch with `params`, then
+ * gives each galaxy the kernel of least predicted cost and keeps that mix
+ * only if it beats the best single kernel.  kinds (nullable, one per galaxy):
+ * the chosen PROFIT_AMD_CONV_* value.  ms (nullable, 6 doubles): forward +
+ * adjoint convolution time per evaluation for brute x 3, fft, bf16x3, chosen. */
+int profit_amd_engine_autotune(profit_amd_engine *engine, const profit_amd_profile *params, int32_t reps,
+                               int32_t *kinds, double *ms);
+
 /* logL per instance.  params: one profit_amd_profile per component,
  * instance-major, in the batch's component order. */
 int profit_amd_engine_like(profit_amd_engine *engine, const profit_amd_profile *params, double *loglike);

@@ -299,6 +299,22 @@ int profit_amd_engine_set_conv_method(profit_amd_engine *engine, int32_t convolv
 	});
 }
 
+int profit_amd_engine_autotune(profit_amd_engine *engine, const profit_amd_profile *params, int32_t reps,
+                               int32_t *kinds, double *ms)
+{
+	return guarded([&] {
+		const profit_amd::Engine::TuneReport r =
+		    engine->eng.autotune(reinterpret_cast<const profit_amd::ProfileParams *>(params), reps);
+		if (kinds)
+			for (size_t g = 0; g < r.kinds.size(); ++g)
+				kinds[g] = r.kinds[g] == profit_amd::CK_FFT    ? PROFIT_AMD_CONV_FFT
+				           : r.kinds[g] == profit_amd::CK_MFMA ? PROFIT_AMD_CONV_BRUTE_BF16X3
+				                                               : PROFIT_AMD_CONV_BRUTE;
+		if (ms)
+			for (int k = 0; k < 6; ++k) ms[k] = r.ms[k];
+	});
+}
+
 int profit_amd_engine_like(profit_amd_engine *engine, const profit_amd_profile *params, double *loglike)
 {
 	return guarded([&] {

@@ -66,6 +66,20 @@ Engine::~Engine()
 	if (stream_) (void)hipStreamDestroy(stream_);
 }
 
+static int next_pow2(int v)
+{
+	int n = 1;
+	while (n < v) n <<= 1;
+	return n;
+}
+
+static int ilog2(int n)
+{
+	int l = 0;
+	while ((1 << l) < n) ++l;
+	return l;
+}
+
 void Engine::set_timing(bool on)
 {
 	timing_ = on;
@@ -122,25 +136,134 @@ void Engine::set_conv_method(int m)
 {
 	if (m < CK_VALU || m > CK_MFMA)
 		throw std::invalid_argument("convolution method must be 0 (brute FP32), 1 (fft) or 2 (brute split-BF16x3)");
-	if (m == conv_method_) return;
+	bool overridden = false;
+	for (int &k : kind_override_) {
+		overridden |= k >= 0;
+		k = -1;
+	}
+	if (m == conv_method_ && !overridden) return;
 	conv_method_ = m;
 	invalidate_graphs(graph_exec_);
 	if (!inst_galaxy_.empty()) set_batch(inst_galaxy_, inst_ncomp_);
 }
 
-static int next_pow2(int v)
+// forward + adjoint convolution time (ms) per evaluation of the current batch
+double Engine::time_conv(const ProfileParams *params, int reps)
 {
-	int n = 1;
-	while (n < v) n <<= 1;
-	return n;
+	const bool was = timing_;
+	std::vector<double> ll(ninst()), grad((size_t)ncomp_total_ * NGRAD);
+	evaluate(params, ll.data(), grad.data(), false);  // warm: attributes, queue size
+	set_timing(true);
+	reset_stats();
+	for (int r = 0; r < reps; ++r) evaluate(params, ll.data(), grad.data(), false);
+	const double ms = (stats_.ms[ST_CONV] + stats_.ms[ST_ADJ_CONV]) / std::max(1, reps);
+	timing_ = was;
+	reset_stats();
+	return ms;
 }
 
-static int ilog2(int n)
+// Autotuner: picks each galaxy's convolution kernel (and the brute-force
+// rows per thread) for the current batch.  Every candidate runs on the whole
+// batch; its measured forward + adjoint convolution time divided by the
+// batch's work in that kernel's own unit (brute force: tile pixels x PSF
+// taps; matrix cores: 128 x 128 tiles x PSF rows x K chunks; FFT: frame
+// points x log2 of the frame) calibrates a cost per unit on this device.
+// Each galaxy then takes the kernel of least predicted cost; the mixed
+// configuration is timed and kept only when it beats the best uniform one.
+Engine::TuneReport Engine::autotune(const ProfileParams *params, int reps)
 {
-	int l = 0;
-	while ((1 << l) < n) ++l;
-	return l;
+	if (ninst() == 0) throw std::invalid_argument("autotune needs a batch");
+	reps = std::max(1, reps);
+	TuneReport rep;
+	const int rows_cand[3] = {4, 8, 16};
+	std::vector<double> w_valu[3], w_fft, w_mfma;
+	const int ng = (int)gal_host_.size();
+	auto per_galaxy_work = [&](int kind, std::vector<double> &w) {
+		w.assign(ng, 0.0);
+		for (int i = 0; i < ninst(); ++i) {
+			const GalaxyDev &g = gal_host_[inst_galaxy_[i]];
+			if (!g.has_psf) continue;
+			double v = 0;
+			if (kind == CK_VALU) v = (double)(g.n_ctiles + g.n_atiles) * 256.0 * conv_rows_ * g.Kx * g.Ky;
+			else if (kind == CK_MFMA) v = (double)(g.n_mtl + g.n_mtc) * g.Ky * (g.mfma_kc + 2);
+			else {
+				const double n1 = next_pow2(g.DW + g.Kx), n2 = next_pow2(g.DH + g.Ky);
+				v = n1 * n2 * std::log2(n1 * n2);
+			}
+			w[inst_galaxy_[i]] += v;
+		}
+	};
+	double best_uniform = 1e300;
+	int best_kind = CK_VALU, best_rows = conv_rows_;
+	double a_valu = 0;
+	for (int k = 0; k < 3; ++k) {
+		set_conv_method(CK_VALU);
+		set_conv_rows(rows_cand[k]);
+		rep.ms[k] = time_conv(params, reps);
+		per_galaxy_work(CK_VALU, w_valu[k]);
+		if (rep.ms[k] < best_uniform) {
+			best_uniform = rep.ms[k];
+			best_kind = CK_VALU;
+			best_rows = rows_cand[k];
+		}
+	}
+	// brute force at the best rows per thread
+	{
+		int kbest = 0;
+		for (int k = 1; k < 3; ++k)
+			if (rep.ms[k] < rep.ms[kbest]) kbest = k;
+		rep.rows = rows_cand[kbest];
+		double tot = 0;
+		for (double v : w_valu[kbest]) tot += v;
+		a_valu = tot > 0 ? rep.ms[kbest] / tot : 0;
+		w_valu[0] = w_valu[kbest];
+	}
+	set_conv_rows(rep.rows);
+	set_conv_method(CK_FFT);
+	rep.ms[3] = time_conv(params, reps);
+	per_galaxy_work(CK_FFT, w_fft);
+	if (rep.ms[3] < best_uniform) {
+		best_uniform = rep.ms[3];
+		best_kind = CK_FFT;
+	}
+	set_conv_method(CK_MFMA);
+	rep.ms[4] = time_conv(params, reps);
+	per_galaxy_work(CK_MFMA, w_mfma);
+	if (rep.ms[4] < best_uniform) {
+		best_uniform = rep.ms[4];
+		best_kind = CK_MFMA;
+	}
+	auto total = [](const std::vector<double> &w) {
+		double t = 0;
+		for (double v : w) t += v;
+		return t;
+	};
+	const double a_fft = total(w_fft) > 0 ? rep.ms[3] / total(w_fft) : 0;
+	const double a_mfma = total(w_mfma) > 0 ? rep.ms[4] / total(w_mfma) : 0;
+	rep.kinds.assign(ng, CK_VALU);
+	for (int g = 0; g < ng; ++g) {
+		if (!gal_host_[g].has_psf) continue;
+		const double c[3] = {a_valu * w_valu[0][g], a_fft * w_fft[g], a_mfma * w_mfma[g]};
+		int k = 0;
+		for (int j = 1; j < 3; ++j)
+			if (c[j] < c[k]) k = j;
+		rep.kinds[g] = k;
+	}
+	// the mixed configuration
+	conv_method_ = best_kind;
+	kind_override_ = rep.kinds;
+	set_batch(inst_galaxy_, inst_ncomp_);
+	rep.ms[5] = time_conv(params, reps);
+	if (rep.ms[5] > best_uniform) {
+		set_conv_rows(best_kind == CK_VALU ? best_rows : rep.rows);
+		set_conv_method(best_kind);
+		for (int g = 0; g < ng; ++g)
+			if (gal_host_[g].has_psf) rep.kinds[g] = best_kind;
+		rep.ms[5] = best_uniform;
+	}
+	return rep;
 }
+
 
 // PSF spectra for the FFT path: g(d) = psf[K-1-K/2+d] about the origin (the
 // brute-force kernel's weights as a function of the offset), wrapped into an
@@ -506,8 +629,10 @@ void Engine::set_batch(const std::vector<int> &inst_galaxy, const std::vector<in
 	hip_check(hipSetDevice(device_), "hipSetDevice");
 	invalidate_graphs(graph_exec_);
 	bool any_fft = false;
-	for (auto &g : gal_host_) {
-		const int k = g.has_psf ? conv_method_ : CK_VALU;
+	kind_override_.resize(gal_host_.size(), -1);
+	for (size_t gi = 0; gi < gal_host_.size(); ++gi) {
+		GalaxyDev &g = gal_host_[gi];
+		const int k = !g.has_psf ? CK_VALU : kind_override_[gi] >= 0 ? kind_override_[gi] : conv_method_;
 		if (k != g.conv_kind) {
 			g.conv_kind = k;
 			gals_dirty_ = true;

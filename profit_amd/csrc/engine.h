@@ -154,6 +154,16 @@ class Engine {
 	// FP64 column sums), 1 FFT (FP64, power-of-two frames up to 4096)
 	void set_conv_method(int m);
 	int conv_method() const { return conv_method_; }
+	// Per-galaxy convolution kernels picked on the current batch (see
+	// autotune in engine.cpp).  ms: the forward + adjoint convolution time per
+	// evaluation of FP32 brute force at 4, 8 and 16 rows per thread, FFT,
+	// split-BF16x3 and the chosen configuration.
+	struct TuneReport {
+		double ms[6] = {0};
+		int rows = 8;
+		std::vector<int> kinds;  // per galaxy (CK_*)
+	};
+	TuneReport autotune(const ProfileParams *params, int reps);
 	// a single instance's canvas supplied directly (no profiles): the model-mode
 	// convolution of `canvas` (DW*DH floats) into out (DW*DH floats)
 	void convolve_canvas(const float *canvas, float *out);
@@ -209,6 +219,8 @@ class Engine {
 	hipStream_t stream_ = nullptr;
 	int conv_rows_ = 8;
 	int conv_method_ = 0;
+	std::vector<int> kind_override_;  // per galaxy: -1 follows conv_method_
+	double time_conv(const ProfileParams *params, int reps);
 	DevBuf d_tw_;
 	int twn_ = 0;
 	int fft_nmax1_ = 0, fft_nmax2_ = 0;

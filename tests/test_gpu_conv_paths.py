@@ -129,9 +129,16 @@ def test_agrees_with_brute_on_a_batch(method):
 def test_psf_less_galaxy_likelihood_and_gradient():
     """A galaxy without a PSF (unit kernel) next to one with a PSF in one batch."""
     wl = W.config1(oracle_model_fn(), magzero=30.0)
+
+    def model_np(w, h, psf, comps, magzero):
+        spec = O.ModelSpec(w, h, magzero=magzero, threads=8)
+        return O.model_evaluate(spec, oracle_profiles(comps, convolve=False))["image"]
+
+    wl_np = W.config1(model_np, magzero=30.0)  # data drawn from the unconvolved model
     comps = [dict(c) for c in wl["components"]]
     comps[0].update(xcen=50.5, mag=15.05, re=8.2, nser=2.45, axrat=0.62)
-    fit_np = O.FitSpec(O.ModelSpec(100, 100, magzero=30.0, threads=8), wl["data"], wl["sigma"], wl["region"])
+    fit_np = O.FitSpec(O.ModelSpec(100, 100, magzero=30.0, threads=8), wl_np["data"], wl_np["sigma"],
+                       wl_np["region"])
     oll_np, ograd_np = O.loglike_grad(fit_np, oracle_profiles(comps, convolve=False))
     fit_p = O.FitSpec(O.ModelSpec(100, 100, psf=wl["psf"], magzero=30.0, threads=8), wl["data"], wl["sigma"],
                       wl["region"])
@@ -139,7 +146,7 @@ def test_psf_less_galaxy_likelihood_and_gradient():
     for method in ["brute"] + METHODS:
         eng = P.Engine()
         eng.set_conv_method(method)
-        g0 = eng.add_galaxy(100, 100, wl["data"], wl["sigma"], magzero=30.0)
+        g0 = eng.add_galaxy(100, 100, wl_np["data"], wl_np["sigma"], magzero=30.0)
         g1 = eng.add_galaxy(100, 100, wl["data"], wl["sigma"], psf=wl["psf"], magzero=30.0)
         eng.set_batch([g0, g1], [1, 1])
         ll, grad = eng.like_grad(gpu_profiles(comps) + gpu_profiles(comps))
