@@ -115,8 +115,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile-rows", type=int, default=8)
     ap.add_argument("--render-fp32", action="store_true", help="direct pixels in FP32 instead of FP64")
-    ap.add_argument("--conv", choices=("brute", "fft", "brute-bf16x3"), default="brute",
-                    help="PSF convolution method")
+    ap.add_argument("--conv", choices=("auto", "brute", "fft", "brute-bf16x3"), default="auto",
+                    help="PSF convolution method (auto: the autotuner's choice among brute and fft)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU figures are not produced",
@@ -137,10 +137,15 @@ def main():
                        likefunc="chisq", magzero=30.0)
     eng.set_tile_rows(args.tile_rows)
     eng.set_render_fp64(not args.render_fp32)
-    eng.set_conv_method(args.conv)
+    eng.set_conv_method("brute" if args.conv == "auto" else args.conv)
     eng.set_batch([g] * B, [2] * B)
     sets = walkers(wl["components"], B)
     params = gpu_params(P, sets)
+    tune = None
+    if args.conv == "auto":
+        tune = eng.autotune(params, reps=2)
+        print(f"[bench] autotune: {tune['kinds'][0]} ({tune['ms']})", file=sys.stderr, flush=True)
+    conv_used = tune["kinds"][0] if tune else args.conv
     print(f"[bench] batch {B}: warmup {args.warmup}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         eng.like_grad(params)
@@ -171,7 +176,7 @@ def main():
     g1 = eng1.add_galaxy(400, 400, wl["data"], wl["sigma"], psf=wl["psf"], calcregion=calc, region=wl["region"],
                          likefunc="chisq", magzero=30.0)
     eng1.set_render_fp64(not args.render_fp32)
-    eng1.set_conv_method(args.conv)
+    eng1.set_conv_method(conv_used)
     eng1.set_batch([g1], [2])
     fd_gradient_gpu(eng1, P, sets[0])  # warm-up (graph capture)
     tf0 = time.perf_counter()
@@ -215,7 +220,8 @@ def main():
             "drawing_canvas": "450x450",
             "tile_rows": args.tile_rows,
             "direct_pixels": "fp32" if args.render_fp32 else "fp64",
-            "convolution": args.conv,
+            "convolution": conv_used,
+            "autotune_ms": tune["ms"] if tune else None,
         },
         "adjoint_vs_fd": {
             "adjoint_graph_replay_value": round(graph_value, 3),

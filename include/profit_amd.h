@@ -161,12 +161,13 @@ int profit_amd_engine_set_batch(profit_amd_engine *engine, const int32_t *inst_g
 /* Tile height of the convolution / rendering tiles: 4*rows (rows = 2, 4, 8, 16). */
 int profit_amd_engine_set_tile_rows(profit_amd_engine *engine, int32_t rows);
 
-/* Precision of directly evaluated pixels in the forward model: 1 = FP64
- * (default), 0 = FP32.  Either way: classification and sub-pixel sample values
+/* Precision of the forward model's profile values: 1 = FP64 (default), 0 =
+ * FP32, for the directly evaluated pixels and the leaf samples of the
+ * sub-pixel recursion alike.  Either way: classification and cell positions
  * are FP64, refinement decisions are FP32 with an FP64 re-test inside a guard
  * band (so they equal the FP64 decisions), the convolution is FP32 with FP64
  * accumulation across PSF columns, likelihood sums are FP64 and the adjoint's
- * per-sample derivatives are FP32. */
+ * per-sample derivatives are FP32 (FP64 for steep cores). */
 int profit_amd_engine_set_render_fp64(profit_amd_engine *engine, int32_t on);
 
 /* Convolution method of the engine's galaxies with a PSF, the choice that
@@ -180,13 +181,16 @@ int profit_amd_engine_set_conv_method(profit_amd_engine *engine, int32_t convolv
 
 /* Autotuner (no libprofit counterpart; libprofit's convolver choice is the
  * caller's): times the FP32 brute force at 4, 8 and 16 rows per thread, the
- * FFT and the split-BF16x3 kernels on the current batch with `params`, then
- * gives each galaxy the kernel of least predicted cost and keeps that mix
- * only if it beats the best single kernel.  kinds (nullable, one per galaxy):
- * the chosen PROFIT_AMD_CONV_* value.  ms (nullable, 6 doubles): forward +
- * adjoint convolution time per evaluation for brute x 3, fft, bf16x3, chosen. */
+ * FFT and (with PROFIT_AMD_TUNE_ALLOW_BF16X3 in flags) the split-BF16x3
+ * kernels on the current batch with `params`, then gives each galaxy the
+ * kernel of least predicted cost and keeps that mix only if it beats the best
+ * single kernel.  kinds (nullable, one per galaxy): the chosen
+ * PROFIT_AMD_CONV_* value.  ms (nullable, 6 doubles): forward + adjoint
+ * convolution time per evaluation for brute x 3, fft, bf16x3 (0 when not
+ * tried), chosen. */
+enum { PROFIT_AMD_TUNE_ALLOW_BF16X3 = 1 };
 int profit_amd_engine_autotune(profit_amd_engine *engine, const profit_amd_profile *params, int32_t reps,
-                               int32_t *kinds, double *ms);
+                               int32_t flags, int32_t *kinds, double *ms);
 
 /* logL per instance.  params: one profit_amd_profile per component,
  * instance-major, in the batch's component order. */

@@ -100,7 +100,7 @@ SYMBOLS = [
     ("profit_amd_engine_set_tile_rows", C.c_int, [C.c_void_p, C.c_int32]),
     ("profit_amd_engine_set_render_fp64", C.c_int, [C.c_void_p, C.c_int32]),
     ("profit_amd_engine_set_conv_method", C.c_int, [C.c_void_p, C.c_int32]),
-    ("profit_amd_engine_autotune", C.c_int, [C.c_void_p, _P(Profile), C.c_int32, _P(C.c_int32),
+    ("profit_amd_engine_autotune", C.c_int, [C.c_void_p, _P(Profile), C.c_int32, C.c_int32, _P(C.c_int32),
                                              _P(C.c_double)]),
     ("profit_amd_engine_like", C.c_int, [C.c_void_p, _P(Profile), _P(C.c_double)]),
     ("profit_amd_engine_like_grad", C.c_int, [C.c_void_p, _P(Profile), _P(C.c_double), _P(C.c_double)]),
@@ -358,14 +358,15 @@ class Engine:
         cores, split-BF16x3) for the galaxies with a PSF."""
         _check(lib().profit_amd_engine_set_conv_method(self._h, CONVOLVERS[method]))
 
-    def autotune(self, params, reps=2):
+    def autotune(self, params, reps=2, allow_bf16x3=False):
         """Per-galaxy convolution kernels for the current batch; returns the
-        chosen convolver name per galaxy and the timings (ms per evaluation)."""
+        chosen convolver name per galaxy and the timings (ms per evaluation).
+        The split-BF16x3 kernel is a candidate only with allow_bf16x3."""
         arr = self._params(params)
         ngal = len(self.galaxies)
         kinds = (C.c_int32 * max(1, ngal))()
         ms = (C.c_double * 6)()
-        _check(lib().profit_amd_engine_autotune(self._h, arr, int(reps), kinds, ms))
+        _check(lib().profit_amd_engine_autotune(self._h, arr, int(reps), int(bool(allow_bf16x3)), kinds, ms))
         names = {v: k for k, v in CONVOLVERS.items()}
         return dict(kinds=[names[kinds[i]] for i in range(ngal)],
                     ms=dict(zip(("brute_rows4", "brute_rows8", "brute_rows16", "fft", "brute_bf16x3", "chosen"),

@@ -300,11 +300,11 @@ int profit_amd_engine_set_conv_method(profit_amd_engine *engine, int32_t convolv
 }
 
 int profit_amd_engine_autotune(profit_amd_engine *engine, const profit_amd_profile *params, int32_t reps,
-                               int32_t *kinds, double *ms)
+                               int32_t flags, int32_t *kinds, double *ms)
 {
 	return guarded([&] {
-		const profit_amd::Engine::TuneReport r =
-		    engine->eng.autotune(reinterpret_cast<const profit_amd::ProfileParams *>(params), reps);
+		const profit_amd::Engine::TuneReport r = engine->eng.autotune(
+		    reinterpret_cast<const profit_amd::ProfileParams *>(params), reps, (flags & PROFIT_AMD_TUNE_ALLOW_BF16X3) != 0);
 		if (kinds)
 			for (size_t g = 0; g < r.kinds.size(); ++g)
 				kinds[g] = r.kinds[g] == profit_amd::CK_FFT    ? PROFIT_AMD_CONV_FFT
@@ -328,13 +328,9 @@ int profit_amd_engine_like_grad(profit_amd_engine *engine, const profit_amd_prof
 	return guarded([&] {
 		if (!grad) throw std::invalid_argument("null gradient buffer");
 		const auto *p = reinterpret_cast<const profit_amd::ProfileParams *>(params);
-		for (int c = 0; c < engine->eng.ncomp(); ++c) {
-			const int t = p[c].type;
-			if (!((t == profit_amd::SERSIC && !p[c].rescale_flux) || t == profit_amd::MOFFAT || t == profit_amd::FERRER ||
-			      t == profit_amd::KING))
-				throw profit::invalid_parameter(
-				    "the gradient supports sersic (without rescale_flux), moffat, ferrer and king profiles");
-		}
+		for (int c = 0; c < engine->eng.ncomp(); ++c)
+			if (p[c].type == profit_amd::SERSIC && p[c].rescale_flux)
+				throw profit::invalid_parameter("the gradient does not support sersic profiles with rescale_flux");
 		engine->eng.evaluate(p, loglike, grad);
 	});
 }

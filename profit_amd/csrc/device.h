@@ -36,7 +36,7 @@ struct GalaxyDev {
 	int32_t has_calc;
 	int32_t n_rtiles, n_ctiles, n_atiles;
 	int32_t conv_rows;     // rows per thread of the convolution tiles
-	int32_t render_fp64;   // direct pixels (and their adjoint) in FP64 instead of FP32
+	int32_t render_fp64;   // profile values (direct pixels, recursion leaves) in FP64 instead of FP32
 	double sx, sy;         // profile pixel scale
 	double img_sx, img_sy; // image pixel scale
 	double magzero;
@@ -76,5 +76,18 @@ struct QEntry {
 	int32_t comp;
 	int32_t pix;
 };
+
+// Sub-pixel work lists: the ordinary entries and the deep centre-pixel entries
+// (resolution 8, depth 10).  count[0], count[1]: their lengths; count[2..7]:
+// hand-out counters (heavy, light) of the forward walker, the adjoint walker
+// and the node-count walker.
+struct WorkQueue {
+	QEntry *light, *heavy;
+	int *count;
+	int lcap, hcap;
+};
+constexpr int kQueueCounters = 8;
+constexpr int kGrab = 8;  // ordinary entries handed to a wavefront at a time
+
 
 }  // namespace profit_amd

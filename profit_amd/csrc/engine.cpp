@@ -170,7 +170,7 @@ double Engine::time_conv(const ProfileParams *params, int reps)
 // points x log2 of the frame) calibrates a cost per unit on this device.
 // Each galaxy then takes the kernel of least predicted cost; the mixed
 // configuration is timed and kept only when it beats the best uniform one.
-Engine::TuneReport Engine::autotune(const ProfileParams *params, int reps)
+Engine::TuneReport Engine::autotune(const ProfileParams *params, int reps, bool allow_bf16x3)
 {
 	if (ninst() == 0) throw std::invalid_argument("autotune needs a batch");
 	reps = std::max(1, reps);
@@ -226,12 +226,15 @@ Engine::TuneReport Engine::autotune(const ProfileParams *params, int reps)
 		best_uniform = rep.ms[3];
 		best_kind = CK_FFT;
 	}
-	set_conv_method(CK_MFMA);
-	rep.ms[4] = time_conv(params, reps);
-	per_galaxy_work(CK_MFMA, w_mfma);
-	if (rep.ms[4] < best_uniform) {
-		best_uniform = rep.ms[4];
-		best_kind = CK_MFMA;
+	w_mfma.assign(ng, 0.0);
+	if (allow_bf16x3) {
+		set_conv_method(CK_MFMA);
+		rep.ms[4] = time_conv(params, reps);
+		per_galaxy_work(CK_MFMA, w_mfma);
+		if (rep.ms[4] < best_uniform) {
+			best_uniform = rep.ms[4];
+			best_kind = CK_MFMA;
+		}
 	}
 	auto total = [](const std::vector<double> &w) {
 		double t = 0;
@@ -245,7 +248,7 @@ Engine::TuneReport Engine::autotune(const ProfileParams *params, int reps)
 		if (!gal_host_[g].has_psf) continue;
 		const double c[3] = {a_valu * w_valu[0][g], a_fft * w_fft[g], a_mfma * w_mfma[g]};
 		int k = 0;
-		for (int j = 1; j < 3; ++j)
+		for (int j = 1; j < (allow_bf16x3 ? 3 : 2); ++j)
 			if (c[j] < c[k]) k = j;
 		rep.kinds[g] = k;
 	}
@@ -615,12 +618,37 @@ void Engine::upload_galaxies()
 	gals_dirty_ = false;
 }
 
-void Engine::ensure_queue(long need)
+void Engine::ensure_queue(long need, long hneed)
 {
-	if (need <= qcap_) return;
-	qcap_ = need;
-	d_queue_.alloc((size_t)qcap_ * sizeof(QEntry));
+	if (need <= qcap_ && hneed <= hcap_) return;
+	if (need > qcap_) {
+		qcap_ = need;
+		d_queue_.alloc((size_t)qcap_ * sizeof(QEntry));
+	}
+	if (hneed > hcap_) {
+		hcap_ = hneed;
+		d_hqueue_.alloc((size_t)hcap_ * sizeof(QEntry));
+	}
 	invalidate_graphs(graph_exec_);
+}
+
+WorkQueue Engine::work_queue()
+{
+	return WorkQueue{d_queue_.as<QEntry>(), d_hqueue_.as<QEntry>(), d_qcount_.as<int>(), (int)qcap_, (int)hcap_};
+}
+
+// whether the last evaluation's work lists fitted
+bool Engine::queue_ok() const
+{
+	const int *q = h_qcount_.as<int>();
+	return q[0] <= qcap_ && q[1] <= hcap_;
+}
+
+// grows the work lists to what the last evaluation needed (drops the graphs)
+void Engine::grow_queue()
+{
+	const int *q = h_qcount_.as<int>();
+	ensure_queue(std::max<long>(qcap_, (long)q[0] + q[0] / 4 + 1024), std::max<long>(hcap_, (long)q[1] + 64));
 }
 
 void Engine::set_batch(const std::vector<int> &inst_galaxy, const std::vector<int> &inst_ncomp)
@@ -649,7 +677,7 @@ void Engine::set_batch(const std::vector<int> &inst_galaxy, const std::vector<in
 	at_off_.assign(n + 1, 0);
 	ncomp_total_ = 0;
 	kx_max_ = 0;
-	long qneed = 1024;
+	long qneed = 1024, hneed = 64;
 	inst_bufs_.clear();
 	std::vector<InstDev> insts(n);
 	std::vector<int> comp_inst;
@@ -675,6 +703,9 @@ void Engine::set_batch(const std::vector<int> &inst_galaxy, const std::vector<in
 		inst_bufs_.push_back(std::move(aux));
 		inst_bufs_.push_back(std::move(fbuf));
 		qneed += (long)inst_ncomp[i] * (long)std::min<size_t>(ndraw, 4096);
+		// centre pixels: within one image pixel of the centre, (2 fs)^2 at most
+		const long fs = std::lround(g.img_sx / g.sx);
+		hneed += (long)inst_ncomp[i] * 4 * std::max(1L, fs * fs);
 	}
 	n_ct_blocks_ = ct_off_[n];
 	n_at_blocks_ = at_off_[n];
@@ -753,14 +784,15 @@ void Engine::set_batch(const std::vector<int> &inst_galaxy, const std::vector<in
 	d_gscale_.alloc(std::max(1, n) * sizeof(double));
 	d_grad_acc_.alloc((size_t)nc * NGRAD * sizeof(double));
 	d_grad_out_.alloc((size_t)nc * NGRAD * sizeof(double));
-	d_qcount_.alloc(sizeof(int));
+	d_qcount_.alloc(kQueueCounters * sizeof(int));
 	h_params_.alloc(nc * sizeof(ProfileParams));
 	h_ll_.alloc(std::max(1, n) * sizeof(double));
 	h_grad_.alloc((size_t)nc * NGRAD * sizeof(double));
 	h_err_.alloc(nc * sizeof(int));
-	h_qcount_.alloc(sizeof(int));
+	h_qcount_.alloc(2 * sizeof(int));
 	qcap_ = 0;
-	ensure_queue(qneed);
+	hcap_ = 0;
+	ensure_queue(qneed, hneed);
 	int ncu = 256;
 	hipDeviceProp_t prop;
 	if (hipGetDeviceProperties(&prop, device_) == hipSuccess) ncu = prop.multiProcessorCount;
@@ -781,15 +813,14 @@ void Engine::record(bool grad)
 	hip_check(launch_setup(d_params_.as<ProfileParams>(), d_comp_inst_.as<int>(), insts, gals, d_comps_.as<Comp>(),
 	                       d_err_.as<int>(), nc, st),
 	          "setup");
-	hip_check(launch_zero_i(d_qcount_.as<int>(), 1, st), "zero");
+	hip_check(launch_zero_i(d_qcount_.as<int>(), kQueueCounters, st), "zero");
 	if (grad) hip_check(launch_zero_d(d_grad_acc_.as<double>(), (long)nc * NGRAD, st), "zero");
 	mark(1);
 	hip_check(launch_render(gals, insts, d_at_off_.as<int>(), n, n_at_blocks_, d_comps_.as<Comp>(),
-	                        d_queue_.as<QEntry>(), d_qcount_.as<int>(), (int)qcap_, -1, 0, nullptr, st),
+	                        work_queue(), -1, 0, nullptr, st),
 	          "render");
 	mark(2);
-	hip_check(launch_subsample(false, subsample_blocks_, gals, insts, d_comps_.as<Comp>(), d_queue_.as<QEntry>(),
-	                           d_qcount_.as<int>(), (int)qcap_, 0, nullptr, st),
+	hip_check(launch_subsample(false, subsample_blocks_, gals, insts, d_comps_.as<Comp>(), work_queue(), 0, nullptr, st),
 	          "subsample");
 	mark(3);
 	conv_forward(0, st);
@@ -805,8 +836,7 @@ void Engine::record(bool grad)
 		                            d_grad_acc_.as<double>(), st),
 		          "adjoint render");
 		mark(7);
-		hip_check(launch_subsample(true, subsample_blocks_, gals, insts, d_comps_.as<Comp>(), d_queue_.as<QEntry>(),
-		                           d_qcount_.as<int>(), (int)qcap_, 0, d_grad_acc_.as<double>(), st),
+		hip_check(launch_subsample(true, subsample_blocks_, gals, insts, d_comps_.as<Comp>(), work_queue(), 0, d_grad_acc_.as<double>(), st),
 		          "adjoint subsample");
 		mark(8);
 		hip_check(launch_grad_finalize(d_comps_.as<Comp>(), d_gscale_.as<double>(), d_grad_acc_.as<double>(),
@@ -820,7 +850,7 @@ void Engine::record(bool grad)
 	hip_check(hipMemcpyAsync(h_ll_.as<double>(), d_ll_.as<double>(), n * sizeof(double), hipMemcpyDeviceToHost, st),
 	          "ll copy");
 	hip_check(hipMemcpyAsync(h_err_.as<int>(), d_err_.as<int>(), nc * sizeof(int), hipMemcpyDeviceToHost, st), "err");
-	hip_check(hipMemcpyAsync(h_qcount_.as<int>(), d_qcount_.as<int>(), sizeof(int), hipMemcpyDeviceToHost, st), "qcount");
+	hip_check(hipMemcpyAsync(h_qcount_.as<int>(), d_qcount_.as<int>(), 2 * sizeof(int), hipMemcpyDeviceToHost, st), "qcount");
 }
 
 void Engine::check_setup_errors()
@@ -845,7 +875,7 @@ void Engine::run(bool grad, bool use_graph)
 				// first call runs eagerly (sets kernel attributes), later calls replay a graph
 				record(grad);
 				hip_check(hipStreamSynchronize(stream_), "sync");
-				if (*h_qcount_.as<int>() <= qcap_) {
+				if (queue_ok()) {
 					hipGraph_t graph;
 					hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "capture");
 					record(grad);
@@ -863,9 +893,8 @@ void Engine::run(bool grad, bool use_graph)
 			record(grad);
 			hip_check(hipStreamSynchronize(stream_), "sync");
 		}
-		const int qc = *h_qcount_.as<int>();
-		stats_.queue_count = qc;
-		if (qc <= qcap_) {
+		stats_.queue_count = h_qcount_.as<int>()[0] + h_qcount_.as<int>()[1];
+		if (queue_ok()) {
 			if (timing_) {
 				const int last = grad ? ST_COUNT : ST_ADJ_CONV;
 				for (int k = 0; k < last; ++k) {
@@ -878,7 +907,7 @@ void Engine::run(bool grad, bool use_graph)
 			stats_.evaluations += 1;
 			return;
 		}
-		ensure_queue((long)qc + qc / 4 + 1024);  // drops the graphs; run again
+		grow_queue();  // drops the graphs; run again
 	}
 	throw std::runtime_error("sub-pixel work queue kept overflowing");
 }
@@ -912,21 +941,19 @@ void Engine::diagnostics(const ProfileParams *params, int *cls, int *nodes)
 		hip_check(launch_setup(d_params_.as<ProfileParams>(), d_comp_inst_.as<int>(), insts, gals, d_comps_.as<Comp>(),
 		                       d_err_.as<int>(), nc, st),
 		          "setup");
-		hip_check(launch_zero_i(d_qcount_.as<int>(), 1, st), "zero");
+		hip_check(launch_zero_i(d_qcount_.as<int>(), kQueueCounters, st), "zero");
 		hip_check(launch_render(gals, insts, d_at_off_.as<int>(), 1, n_at_blocks_, d_comps_.as<Comp>(),
-		                        d_queue_.as<QEntry>(), d_qcount_.as<int>(), (int)qcap_, -1, 0, dcls.as<int>(), st),
+		                        work_queue(), -1, 0, dcls.as<int>(), st),
 		          "render");
-		hip_check(launch_subsample_nodes(subsample_blocks_, gals, insts, d_comps_.as<Comp>(), d_queue_.as<QEntry>(),
-		                                 d_qcount_.as<int>(), (int)qcap_, dnodes.as<int>(), st),
+		hip_check(launch_subsample_nodes(subsample_blocks_, gals, insts, d_comps_.as<Comp>(), work_queue(), dnodes.as<int>(), st),
 		          "nodes");
 		hip_check(hipMemcpyAsync(h_err_.as<int>(), d_err_.as<int>(), nc * sizeof(int), hipMemcpyDeviceToHost, st), "err");
-		hip_check(hipMemcpyAsync(h_qcount_.as<int>(), d_qcount_.as<int>(), sizeof(int), hipMemcpyDeviceToHost, st),
+		hip_check(hipMemcpyAsync(h_qcount_.as<int>(), d_qcount_.as<int>(), 2 * sizeof(int), hipMemcpyDeviceToHost, st),
 		          "qcount");
 		hip_check(hipStreamSynchronize(st), "sync");
 		check_setup_errors();
-		const int qc = *h_qcount_.as<int>();
-		if (qc <= qcap_) break;
-		ensure_queue((long)qc + 1024);
+		if (queue_ok()) break;
+		grow_queue();
 	}
 	hip_check(hipMemcpy(cls, dcls.as<int>(), plane * nc * sizeof(int), hipMemcpyDeviceToHost), "cls");
 	hip_check(hipMemcpy(nodes, dnodes.as<int>(), plane * nc * sizeof(int), hipMemcpyDeviceToHost), "nodes");
@@ -947,22 +974,20 @@ void Engine::model_images(const ProfileParams *params, float *out)
 		hip_check(launch_setup(d_params_.as<ProfileParams>(), d_comp_inst_.as<int>(), insts, gals, d_comps_.as<Comp>(),
 		                       d_err_.as<int>(), nc, st),
 		          "setup");
-		hip_check(launch_zero_i(d_qcount_.as<int>(), 1, st), "zero");
+		hip_check(launch_zero_i(d_qcount_.as<int>(), kQueueCounters, st), "zero");
 		hip_check(launch_render(gals, insts, d_at_off_.as<int>(), n, n_at_blocks_, d_comps_.as<Comp>(),
-		                        d_queue_.as<QEntry>(), d_qcount_.as<int>(), (int)qcap_, -1, 0, nullptr, st),
+		                        work_queue(), -1, 0, nullptr, st),
 		          "render");
-		hip_check(launch_subsample(false, subsample_blocks_, gals, insts, d_comps_.as<Comp>(), d_queue_.as<QEntry>(),
-		                           d_qcount_.as<int>(), (int)qcap_, 0, nullptr, st),
+		hip_check(launch_subsample(false, subsample_blocks_, gals, insts, d_comps_.as<Comp>(), work_queue(), 0, nullptr, st),
 		          "subsample");
 		conv_forward(1, st);
 		hip_check(hipMemcpyAsync(h_err_.as<int>(), d_err_.as<int>(), nc * sizeof(int), hipMemcpyDeviceToHost, st), "err");
-		hip_check(hipMemcpyAsync(h_qcount_.as<int>(), d_qcount_.as<int>(), sizeof(int), hipMemcpyDeviceToHost, st),
+		hip_check(hipMemcpyAsync(h_qcount_.as<int>(), d_qcount_.as<int>(), 2 * sizeof(int), hipMemcpyDeviceToHost, st),
 		          "qcount");
 		hip_check(hipStreamSynchronize(st), "sync");
 		check_setup_errors();
-		const int qc = *h_qcount_.as<int>();
-		if (qc <= qcap_) break;
-		ensure_queue((long)qc + 1024);
+		if (queue_ok()) break;
+		grow_queue();
 	}
 	size_t off = 0;
 	for (int i = 0; i < n; ++i) {
@@ -994,27 +1019,25 @@ void Engine::model_canvas(const ProfileParams *params, float *out)
 		hip_check(launch_setup(d_params_.as<ProfileParams>(), d_comp_inst_.as<int>(), insts, gals, d_comps_.as<Comp>(),
 		                       d_err_.as<int>(), nc, st),
 		          "setup");
-		hip_check(launch_zero_i(d_qcount_.as<int>(), 1, st), "zero");
+		hip_check(launch_zero_i(d_qcount_.as<int>(), kQueueCounters, st), "zero");
 		// convolved profiles into the canvas, the others into the aux canvas
 		hip_check(launch_render(gals, insts, d_at_off_.as<int>(), 1, n_at_blocks_, d_comps_.as<Comp>(),
-		                        d_queue_.as<QEntry>(), d_qcount_.as<int>(), (int)qcap_, conv ? 1 : -1, 0, nullptr, st),
+		                        work_queue(), conv ? 1 : -1, 0, nullptr, st),
 		          "render");
 		if (conv)
 			hip_check(launch_render(gals, insts, d_at_off_.as<int>(), 1, n_at_blocks_, d_comps_.as<Comp>(),
-			                        d_queue_.as<QEntry>(), d_qcount_.as<int>(), (int)qcap_, 0, 1, nullptr, st),
+			                        work_queue(), 0, 1, nullptr, st),
 			          "render");
-		hip_check(launch_subsample(false, subsample_blocks_, gals, insts, d_comps_.as<Comp>(), d_queue_.as<QEntry>(),
-		                           d_qcount_.as<int>(), (int)qcap_, conv ? 1 : 0, nullptr, st),
+		hip_check(launch_subsample(false, subsample_blocks_, gals, insts, d_comps_.as<Comp>(), work_queue(), conv ? 1 : 0, nullptr, st),
 		          "subsample");
 		if (conv) conv_forward(1, st);
 		hip_check(hipMemcpyAsync(h_err_.as<int>(), d_err_.as<int>(), nc * sizeof(int), hipMemcpyDeviceToHost, st), "err");
-		hip_check(hipMemcpyAsync(h_qcount_.as<int>(), d_qcount_.as<int>(), sizeof(int), hipMemcpyDeviceToHost, st),
+		hip_check(hipMemcpyAsync(h_qcount_.as<int>(), d_qcount_.as<int>(), 2 * sizeof(int), hipMemcpyDeviceToHost, st),
 		          "qcount");
 		hip_check(hipStreamSynchronize(st), "sync");
 		check_setup_errors();
-		const int qc = *h_qcount_.as<int>();
-		if (qc <= qcap_) break;
-		ensure_queue((long)qc + 1024);
+		if (queue_ok()) break;
+		grow_queue();
 	}
 	const size_t nd = (size_t)g.DW * g.DH;
 	if (conv) {

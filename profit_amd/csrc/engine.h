@@ -19,12 +19,11 @@ int launch_setup(const ProfileParams *params, const int *comp_inst, const InstDe
 int launch_zero_d(double *p, long n, hipStream_t st);
 int launch_zero_i(int *p, long n, hipStream_t st);
 int launch_render(const GalaxyDev *gals, const InstDev *insts, const int *tile_off, int ninst, int nblocks,
-                  const Comp *comps, QEntry *queue, int *qcount, int qcap, int filter, int target, int *diag_cls,
-                  hipStream_t st);
+                  const Comp *comps, WorkQueue wq, int filter, int target, int *diag_cls, hipStream_t st);
 int launch_subsample(bool grad, int nblocks, const GalaxyDev *gals, const InstDev *insts, const Comp *comps,
-                     const QEntry *queue, const int *qcount, int qcap, int split, double *grad_acc, hipStream_t st);
+                     WorkQueue wq, int split, double *grad_acc, hipStream_t st);
 int launch_subsample_nodes(int nblocks, const GalaxyDev *gals, const InstDev *insts, const Comp *comps,
-                           const QEntry *queue, const int *qcount, int qcap, int *nodes, hipStream_t st);
+                           WorkQueue wq, int *nodes, hipStream_t st);
 int launch_conv(int mode, int rows, int nblocks, int kx_max, int kc, const GalaxyDev *gals, const InstDev *insts,
                 const int *tile_off, int ninst, const int *lk_base, double *partial, hipStream_t st);
 int launch_like_finalize(const GalaxyDev *gals, const InstDev *insts, const int *lk_off, int ninst,
@@ -163,7 +162,7 @@ class Engine {
 		int rows = 8;
 		std::vector<int> kinds;  // per galaxy (CK_*)
 	};
-	TuneReport autotune(const ProfileParams *params, int reps);
+	TuneReport autotune(const ProfileParams *params, int reps, bool allow_bf16x3 = false);
 	// a single instance's canvas supplied directly (no profiles): the model-mode
 	// convolution of `canvas` (DW*DH floats) into out (DW*DH floats)
 	void convolve_canvas(const float *canvas, float *out);
@@ -212,7 +211,10 @@ class Engine {
 	void upload_galaxies();
 	void run(bool grad, bool use_graph);
 	void record(bool grad);
-	void ensure_queue(long need);
+	void ensure_queue(long need, long hneed);
+	WorkQueue work_queue();
+	bool queue_ok() const;
+	void grow_queue();
 	void check_setup_errors();
 
 	int device_ = 0;
@@ -252,7 +254,8 @@ class Engine {
 	DevBuf d_insts_, d_ct_off_, d_at_off_, d_comp_inst_;
 	std::vector<DevBuf> inst_bufs_;
 	DevBuf d_params_, d_comps_, d_err_, d_queue_, d_qcount_, d_partial_, d_ll_, d_gscale_, d_grad_acc_, d_grad_out_;
-	long qcap_ = 0;
+	long qcap_ = 0, hcap_ = 0;
+	DevBuf d_hqueue_;
 	HostBuf h_params_, h_ll_, h_grad_, h_err_, h_qcount_;
 
 	hipGraphExec_t graph_exec_[2] = {nullptr, nullptr};

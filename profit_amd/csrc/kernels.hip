@@ -95,13 +95,35 @@ __device__ inline PixelClass classify(const Comp &c, double x, double y)
 
 // ------------------------------------------------------------------ forward render
 
+// sersic.cpp:331-347: pixels within one image pixel of the centre of an
+// nser > 1 Sersic profile are sub-sampled at resolution 8, depth 10
+__device__ __forceinline__ bool centre_pixel(const Comp &c, double x, double y)
+{
+	return c.centre_rule && fabs(x - c.xc) < c.img_sx && fabs(y - c.yc) < c.img_sy;
+}
+
+// wave-aggregated append of the lanes with `take` to a work list
+__device__ __forceinline__ void append_entry(bool take, int lane, QEntry *list, int *count, int cap, QEntry e)
+{
+	const unsigned long long m = __ballot(take);
+	if (!m) return;
+	const int leader = __ffsll((long long)m) - 1;
+	int base = 0;
+	if (lane == leader) base = atomicAdd(count, __popcll(m));
+	base = __shfl(base, leader);
+	if (take) {
+		const int slot = base + __popcll(m & ((1ull << lane) - 1ull));
+		if (slot < cap) list[slot] = e;
+	}
+}
+
 // Tiles are 64 x (4*rows) pixels; thread (tx, ty) covers column tx and rows
 // ty*rows .. ty*rows+rows-1.  filter: -1 every component, 0/1 only those whose
 // convolve flag equals it.  target: 0 canvas, 1 the aux canvas.
 __global__ __launch_bounds__(256) void k_render(const GalaxyDev *__restrict__ gals, const InstDev *__restrict__ insts,
                                                 const int *__restrict__ tile_off, int ninst,
-                                                const Comp *__restrict__ comps, QEntry *queue, int *qcount,
-                                                int qcap, int filter, int target, int *diag_cls)
+                                                const Comp *__restrict__ comps, WorkQueue wq, int filter, int target,
+                                                int *diag_cls)
 {
 	const int b = blockIdx.x;
 	const int inst = owner_of(tile_off, ninst, b);
@@ -126,7 +148,7 @@ __global__ __launch_bounds__(256) void k_render(const GalaxyDev *__restrict__ ga
 		for (int k = 0; k < I.ncomp; ++k) {
 			const Comp &c = comps[I.comp0 + k];
 			if (filter >= 0 && c.convolve != filter) continue;
-			bool sub = false;
+			bool sub = false, heavy = false;
 			int cls = 0;
 			if (active) {
 				const PixelClass pc = classify(c, x, y);
@@ -135,20 +157,15 @@ __global__ __launch_bounds__(256) void k_render(const GalaxyDev *__restrict__ ga
 					if (g.render_fp64) acc += (float)(c.flux_scale * eval_at<double>(c, pc.xp, pc.yp));
 					else acc += (float)c.flux_scale * eval_at<float>(c, (float)pc.xp, (float)pc.yp);
 				}
-				else if (pc.cls == 3) sub = true;
-			}
-			if (diag_cls && inb) diag_cls[(long)(I.comp0 + k) * g.DW * g.DH + pix] = cls;
-			const unsigned long long m = __ballot(sub);
-			if (m) {
-				const int leader = __ffsll((long long)m) - 1;
-				int base = 0;
-				if (lane == leader) base = atomicAdd(qcount, __popcll(m));
-				base = __shfl(base, leader);
-				if (sub) {
-					const int slot = base + __popcll(m & ((1ull << lane) - 1ull));
-					if (slot < qcap) queue[slot] = QEntry{I.comp0 + k, pix};
+				else if (pc.cls == 3) {
+					// the deep centre-pixel trees (sersic.cpp:331-347) go to their own list
+					heavy = centre_pixel(c, x, y);
+					sub = !heavy;
 				}
 			}
+			if (diag_cls && inb) diag_cls[(long)(I.comp0 + k) * g.DW * g.DH + pix] = cls;
+			append_entry(sub, lane, wq.light, wq.count, wq.lcap, QEntry{I.comp0 + k, pix});
+			append_entry(heavy, lane, wq.heavy, wq.count + 1, wq.hcap, QEntry{I.comp0 + k, pix});
 		}
 		if (inb) out[pix] = acc;
 	}
@@ -161,7 +178,7 @@ struct WalkLevel {
 	unsigned long long mask[4];
 };
 
-__device__ inline double cell_centre(double lo, double half, int idx)
+__device__ __forceinline__ double cell_centre(double lo, double half, int idx)
 {
 	// the same sequence of additions as libprofit's loops (radial.cpp:86-106)
 	double v = lo;
@@ -172,24 +189,31 @@ __device__ inline double cell_centre(double lo, double half, int idx)
 	return v + half;
 }
 
-// d(f)/d slot of one sample in FP32, chain rule to the component slots,
-// accumulated with weight wgt (FP64) into gs.
-__device__ inline float sample_accumulate_f32(const Comp &c, double xp, double yp, double wgt, double *gs)
+// d(f)/d slot of one sample in precision T, chain rule to the component
+// slots, accumulated with weight wgt (FP64) into gs; returns f.
+template <class T>
+__device__ __forceinline__ double sample_accumulate(const Comp &c, double xp, double yp, double wgt, double *gs)
 {
-	float o[9];
-	const float fxp = (float)xp, fyp = (float)yp;
-	sample_grad<float>(c, fxp, fyp, o);
-	const float q = (float)c.axrat;
-	const float ca = (float)c.cos_ang, sa = (float)c.sin_ang;
-	const float fx = o[1], fy = o[2];
+	T o[9];
+	const T fxp = (T)xp, fyp = (T)yp;
+	sample_grad<T>(c, fxp, fyp, o);
+	const T q = (T)c.axrat;
+	const T ca = (T)c.cos_ang, sa = (T)c.sin_ang;
+	const T fx = o[1], fy = o[2];
 	gs[G_XCEN] += wgt * (double)(fx * -ca + fy * (sa / q));
 	gs[G_YCEN] += wgt * (double)(fx * -sa + fy * (-ca / q));
-	gs[G_ANG] += wgt * (double)((fx * (q * fyp) + fy * (-fxp / q)) * (3.14159265358979323846f / 180.f));
+	gs[G_ANG] += wgt * (double)((fx * (q * fyp) + fy * (-fxp / q)) * (T(3.14159265358979323846) / T(180)));
 	gs[G_AXRAT] += wgt * (double)(fy * (-fyp / q));
 	gs[G_BOX] += wgt * (double)o[3];
 #pragma unroll
 	for (int s = 0; s < 5; ++s) gs[G_S0 + s] += wgt * (double)o[4 + s];
-	return o[0];
+	return (double)o[0];
+}
+
+__device__ __forceinline__ double sample_accumulate_any(const Comp &c, double xp, double yp, double wgt, double *gs)
+{
+	if (c.grad64) return sample_accumulate<double>(c, xp, yp, wgt, gs);
+	return sample_accumulate<float>(c, xp, yp, wgt, gs);
 }
 
 // libprofit's refinement test |test - val| > acc * val (radial.cpp:92-100).
@@ -197,7 +221,7 @@ __device__ inline float sample_accumulate_f32(const Comp &c, double xp, double y
 // of 1e-3 * val around the threshold (far wider than the FP32 evaluation
 // error) is it re-evaluated in FP64, so the decision equals the FP64 one.
 // Returns the decision; *v64 receives the FP64 value when it was computed.
-__device__ inline bool refine_cell(const Comp &c, double xp, double yp, double dyp, bool recurse, float v32,
+__device__ __forceinline__ bool refine_cell(const Comp &c, double xp, double yp, double dyp, bool recurse, float v32,
                                    double *v64, bool *have64)
 {
 	*have64 = false;
@@ -217,8 +241,8 @@ __device__ inline bool refine_cell(const Comp &c, double xp, double yp, double d
 // derivative sums (FP32 per-sample derivatives); `scale` multiplies the
 // weights (G(q) * flux_scale in the adjoint, 1 otherwise).
 template <bool kGrad>
-__device__ void walk_pixel(const Comp &c, WalkLevel *L, int lane, double x0, double x1, double y0, double y1, int res,
-                           int maxrec, double scale, double &sum, double *gs)
+__device__ __forceinline__ void walk_pixel(const Comp &c, WalkLevel *L, int lane, double x0, double x1, double y0, double y1, int res,
+                           int maxrec, double scale, bool fp64, double &sum, double *gs)
 {
 	const int ncell = res * res;
 	const int nchunk = (ncell + 63) >> 6;
@@ -233,7 +257,9 @@ __device__ void walk_pixel(const Comp &c, WalkLevel *L, int lane, double x0, dou
 		const bool recurse = res > 1 && level < maxrec;
 		const double dyp = -xbin * c.sin_over + ybin * c.cos_over;
 		unsigned long long m[4] = {0ull, 0ull, 0ull, 0ull};
-		for (int k = 0; k < nchunk; ++k) {
+#pragma unroll
+		for (int k = 0; k < 4; ++k) {
+			if (k >= nchunk) break;
 			const int cell = k * 64 + lane;
 			bool fail = false;
 			if (cell < ncell) {
@@ -250,11 +276,12 @@ __device__ void walk_pixel(const Comp &c, WalkLevel *L, int lane, double x0, dou
 				if (!fail) {
 					if (kGrad) {
 						// value (for the flux-scale derivatives) and derivatives in FP32
-						const float f = sample_accumulate_f32(c, xp, yp, weight * scale, gs);
-						sum += (double)f * (weight * scale);
+						const double f = sample_accumulate_any(c, xp, yp, weight * scale, gs);
+						sum += f * (weight * scale);
 					}
 					else {
-						if (!have64) v64 = eval_at<double>(c, xp, yp);
+						// leaf values in FP64 (default) or the FP32 value already at hand
+						if (!have64) v64 = fp64 ? eval_at<double>(c, xp, yp) : (double)v32;
 						sum += v64 * (weight * scale);
 					}
 				}
@@ -316,8 +343,7 @@ __device__ void walk_pixel(const Comp &c, WalkLevel *L, int lane, double x0, dou
 template <bool kGrad>
 __global__ __launch_bounds__(256) void k_subsample(const GalaxyDev *__restrict__ gals,
                                                    const InstDev *__restrict__ insts,
-                                                   const Comp *__restrict__ comps, const QEntry *__restrict__ queue,
-                                                   const int *__restrict__ qcount, int qcap, int chunk,
+                                                   const Comp *__restrict__ comps, WorkQueue wq, int counter,
                                                    int split, double *grad_acc)
 {
 	__shared__ WalkLevel levels[4][kMaxLevels];
@@ -325,18 +351,42 @@ __global__ __launch_bounds__(256) void k_subsample(const GalaxyDev *__restrict__
 	// entries and component fields are read through the scalar cache
 	const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 	WalkLevel *L = levels[wv];
-	const int gw = blockIdx.x * 4 + wv, nw = gridDim.x * 4;
-	int count = *qcount;
-	if (count > qcap) count = qcap;
+	const int nl = min(wq.count[0], wq.lcap), nh = min(wq.count[1], wq.hcap);
 	double gs[NGRAD];
 #pragma unroll
 	for (int s = 0; s < NGRAD; ++s) gs[s] = 0;
 	double vsum = 0;
 	int cur = -1;
-	for (int base = gw * chunk; base < count; base += nw * chunk) {
-		const int end = min(base + chunk, count);
-		for (int e = base; e < end; ++e) {
-			const QEntry q = queue[e];
+	// dynamic hand-out: the deep centre trees one at a time first, then the
+	// other entries in grabs of kGrab, so no wave is left with a long tail
+	bool heavy_left = nh > 0;
+	for (;;) {
+		int base = 0, take = 0;
+		const QEntry *list;
+		{
+			int h = nh;
+			if (heavy_left) {
+				if (lane == 0) h = atomicAdd(&wq.count[counter], 1);
+				h = __shfl(h, 0);
+			}
+			if (h < nh) {
+				base = h;
+				take = 1;
+				list = wq.heavy;
+			}
+			else {
+				heavy_left = false;
+				int l = 0;
+				if (lane == 0) l = atomicAdd(&wq.count[counter + 1], kGrab);
+				l = __shfl(l, 0);
+				if (l >= nl) break;
+				base = l;
+				take = min(kGrab, nl - l);
+				list = wq.light;
+			}
+		}
+		for (int e = base; e < base + take; ++e) {
+			const QEntry q = list[e];
 			const Comp &c = comps[q.comp];
 			if (kGrad && q.comp != cur) {
 				if (cur >= 0) {
@@ -365,13 +415,13 @@ __global__ __launch_bounds__(256) void k_subsample(const GalaxyDev *__restrict__
 			const double y = hy + j * g.sx;
 			const double x = hx + i * g.sy;
 			int res = c.res, maxrec = c.maxrec;
-			// sersic.cpp:331-347: centre pixels of nser > 1 profiles
-			if (c.centre_rule && fabs(x - c.xc) < c.img_sx && fabs(y - c.yc) < c.img_sy) {
+			if (centre_pixel(c, x, y)) {
 				res = 8;
 				maxrec = 10;
 			}
 			double s_local = 0;
-			walk_pixel<kGrad>(c, L, lane, x - hx, x + hx, y - hy, y + hy, res, maxrec, scale, s_local, gs);
+			walk_pixel<kGrad>(c, L, lane, x - hx, x + hx, y - hy, y + hy, res, maxrec, scale, g.render_fp64 != 0, s_local,
+			                  gs);
 			if (kGrad) {
 				vsum += s_local;
 			}
@@ -398,8 +448,7 @@ __global__ __launch_bounds__(256) void k_subsample(const GalaxyDev *__restrict__
 __global__ __launch_bounds__(256) void k_subsample_nodes(const GalaxyDev *__restrict__ gals,
                                                          const InstDev *__restrict__ insts,
                                                          const Comp *__restrict__ comps,
-                                                         const QEntry *__restrict__ queue,
-                                                         const int *__restrict__ qcount, int qcap, int *nodes_out)
+                                                         WorkQueue wq, int *nodes_out)
 {
 	__shared__ WalkLevel levels[4][kMaxLevels];
 	// the wavefront index is uniform: make that visible to the compiler so queue
@@ -407,10 +456,9 @@ __global__ __launch_bounds__(256) void k_subsample_nodes(const GalaxyDev *__rest
 	const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 	WalkLevel *L = levels[wv];
 	const int gw = blockIdx.x * 4 + wv, nw = gridDim.x * 4;
-	int count = *qcount;
-	if (count > qcap) count = qcap;
-	for (int e = gw; e < count; e += nw) {
-		const QEntry q = queue[e];
+	const int nl = min(wq.count[0], wq.lcap), nh = min(wq.count[1], wq.hcap);
+	for (int e = gw; e < nl + nh; e += nw) {
+		const QEntry q = e < nh ? wq.heavy[e] : wq.light[e - nh];
 		const Comp &c = comps[q.comp];
 		const InstDev &I = insts[c.instance];
 		const GalaxyDev &g = gals[I.galaxy];
@@ -731,21 +779,8 @@ __global__ __launch_bounds__(256) void k_adj_render(const GalaxyDev *__restrict_
 			const PixelClass pc = classify(c, x, y);
 			if (pc.cls != 2) continue;
 			const double wq = (double)Gq * c.flux_scale;
-			// per-sample derivatives in FP32 (the forward pixels may be FP64)
-			float o[9];
-			const float xp = (float)pc.xp, yp = (float)pc.yp;
-			sample_grad<float>(c, xp, yp, o);
-			const float q = (float)c.axrat;
-			const float ca = (float)c.cos_ang, sa = (float)c.sin_ang;
-			const float fx = o[1], fy = o[2];
-			const double f = o[0];
-			gl[G_XCEN] += wq * (double)(fx * -ca + fy * (sa / q));
-			gl[G_YCEN] += wq * (double)(fx * -sa + fy * (-ca / q));
-			gl[G_ANG] += wq * (double)((fx * (q * yp) + fy * (-xp / q)) * (3.14159265358979323846f / 180.f));
-			gl[G_AXRAT] += wq * (double)(fy * (-yp / q));
-			gl[G_BOX] += wq * (double)o[3];
-#pragma unroll
-			for (int s = 0; s < 5; ++s) gl[G_S0 + s] += wq * (double)o[4 + s];
+			// per-sample derivatives in FP32 (FP64 for steep cores)
+			const double f = sample_accumulate_any(c, pc.xp, pc.yp, wq, gl);
 			// flux-scale dependence: d fs / d slot = fs * dlnfs
 #pragma unroll
 			for (int s = 0; s < NGRAD; ++s) gl[s] += wq * f * c.dlnfs[s];
@@ -831,31 +866,27 @@ int launch_zero_i(int *p, long n, hipStream_t st)
 }
 
 int launch_render(const GalaxyDev *gals, const InstDev *insts, const int *tile_off, int ninst, int nblocks,
-                  const Comp *comps, QEntry *queue, int *qcount, int qcap, int filter, int target, int *diag_cls,
-                  hipStream_t st)
+                  const Comp *comps, WorkQueue wq, int filter, int target, int *diag_cls, hipStream_t st)
 {
 	if (nblocks <= 0) return 0;
-	k_render<<<nblocks, 256, 0, st>>>(gals, insts, tile_off, ninst, comps, queue, qcount, qcap, filter, target,
-	                                  diag_cls);
+	k_render<<<nblocks, 256, 0, st>>>(gals, insts, tile_off, ninst, comps, wq, filter, target, diag_cls);
 	return (int)hipGetLastError();
 }
 
 int launch_subsample(bool grad, int nblocks, const GalaxyDev *gals, const InstDev *insts, const Comp *comps,
-                     const QEntry *queue, const int *qcount, int qcap, int split, double *grad_acc, hipStream_t st)
+                     WorkQueue wq, int split, double *grad_acc, hipStream_t st)
 {
 	if (nblocks <= 0) return 0;
-	if (grad)
-		k_subsample<true><<<nblocks, 256, 0, st>>>(gals, insts, comps, queue, qcount, qcap, 4, split, grad_acc);
-	else
-		k_subsample<false><<<nblocks, 256, 0, st>>>(gals, insts, comps, queue, qcount, qcap, 1, split, grad_acc);
+	if (grad) k_subsample<true><<<nblocks, 256, 0, st>>>(gals, insts, comps, wq, 4, split, grad_acc);
+	else k_subsample<false><<<nblocks, 256, 0, st>>>(gals, insts, comps, wq, 2, split, grad_acc);
 	return (int)hipGetLastError();
 }
 
 int launch_subsample_nodes(int nblocks, const GalaxyDev *gals, const InstDev *insts, const Comp *comps,
-                           const QEntry *queue, const int *qcount, int qcap, int *nodes, hipStream_t st)
+                           WorkQueue wq, int *nodes, hipStream_t st)
 {
 	if (nblocks <= 0) return 0;
-	k_subsample_nodes<<<nblocks, 256, 0, st>>>(gals, insts, comps, queue, qcount, qcap, nodes);
+	k_subsample_nodes<<<nblocks, 256, 0, st>>>(gals, insts, comps, wq, nodes);
 	return (int)hipGetLastError();
 }
 

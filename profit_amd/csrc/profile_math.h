@@ -22,6 +22,8 @@
 namespace profit_amd {
 
 #define PA_HD __host__ __device__ inline
+// hot-path profile math: always inlined into the render and walker kernels
+#define PA_HDI __host__ __device__ __forceinline__
 
 enum ProfileType : int32_t { SERSIC = 0, MOFFAT = 1, FERRER = 2, KING = 3, CORESERSIC = 4, BROKENEXP = 5 };
 
@@ -79,6 +81,8 @@ struct Comp {
 	int32_t type, kind, res, maxrec;
 	int32_t rough, centre_rule, boxy, has_rmax;
 	int32_t convolve, instance, grad_ok, adjust;
+	int32_t grad64;  // per-sample adjoint derivatives in FP64 (steep cores)
+	int32_t pad_;
 	double xc, yc;                    // centre after the PSF-padding shift
 	double cos_ang, sin_ang, cos_over, sin_over;
 	double rmax_r, rsw_r;             // rscale_max*rscale, rscale_switch*rscale
@@ -176,7 +180,7 @@ PA_HD int validate_params(const ProfileParams &p)
 // ------------------------------------------------------------------ evaluate_at
 
 template <class T>
-PA_HD T boxy_r(const Comp &c, T x, T y)
+PA_HDI T boxy_r(const Comp &c, T x, T y)
 {
 	if (!c.boxy) return ::sqrt(x * x + y * y);
 	const T b2 = T(c.box) + T(2);
@@ -184,7 +188,7 @@ PA_HD T boxy_r(const Comp &c, T x, T y)
 }
 
 template <class T>
-PA_HD T sersic_at(const Comp &c, T x, T y)
+PA_HDI T sersic_at(const Comp &c, T x, T y)
 {
 	const T re = T(c.s[0]), nser = T(c.s[1]), bn = T(c.bn);
 	T base, rf;
@@ -219,13 +223,13 @@ PA_HD T sersic_at(const Comp &c, T x, T y)
 }
 
 template <class T>
-PA_HD T king_core(T r, T rc, T rt, T a)
+PA_HDI T king_core(T r, T rc, T rt, T a)
 {
 	return ::pow(T(1) / ::pow(T(1) + ::pow(r / rc, T(2)), T(1) / a) - T(1) / ::pow(T(1) + ::pow(rt / rc, T(2)), T(1) / a), a);
 }
 
 template <class T>
-PA_HD T coresersic_core(const Comp &c, T r)
+PA_HDI T coresersic_core(const Comp &c, T r)
 {
 	const T re = T(c.s[0]), rb = T(c.s[1]), nser = T(c.s[2]), a = T(c.s[3]), b = T(c.s[4]);
 	return ::pow(T(1) + ::pow(r / rb, -a), b / a) *
@@ -233,7 +237,7 @@ PA_HD T coresersic_core(const Comp &c, T r)
 }
 
 template <class T>
-PA_HD T brokenexp_core(T r, T h1, T h2, T rb, T a)
+PA_HDI T brokenexp_core(T r, T h1, T h2, T rb, T a)
 {
 	T base = r - rb;
 	const T expo = T(1) / h1 - T(1) / h2;
@@ -245,7 +249,7 @@ PA_HD T brokenexp_core(T r, T h1, T h2, T rb, T a)
 // f = (1 + (r/rb)^-a)^(b/a) exp(-bn ((r^a + rb^a)/re^a)^(1/(n a))),
 // shape = (re, rb, nser, a, b); bn = qgamma(0.5, 2 nser), dbn its n-derivative.
 template <class T>
-PA_HD T coresersic_dlog(const Comp &c, T r, T *ds)
+PA_HDI T coresersic_dlog(const Comp &c, T r, T *ds)
 {
 	const T re = T(c.s[0]), rb = T(c.s[1]), n = T(c.s[2]), a = T(c.s[3]), b = T(c.s[4]), bn = T(c.bn);
 	const T t = ::pow(r / rb, -a), Q = T(1) + t;
@@ -266,7 +270,7 @@ PA_HD T coresersic_dlog(const Comp &c, T r, T *ds)
 // shape = (h1, h2, rb, a); the soft break switches to its asymptote when
 // a (r - rb) >= 40 as the profile itself does.
 template <class T>
-PA_HD T brokenexp_dlog(T r, T h1, T h2, T rb, T a, T *ds)
+PA_HDI T brokenexp_dlog(T r, T h1, T h2, T rb, T a, T *ds)
 {
 	const T z = a * (r - rb);
 	const T expo = T(1) / h1 - T(1) / h2;
@@ -290,7 +294,7 @@ PA_HD T brokenexp_dlog(T r, T h1, T h2, T rb, T a, T *ds)
 }
 
 template <class T>
-PA_HD T eval_at(const Comp &c, T x, T y)
+PA_HDI T eval_at(const Comp &c, T x, T y)
 {
 	switch (c.type) {
 	case SERSIC: return sersic_at<T>(c, x, y);
@@ -317,7 +321,7 @@ PA_HD T eval_at(const Comp &c, T x, T y)
 // ------------------------------------------------------------------ derivatives
 
 template <class T>
-PA_HD void dr_terms(const Comp &c, T x, T y, T r, T *drx, T *dry, T *drB)
+PA_HDI void dr_terms(const Comp &c, T x, T y, T r, T *drx, T *dry, T *drB)
 {
 	*drx = *dry = *drB = T(0);
 	if (!(r > T(0))) return;
@@ -339,7 +343,7 @@ PA_HD void dr_terms(const Comp &c, T x, T y, T r, T *drx, T *dry, T *drB)
 }
 
 template <class T>
-PA_HD T king_dcore(T r, T rc, T rt, T a, int which)
+PA_HDI T king_dcore(T r, T rc, T rt, T a, int which)
 {
 	const T u2 = (r / rc) * (r / rc), w2 = (rt / rc) * (rt / rc);
 	const T A = ::pow(T(1) + u2, -T(1) / a), C = ::pow(T(1) + w2, -T(1) / a), D = A - C;
@@ -355,7 +359,7 @@ PA_HD T king_dcore(T r, T rc, T rt, T a, int which)
 
 // o[0] = f, o[1] = df/dx_prof, o[2] = df/dy_prof, o[3] = df/dbox, o[4..8] = df/dshape
 template <class T>
-PA_HD void sample_grad(const Comp &c, T x, T y, T *o)
+PA_HDI void sample_grad(const Comp &c, T x, T y, T *o)
 {
 	for (int i = 0; i < 9; ++i) o[i] = T(0);
 	switch (c.type) {
@@ -524,6 +528,11 @@ PA_HD int setup_component(const ProfileParams &p, double magzero, double sx, dou
 	c->img_sx = img_sx;
 	c->img_sy = img_sy;
 	c->grad_ok = !(p.type == SERSIC && p.rescale_flux);
+	// steep cores: the centroid derivatives of a cusp (core-Sersic with an inner
+	// slope, Sersic n > 4) cancel over many large samples, so their per-sample
+	// derivatives are taken in FP64
+	c->grad64 = (p.type == CORESERSIC && s[4] != 0) || (p.type == SERSIC && s[1] > 4);
+	c->pad_ = 0;
 	for (int i = 0; i < 5; ++i) c->s[i] = s[i];
 	for (int i = 0; i < NGRAD; ++i) c->dlnfs[i] = 0;
 	c->bn = 0;
@@ -720,5 +729,6 @@ PA_HD int setup_component(const ProfileParams &p, double magzero, double sx, dou
 }
 
 #undef PA_HD
+#undef PA_HDI
 
 }  // namespace profit_amd

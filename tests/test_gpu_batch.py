@@ -3,7 +3,7 @@ component counts and profile types evaluated in one launch sequence, each
 instance checked against the FP64 oracle: model image (per-pixel relative
 error <= 1e-5 above 1e-6 of the peak), logL (<= 1e-3 absolute) and the
 adjoint gradient against the oracle's central differences on the frozen
-recursion tree (h = 1e-4 |theta|)."""
+recursion tree (h = 1e-6 |theta|, 1e-8 |theta| for the centroid)."""
 import numpy as np
 import pytest
 
@@ -49,7 +49,7 @@ def _walker(comps, k):
     for c in out:
         c["xcen"] += 0.13 * (k % 3 - 1)
         c["mag"] += 0.04
-        c["axrat"] = min(1.0, c["axrat"] * 1.02)
+        c["axrat"] = min(0.98, c["axrat"] * 1.02)
     return out
 
 
@@ -78,7 +78,12 @@ def test_heterogeneous_batch_gradients(batch):
     ll, grad = eng.like_grad(params)
     row = 0
     for i, (g, f, s) in enumerate(zip(gals, fits, sets)):
-        oll, ograd = O.loglike_grad_fd(f, oracle_profiles(s), rel_step=1e-4)
+        # cusps (core-Sersic inner slopes, Sersic n > 1) make the centroid
+        # derivative converge only for steps far below the deepest sub-pixel
+        # cells (n = 7.4 needs 1e-8 |theta|); 1e-6 |theta| elsewhere
+        steps = np.full((len(s), O.NGRAD), 1e-6)
+        steps[:, :2] = 1e-8
+        oll, ograd = O.loglike_grad_fd(f, oracle_profiles(s), rel_step=steps)
         assert abs(ll[i] - oll) <= 1e-3, i
         for k, c in enumerate(s):
             slots = [0, 1, 2, 3, 4] + [6 + j for j in range(SHAPES[c["type"]])]
@@ -89,3 +94,21 @@ def test_heterogeneous_batch_gradients(batch):
                   f"fd {np.array2string(o, precision=4)}", flush=True)
             assert np.all(np.abs(gv - o) <= 2e-3 * np.maximum(np.abs(o), 5e-2 * big)), (i, k)
         row += len(s)
+
+
+def test_autotuned_batch_keeps_parity(batch):
+    """The autotuner's per-galaxy kernel mix evaluates the same logL (FP32 brute
+    force and FFT are both inside the gates)."""
+    gals, fits = batch
+    eng = _engine(gals, "brute")
+    params = [p for g in gals for p in gpu_profiles(g["components"])]
+    rep = eng.autotune(params, reps=2)
+    print(f"[batch] autotune kinds {rep['kinds']} ms {rep['ms']}", flush=True)
+    assert len(rep["kinds"]) == len(gals)
+    assert set(rep["kinds"]) <= {"brute", "fft"}
+    assert rep["ms"]["chosen"] <= min(rep["ms"]["brute_rows4"], rep["ms"]["brute_rows8"], rep["ms"]["brute_rows16"],
+                                      rep["ms"]["fft"]) * 1.0001
+    ll = eng.like(params)
+    for i, (g, f) in enumerate(zip(gals, fits)):
+        oll = O.loglike(f, oracle_profiles(g["components"]))
+        assert abs(ll[i] - oll) <= 1e-3, i

@@ -1,0 +1,95 @@
+"""Config 3: the adjoint of config 1 (14 = 7 parameters here, one Sersic)
+driving 1000 HMC leapfrog steps, step 0.01 in unit-scaled space, from
+starts perturbed by +-10%; the gradient at the starts is checked against
+central differences of the GPU forward path (h = 1e-4 |theta|) and against the
+FP64 oracle's gradient."""
+import numpy as np
+import pytest
+
+import oracle as O
+import profit_amd as P
+from profit_amd import hmc as H
+from profit_amd import workloads as W
+from tests.helpers import oracle_model_fn, oracle_profiles
+
+pytestmark = pytest.mark.gpu
+
+FREE = ["xcen", "ycen", "mag", "re", "nser", "ang", "axrat"]
+
+
+def _setup(nchain, magzero=30.0):
+    """Unit-scaled space: theta = theta_true + width * u with the posterior
+    widths from the curvature of logL at the truth."""
+    wl = W.config1(oracle_model_fn(), magzero=magzero)
+    eng = P.Engine()
+    g = eng.add_galaxy(100, 100, wl["data"], wl["sigma"], psf=wl["psf"], region=wl["region"], magzero=magzero)
+    comps = wl["components"]
+    free = [(0, n) for n in FREE]
+    scale = H.unit_scales(eng, g, comps, free)
+    prob = H.Problem(eng, g, comps, free, scale, nchain)
+    return wl, prob
+
+
+def _starts(prob, nchain, seed=3):
+    """Starts with every parameter perturbed by up to +-10%."""
+    rng = np.random.default_rng(seed)
+    th = prob.theta0[None, :] * (1 + rng.uniform(-0.1, 0.1, size=(nchain, len(FREE))))
+    return (th - prob.theta0[None, :]) / prob.scale[None, :]
+
+
+def test_gradient_at_perturbed_starts_vs_fd_and_oracle():
+    """The adjoint differentiates the model on a fixed recursion tree: it must
+    match the FP64 oracle's fixed-tree gradient and the oracle's central
+    differences on the frozen tree tightly.  Central differences through the
+    plain GPU forward path also re-decide the tree, so far from the optimum
+    (large residuals) they differ from it by the jumps of those decisions: a
+    looser gate there."""
+    nchain = 4
+    wl, prob = _setup(nchain)
+    u = _starts(prob, nchain)
+    ll, g = prob.logp_grad(u)
+    fit = O.FitSpec(O.ModelSpec(100, 100, psf=wl["psf"], magzero=30.0, threads=8), wl["data"], wl["sigma"],
+                    wl["region"])
+    th = prob.theta(u)
+    for c in range(nchain):
+        comps = [dict(wl["components"][0])]
+        for j, n in enumerate(FREE):
+            comps[0][n] = th[c, j]
+        oll, og = O.loglike_grad(fit, oracle_profiles(comps))
+        # centroid steps far below the deepest centre-pixel cells (the Sersic
+        # n > 1 cusp), 1e-4 |theta| elsewhere
+        steps = np.full((1, O.NGRAD), 1e-4)
+        steps[:, :2] = 1e-7
+        _, ofd = O.loglike_grad_fd(fit, oracle_profiles(comps), rel_step=steps)
+        # far from the optimum |logL| ~ 1e6: the gate scales with it (FP32 canvas)
+        assert abs(ll[c] - oll) <= max(1e-3, 1e-7 * abs(oll))
+        big = np.abs(g[c]).max()
+        for j, n in enumerate(FREE):
+            s = H.grad_slot("sersic", n)
+            o, f = og[0, s] * prob.scale[j], ofd[0, s] * prob.scale[j]
+            print(f"[hmc] chain {c} {n}: adjoint {g[c, j]:.6e} oracle {o:.6e} oracle-fd {f:.6e}", flush=True)
+            assert abs(g[c, j] - o) <= 1e-3 * max(abs(o), 1e-2 * big), (c, n)
+            assert abs(g[c, j] - f) <= 2e-3 * max(abs(f), 1e-2 * big), (c, n)
+    for j in range(len(FREE)):
+        h = 1e-4 * np.abs(th[:, j]) / prob.scale[j]
+        up, um = u.copy(), u.copy()
+        up[:, j] += h
+        um[:, j] -= h
+        fd = (prob.logp(up) - prob.logp(um)) / (2 * h)
+        for c in range(nchain):
+            print(f"[hmc] chain {c} {FREE[j]}: adjoint {g[c, j]:.6e} gpu-fd {fd[c]:.6e}", flush=True)
+            assert abs(g[c, j] - fd[c]) <= 2e-2 * max(abs(fd[c]), 5e-2 * np.abs(g[c]).max()), (c, j)
+
+
+def test_hmc_1000_leapfrog_steps():
+    nchain = 32
+    _, prob = _setup(nchain)
+    u0 = _starts(prob, nchain)
+    ll0 = prob.logp(u0)
+    out = H.hmc(prob, u0, n_leapfrog=1000, step=0.01, path=10, seed=5)
+    dH = out["dH"]
+    print(f"[hmc] acceptance {out['acceptance']:.3f}, median |dH| {np.median(np.abs(dH)):.3e}, "
+          f"logL start {np.median(ll0):.2f} end {np.median(out['ll']):.2f}", flush=True)
+    assert np.all(np.isfinite(out["ll"]))
+    assert out["acceptance"] > 0.05
+    assert np.median(out["ll"]) > np.median(ll0)
