@@ -301,6 +301,7 @@ void Engine::ensure_fft_spectra()
 		g.fft_n2 = n2;
 		g.fft_l1 = ilog2(n1);
 		g.fft_l2 = ilog2(n2);
+		const int m1 = n1 / 2 + 1;  // half-spectrum width
 		std::vector<double> frame((size_t)n1 * n2, 0.0);
 		const int ax = g.Kx - 1 - g.Kx / 2, ay = g.Ky - 1 - g.Ky / 2;
 		for (int ky = 0; ky < g.Ky; ++ky)
@@ -308,7 +309,7 @@ void Engine::ensure_fft_spectra()
 				const int dx = kx - ax, dy = ky - ay;
 				frame[(size_t)((dy + n2) % n2) * n1 + (dx + n1) % n1] = b.psf64[(size_t)ky * g.Kx + kx];
 			}
-		b.fft_spec.alloc(frame.size() * sizeof(double2));
+		b.fft_spec.alloc((size_t)m1 * n2 * sizeof(double2));
 		DevBuf src(frame.size() * sizeof(double));
 		hip_check(hipMemcpy(src.as<double>(), frame.data(), frame.size() * sizeof(double), hipMemcpyHostToDevice), "psf");
 		g.fft_spec = b.fft_spec.as<double2>();
@@ -318,14 +319,12 @@ void Engine::ensure_fft_spectra()
 		I.galaxy = 0;
 		hip_check(hipMemcpy(dg.as<GalaxyDev>(), &g, sizeof(g), hipMemcpyHostToDevice), "gal");
 		hip_check(hipMemcpy(di.as<InstDev>(), &I, sizeof(I), hipMemcpyHostToDevice), "inst");
-		int off[2] = {0, n2};
-		hip_check(hipMemcpy(doff.as<int>(), off, sizeof(off), hipMemcpyHostToDevice), "off");
-		hip_check(launch_fft_rows_fwd(n2, n1, dg.as<GalaxyDev>(), di.as<InstDev>(), doff.as<int>(), 1, 2,
-		                              d_tw_.as<double2>(), twn_, src.as<double>(), b.fft_spec.as<double2>(), stream_),
-		          "psf rows");
 		const int cb = fft_cols_per_block(n2);
-		off[1] = (n1 + cb - 1) / cb;
+		const int off[2] = {0, (m1 + cb - 1) / cb};
 		hip_check(hipMemcpy(doff.as<int>(), off, sizeof(off), hipMemcpyHostToDevice), "off");
+		hip_check(launch_fft_psf_rows(n1, n2, dg.as<GalaxyDev>(), d_tw_.as<double2>(), twn_, src.as<double>(),
+		                              b.fft_spec.as<double2>(), stream_),
+		          "psf rows");
 		hip_check(launch_fft_cols(off[1], n2, cb, dg.as<GalaxyDev>(), di.as<InstDev>(), doff.as<int>(), 1, 0, 2, 0,
 		                          d_tw_.as<double2>(), twn_, b.fft_spec.as<double2>(), stream_),
 		          "psf cols");
@@ -364,7 +363,7 @@ void Engine::conv_forward(int mode, hipStream_t st)
 	}
 	if (rf0_off_[n] == 0) return;
 	hip_check(launch_fft_rows_fwd(rf0_off_[n], fft_nmax1_, gals, insts, d_rf0_off_.as<int>(), n, 0, d_tw_.as<double2>(),
-	                              twn_, nullptr, nullptr, st),
+	                              twn_, st),
 	          "fft rows");
 	hip_check(launch_fft_cols(col_off_[n], fft_nmax2_, fft_cb_, gals, insts, d_col_off_.as<int>(), n, 1, 0,
 	                          mode == 0 ? 0 : 1, d_tw_.as<double2>(), twn_, nullptr, st),
@@ -393,7 +392,7 @@ void Engine::conv_adjoint(hipStream_t st)
 	          "adjoint conv mfma");
 	if (rf1_off_[n] == 0) return;
 	hip_check(launch_fft_rows_fwd(rf1_off_[n], fft_nmax1_, gals, insts, d_rf1_off_.as<int>(), n, 1, d_tw_.as<double2>(),
-	                              twn_, nullptr, nullptr, st),
+	                              twn_, st),
 	          "fft rows");
 	hip_check(launch_fft_cols(col_off_[n], fft_nmax2_, fft_cb_, gals, insts, d_col_off_.as<int>(), n, 2, 1, 1,
 	                          d_tw_.as<double2>(), twn_, nullptr, st),
@@ -694,7 +693,7 @@ void Engine::set_batch(const std::vector<int> &inst_galaxy, const std::vector<in
 		if (g.conv_kind == CK_VALU) kx_max_ = std::max(kx_max_, g.Kx);
 		const size_t ndraw = (size_t)g.DW * g.DH, npix = (size_t)g.W * g.H;
 		DevBuf canvas(ndraw * 4), w(npix * 4), G(ndraw * 4), aux(ndraw * 4), fbuf;
-		if (g.conv_kind == CK_FFT) fbuf.alloc((size_t)g.fft_n1 * g.fft_n2 * sizeof(double2));
+		if (g.conv_kind == CK_FFT) fbuf.alloc((size_t)(g.fft_n1 / 2 + 1) * g.fft_n2 * sizeof(double2));
 		insts[i] = InstDev{gi,          inst_comp0_[i], inst_ncomp[i],   0,
 		                   canvas.as<float>(), w.as<float>(), G.as<float>(), aux.as<float>(), fbuf.as<double2>()};
 		inst_bufs_.push_back(std::move(canvas));
@@ -744,12 +743,15 @@ void Engine::set_batch(const std::vector<int> &inst_galaxy, const std::vector<in
 		bt_off_[i + 1] = bt_off_[i] + (g.conv_kind == CK_VALU ? g.n_atiles : 0);
 		ml_off_[i + 1] = ml_off_[i] + (g.conv_kind == CK_MFMA ? g.n_mtl : 0);
 		mc_off_[i + 1] = mc_off_[i] + (g.conv_kind == CK_MFMA ? g.n_mtc : 0);
-		lk_off_[i + 1] = lk_off_[i] + (g.conv_kind == CK_VALU ? g.n_ctiles : g.conv_kind == CK_MFMA ? g.n_mtl : g.H);
-		rf0_off_[i + 1] = rf0_off_[i] + (f ? g.DH : 0);
-		rf1_off_[i + 1] = rf1_off_[i] + (f ? g.H : 0);
-		col_off_[i + 1] = col_off_[i] + (f ? (g.fft_n1 + fft_cb_ - 1) / fft_cb_ : 0);
-		ri0_off_[i + 1] = ri0_off_[i] + (f ? g.H : 0);
-		ri1_off_[i + 1] = ri1_off_[i] + (f ? g.DH : 0);
+		lk_off_[i + 1] = lk_off_[i] + (g.conv_kind == CK_VALU   ? g.n_ctiles
+		                               : g.conv_kind == CK_MFMA ? g.n_mtl
+		                                                        : (g.H + 1) / 2);
+		// row passes take two rows per block; columns span the half-spectrum
+		rf0_off_[i + 1] = rf0_off_[i] + (f ? (g.DH + 1) / 2 : 0);
+		rf1_off_[i + 1] = rf1_off_[i] + (f ? (g.H + 1) / 2 : 0);
+		col_off_[i + 1] = col_off_[i] + (f ? (g.fft_n1 / 2 + 1 + fft_cb_ - 1) / fft_cb_ : 0);
+		ri0_off_[i + 1] = ri0_off_[i] + (f ? (g.H + 1) / 2 : 0);
+		ri1_off_[i + 1] = ri1_off_[i] + (f ? (g.DH + 1) / 2 : 0);
 	}
 	n_bt_blocks_ = bt_off_[n];
 	auto upload_off = [&](DevBuf &d, const std::vector<int> &h) {

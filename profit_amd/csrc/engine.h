@@ -35,8 +35,9 @@ int launch_grad_finalize(const Comp *comps, const double *gscale, const double *
 int launch_dilate(const uint8_t *in, uint8_t *tmp, uint8_t *out, int w, int h, int padx, int pady, hipStream_t st);
 // FFT path (fft.hip)
 int launch_fft_rows_fwd(int nblocks, int nmax, const GalaxyDev *gals, const InstDev *insts, const int *row_off,
-                        int ninst, int mode, const double2 *tw, int twn, const double *src64, double2 *dst64,
-                        hipStream_t st);
+                        int ninst, int mode, const double2 *tw, int twn, hipStream_t st);
+int launch_fft_psf_rows(int n1, int n2, const GalaxyDev *gal, const double2 *tw, int twn, const double *src64,
+                        double2 *dst64, hipStream_t st);
 int launch_fft_cols(int nblocks, int n2max, int cb, const GalaxyDev *gals, const InstDev *insts, const int *col_off,
                     int ninst, int op, int in_mode, int out_mode, const double2 *tw, int twn, double2 *dst64,
                     hipStream_t st);

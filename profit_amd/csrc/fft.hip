@@ -2,17 +2,22 @@
 //
 // libprofit computes the 2-D linear convolution through a 1-D real FFT of the
 // flattened, zero-extended image (FFTW).  Here the same linear convolution is
-// computed with 2-D power-of-two complex FFTs in FP64 on the GPU, in three
-// passes over an N1 x N2 frame per instance:
-//   k_fft_rows_fwd   row FFTs of the zero-padded real input (canvas or w)
-//   k_fft_cols_conv  column FFT, product with the PSF spectrum (its conjugate
-//                    for the adjoint correlation), inverse column FFT
-//   k_fft_rows_inv   inverse row FFTs with the epilogue of the brute-force
-//                    path: fused likelihood, masked model image, or the
-//                    adjoint image G
-// Every transform runs in LDS (in-place radix-2 after a bit-reversed load).
-// FP64 keeps the error relative to the image maximum near 1e-15, so faint
-// pixels keep their relative accuracy.
+// computed with 2-D power-of-two FFTs of a real frame in FP64 on the GPU, in
+// three passes over an N1 x N2 frame per instance that keeps only the
+// half-spectrum (N1/2 + 1 columns, the rest follows from Hermitian symmetry):
+//   k_fft_rows_fwd  two real rows packed as one complex FFT, split into
+//                   their half-spectra
+//   k_fft_cols      column FFT of the half-spectrum, product with the PSF
+//                   spectrum (its conjugate for the adjoint correlation),
+//                   inverse column FFT
+//   k_fft_rows_inv  two half-spectrum rows packed as one complex inverse FFT,
+//                   with the epilogue of the brute-force path: fused
+//                   likelihood, masked model image, or the adjoint image G
+// Every transform runs in LDS: radix-2 decimation in time from bit-reversed
+// input for the forward transforms, decimation in frequency to bit-reversed
+// output for the inverse ones, twiddles staged in LDS.  FP64 keeps the error
+// relative to the image maximum near 1e-15, so faint pixels keep their
+// relative accuracy.
 #include <hip/hip_runtime.h>
 
 #include "device.h"
@@ -34,32 +39,59 @@ __device__ inline int owner(const int *__restrict__ off, int n, int b)
 
 __device__ inline unsigned bitrev(unsigned i, int logn) { return __brev(i) >> (32 - logn); }
 
+__device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 __device__ inline double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ inline double2 conjg(double2 a) { return make_double2(a.x, -a.y); }
 
 __device__ inline double2 cmulc(double2 a, double2 b)  // a * conj(b)
 {
 	return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
 }
 
-// In-place iterative radix-2 transforms of `ncol` sequences of length n in
-// LDS (sequence c at data + c*stride), input in bit-reversed order.
-// tw[t] = exp(-2 pi i t / twn) for t < twn/2.  inverse: conjugate twiddles.
-__device__ inline void lds_fft(double2 *data, int n, int logn, int ncol, int stride, const double2 *__restrict__ tw,
-                               int twn, bool inverse)
+// twiddles of an n-point transform in LDS: w[t] = exp(-2 pi i t / n), t < n/2,
+// taken from the global table of the largest transform (twn points)
+__device__ inline void stage_twiddles(double2 *w, int n, const double2 *__restrict__ tw, int twn)
+{
+	const int step = twn / n;
+	for (int t = threadIdx.x; t < n / 2; t += blockDim.x) w[t] = tw[t * step];
+}
+
+// In-place forward radix-2 DIT of `ncol` sequences of length n (sequence c at
+// data + c*stride), bit-reversed input, natural-order output.
+__device__ inline void fft_dit(double2 *data, int n, int logn, int ncol, int stride, const double2 *w)
 {
 	const int half_total = ncol * (n >> 1);
 	for (int s = 1; s <= logn; ++s) {
-		const int len = 1 << s, half = len >> 1;
-		const int step = twn >> s;  // twn / len
+		const int half = 1 << (s - 1);
+		const int shift = logn - s;  // twiddle index = k * (n / len)
 		for (int b = threadIdx.x; b < half_total; b += blockDim.x) {
 			const int c = b >> (logn - 1), bb = b - (c << (logn - 1));
 			const int grp = bb >> (s - 1), k = bb - (grp << (s - 1));
-			const int i = c * stride + grp * len + k, j = i + half;
-			double2 w = tw[k * step];
-			if (inverse) w.y = -w.y;
-			const double2 u = data[i], v = cmul(data[j], w);
-			data[i] = make_double2(u.x + v.x, u.y + v.y);
-			data[j] = make_double2(u.x - v.x, u.y - v.y);
+			const int i = c * stride + (grp << s) + k, j = i + half;
+			const double2 u = data[i], v = cmul(data[j], w[k << shift]);
+			data[i] = cadd(u, v);
+			data[j] = csub(u, v);
+		}
+		__syncthreads();
+	}
+}
+
+// In-place inverse (conjugate twiddles, unscaled) radix-2 DIF: natural-order
+// input, bit-reversed output.
+__device__ inline void ifft_dif(double2 *data, int n, int logn, int ncol, int stride, const double2 *w)
+{
+	const int half_total = ncol * (n >> 1);
+	for (int s = logn; s >= 1; --s) {
+		const int half = 1 << (s - 1);
+		const int shift = logn - s;
+		for (int b = threadIdx.x; b < half_total; b += blockDim.x) {
+			const int c = b >> (logn - 1), bb = b - (c << (logn - 1));
+			const int grp = bb >> (s - 1), k = bb - (grp << (s - 1));
+			const int i = c * stride + (grp << s) + k, j = i + half;
+			const double2 u = data[i], v = data[j];
+			data[i] = cadd(u, v);
+			data[j] = cmul(csub(u, v), conjg(w[k << shift]));
 		}
 		__syncthreads();
 	}
@@ -67,61 +99,80 @@ __device__ inline void lds_fft(double2 *data, int n, int logn, int ncol, int str
 
 }  // namespace
 
-// mode 0: input = canvas (DW x DH) at frame (0,0), one block per canvas row.
-// mode 1: input = w (W x H) at frame (padx, pady), one block per image row.
-// mode 2: input = real FP64 frame (N1 x N2) `src64`, one block per frame row.
+// Forward transform of one pair of real rows (a = src[r], b = src[r + 1] when
+// `second`), columns [x0, x0 + nx) of the frame holding data, into the
+// half-spectrum rows `outa` and `outa + M`:
+// A = (Z[k] + conj Z[N-k]) / 2 and B = (Z[k] - conj Z[N-k]) / 2i, k = 0..N/2.
+template <class T>
+__device__ __forceinline__ void rows_fwd_pair(double2 *sh, const double2 *w, int N, int L, const T *src, long ld,
+                                              int x0, int nx, bool second, double2 *outa)
+{
+	const int M = N / 2 + 1;
+	for (int t = threadIdx.x; t < N; t += blockDim.x) {
+		double a = 0, bv = 0;
+		const int xi = t - x0;
+		if (xi >= 0 && xi < nx) {
+			a = (double)src[xi];
+			if (second) bv = (double)src[ld + xi];
+		}
+		sh[bitrev(t, L)] = make_double2(a, bv);
+	}
+	__syncthreads();
+	fft_dit(sh, N, L, 1, N, w);
+	double2 *outb = outa + M;
+	for (int k = threadIdx.x; k < M; k += blockDim.x) {
+		const double2 z = sh[k], zc = conjg(sh[(N - k) & (N - 1)]);
+		outa[k] = make_double2(0.5 * (z.x + zc.x), 0.5 * (z.y + zc.y));
+		// (z - zc) / 2i = (-i / 2) (z - zc)
+		if (second) outb[k] = make_double2(0.5 * (z.y - zc.y), -0.5 * (z.x - zc.x));
+	}
+}
+
+// Forward rows, two input rows per block packed as z = a + i b:
+//   mode 0: canvas rows (DW x DH) into frame rows [0, DH)
+//   mode 1: w rows (W x H) at column padx into frame rows [pady, pady + H)
 __global__ __launch_bounds__(256) void k_fft_rows_fwd(const GalaxyDev *__restrict__ gals, const InstDev *__restrict__ insts,
                                                       const int *__restrict__ row_off, int ninst, int mode,
-                                                      const double2 *__restrict__ tw, int twn,
-                                                      const double *__restrict__ src64, double2 *dst64)
+                                                      const double2 *__restrict__ tw, int twn)
 {
 	extern __shared__ double2 sh[];
 	const int b = blockIdx.x;
 	const int inst = owner(row_off, ninst, b);
 	const InstDev &I = insts[inst];
 	const GalaxyDev &g = gals[I.galaxy];
-	const int N = g.fft_n1, L = g.fft_l1;
-	const int r = b - row_off[inst];
-	int y, x0, nx;
-	const float *src32 = nullptr;
-	const double *srcd = nullptr;
-	if (mode == 0) {
-		y = r;
-		x0 = 0;
-		nx = g.DW;
-		src32 = I.canvas + (long)r * g.DW;
-	}
-	else if (mode == 1) {
-		y = r + g.pady;
-		x0 = g.padx;
-		nx = g.W;
-		src32 = I.w + (long)r * g.W;
-	}
-	else {
-		y = r;
-		x0 = 0;
-		nx = N;
-		srcd = src64 + (long)r * N;
-	}
-	for (int t = threadIdx.x; t < N; t += blockDim.x) {
-		double v = 0;
-		const int xi = t - x0;
-		if (xi >= 0 && xi < nx) v = src32 ? (double)src32[xi] : srcd[xi];
-		sh[bitrev(t, L)] = make_double2(v, 0.0);
-	}
-	__syncthreads();
-	lds_fft(sh, N, L, 1, N, tw, twn, false);
-	double2 *out = (mode == 2 ? dst64 : I.fbuf) + (long)y * N;
-	for (int t = threadIdx.x; t < N; t += blockDim.x) out[t] = sh[t];
+	const int N = g.fft_n1, L = g.fft_l1, M = N / 2 + 1;
+	double2 *w = sh + N;
+	stage_twiddles(w, N, tw, twn);
+	const int r = 2 * (b - row_off[inst]);  // first input row of the pair
+	// straight-line selects of the mode's geometry
+	const bool adj = mode == 1;
+	const int nrows = adj ? g.H : g.DH;
+	const int width = adj ? g.W : g.DW;
+	const int x0 = adj ? g.padx : 0;
+	const int y0 = adj ? g.pady : 0;
+	const float *src = (adj ? I.w : I.canvas) + (long)r * width;
+	rows_fwd_pair<float>(sh, w, N, L, src, width, x0, width, r + 1 < nrows, I.fbuf + (long)(y0 + r) * M);
 }
 
-// Column pass over CB adjacent columns of one instance.
-// in rows [in_lo, in_hi) of the frame hold the row spectra, the others are
-// zero.  op 0: forward only (writes all N2 rows of `dst64`, used to build the
-// PSF spectrum); op 1: product with the spectrum; op 2: with its conjugate.
-// After the inverse column FFT, rows [out_lo, out_hi) are written back.
-// Global accesses run along rows (CB adjacent columns per row); the columns
-// sit in LDS at a stride of N2 + 1 elements to spread them over the banks.
+// Forward rows of a galaxy's real FP64 PSF frame (N1 x N2, `src64`) into the
+// half-spectrum frame `dst64`, two rows per block.
+__global__ __launch_bounds__(256) void k_fft_psf_rows(const GalaxyDev *__restrict__ g, const double2 *__restrict__ tw,
+                                                      int twn, const double *__restrict__ src64, double2 *dst64)
+{
+	extern __shared__ double2 sh[];
+	const int N = g->fft_n1, L = g->fft_l1, M = N / 2 + 1;
+	double2 *w = sh + N;
+	stage_twiddles(w, N, tw, twn);
+	const int r = 2 * blockIdx.x;
+	rows_fwd_pair<double>(sh, w, N, L, src64 + (long)r * N, N, 0, N, r + 1 < g->fft_n2, dst64 + (long)r * M);
+}
+
+// Column pass over CB adjacent half-spectrum columns of one instance.  Frame
+// rows [in_lo, in_hi) hold row spectra, the others are zero.  op 0: forward
+// only (all N2 rows of `dst64` written: the PSF spectrum); op 1: product with
+// the PSF spectrum; op 2: with its conjugate.  The inverse column transform
+// then leaves rows [out_lo, out_hi) in the frame.  Global accesses run along
+// rows (CB adjacent entries); columns sit in LDS at a stride of N2 + 1.
 __global__ __launch_bounds__(256) void k_fft_cols(const GalaxyDev *__restrict__ gals, const InstDev *__restrict__ insts,
                                                   const int *__restrict__ col_off, int ninst, int cb, int op,
                                                   int in_mode, int out_mode, const double2 *__restrict__ tw, int twn,
@@ -132,8 +183,10 @@ __global__ __launch_bounds__(256) void k_fft_cols(const GalaxyDev *__restrict__ 
 	const int inst = owner(col_off, ninst, b);
 	const InstDev &I = insts[inst];
 	const GalaxyDev &g = gals[I.galaxy];
-	const int N1 = g.fft_n1, N = g.fft_n2, L = g.fft_l2;
+	const int M = g.fft_n1 / 2 + 1, N = g.fft_n2, L = g.fft_l2;
 	const int S = N + 1;
+	double2 *w = sh + cb * S;
+	stage_twiddles(w, N, tw, twn);
 	const int c0 = (b - col_off[inst]) * cb;
 	double2 *frame = (op == 0) ? dst64 : I.fbuf;
 	// rows holding data: 0 canvas rows [0, DH); 1 image rows [pady, pady + H); 2 all rows
@@ -142,46 +195,46 @@ __global__ __launch_bounds__(256) void k_fft_cols(const GalaxyDev *__restrict__ 
 	for (int t = threadIdx.x; t < cb * N; t += blockDim.x) {
 		const int y = t / cb, c = t - y * cb;
 		double2 v = make_double2(0.0, 0.0);
-		if (y >= in_lo && y < in_hi && c0 + c < N1) v = frame[(long)y * N1 + c0 + c];
+		if (y >= in_lo && y < in_hi && c0 + c < M) v = frame[(long)y * M + c0 + c];
 		sh[c * S + bitrev(y, L)] = v;
 	}
 	__syncthreads();
-	lds_fft(sh, N, L, cb, S, tw, twn, false);
+	fft_dit(sh, N, L, cb, S, w);
 	if (op == 0) {
 		for (int t = threadIdx.x; t < cb * N; t += blockDim.x) {
 			const int y = t / cb, c = t - y * cb;
-			if (c0 + c < N1) frame[(long)y * N1 + c0 + c] = sh[c * S + y];
+			if (c0 + c < M) frame[(long)y * M + c0 + c] = sh[c * S + y];
 		}
 		return;
 	}
-	// product with the PSF spectrum, written straight back in bit-reversed
-	// order for the inverse transform
-	double2 *tmp = sh + cb * S;
 	for (int t = threadIdx.x; t < cb * N; t += blockDim.x) {
 		const int y = t / cb, c = t - y * cb;
-		double2 v = sh[c * S + y];
-		if (c0 + c < N1) {
-			const double2 s = g.fft_spec[(long)y * N1 + c0 + c];
-			v = (op == 1) ? cmul(v, s) : cmulc(v, s);
+		if (c0 + c < M) {
+			const double2 s = g.fft_spec[(long)y * M + c0 + c];
+			const double2 v = sh[c * S + y];
+			sh[c * S + y] = (op == 1) ? cmul(v, s) : cmulc(v, s);
 		}
-		tmp[c * S + bitrev(y, L)] = v;
 	}
 	__syncthreads();
-	lds_fft(tmp, N, L, cb, S, tw, twn, true);
+	ifft_dif(sh, N, L, cb, S, w);
 	// rows written back: 0 image rows [pady, pady + H); 1 canvas rows [0, DH)
 	const int out_lo = out_mode == 0 ? g.pady : 0;
 	const int out_hi = out_mode == 0 ? g.pady + g.H : g.DH;
 	for (int t = threadIdx.x; t < cb * N; t += blockDim.x) {
 		const int y = t / cb, c = t - y * cb;
-		if (y >= out_lo && y < out_hi && c0 + c < N1) frame[(long)y * N1 + c0 + c] = tmp[c * S + y];
+		if (y >= out_lo && y < out_hi && c0 + c < M) frame[(long)y * M + c0 + c] = sh[c * S + bitrev(y, L)];
 	}
 }
 
-// Inverse row pass with the epilogues.
-// mode 0 (likelihood): one block per image row; model = Re / (N1 N2) at frame
-//   (padx + x, pady + y), likelihood terms into partial[b], w written.
-// mode 1 (model image): one block per canvas row; G = model masked by calc.
-// mode 2 (adjoint): one block per canvas row; G = Re / (N1 N2).
+// Inverse rows, two frame rows (half-spectra A, B) per block packed as
+// Z = A + i B over the full length (Z[k] = conj A[N-k] + i conj B[N-k] above
+// N/2); the real part of the inverse is the first row, the imaginary part the
+// second.  Epilogues:
+//   mode 0 (likelihood): image rows r, r + 1 (frame rows pady + r, ...); model
+//     = value / (N1 N2) at column padx + x; likelihood terms into one partial
+//     per block, w written
+//   mode 1 (model image): canvas rows; G = model masked by calc
+//   mode 2 (adjoint): canvas rows; G = value / (N1 N2)
 __global__ __launch_bounds__(256) void k_fft_rows_inv(const GalaxyDev *__restrict__ gals, const InstDev *__restrict__ insts,
                                                       const int *__restrict__ row_off, int ninst, int mode,
                                                       const double2 *__restrict__ tw, int twn,
@@ -192,24 +245,44 @@ __global__ __launch_bounds__(256) void k_fft_rows_inv(const GalaxyDev *__restric
 	const int inst = owner(row_off, ninst, b);
 	const InstDev &I = insts[inst];
 	const GalaxyDev &g = gals[I.galaxy];
-	const int N = g.fft_n1, L = g.fft_l1;
-	const int r = b - row_off[inst];
+	const int N = g.fft_n1, L = g.fft_l1, M = N / 2 + 1;
+	double2 *w = sh + N;
+	stage_twiddles(w, N, tw, twn);
+	const int p = b - row_off[inst];
+	const int r = 2 * p;  // first output row of the pair (image rows in mode 0)
+	const int nrows = mode == 0 ? g.H : g.DH;
 	const int y = mode == 0 ? r + g.pady : r;
-	const double2 *row = I.fbuf + (long)y * N;
-	for (int t = threadIdx.x; t < N; t += blockDim.x) sh[bitrev(t, L)] = row[t];
+	const bool second = r + 1 < nrows;
+	const double2 *ra = I.fbuf + (long)y * M;
+	const double2 *rb = ra + M;
+	for (int k = threadIdx.x; k < N; k += blockDim.x) {
+		double2 z;
+		if (k < M) {
+			const double2 a = ra[k], bv = second ? rb[k] : make_double2(0.0, 0.0);
+			z = make_double2(a.x - bv.y, a.y + bv.x);  // A + i B
+		}
+		else {
+			const double2 a = conjg(ra[N - k]), bv = second ? conjg(rb[N - k]) : make_double2(0.0, 0.0);
+			z = make_double2(a.x - bv.y, a.y + bv.x);
+		}
+		sh[k] = z;
+	}
 	__syncthreads();
-	lds_fft(sh, N, L, 1, N, tw, twn, true);
+	ifft_dif(sh, N, L, 1, N, w);
 	const double scale = 1.0 / ((double)g.fft_n1 * (double)g.fft_n2);
 	if (mode == 0) {
 		double part = 0;
 		const bool pois = g.likefunc == LIKE_POIS;
 		const double s = g.pois_scale;
-		for (int x = threadIdx.x; x < g.W; x += blockDim.x) {
-			const long o = (long)r * g.W + x;
+		for (int t = threadIdx.x; t < 2 * g.W; t += blockDim.x) {
+			const int row = t >= g.W ? 1 : 0, x = t - row * g.W;
+			if (row == 1 && !second) continue;
+			const long o = (long)(r + row) * g.W + x;
 			float wv = 0.f;
 			if (g.region[o]) {
-				const bool in_calc = !g.has_calc || g.calc[(long)y * g.DW + x + g.padx];
-				const double m = in_calc ? sh[x + g.padx].x * scale : 0.0;
+				const bool in_calc = !g.has_calc || g.calc[(long)(y + row) * g.DW + x + g.padx];
+				const double2 z = sh[bitrev(x + g.padx, L)];
+				const double m = in_calc ? (row ? z.y : z.x) * scale : 0.0;
 				const double d = g.data[o];
 				if (pois) {
 					const double mm = s * m, dd = s * d;
@@ -234,15 +307,18 @@ __global__ __launch_bounds__(256) void k_fft_rows_inv(const GalaxyDev *__restric
 		if (threadIdx.x == 0) {
 			double t = 0;
 			for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t += red[k];
-			partial[lk_base[inst] + r] = t;
+			partial[lk_base[inst] + p] = t;
 		}
 		return;
 	}
-	float *out = I.G + (long)y * g.DW;
-	for (int x = threadIdx.x; x < g.DW; x += blockDim.x) {
-		double v = sh[x].x * scale;
-		if (mode == 1 && g.has_calc && !g.calc[(long)y * g.DW + x]) v = 0;
-		out[x] = (float)v;
+	for (int t = threadIdx.x; t < 2 * g.DW; t += blockDim.x) {
+		const int row = t >= g.DW ? 1 : 0, x = t - row * g.DW;
+		if (row == 1 && !second) continue;
+		const double2 z = sh[bitrev(x, L)];
+		double v = (row ? z.y : z.x) * scale;
+		const long o = (long)(y + row) * g.DW + x;
+		if (mode == 1 && g.has_calc && !g.calc[o]) v = 0;
+		I.G[o] = (float)v;
 	}
 }
 
@@ -263,18 +339,32 @@ static int ensure_lds(const void *fn, size_t lds, size_t *done)
 	done[dev] = lds;
 	return 0;
 }
-static size_t lds_rows_fwd[64], lds_cols[64], lds_rows_inv[64];
+static size_t lds_rows_fwd[64], lds_psf_rows[64], lds_cols[64], lds_rows_inv[64];
 
-static int cols_per_block(int n2) { return n2 >= 2048 ? 1 : (2048 / n2 > 8 ? 8 : 2048 / n2); }
+static int cols_per_block(int n2) { return n2 >= 4096 ? 1 : (n2 >= 2048 ? 2 : (4096 / n2 > 8 ? 8 : 4096 / n2)); }
+
+// LDS of a row pass: the row plus half as many twiddles
+static size_t row_lds(int nmax) { return sizeof(double2) * (size_t)(nmax + nmax / 2); }
 
 int launch_fft_rows_fwd(int nblocks, int nmax, const GalaxyDev *gals, const InstDev *insts, const int *row_off,
-                        int ninst, int mode, const double2 *tw, int twn, const double *src64, double2 *dst64,
-                        hipStream_t st)
+                        int ninst, int mode, const double2 *tw, int twn, hipStream_t st)
 {
 	if (nblocks <= 0) return 0;
-	const size_t lds = sizeof(double2) * (size_t)nmax;
+	if (mode != 0 && mode != 1) return (int)hipErrorInvalidValue;
+	const size_t lds = row_lds(nmax);
+	if (lds > 160 * 1024) return (int)hipErrorInvalidValue;
 	if (int e = ensure_lds((const void *)k_fft_rows_fwd, lds, lds_rows_fwd)) return e;
-	k_fft_rows_fwd<<<nblocks, 256, lds, st>>>(gals, insts, row_off, ninst, mode, tw, twn, src64, dst64);
+	k_fft_rows_fwd<<<nblocks, 256, lds, st>>>(gals, insts, row_off, ninst, mode, tw, twn);
+	return (int)hipGetLastError();
+}
+
+int launch_fft_psf_rows(int n1, int n2, const GalaxyDev *gal, const double2 *tw, int twn, const double *src64,
+                        double2 *dst64, hipStream_t st)
+{
+	const size_t lds = row_lds(n1);
+	if (lds > 160 * 1024 || n2 < 2) return (int)hipErrorInvalidValue;
+	if (int e = ensure_lds((const void *)k_fft_psf_rows, lds, lds_psf_rows)) return e;
+	k_fft_psf_rows<<<n2 / 2, 256, lds, st>>>(gal, tw, twn, src64, dst64);
 	return (int)hipGetLastError();
 }
 
@@ -283,7 +373,7 @@ int launch_fft_cols(int nblocks, int n2max, int cb, const GalaxyDev *gals, const
                     hipStream_t st)
 {
 	if (nblocks <= 0) return 0;
-	const size_t lds = sizeof(double2) * (size_t)cb * (n2max + 1) * (op == 0 ? 1 : 2);
+	const size_t lds = sizeof(double2) * ((size_t)cb * (n2max + 1) + n2max / 2);
 	if (lds > 160 * 1024) return (int)hipErrorInvalidValue;
 	if (int e = ensure_lds((const void *)k_fft_cols, lds, lds_cols)) return e;
 	k_fft_cols<<<nblocks, 256, lds, st>>>(gals, insts, col_off, ninst, cb, op, in_mode, out_mode, tw, twn, dst64);
@@ -295,7 +385,8 @@ int launch_fft_rows_inv(int nblocks, int nmax, const GalaxyDev *gals, const Inst
                         hipStream_t st)
 {
 	if (nblocks <= 0) return 0;
-	const size_t lds = sizeof(double2) * (size_t)nmax;
+	const size_t lds = row_lds(nmax);
+	if (lds > 160 * 1024) return (int)hipErrorInvalidValue;
 	if (int e = ensure_lds((const void *)k_fft_rows_inv, lds, lds_rows_inv)) return e;
 	k_fft_rows_inv<<<nblocks, 256, lds, st>>>(gals, insts, row_off, ninst, mode, tw, twn, lk_base, partial);
 	return (int)hipGetLastError();

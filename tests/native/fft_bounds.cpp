@@ -1,0 +1,116 @@
+// Runs the FFT convolution kernels on the CPU (hip_cpu stand-in) for a set of
+// geometries with every device buffer at its exact size, so that
+// AddressSanitizer reports any access outside one.  The host-side numbers
+// (frame sizes, blocks per pass, LDS sizes) are computed the way the engine
+// computes them.  Prints "ok <geometry>" per case.
+#include <cstdio>
+#include <cstdlib>
+
+#include "fft_kernels.inc"  // fft.hip up to its launchers
+
+namespace profit_amd {
+double2 sh[1 << 17];  // the kernels' dynamic LDS (blocks run one at a time)
+}
+
+using namespace profit_amd;
+
+static int next_pow2(int v)
+{
+	int n = 1;
+	while (n < v) n <<= 1;
+	return n;
+}
+static int ilog2(int n)
+{
+	int l = 0;
+	while ((1 << l) < n) ++l;
+	return l;
+}
+template <class T> static T *dev(size_t n)
+{
+	T *p = (T *)std::malloc(n * sizeof(T));
+	std::memset(p, 0, n * sizeof(T));
+	return p;
+}
+
+static void run_case(int W, int H, int K, int pad)
+{
+	GalaxyDev g{};
+	g.W = W;
+	g.H = H;
+	g.padx = g.pady = pad;
+	g.DW = W + 2 * pad;
+	g.DH = H + 2 * pad;
+	g.Kx = g.Ky = K;
+	g.has_calc = 1;
+	g.likefunc = LIKE_CHISQ;
+	g.pois_scale = 1;
+	const int n1 = next_pow2(g.DW + K), n2 = next_pow2(g.DH + K), m1 = n1 / 2 + 1;
+	g.fft_n1 = n1;
+	g.fft_n2 = n2;
+	g.fft_l1 = ilog2(n1);
+	g.fft_l2 = ilog2(n2);
+	const int twn = n1 > n2 ? n1 : n2;
+	double2 *tw = dev<double2>(twn / 2);
+	for (int t = 0; t < twn / 2; ++t) tw[t] = make_double2(std::cos(2 * M_PI * t / twn), -std::sin(2 * M_PI * t / twn));
+	double2 *spec = dev<double2>((size_t)m1 * n2);
+	double *src = dev<double>((size_t)n1 * n2);
+	src[0] = 1;
+	g.fft_spec = spec;
+	uint8_t *calc = dev<uint8_t>((size_t)g.DW * g.DH), *region = dev<uint8_t>((size_t)W * H);
+	std::memset(calc, 1, (size_t)g.DW * g.DH);
+	std::memset(region, 1, (size_t)W * H);
+	float *data = dev<float>((size_t)W * H), *ivar = dev<float>((size_t)W * H);
+	for (int i = 0; i < W * H; ++i) ivar[i] = 1;
+	g.calc = calc;
+	g.region = region;
+	g.data = data;
+	g.ivar = ivar;
+	GalaxyDev *gals = dev<GalaxyDev>(1);
+	gals[0] = g;
+	InstDev *insts = dev<InstDev>(1);
+	insts[0].galaxy = 0;
+	insts[0].canvas = dev<float>((size_t)g.DW * g.DH);
+	insts[0].w = dev<float>((size_t)W * H);
+	insts[0].G = dev<float>((size_t)g.DW * g.DH);
+	insts[0].fbuf = dev<double2>((size_t)m1 * n2);
+	const int cb = n2 >= 4096 ? 1 : (n2 >= 2048 ? 2 : (4096 / n2 > 8 ? 8 : 4096 / n2));
+	int *off = dev<int>(2);
+	int *lk = dev<int>(2);
+	double *partial = dev<double>((H + 1) / 2);
+	lk[1] = (H + 1) / 2;
+	auto rows_fwd = [&](int mode, int nblocks) {
+		off[1] = nblocks;
+		hip_cpu::launch(nblocks, 256, [&] { k_fft_rows_fwd(gals, insts, off, 1, mode, tw, twn); });
+	};
+	auto cols = [&](int op, int in_mode, int out_mode, double2 *d64) {
+		const int nb = (m1 + cb - 1) / cb;
+		off[1] = nb;
+		hip_cpu::launch(nb, 256, [&] { k_fft_cols(gals, insts, off, 1, cb, op, in_mode, out_mode, tw, twn, d64); });
+	};
+	auto rows_inv = [&](int mode, int nblocks) {
+		off[1] = nblocks;
+		hip_cpu::launch(nblocks, 256, [&] { k_fft_rows_inv(gals, insts, off, 1, mode, tw, twn, lk, partial); });
+	};
+	hip_cpu::launch(n2 / 2, 256, [&] { k_fft_psf_rows(gals, tw, twn, src, spec); });
+	cols(0, 2, 0, spec);
+	rows_fwd(0, (g.DH + 1) / 2);
+	cols(1, 0, 1, nullptr);
+	rows_inv(1, (g.DH + 1) / 2);
+	rows_fwd(0, (g.DH + 1) / 2);
+	cols(1, 0, 0, nullptr);
+	rows_inv(0, (H + 1) / 2);
+	rows_fwd(1, (H + 1) / 2);
+	cols(2, 1, 1, nullptr);
+	rows_inv(2, (g.DH + 1) / 2);
+	std::printf("ok W=%d H=%d K=%d pad=%d frame=%dx%d\n", W, H, K, pad, n1, n2);
+}
+
+int main()
+{
+	run_case(70, 90, 5, 0);
+	run_case(33, 130, 9, 0);
+	run_case(64, 40, 25, 12);
+	run_case(31, 17, 7, 3);
+	return 0;
+}
