@@ -29,6 +29,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "lik+grad evals/s, 400² bulge+disk ⊛ PSF: adjoint vs FD, ≤1e-5 rel pixel err"
 PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
+PEAK_BF16_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
 
 
@@ -97,7 +99,7 @@ def cpu_baseline(wl, calc, seconds, threads):
         O.loglike_grad_fd(fit, oracle_profiles(comps), rel_step=1e-4, slots=slots, frozen=False)
         n += 1
         el = time.perf_counter() - t0
-        if el >= seconds or n >= 3:
+        if el >= seconds:
             break
     ll, model = O.loglike(fit, oracle_profiles(comps), want_model=True)
     _, grad = O.loglike_grad(fit, oracle_profiles(comps))
@@ -115,9 +117,22 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile-rows", type=int, default=8)
     ap.add_argument("--render-fp32", action="store_true", help="direct pixels in FP32 instead of FP64")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy 256 MiB device-to-device three times (the reference kernel for scaling "
+                         "the FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--conv", choices=("auto", "brute", "fft", "brute-bf16x3"), default="auto",
                     help="PSF convolution method (auto: the autotuner's choice among brute and fft)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = 64 * 1024 * 1024
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": n * 4}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU figures are not produced",
                           "n_gpus": args.gpus}))
@@ -178,25 +193,73 @@ def main():
     eng1.set_render_fp64(not args.render_fp32)
     eng1.set_conv_method(conv_used)
     eng1.set_batch([g1], [2])
-    fd_gradient_gpu(eng1, P, sets[0])  # warm-up (graph capture)
-    tf0 = time.perf_counter()
-    for i in range(args.fd_evals):
-        fd_gradient_gpu(eng1, P, sets[i % B])
-    tf1 = time.perf_counter()
-    fd_value = args.fd_evals / (tf1 - tf0)
-    print(f"[bench] plain forward + FD: {fd_value:.1f} lik+grad/s", file=sys.stderr, flush=True)
+    fd_value = None
+    if args.fd_evals > 0:
+        fd_gradient_gpu(eng1, P, sets[0])  # warm-up (graph capture)
+        tf0 = time.perf_counter()
+        for i in range(args.fd_evals):
+            fd_gradient_gpu(eng1, P, sets[i % B])
+        tf1 = time.perf_counter()
+        fd_value = args.fd_evals / (tf1 - tf0)
+    if fd_value:
+        print(f"[bench] plain forward + FD: {fd_value:.1f} lik+grad/s", file=sys.stderr, flush=True)
 
-    # roofline of the dominant kernel (per launch = one batch)
+    # roofline of the dominant kernel (one launch = the whole batch), with the
+    # algorithmic work per unit of each kernel family:
+    #   rendering (k_render, the recursion walkers, the adjoint render):
+    #     5 B per canvas pixel (4 B FP32 write + 1 B mask read), HBM-bound
+    #   brute-force convolution: 2 K^2 FLOP per output pixel (FP32 vector peak)
+    #   FFT convolution (3 kernels): 3 transforms x 5 N log2 N FLOP, N = N1 N2
+    #     (FP64 vector peak)
     sm = stats["stage_ms"]
-    calls = max(1, stats["stage_calls"]["conv_like"])
-    fwd_ms = sm["conv_like"] / calls
-    adj_ms = sm["adjoint_conv"] / max(1, stats["stage_calls"]["adjoint_conv"])
-    if adj_ms >= fwd_ms:
-        kname, flops, kms = "k_conv<8,CONV_ADJ> (adjoint correlation)", stats["adjoint_flops"], adj_ms
+    ncalls = max(1, stats["stage_calls"]["subsample"])
+    stage_ms = {k: v / ncalls for k, v in sm.items()}
+    dw, dh, _, _ = eng.geometry(g)
+    K = wl["psf"].shape[0]
+    n1 = 1 << int(np.ceil(np.log2(dw + K)))
+    n2 = 1 << int(np.ceil(np.log2(dh + K)))
+    n_fft = n1 * n2
+    render_bytes = B * dw * dh * 5.0
+    kernels = {
+        "render": ("k_render (classification + direct pixels)", "hbm", render_bytes, PEAK_HBM_GBS, "GB/s"),
+        "subsample": ("k_subsample<false> (recursion walker)", "hbm", render_bytes, PEAK_HBM_GBS, "GB/s"),
+        "adjoint_render": ("k_adj_render (adjoint of direct pixels)", "hbm", render_bytes, PEAK_HBM_GBS, "GB/s"),
+        "adjoint_subsample": ("k_subsample<true> (adjoint recursion walker)", "hbm", render_bytes, PEAK_HBM_GBS,
+                              "GB/s"),
+    }
+    if conv_used == "fft":
+        fft_flops = B * 3 * 5.0 * n_fft * np.log2(n_fft)
+        kernels["conv_like"] = ("FFT convolution + likelihood (k_fft_rows_fwd, k_fft_cols, k_fft_rows_inv)", "mfma",
+                                fft_flops, PEAK_FP64_TFLOPS, "TFLOP/s")
+        kernels["adjoint_conv"] = ("FFT adjoint correlation (k_fft_rows_fwd, k_fft_cols, k_fft_rows_inv)", "mfma",
+                                   fft_flops, PEAK_FP64_TFLOPS, "TFLOP/s")
     else:
-        kname, flops, kms = "k_conv<8,CONV_LIKE> (convolution + likelihood)", stats["conv_flops"], fwd_ms
-    achieved = flops / (kms * 1e-3) / 1e12
+        peak = PEAK_BF16_TFLOPS if conv_used == "brute-bf16x3" else PEAK_FP32_TFLOPS
+        kname = "k_conv_mfma" if conv_used == "brute-bf16x3" else "k_conv"
+        kernels["conv_like"] = (f"{kname}<CONV_LIKE> (convolution + likelihood)", "mfma", stats["conv_flops"], peak,
+                                "TFLOP/s")
+        kernels["adjoint_conv"] = (f"{kname}<CONV_ADJ> (adjoint correlation)", "mfma", stats["adjoint_flops"], peak,
+                                   "TFLOP/s")
+    dom = max(kernels, key=lambda k: stage_ms[k])
+    kname, bound, work, peak, unit = kernels[dom]
+    # HBM bytes per launch of that kernel from the committed PMC summary of the
+    # same workload (FETCH_SIZE / WRITE_SIZE passes scaled by a copy kernel)
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_config2_batch{B}.json")
+    if os.path.exists(tfile) and conv_used == "fft":
+        with open(tfile) as f:
+            tk = json.load(f)["kernels"]
+        key = kname.split(" ")[0]
+        if key in tk:
+            traffic = tk[key]["hbm_bytes_per_launch"]
+    kms = stage_ms[dom]
+    achieved = work / (kms * 1e-3) / (1e9 if unit == "GB/s" else 1e12)
     stage_per_step = {k: v / args.steps for k, v in sm.items()}
+    per_kernel = {}
+    for k, (kn, bd, wk, pk, un) in kernels.items():
+        a = wk / (stage_ms[k] * 1e-3) / (1e9 if un == "GB/s" else 1e12)
+        per_kernel[k] = {"kernel": kn, "bound": bd, "ms": round(stage_ms[k], 4), "achieved": round(a, 3),
+                         "unit": un, "frac": round(a / pk, 5)}
 
     out = {
         "metric": METRIC,
@@ -209,7 +272,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (render, convolution) + f64 (classification, recursion, likelihood sums)",
+        "dtype": ("f64 (FFT convolution, direct pixels, recursion leaves, likelihood sums) + f32 (refinement "
+                  "tests, adjoint derivatives)") if conv_used == "fft" else
+                 ("f32 (convolution) + f64 (direct pixels, recursion leaves, likelihood sums)"),
         "data": "synthetic (noiseless model from this GPU path plus seeded SplitMix64/Box-Muller noise)",
         "config": {
             "workload": "400x400 bulge+disk (Sersic n=4 re=3 mag=17 axrat=0.9; Sersic n=1 re=15 mag=16 "
@@ -225,23 +290,23 @@ def main():
         },
         "adjoint_vs_fd": {
             "adjoint_graph_replay_value": round(graph_value, 3),
-            "fd_plain_forward_value": round(fd_value, 3),
-            "ratio": round(value / fd_value, 2),
+            "fd_plain_forward_value": round(fd_value, 3) if fd_value else None,
+            "ratio": round(value / fd_value, 2) if fd_value else None,
             "fd_forward_calls_per_lik_grad": 29,
         },
         "stage_ms_per_step": {k: round(v, 4) for k, v in stage_per_step.items()},
         "roofline": {
-            "bound": "mfma",
+            "bound": bound,
             "achieved": round(achieved, 3),
-            "peak": PEAK_FP32_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-            "traffic": None,
+            "peak": peak,
+            "unit": unit,
+            "frac": round(achieved / peak, 5),
+            "traffic": traffic,
             "kernel": kname,
-            "flops_per_launch": flops,
+            "work_per_launch": work,
             "avg_launch_ms": round(kms, 4),
-            "note": "FP32 FMA on the vector ALUs (packed FP32 disabled); peak quoted is MI355X FP32 157.3 TF",
         },
+        "roofline_by_kernel": per_kernel,
     }
 
     # FP64 CPU oracle leg: baseline timing and parity of the GPU result

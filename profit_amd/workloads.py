@@ -170,3 +170,27 @@ def config4_attach_data(gals, models, seed=40, read_noise=5.0):
         g.update(model=m, sigma=sigma,
                  data=m + normal_draws(seed * 1000003 + i, m.size).reshape(m.shape) * sigma)
     return gals
+
+
+def config5(model_fn=None, seed=5, nsrc=500, magzero=30.0, read_noise=5.0):
+    """Config 5, the precision stress field: 2048 x 2048 with 500 Sersic
+    sources (nser log-uniform 0.5-8, re U(0.5, 20) px, mag U(16, 24),
+    positions uniform, ang U(0, 180), axrat U(0.2, 1)), sub-pixel recursion
+    with max_recursions = 4, a 101 x 101 Moffat PSF (FWHM 5, con 2.5), all
+    pixels in the likelihood, sigma = sqrt(max(model, 0) + read_noise^2)."""
+    w = h = 2048
+    psf = moffat_psf(101, 5.0, 2.5)
+    u = uniforms(seed * 7919 + 1, 8 * nsrc).reshape(nsrc, 8)
+    comps = []
+    for v in u:
+        comps.append(dict(type="sersic", xcen=w * v[0], ycen=h * v[1], mag=16.0 + 8.0 * v[2],
+                          re=0.5 + 19.5 * v[3], nser=float(np.exp(np.log(0.5) + np.log(16.0) * v[4])),
+                          ang=180.0 * v[5], axrat=0.2 + 0.8 * v[6], box=0.0, max_recursions=4))
+    region = np.ones((h, w), dtype=np.uint8)
+    out = dict(width=w, height=h, psf=psf, components=comps, region=region, magzero=magzero, likefunc="chisq")
+    if model_fn is not None:
+        model = model_fn(w, h, psf, comps, magzero)
+        sigma = np.sqrt(np.maximum(model, 0) + read_noise ** 2)
+        data = model + normal_draws(seed, w * h).reshape(h, w) * sigma
+        out.update(model=model, data=data, sigma=sigma)
+    return out
