@@ -141,3 +141,36 @@ def test_brute_convolution(ksize):
     _report(f"brute conv k={ksize}", err, n)
     assert err <= REL_TOL
     assert np.all(got[mask == 0] == 0)
+
+
+def test_fp32_values_meet_the_gates():
+    """The FP32 value option (direct pixels and recursion leaves in FP32,
+    decisions still FP64-exact) against the same gates: config 1 at magzero 30
+    (the logL most sensitive to the model) and config 2."""
+    wl = W.config1(oracle_model_fn(), magzero=30.0)
+    fit = O.FitSpec(O.ModelSpec(100, 100, psf=wl["psf"], magzero=30.0, threads=8), wl["data"], wl["sigma"],
+                    wl["region"])
+    oll, omodel = O.loglike(fit, oracle_profiles(wl["components"]), want_model=True)
+    eng = P.Engine()
+    eng.set_render_fp64(False)
+    g = eng.add_galaxy(100, 100, wl["data"], wl["sigma"], psf=wl["psf"], region=wl["region"], magzero=30.0)
+    eng.set_batch([g], [1])
+    params = gpu_profiles(wl["components"])
+    err, n = rel_err(eng.model_images(params)[0], omodel)
+    gll = eng.like(params)[0]
+    print(f"[parity] fp32 values config1 magzero 30: pixel err {err:.3e}, logL diff {gll - oll:.3e}", flush=True)
+    assert err <= REL_TOL
+    assert abs(gll - oll) <= 1e-3
+    wl2, calc, fit2 = _config2_fit()
+    oll2, omodel2 = O.loglike(fit2, oracle_profiles(wl2["components"]), want_model=True)
+    eng2 = P.Engine()
+    eng2.set_render_fp64(False)
+    g2 = eng2.add_galaxy(400, 400, wl2["data"], wl2["sigma"], psf=wl2["psf"], calcregion=calc, region=wl2["region"],
+                         magzero=30.0)
+    eng2.set_batch([g2], [2])
+    params2 = gpu_profiles(wl2["components"])
+    err2, _ = rel_err(eng2.model_images(params2)[0], omodel2)
+    gll2 = eng2.like(params2)[0]
+    print(f"[parity] fp32 values config2: pixel err {err2:.3e}, logL diff {gll2 - oll2:.3e}", flush=True)
+    assert err2 <= REL_TOL
+    assert abs(gll2 - oll2) <= 1e-3
