@@ -116,7 +116,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile-rows", type=int, default=8)
-    ap.add_argument("--render-fp32", action="store_true", help="direct pixels in FP32 instead of FP64")
+    ap.add_argument("--render-fp64", action="store_true",
+                    help="profile values (direct pixels, recursion leaves) in FP64 instead of FP32")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 256 MiB device-to-device three times (the reference kernel for scaling "
                          "the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -151,7 +152,7 @@ def main():
     g = eng.add_galaxy(400, 400, wl["data"], wl["sigma"], psf=wl["psf"], calcregion=calc, region=wl["region"],
                        likefunc="chisq", magzero=30.0)
     eng.set_tile_rows(args.tile_rows)
-    eng.set_render_fp64(not args.render_fp32)
+    eng.set_render_fp64(args.render_fp64)
     eng.set_conv_method("brute" if args.conv == "auto" else args.conv)
     eng.set_batch([g] * B, [2] * B)
     sets = walkers(wl["components"], B)
@@ -190,7 +191,7 @@ def main():
     eng1 = P.Engine()
     g1 = eng1.add_galaxy(400, 400, wl["data"], wl["sigma"], psf=wl["psf"], calcregion=calc, region=wl["region"],
                          likefunc="chisq", magzero=30.0)
-    eng1.set_render_fp64(not args.render_fp32)
+    eng1.set_render_fp64(args.render_fp64)
     eng1.set_conv_method(conv_used)
     eng1.set_batch([g1], [2])
     fd_value = None
@@ -272,9 +273,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("f64 (FFT convolution, direct pixels, recursion leaves, likelihood sums) + f32 (refinement "
-                  "tests, adjoint derivatives)") if conv_used == "fft" else
-                 ("f32 (convolution) + f64 (direct pixels, recursion leaves, likelihood sums)"),
+        "dtype": (("f64 (FFT convolution" if conv_used == "fft" else "f32 (convolution") +
+                  (", profile values" if args.render_fp64 else "") + ", cell positions, likelihood sums) + f32 (" +
+                  ("" if args.render_fp64 else "profile values, ") + "refinement tests, adjoint derivatives)"),
         "data": "synthetic (noiseless model from this GPU path plus seeded SplitMix64/Box-Muller noise)",
         "config": {
             "workload": "400x400 bulge+disk (Sersic n=4 re=3 mag=17 axrat=0.9; Sersic n=1 re=15 mag=16 "
@@ -284,7 +285,7 @@ def main():
             "batch": B,
             "drawing_canvas": "450x450",
             "tile_rows": args.tile_rows,
-            "direct_pixels": "fp32" if args.render_fp32 else "fp64",
+            "profile_values": "fp64" if args.render_fp64 else "fp32",
             "convolution": conv_used,
             "autotune_ms": tune["ms"] if tune else None,
         },

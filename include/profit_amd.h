@@ -161,8 +161,8 @@ int profit_amd_engine_set_batch(profit_amd_engine *engine, const int32_t *inst_g
 /* Tile height of the convolution / rendering tiles: 4*rows (rows = 2, 4, 8, 16). */
 int profit_amd_engine_set_tile_rows(profit_amd_engine *engine, int32_t rows);
 
-/* Precision of the forward model's profile values: 1 = FP64 (default), 0 =
- * FP32, for the directly evaluated pixels and the leaf samples of the
+/* Precision of the forward model's profile values: 0 = FP32 (default), 1 =
+ * FP64, for the directly evaluated pixels and the leaf samples of the
  * sub-pixel recursion alike.  Either way: classification and cell positions
  * are FP64, refinement decisions are FP32 with an FP64 re-test inside a guard
  * band (so they equal the FP64 decisions), the convolution is FP32 with FP64

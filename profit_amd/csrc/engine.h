@@ -185,7 +185,7 @@ class Engine {
 	int ncomp() const { return ncomp_total_; }
 	int conv_rows() const { return conv_rows_; }
 	void set_conv_rows(int r);
-	// direct pixels (and their adjoint) in FP64 (default) or FP32
+	// profile values (direct pixels, recursion leaves) in FP32 (default) or FP64
 	void set_render_fp64(bool on);
 	bool render_fp64() const { return render_fp64_; }
 	void set_timing(bool on);
@@ -237,7 +237,7 @@ class Engine {
 	int n_bt_blocks_ = 0;
 	int mfma_pitch_ = 0, mfma_kch_ = 0;
 	int fft_cb_ = 8;
-	bool render_fp64_ = true;
+	bool render_fp64_ = false;
 	int kc_max_ = 64;
 	bool timing_ = false;
 	EngineStats stats_;

@@ -13,9 +13,10 @@
 //   k_fft_rows_inv  two half-spectrum rows packed as one complex inverse FFT,
 //                   with the epilogue of the brute-force path: fused
 //                   likelihood, masked model image, or the adjoint image G
-// Every transform runs in LDS: radix-2 decimation in time from bit-reversed
-// input for the forward transforms, decimation in frequency to bit-reversed
-// output for the inverse ones, twiddles staged in LDS.  FP64 keeps the error
+// Every transform runs in LDS: decimation in time from bit-reversed input for
+// the forward transforms, decimation in frequency to bit-reversed output for
+// the inverse ones, three radix-2 stages per pass in registers, twiddles
+// staged in LDS.  FP64 keeps the error
 // relative to the image maximum near 1e-15, so faint pixels keep their
 // relative accuracy.
 #include <hip/hip_runtime.h>
@@ -57,44 +58,100 @@ __device__ inline void stage_twiddles(double2 *w, int n, const double2 *__restri
 	for (int t = threadIdx.x; t < n / 2; t += blockDim.x) w[t] = tw[t * step];
 }
 
-// In-place forward radix-2 DIT of `ncol` sequences of length n (sequence c at
-// data + c*stride), bit-reversed input, natural-order output.
-__device__ inline void fft_dit(double2 *data, int n, int logn, int ncol, int stride, const double2 *w)
+// In-place forward DIT of `ncol` sequences of length n (sequence c at
+// data + c*stride), bit-reversed input, natural-order output.  The log2(n)
+// radix-2 stages run in passes of up to three: each thread holds the eight
+// (four, two) elements that the pass's stages combine in registers, so LDS is
+// read and written once per pass instead of once per stage.  Stage S
+// combines i and i + 2^(S-1) with twiddle w[(i mod 2^(S-1)) * n / 2^S].
+template <int K>
+__device__ __forceinline__ void dit_pass(double2 *data, int logn, int ncol, int stride, int s, const double2 *w)
 {
-	const int half_total = ncol * (n >> 1);
-	for (int s = 1; s <= logn; ++s) {
-		const int half = 1 << (s - 1);
-		const int shift = logn - s;  // twiddle index = k * (n / len)
-		for (int b = threadIdx.x; b < half_total; b += blockDim.x) {
-			const int c = b >> (logn - 1), bb = b - (c << (logn - 1));
-			const int grp = bb >> (s - 1), k = bb - (grp << (s - 1));
-			const int i = c * stride + (grp << s) + k, j = i + half;
-			const double2 u = data[i], v = cmul(data[j], w[k << shift]);
-			data[i] = cadd(u, v);
-			data[j] = csub(u, v);
+	constexpr int R = 1 << K;
+	const int sub = 1 << s;                     // 2^s: element spacing inside a group
+	const int groups_per_col = 1 << (logn - K);  // n / R
+	const int total = ncol * groups_per_col;
+	for (int gi = threadIdx.x; gi < total; gi += blockDim.x) {
+		const int c = gi >> (logn - K), g = gi - (c << (logn - K));
+		const int lo = g & (sub - 1), hi = g >> s;
+		double2 *base = data + c * stride + (hi << (s + K)) + lo;
+		double2 x[R];
+#pragma unroll
+		for (int j = 0; j < R; ++j) x[j] = base[j << s];
+#pragma unroll
+		for (int t = 1; t <= K; ++t) {
+			const int h = 1 << (t - 1);
+			const int S = s + t;
+#pragma unroll
+			for (int j = 0; j < R; ++j) {
+				if (j & h) continue;
+				const int kS = ((j & (h - 1)) << s) + lo;  // index within the stage's half block
+				const double2 u = x[j], v = cmul(x[j + h], w[kS << (logn - S)]);
+				x[j] = cadd(u, v);
+				x[j + h] = csub(u, v);
+			}
 		}
-		__syncthreads();
+#pragma unroll
+		for (int j = 0; j < R; ++j) base[j << s] = x[j];
 	}
+	__syncthreads();
 }
 
-// In-place inverse (conjugate twiddles, unscaled) radix-2 DIF: natural-order
-// input, bit-reversed output.
+__device__ inline void fft_dit(double2 *data, int n, int logn, int ncol, int stride, const double2 *w)
+{
+	(void)n;
+	int s = 0;
+	for (; s + 3 <= logn; s += 3) dit_pass<3>(data, logn, ncol, stride, s, w);
+	if (logn - s == 2) dit_pass<2>(data, logn, ncol, stride, s, w);
+	else if (logn - s == 1) dit_pass<1>(data, logn, ncol, stride, s, w);
+}
+
+// In-place inverse (conjugate twiddles, unscaled) DIF: natural-order input,
+// bit-reversed output.  Stage S (from log2(n) down to 1) maps (u, v) at i and
+// i + 2^(S-1) to (u + v, (u - v) conj w[(i mod 2^(S-1)) * n / 2^S]); passes of
+// up to three stages as in the forward transform.
+template <int K>
+__device__ __forceinline__ void dif_pass(double2 *data, int logn, int ncol, int stride, int top, const double2 *w)
+{
+	// stages top, top-1, ..., top-K+1 combine indices differing in bits
+	// top-1 .. top-K, i.e. spacing 2^(top-K)
+	constexpr int R = 1 << K;
+	const int s = top - K;
+	const int sub = 1 << s;
+	const int total = ncol * (1 << (logn - K));
+	for (int gi = threadIdx.x; gi < total; gi += blockDim.x) {
+		const int c = gi >> (logn - K), g = gi - (c << (logn - K));
+		const int lo = g & (sub - 1), hi = g >> s;
+		double2 *base = data + c * stride + (hi << top) + lo;
+		double2 x[R];
+#pragma unroll
+		for (int j = 0; j < R; ++j) x[j] = base[j << s];
+#pragma unroll
+		for (int t = K; t >= 1; --t) {
+			const int h = 1 << (t - 1);
+			const int S = s + t;
+#pragma unroll
+			for (int j = 0; j < R; ++j) {
+				if (j & h) continue;
+				const int kS = ((j & (h - 1)) << s) + lo;
+				const double2 u = x[j], v = x[j + h];
+				x[j] = cadd(u, v);
+				x[j + h] = cmul(csub(u, v), conjg(w[kS << (logn - S)]));
+			}
+		}
+#pragma unroll
+		for (int j = 0; j < R; ++j) base[j << s] = x[j];
+	}
+	__syncthreads();
+}
+
 __device__ inline void ifft_dif(double2 *data, int n, int logn, int ncol, int stride, const double2 *w)
 {
-	const int half_total = ncol * (n >> 1);
-	for (int s = logn; s >= 1; --s) {
-		const int half = 1 << (s - 1);
-		const int shift = logn - s;
-		for (int b = threadIdx.x; b < half_total; b += blockDim.x) {
-			const int c = b >> (logn - 1), bb = b - (c << (logn - 1));
-			const int grp = bb >> (s - 1), k = bb - (grp << (s - 1));
-			const int i = c * stride + (grp << s) + k, j = i + half;
-			const double2 u = data[i], v = data[j];
-			data[i] = cadd(u, v);
-			data[j] = cmul(csub(u, v), conjg(w[k << shift]));
-		}
-		__syncthreads();
-	}
+	(void)n;
+	int top = logn;
+	for (; top - 3 >= 0; top -= 3) dif_pass<3>(data, logn, ncol, stride, top, w);
+	if (top == 2) dif_pass<2>(data, logn, ncol, stride, top, w);
+	else if (top == 1) dif_pass<1>(data, logn, ncol, stride, top, w);
 }
 
 }  // namespace

@@ -94,16 +94,38 @@ static void run_case(int W, int H, int K, int pad)
 	};
 	hip_cpu::launch(n2 / 2, 256, [&] { k_fft_psf_rows(gals, tw, twn, src, spec); });
 	cols(0, 2, 0, spec);
+	// the PSF frame is a unit impulse at the origin, so every convolution is
+	// the identity: model = canvas, adjoint = w placed at (padx, pady)
+	unsigned st = 12345u;
+	auto rnd = [&] {
+		st = st * 1664525u + 1013904223u;
+		return (float)((st >> 8) & 0xffff) / 65536.f;
+	};
+	for (int i = 0; i < g.DW * g.DH; ++i) insts[0].canvas[i] = rnd();
+	for (int i = 0; i < W * H; ++i) insts[0].w[i] = rnd();
+	double err = 0;
 	rows_fwd(0, (g.DH + 1) / 2);
 	cols(1, 0, 1, nullptr);
 	rows_inv(1, (g.DH + 1) / 2);
+	for (int i = 0; i < g.DW * g.DH; ++i) err = std::fmax(err, std::fabs(insts[0].G[i] - insts[0].canvas[i]));
 	rows_fwd(0, (g.DH + 1) / 2);
 	cols(1, 0, 0, nullptr);
 	rows_inv(0, (H + 1) / 2);
+	for (int y = 0; y < H; ++y)
+		for (int x = 0; x < W; ++x)
+			err = std::fmax(err, std::fabs(insts[0].w[y * W + x] + insts[0].canvas[(y + pad) * g.DW + x + pad]));
+	for (int i = 0; i < W * H; ++i) insts[0].w[i] = rnd();
 	rows_fwd(1, (H + 1) / 2);
 	cols(2, 1, 1, nullptr);
 	rows_inv(2, (g.DH + 1) / 2);
-	std::printf("ok W=%d H=%d K=%d pad=%d frame=%dx%d\n", W, H, K, pad, n1, n2);
+	for (int y = 0; y < g.DH; ++y)
+		for (int x = 0; x < g.DW; ++x) {
+			const bool in = y >= pad && y < pad + H && x >= pad && x < pad + W;
+			const float ref = in ? insts[0].w[(y - pad) * W + x - pad] : 0.f;
+			err = std::fmax(err, std::fabs(insts[0].G[y * g.DW + x] - ref));
+		}
+	std::printf("%s W=%d H=%d K=%d pad=%d frame=%dx%d max err %.3e\n", err < 1e-5 ? "ok" : "BAD", W, H, K, pad, n1,
+	            n2, err);
 }
 
 int main()

@@ -61,8 +61,9 @@ def test_gradient_at_perturbed_starts_vs_fd_and_oracle():
         steps = np.full((1, O.NGRAD), 1e-4)
         steps[:, :2] = 1e-7
         _, ofd = O.loglike_grad_fd(fit, oracle_profiles(comps), rel_step=steps)
-        # far from the optimum |logL| ~ 1e6: the gate scales with it (FP32 canvas)
-        assert abs(ll[c] - oll) <= max(1e-3, 1e-7 * abs(oll))
+        # far from the optimum |logL| ~ 1e6: the gate scales with it (FP32
+        # profile values and canvas, ~1e-7 relative)
+        assert abs(ll[c] - oll) <= max(1e-3, 5e-7 * abs(oll))
         big = np.abs(g[c]).max()
         for j, n in enumerate(FREE):
             s = H.grad_slot("sersic", n)
