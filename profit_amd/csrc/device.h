@@ -70,6 +70,9 @@ struct InstDev {
 	float *G;              // DW*DH  correlated gradient image / convolved model
 	float *aux;            // DW*DH  profiles drawn without convolution (or null)
 	double2 *fbuf;         // N2 x N1 FFT frame (FFT path only)
+	// per component, canvas row and 64-pixel tile column: bit set where the
+	// forward render evaluated the pixel directly (ncomp x DH x ceil(DW/64))
+	uint64_t *dmask;
 };
 
 struct QEntry {

@@ -694,13 +694,16 @@ void Engine::set_batch(const std::vector<int> &inst_galaxy, const std::vector<in
 		const size_t ndraw = (size_t)g.DW * g.DH, npix = (size_t)g.W * g.H;
 		DevBuf canvas(ndraw * 4), w(npix * 4), G(ndraw * 4), aux(ndraw * 4), fbuf;
 		if (g.conv_kind == CK_FFT) fbuf.alloc((size_t)(g.fft_n1 / 2 + 1) * g.fft_n2 * sizeof(double2));
+		DevBuf dmask(std::max<size_t>(8, (size_t)inst_ncomp[i] * g.DH * ((g.DW + 63) / 64) * sizeof(uint64_t)));
 		insts[i] = InstDev{gi,          inst_comp0_[i], inst_ncomp[i],   0,
-		                   canvas.as<float>(), w.as<float>(), G.as<float>(), aux.as<float>(), fbuf.as<double2>()};
+		                   canvas.as<float>(), w.as<float>(), G.as<float>(), aux.as<float>(), fbuf.as<double2>(),
+		                   dmask.as<uint64_t>()};
 		inst_bufs_.push_back(std::move(canvas));
 		inst_bufs_.push_back(std::move(w));
 		inst_bufs_.push_back(std::move(G));
 		inst_bufs_.push_back(std::move(aux));
 		inst_bufs_.push_back(std::move(fbuf));
+		inst_bufs_.push_back(std::move(dmask));
 		qneed += (long)inst_ncomp[i] * (long)std::min<size_t>(ndraw, 4096);
 		// centre pixels: within one image pixel of the centre, (2 fs)^2 at most
 		const long fs = std::lround(g.img_sx / g.sx);
@@ -995,7 +998,7 @@ void Engine::model_images(const ProfileParams *params, float *out)
 	for (int i = 0; i < n; ++i) {
 		const GalaxyDev &g = gal_host_[inst_galaxy_[i]];
 		std::vector<float> canvas((size_t)g.DW * g.DH);
-		const float *src = inst_bufs_[5 * i + 2].as<float>();
+		const float *src = inst_bufs_[kInstBufs * i + 2].as<float>();
 		hip_check(hipMemcpy(canvas.data(), src, canvas.size() * 4, hipMemcpyDeviceToHost), "model");
 		for (int j = 0; j < g.H; ++j)
 			for (int k = 0; k < g.W; ++k) out[off + (size_t)j * g.W + k] = canvas[(size_t)(j + g.pady) * g.DW + k + g.padx];
@@ -1146,7 +1149,7 @@ void convolve_image(const double *src, int w, int h, const double *krn, int kw, 
 	g.atiles = d_tiles.as<int2>();
 	g.n_atiles = (int)tiles.size();
 	g.conv_rows = rows;
-	InstDev I{0, 0, 0, 0, d_src.as<float>(), nullptr, d_out.as<float>(), nullptr};
+	InstDev I{0, 0, 0, 0, d_src.as<float>(), nullptr, d_out.as<float>(), nullptr, nullptr, nullptr};
 	const int off[2] = {0, (int)tiles.size()};
 	hip_check(hipMemcpy(d_gal.as<GalaxyDev>(), &g, sizeof(g), hipMemcpyHostToDevice), "gal");
 	hip_check(hipMemcpy(d_inst.as<InstDev>(), &I, sizeof(I), hipMemcpyHostToDevice), "inst");

@@ -253,6 +253,8 @@ class Engine {
 	int n_ct_blocks_ = 0, n_at_blocks_ = 0;
 	int kx_max_ = 0;
 	DevBuf d_insts_, d_ct_off_, d_at_off_, d_comp_inst_;
+	// per instance: canvas, w, G, aux, FFT frame, direct-pixel mask
+	static constexpr int kInstBufs = 6;
 	std::vector<DevBuf> inst_bufs_;
 	DevBuf d_params_, d_comps_, d_err_, d_queue_, d_qcount_, d_partial_, d_ll_, d_gscale_, d_grad_acc_, d_grad_out_;
 	long qcap_ = 0, hcap_ = 0;

@@ -164,6 +164,10 @@ __global__ __launch_bounds__(256) void k_render(const GalaxyDev *__restrict__ ga
 				}
 			}
 			if (diag_cls && inb) diag_cls[(long)(I.comp0 + k) * g.DW * g.DH + pix] = cls;
+			// direct-pixel mask for the adjoint render
+			const unsigned long long dm = __ballot(cls == 2);
+			if (I.dmask && lane == 0 && j < g.DH)
+				I.dmask[((long)k * g.DH + j) * ((g.DW + 63) >> 6) + t.x] = dm;
 			append_entry(sub, lane, wq.light, wq.count, wq.lcap, QEntry{I.comp0 + k, pix});
 			append_entry(heavy, lane, wq.heavy, wq.count + 1, wq.hcap, QEntry{I.comp0 + k, pix});
 		}
@@ -747,6 +751,30 @@ __global__ void k_like_finalize(const GalaxyDev *__restrict__ gals, const InstDe
 
 // ------------------------------------------------------------------ adjoint render
 
+// d(f)/d slot of one direct pixel in FP32, chain rule to the component slots,
+// accumulated with weight wgt into FP32 per-thread sums (a thread's few
+// pixels); returns f.
+__device__ __forceinline__ float pixel_accumulate_f32(const Comp &c, float xp, float yp, float wgt, float *gs)
+{
+	float o[9];
+	sample_grad<float>(c, xp, yp, o);
+	const float q = (float)c.axrat;
+	const float ca = (float)c.cos_ang, sa = (float)c.sin_ang;
+	const float fx = o[1], fy = o[2];
+	gs[G_XCEN] += wgt * (fx * -ca + fy * (sa / q));
+	gs[G_YCEN] += wgt * (fx * -sa + fy * (-ca / q));
+	gs[G_ANG] += wgt * ((fx * (q * yp) + fy * (-xp / q)) * (3.14159265358979323846f / 180.f));
+	gs[G_AXRAT] += wgt * (fy * (-yp / q));
+	gs[G_BOX] += wgt * o[3];
+#pragma unroll
+	for (int s = 0; s < 5; ++s) gs[G_S0 + s] += wgt * o[4 + s];
+	return o[0];
+}
+
+// Adjoint of the directly evaluated pixels: sum_q G(q) fs d f(q) / d slot over
+// the pixels the forward render marked direct (so no classification is
+// repeated), per-thread sums in FP32 (FP64 for steep cores), the flux-scale
+// term applied once per component from sum_q G(q) fs f(q).
 __global__ __launch_bounds__(256) void k_adj_render(const GalaxyDev *__restrict__ gals,
                                                     const InstDev *__restrict__ insts,
                                                     const int *__restrict__ tile_off, int ninst,
@@ -759,35 +787,63 @@ __global__ __launch_bounds__(256) void k_adj_render(const GalaxyDev *__restrict_
 	const GalaxyDev &g = gals[I.galaxy];
 	const int rows = g.conv_rows;
 	const int2 t = g.atiles[b - tile_off[inst]];
-	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+	const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 	const int i = t.x * kTileW + lane;
 	const int j0 = t.y * 4 * rows + wv * rows;
+	const int wpr = (g.DW + 63) >> 6;
 	const double hx = g.sx / 2., hy = g.sy / 2.;
+	const double x = hx + i * g.sy;
 	for (int k = 0; k < I.ncomp; ++k) {
 		const Comp &c = comps[I.comp0 + k];
 		double gl[NGRAD];
 #pragma unroll
 		for (int s = 0; s < NGRAD; ++s) gl[s] = 0;
-		for (int r = 0; r < rows; ++r) {
-			const int j = j0 + r;
-			if (i >= g.DW || j >= g.DH) continue;
-			const int pix = i + j * g.DW;
-			if (g.has_calc && !g.calc[pix]) continue;
-			const float Gq = I.G[pix];
-			if (Gq == 0.f) continue;
-			const double y = hy + j * g.sx, x = hx + i * g.sy;
-			const PixelClass pc = classify(c, x, y);
-			if (pc.cls != 2) continue;
-			const double wq = (double)Gq * c.flux_scale;
-			// per-sample derivatives in FP32 (FP64 for steep cores)
-			const double f = sample_accumulate_any(c, pc.xp, pc.yp, wq, gl);
-			// flux-scale dependence: d fs / d slot = fs * dlnfs
-#pragma unroll
-			for (int s = 0; s < NGRAD; ++s) gl[s] += wq * f * c.dlnfs[s];
+		double vsum = 0;
+		if (c.grad64) {
+			for (int r = 0; r < rows; ++r) {
+				const int j = j0 + r;
+				if (j >= g.DH) break;
+				const unsigned long long m = I.dmask[((long)k * g.DH + j) * wpr + t.x];
+				if (!((m >> lane) & 1ull)) continue;
+				const float Gq = I.G[i + j * g.DW];
+				if (Gq == 0.f) continue;
+				const double y = hy + j * g.sx;
+				const double xr = x - c.xc, yr = y - c.yc;
+				const double xp = xr * c.cos_ang + yr * c.sin_ang;
+				const double yp = -xr * c.sin_over + yr * c.cos_over;
+				const double wq = (double)Gq * c.flux_scale;
+				vsum += wq * sample_accumulate<double>(c, xp, yp, wq, gl);
+			}
 		}
+		else {
+			float gf[NGRAD];
+#pragma unroll
+			for (int s = 0; s < NGRAD; ++s) gf[s] = 0.f;
+			float vf = 0.f;
+			const float fs = (float)c.flux_scale;
+			for (int r = 0; r < rows; ++r) {
+				const int j = j0 + r;
+				if (j >= g.DH) break;
+				const unsigned long long m = I.dmask[((long)k * g.DH + j) * wpr + t.x];
+				if (!m) continue;
+				if (!((m >> lane) & 1ull)) continue;
+				const float Gq = I.G[i + j * g.DW];
+				if (Gq == 0.f) continue;
+				const double y = hy + j * g.sx;
+				const double xr = x - c.xc, yr = y - c.yc;
+				const float xp = (float)(xr * c.cos_ang + yr * c.sin_ang);
+				const float yp = (float)(-xr * c.sin_over + yr * c.cos_over);
+				const float wq = Gq * fs;
+				vf += wq * pixel_accumulate_f32(c, xp, yp, wq, gf);
+			}
+#pragma unroll
+			for (int s = 0; s < NGRAD; ++s) gl[s] = (double)gf[s];
+			vsum = (double)vf;
+		}
+		// flux-scale dependence: d fs / d slot = fs * dlnfs
 #pragma unroll
 		for (int s = 0; s < NGRAD; ++s) {
-			const double v = wave_sum(gl[s]);
+			const double v = wave_sum(gl[s] + vsum * c.dlnfs[s]);
 			if (lane == 0) red[wv][s] = v;
 		}
 		__syncthreads();
