@@ -1420,15 +1420,42 @@ int or_loglike_image(int likefunc, const double *data, const double *sigma,
 			}
 		var /= (double)(N - 1);
 		double dof = 2 * var / (var - 1);
-		if (!(dof > 1)) dof = 1; /* max(1, min(Inf, dof)) */
-		double s = 0;
+		/* max(1, min(Inf, dof, na.rm=TRUE)): NaN (N < 2) gives Inf, below 1 gives 1 */
+		int clamped = 0;
+		if (N < 2 || isnan(dof)) dof = INFINITY;
+		else if (!(dof > 1)) {
+			dof = 1;
+			clamped = 1;
+		}
+		double s = 0, sl = 0, sc = 0;
 		for (unsigned t = 0; t < n; t++) {
 			if (region && !region[t]) continue;
 			double z = (data[t] - model[t]) / sigma[t];
 			if (isinf(dof)) s += -(ln_sqrt_2pi + 0.5 * z * z);
-			else s += lgamma((dof + 1) / 2) - lgamma(dof / 2) - 0.5 * log(dof * M_PI) - (dof + 1) / 2 * log1p(z * z / dof);
+			else {
+				s += lgamma((dof + 1) / 2) - lgamma(dof / 2) - 0.5 * log(dof * M_PI) - (dof + 1) / 2 * log1p(z * z / dof);
+				sl += log1p(z * z / dof);
+				sc += z * z / (dof * (dof + z * z));
+			}
 		}
 		*ll = s;
+		if (dll) {
+			/* d LL / d z_t at fixed dof, plus the dof's dependence on z through
+			 * the sample variance: dof = 2 v / (v - 1), dv/dz_t = 2 (z_t - mean) / (N - 1) */
+			double kdof = 0;
+			if (!isinf(dof) && !clamped) {
+				double dLdnu = (double)N * (0.5 * or_digamma((dof + 1) / 2) - 0.5 * or_digamma(dof / 2) - 0.5 / dof) -
+				               0.5 * sl + 0.5 * (dof + 1) * sc;
+				double dnudv = -2 / ((var - 1) * (var - 1));
+				kdof = dLdnu * dnudv * 2 / (double)(N - 1);
+			}
+			for (unsigned t = 0; t < n; t++) {
+				if (region && !region[t]) continue;
+				double z = (data[t] - model[t]) / sigma[t];
+				double dz = isinf(dof) ? -z : -(dof + 1) * z / (dof + z * z) + kdof * (z - mean);
+				dll[t] = -dz / sigma[t];
+			}
+		}
 		return 0;
 	}
 	if (likefunc == OR_LIKE_POIS) {
@@ -1486,7 +1513,6 @@ static double *slot_ptr(or_profile *p, int slot)
 static int check_grad_support(const or_fit *fit, const or_profile *profiles, int nprof)
 {
 	if (fit->model.finesampling != 1) return fail("gradient supports finesampling = 1 only");
-	if (fit->likefunc == OR_LIKE_T) return fail("gradient of the t likelihood is not supported");
 	for (int k = 0; k < nprof; k++) {
 		int t = profiles[k].type;
 		if (!(t == OR_SERSIC || t == OR_MOFFAT || t == OR_FERRER || t == OR_KING))

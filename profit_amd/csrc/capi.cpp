@@ -259,8 +259,6 @@ int profit_amd_engine_add_galaxy(profit_amd_engine *engine, const profit_amd_gal
 		s.data = g->data;
 		s.sigma = g->sigma;
 		s.likefunc = g->likefunc;
-		if (s.likefunc == PROFIT_AMD_LIKE_T)
-			throw profit::invalid_parameter("the t likelihood is not available in the batched engine");
 		if (s.likefunc < 0 || s.likefunc > 3) throw profit::invalid_parameter("unknown likelihood function");
 		const int id = engine->eng.add_galaxy(s);
 		if (galaxy_id) *galaxy_id = id;

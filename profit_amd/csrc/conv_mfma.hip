@@ -200,7 +200,7 @@ __global__ __launch_bounds__(256) void k_conv_mfma(const GalaxyDev *__restrict__
 	// outputs: block q = 2*yb + xb, item r -> x = r32, y = (r&3) + 8*(r>>2) + 4*g32
 	if (MODE == CONV_LIKE) {
 		double part = 0;
-		const bool pois = g.likefunc == LIKE_POIS;
+		const bool pois = g.likefunc == LIKE_POIS, tl = g.likefunc == LIKE_T;
 		const double s = g.pois_scale;
 #pragma unroll
 		for (int q = 0; q < 4; ++q) {
@@ -225,8 +225,10 @@ __global__ __launch_bounds__(256) void k_conv_mfma(const GalaxyDev *__restrict__
 							const double iv = g.ivar[o];
 							part += diff * diff * iv;
 							wvv = (float)(diff * iv);
+							// t: the standardised residual, for k_like_t
+							if (tl) wvv = (float)(diff * sqrt(iv));
 						}
-						if (!in_calc) wvv = 0.f;
+						if (!in_calc && !tl) wvv = 0.f;
 					}
 					I.w[o] = wvv;
 				}

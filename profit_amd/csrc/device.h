@@ -23,6 +23,11 @@ enum ConvMode : int { CONV_LIKE = 0, CONV_MODEL = 1, CONV_ADJ = 2 };
 // convolution kernel of a galaxy
 enum ConvKind { CK_VALU = 0, CK_FFT = 1, CK_MFMA = 2 };
 
+// Row pitch (in complex entries) of an FFT half-spectrum frame of width
+// n1 / 2 + 1: rounded up to 8 entries (128 B) so that a column block's
+// eight adjacent entries of a row fill one cache line.
+__host__ __device__ constexpr int fft_pitch(int n1) { return ((n1 / 2 + 1) + 7) & ~7; }
+
 // One galaxy: image geometry, PSF, masks and data.  Several instances (parameter
 // sets) may share a galaxy.
 struct GalaxyDev {
@@ -47,11 +52,11 @@ struct GalaxyDev {
 	const float *psf_adj;  // PSF as given, weights of the adjoint correlation
 	const uint8_t *calc;   // DW*DH drawing mask (null: every pixel)
 	const uint8_t *region; // W*H likelihood mask (already intersected with calc)
-	const float *data;     // W*H
+	const double *data;    // W*H, FP64: the Poisson term d log m needs the data's own precision
 	const float *ivar;     // W*H, 1/sigma^2
 	int32_t fft_n1, fft_n2;   // FFT frame (powers of two) when the FFT path is on
 	int32_t fft_l1, fft_l2;   // their base-2 logarithms
-	const double2 *fft_spec;  // N2 x N1 spectrum of the PSF, wrapped about the origin
+	const double2 *fft_spec;  // half-spectrum of the PSF (N2 rows at fft_pitch(N1)), wrapped about the origin
 	int32_t mfma_kc;          // 16-wide K chunks of one Toeplitz row block (split-BF16x3 path)
 	int32_t n_mtl, n_mtc;     // 128 x 128 tiles: likelihood (image) and canvas
 	const uint4 *mfrag_fwd;   // B fragments of the forward weights, [ky][kc][hi|lo][lane]

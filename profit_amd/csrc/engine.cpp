@@ -309,7 +309,7 @@ void Engine::ensure_fft_spectra()
 				const int dx = kx - ax, dy = ky - ay;
 				frame[(size_t)((dy + n2) % n2) * n1 + (dx + n1) % n1] = b.psf64[(size_t)ky * g.Kx + kx];
 			}
-		b.fft_spec.alloc((size_t)m1 * n2 * sizeof(double2));
+		b.fft_spec.alloc((size_t)fft_pitch(n1) * n2 * sizeof(double2));
 		DevBuf src(frame.size() * sizeof(double));
 		hip_check(hipMemcpy(src.as<double>(), frame.data(), frame.size() * sizeof(double), hipMemcpyHostToDevice), "psf");
 		g.fft_spec = b.fft_spec.as<double2>();
@@ -504,19 +504,20 @@ int Engine::add_galaxy(const GalaxySpec &s)
 	hip_check(hipMemcpy(bufs.region.as<uint8_t>(), region.data(), npix, hipMemcpyHostToDevice), "region");
 	g.pois_scale = 1;
 	if (s.data && s.sigma) {
-		std::vector<float> d(npix), iv(npix);
+		std::vector<double> d(npix);
+		std::vector<float> iv(npix);
 		double smax = 0;
 		for (size_t k = 0; k < npix; ++k) {
-			d[k] = (float)s.data[k];
+			d[k] = s.data[k];
 			const double sg = s.sigma[k];
 			iv[k] = (float)(1.0 / (sg * sg));
 			if (region[k]) smax = std::max(smax, std::fabs(s.data[k]) / (std::fabs(sg) * std::fabs(sg)));
 		}
 		// R/profitLikeModel.R:324-328
 		if (smax < 0.1 || smax > 10) g.pois_scale = smax;
-		bufs.data.alloc(npix * 4);
+		bufs.data.alloc(npix * sizeof(double));
 		bufs.ivar.alloc(npix * 4);
-		hip_check(hipMemcpy(bufs.data.as<float>(), d.data(), npix * 4, hipMemcpyHostToDevice), "data");
+		hip_check(hipMemcpy(bufs.data.as<double>(), d.data(), npix * sizeof(double), hipMemcpyHostToDevice), "data");
 		hip_check(hipMemcpy(bufs.ivar.as<float>(), iv.data(), npix * 4, hipMemcpyHostToDevice), "ivar");
 	}
 	g.psf_fwd = bufs.psf_fwd.as<float>();
@@ -525,7 +526,7 @@ int Engine::add_galaxy(const GalaxySpec &s)
 	g.mfrag_adj = bufs.mfrag_adj.as<uint4>();
 	g.calc = bufs.calc.as<uint8_t>();
 	g.region = bufs.region.as<uint8_t>();
-	g.data = bufs.data.as<float>();
+	g.data = bufs.data.as<double>();
 	g.ivar = bufs.ivar.as<float>();
 	gal_host_.push_back(g);
 	gal_bufs_.push_back(std::move(bufs));
@@ -675,6 +676,7 @@ void Engine::set_batch(const std::vector<int> &inst_galaxy, const std::vector<in
 	ct_off_.assign(n + 1, 0);
 	at_off_.assign(n + 1, 0);
 	ncomp_total_ = 0;
+	any_t_ = false;
 	kx_max_ = 0;
 	long qneed = 1024, hneed = 64;
 	inst_bufs_.clear();
@@ -685,6 +687,7 @@ void Engine::set_batch(const std::vector<int> &inst_galaxy, const std::vector<in
 		if (gi < 0 || gi >= (int)gal_host_.size()) throw std::invalid_argument("unknown galaxy id");
 		if (inst_ncomp[i] < 0) throw std::invalid_argument("negative component count");
 		const GalaxyDev &g = gal_host_[gi];
+		any_t_ |= g.likefunc == LIKE_T;
 		inst_comp0_[i] = ncomp_total_;
 		ncomp_total_ += inst_ncomp[i];
 		for (int k = 0; k < inst_ncomp[i]; ++k) comp_inst.push_back(i);
@@ -693,7 +696,7 @@ void Engine::set_batch(const std::vector<int> &inst_galaxy, const std::vector<in
 		if (g.conv_kind == CK_VALU) kx_max_ = std::max(kx_max_, g.Kx);
 		const size_t ndraw = (size_t)g.DW * g.DH, npix = (size_t)g.W * g.H;
 		DevBuf canvas(ndraw * 4), w(npix * 4), G(ndraw * 4), aux(ndraw * 4), fbuf;
-		if (g.conv_kind == CK_FFT) fbuf.alloc((size_t)(g.fft_n1 / 2 + 1) * g.fft_n2 * sizeof(double2));
+		if (g.conv_kind == CK_FFT) fbuf.alloc((size_t)fft_pitch(g.fft_n1) * g.fft_n2 * sizeof(double2));
 		DevBuf dmask(std::max<size_t>(8, (size_t)inst_ncomp[i] * g.DH * ((g.DW + 63) / 64) * sizeof(uint64_t)));
 		insts[i] = InstDev{gi,          inst_comp0_[i], inst_ncomp[i],   0,
 		                   canvas.as<float>(), w.as<float>(), G.as<float>(), aux.as<float>(), fbuf.as<double2>(),
@@ -757,6 +760,12 @@ void Engine::set_batch(const std::vector<int> &inst_galaxy, const std::vector<in
 		ri1_off_[i + 1] = ri1_off_[i] + (f ? (g.DH + 1) / 2 : 0);
 	}
 	n_bt_blocks_ = bt_off_[n];
+	// the adjoint render gives each thread about 32 canvas pixels of a component
+	adj_grp_ = std::max(1, 32 / conv_rows_);
+	ag_off_.assign(n + 1, 0);
+	for (int i = 0; i < n; ++i)
+		ag_off_[i + 1] = ag_off_[i] + (gal_host_[inst_galaxy[i]].n_atiles + adj_grp_ - 1) / adj_grp_;
+	n_ag_blocks_ = ag_off_[n];
 	auto upload_off = [&](DevBuf &d, const std::vector<int> &h) {
 		d.alloc((n + 1) * sizeof(int));
 		hip_check(hipMemcpy(d.as<int>(), h.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice), "off");
@@ -770,6 +779,7 @@ void Engine::set_batch(const std::vector<int> &inst_galaxy, const std::vector<in
 	upload_off(d_col_off_, col_off_);
 	upload_off(d_ri0_off_, ri0_off_);
 	upload_off(d_ri1_off_, ri1_off_);
+	upload_off(d_ag_off_, ag_off_);
 	d_insts_.alloc(std::max(1, n) * sizeof(InstDev));
 	if (n) hip_check(hipMemcpy(d_insts_.as<InstDev>(), insts.data(), n * sizeof(InstDev), hipMemcpyHostToDevice), "insts");
 	d_ct_off_.alloc((n + 1) * sizeof(int));
@@ -833,11 +843,13 @@ void Engine::record(bool grad)
 	hip_check(launch_like_finalize(gals, insts, d_lk_off_.as<int>(), n, d_partial_.as<double>(), d_ll_.as<double>(),
 	                               d_gscale_.as<double>(), st),
 	          "like");
+	if (any_t_)
+		hip_check(launch_like_t(gals, insts, n, grad ? 1 : 0, d_ll_.as<double>(), d_gscale_.as<double>(), st), "t like");
 	mark(5);
 	if (grad) {
 		conv_adjoint(st);
 		mark(6);
-		hip_check(launch_adj_render(gals, insts, d_at_off_.as<int>(), n, n_at_blocks_, d_comps_.as<Comp>(),
+		hip_check(launch_adj_render(gals, insts, d_ag_off_.as<int>(), n, n_ag_blocks_, adj_grp_, d_comps_.as<Comp>(),
 		                            d_grad_acc_.as<double>(), st),
 		          "adjoint render");
 		mark(7);

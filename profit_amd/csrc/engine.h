@@ -28,7 +28,9 @@ int launch_conv(int mode, int rows, int nblocks, int kx_max, int kc, const Galax
                 const int *tile_off, int ninst, const int *lk_base, double *partial, hipStream_t st);
 int launch_like_finalize(const GalaxyDev *gals, const InstDev *insts, const int *lk_off, int ninst,
                          const double *partial, double *ll, double *gscale, hipStream_t st);
-int launch_adj_render(const GalaxyDev *gals, const InstDev *insts, const int *tile_off, int ninst, int nblocks,
+int launch_like_t(const GalaxyDev *gals, const InstDev *insts, int ninst, int grad, double *ll, double *gscale,
+                  hipStream_t st);
+int launch_adj_render(const GalaxyDev *gals, const InstDev *insts, const int *grp_off, int ninst, int nblocks, int grp,
                       const Comp *comps, double *grad_acc, hipStream_t st);
 int launch_grad_finalize(const Comp *comps, const double *gscale, const double *grad_acc, double *grad_out,
                          int ncomp, hipStream_t st);
@@ -249,10 +251,12 @@ class Engine {
 
 	std::vector<int> inst_galaxy_, inst_ncomp_, inst_comp0_;
 	int ncomp_total_ = 0;
-	std::vector<int> ct_off_, at_off_;
-	int n_ct_blocks_ = 0, n_at_blocks_ = 0;
+	bool any_t_ = false;  // an instance of the batch uses the t likelihood
+	std::vector<int> ct_off_, at_off_, ag_off_;
+	int n_ct_blocks_ = 0, n_at_blocks_ = 0, n_ag_blocks_ = 0;
+	int adj_grp_ = 1;  // canvas tiles per block of the adjoint render
 	int kx_max_ = 0;
-	DevBuf d_insts_, d_ct_off_, d_at_off_, d_comp_inst_;
+	DevBuf d_insts_, d_ct_off_, d_at_off_, d_ag_off_, d_comp_inst_;
 	// per instance: canvas, w, G, aux, FFT frame, direct-pixel mask
 	static constexpr int kInstBufs = 6;
 	std::vector<DevBuf> inst_bufs_;

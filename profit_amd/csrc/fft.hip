@@ -158,11 +158,11 @@ __device__ inline void ifft_dif(double2 *data, int n, int logn, int ncol, int st
 
 // Forward transform of one pair of real rows (a = src[r], b = src[r + 1] when
 // `second`), columns [x0, x0 + nx) of the frame holding data, into the
-// half-spectrum rows `outa` and `outa + M`:
+// half-spectrum rows `outa` and `outa + pitch`:
 // A = (Z[k] + conj Z[N-k]) / 2 and B = (Z[k] - conj Z[N-k]) / 2i, k = 0..N/2.
 template <class T>
 __device__ __forceinline__ void rows_fwd_pair(double2 *sh, const double2 *w, int N, int L, const T *src, long ld,
-                                              int x0, int nx, bool second, double2 *outa)
+                                              int x0, int nx, bool second, double2 *outa, int pitch)
 {
 	const int M = N / 2 + 1;
 	for (int t = threadIdx.x; t < N; t += blockDim.x) {
@@ -176,7 +176,7 @@ __device__ __forceinline__ void rows_fwd_pair(double2 *sh, const double2 *w, int
 	}
 	__syncthreads();
 	fft_dit(sh, N, L, 1, N, w);
-	double2 *outb = outa + M;
+	double2 *outb = outa + pitch;
 	for (int k = threadIdx.x; k < M; k += blockDim.x) {
 		const double2 z = sh[k], zc = conjg(sh[(N - k) & (N - 1)]);
 		outa[k] = make_double2(0.5 * (z.x + zc.x), 0.5 * (z.y + zc.y));
@@ -197,7 +197,7 @@ __global__ __launch_bounds__(256) void k_fft_rows_fwd(const GalaxyDev *__restric
 	const int inst = owner(row_off, ninst, b);
 	const InstDev &I = insts[inst];
 	const GalaxyDev &g = gals[I.galaxy];
-	const int N = g.fft_n1, L = g.fft_l1, M = N / 2 + 1;
+	const int N = g.fft_n1, L = g.fft_l1, P = fft_pitch(N);
 	double2 *w = sh + N;
 	stage_twiddles(w, N, tw, twn);
 	const int r = 2 * (b - row_off[inst]);  // first input row of the pair
@@ -208,7 +208,7 @@ __global__ __launch_bounds__(256) void k_fft_rows_fwd(const GalaxyDev *__restric
 	const int x0 = adj ? g.padx : 0;
 	const int y0 = adj ? g.pady : 0;
 	const float *src = (adj ? I.w : I.canvas) + (long)r * width;
-	rows_fwd_pair<float>(sh, w, N, L, src, width, x0, width, r + 1 < nrows, I.fbuf + (long)(y0 + r) * M);
+	rows_fwd_pair<float>(sh, w, N, L, src, width, x0, width, r + 1 < nrows, I.fbuf + (long)(y0 + r) * P, P);
 }
 
 // Forward rows of a galaxy's real FP64 PSF frame (N1 x N2, `src64`) into the
@@ -217,11 +217,11 @@ __global__ __launch_bounds__(256) void k_fft_psf_rows(const GalaxyDev *__restric
                                                       int twn, const double *__restrict__ src64, double2 *dst64)
 {
 	extern __shared__ double2 sh[];
-	const int N = g->fft_n1, L = g->fft_l1, M = N / 2 + 1;
+	const int N = g->fft_n1, L = g->fft_l1, P = fft_pitch(N);
 	double2 *w = sh + N;
 	stage_twiddles(w, N, tw, twn);
 	const int r = 2 * blockIdx.x;
-	rows_fwd_pair<double>(sh, w, N, L, src64 + (long)r * N, N, 0, N, r + 1 < g->fft_n2, dst64 + (long)r * M);
+	rows_fwd_pair<double>(sh, w, N, L, src64 + (long)r * N, N, 0, N, r + 1 < g->fft_n2, dst64 + (long)r * P, P);
 }
 
 // Column pass over CB adjacent half-spectrum columns of one instance.  Frame
@@ -240,7 +240,7 @@ __global__ __launch_bounds__(256) void k_fft_cols(const GalaxyDev *__restrict__ 
 	const int inst = owner(col_off, ninst, b);
 	const InstDev &I = insts[inst];
 	const GalaxyDev &g = gals[I.galaxy];
-	const int M = g.fft_n1 / 2 + 1, N = g.fft_n2, L = g.fft_l2;
+	const int M = g.fft_n1 / 2 + 1, P = fft_pitch(g.fft_n1), N = g.fft_n2, L = g.fft_l2;
 	const int S = N + 1;
 	double2 *w = sh + cb * S;
 	stage_twiddles(w, N, tw, twn);
@@ -252,7 +252,7 @@ __global__ __launch_bounds__(256) void k_fft_cols(const GalaxyDev *__restrict__ 
 	for (int t = threadIdx.x; t < cb * N; t += blockDim.x) {
 		const int y = t / cb, c = t - y * cb;
 		double2 v = make_double2(0.0, 0.0);
-		if (y >= in_lo && y < in_hi && c0 + c < M) v = frame[(long)y * M + c0 + c];
+		if (y >= in_lo && y < in_hi && c0 + c < M) v = frame[(long)y * P + c0 + c];
 		sh[c * S + bitrev(y, L)] = v;
 	}
 	__syncthreads();
@@ -260,14 +260,14 @@ __global__ __launch_bounds__(256) void k_fft_cols(const GalaxyDev *__restrict__ 
 	if (op == 0) {
 		for (int t = threadIdx.x; t < cb * N; t += blockDim.x) {
 			const int y = t / cb, c = t - y * cb;
-			if (c0 + c < M) frame[(long)y * M + c0 + c] = sh[c * S + y];
+			if (c0 + c < M) frame[(long)y * P + c0 + c] = sh[c * S + y];
 		}
 		return;
 	}
 	for (int t = threadIdx.x; t < cb * N; t += blockDim.x) {
 		const int y = t / cb, c = t - y * cb;
 		if (c0 + c < M) {
-			const double2 s = g.fft_spec[(long)y * M + c0 + c];
+			const double2 s = g.fft_spec[(long)y * P + c0 + c];
 			const double2 v = sh[c * S + y];
 			sh[c * S + y] = (op == 1) ? cmul(v, s) : cmulc(v, s);
 		}
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(256) void k_fft_cols(const GalaxyDev *__restrict__ 
 	const int out_hi = out_mode == 0 ? g.pady + g.H : g.DH;
 	for (int t = threadIdx.x; t < cb * N; t += blockDim.x) {
 		const int y = t / cb, c = t - y * cb;
-		if (y >= out_lo && y < out_hi && c0 + c < M) frame[(long)y * M + c0 + c] = sh[c * S + bitrev(y, L)];
+		if (y >= out_lo && y < out_hi && c0 + c < M) frame[(long)y * P + c0 + c] = sh[c * S + bitrev(y, L)];
 	}
 }
 
@@ -310,8 +310,8 @@ __global__ __launch_bounds__(256) void k_fft_rows_inv(const GalaxyDev *__restric
 	const int nrows = mode == 0 ? g.H : g.DH;
 	const int y = mode == 0 ? r + g.pady : r;
 	const bool second = r + 1 < nrows;
-	const double2 *ra = I.fbuf + (long)y * M;
-	const double2 *rb = ra + M;
+	const double2 *ra = I.fbuf + (long)y * fft_pitch(N);
+	const double2 *rb = ra + fft_pitch(N);
 	for (int k = threadIdx.x; k < N; k += blockDim.x) {
 		double2 z;
 		if (k < M) {
@@ -329,7 +329,7 @@ __global__ __launch_bounds__(256) void k_fft_rows_inv(const GalaxyDev *__restric
 	const double scale = 1.0 / ((double)g.fft_n1 * (double)g.fft_n2);
 	if (mode == 0) {
 		double part = 0;
-		const bool pois = g.likefunc == LIKE_POIS;
+		const bool pois = g.likefunc == LIKE_POIS, tl = g.likefunc == LIKE_T;
 		const double s = g.pois_scale;
 		for (int t = threadIdx.x; t < 2 * g.W; t += blockDim.x) {
 			const int row = t >= g.W ? 1 : 0, x = t - row * g.W;
@@ -351,8 +351,10 @@ __global__ __launch_bounds__(256) void k_fft_rows_inv(const GalaxyDev *__restric
 					const double iv = g.ivar[o];
 					part += diff * diff * iv;
 					wv = (float)(diff * iv);
+					// t: the standardised residual, for k_like_t
+					if (tl) wv = (float)(diff * sqrt(iv));
 				}
-				if (!in_calc) wv = 0.f;
+				if (!in_calc && !tl) wv = 0.f;
 			}
 			I.w[o] = wv;
 		}

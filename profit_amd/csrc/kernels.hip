@@ -652,7 +652,7 @@ __global__ __launch_bounds__(256) void k_conv(const GalaxyDev *__restrict__ gals
 	const int oxx = x0 + tx;
 	if (MODE == CONV_LIKE) {
 		double part = 0;
-		const bool pois = g.likefunc == LIKE_POIS;
+		const bool pois = g.likefunc == LIKE_POIS, tl = g.likefunc == LIKE_T;
 		const double s = g.pois_scale;
 #pragma unroll
 		for (int r = 0; r < R; ++r) {
@@ -676,8 +676,10 @@ __global__ __launch_bounds__(256) void k_conv(const GalaxyDev *__restrict__ gals
 						const double iv = g.ivar[o];
 						part += diff * diff * iv;
 						wv = (float)(diff * iv);
+						// t: the standardised residual, for k_like_t
+						if (tl) wv = (float)(diff * sqrt(iv));
 					}
-					if (!in_calc) wv = 0.f;
+					if (!in_calc && !tl) wv = 0.f;
 				}
 				I.w[o] = wv;
 			}
@@ -749,6 +751,93 @@ __global__ void k_like_finalize(const GalaxyDev *__restrict__ gals, const InstDe
 	gscale[i] = gsc;
 }
 
+// Sum over the block (blockDim.x a multiple of 64, at most 1024) in a fixed
+// order; every thread receives the total.
+__device__ double block_sum(double v, double *sh)
+{
+	v = wave_sum(v);
+	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+	__syncthreads();
+	if (lane == 0) sh[wv] = v;
+	__syncthreads();
+	double t = 0;
+	for (int k = 0; k < nw; ++k) t += sh[k];
+	return t;
+}
+
+// Student-t likelihood (R/profitLikeModel.R:281-286), one block per instance.
+// The convolution epilogue has left the standardised residuals
+// z = (data - model) / sigma of the likelihood pixels in w.  The degrees of
+// freedom follow from their sample variance v, dof = max(1, 2v / (v - 1))
+// (infinite, i.e. the normal density, when v = 1 or fewer than two pixels);
+// logL = sum log dt(z, dof).  With `grad`, w becomes d logL / d model,
+// including the dependence of dof on every residual:
+//   d logL / dz = -(dof + 1) z / (dof + z^2) + dL/d dof * d dof/dv * 2 (z - mean) / (N - 1).
+__global__ __launch_bounds__(1024) void k_like_t(const GalaxyDev *__restrict__ gals, const InstDev *__restrict__ insts,
+                                                 int grad, double *ll, double *gscale)
+{
+	__shared__ double sh[16];
+	const int inst = blockIdx.x;
+	const InstDev I = insts[inst];
+	const GalaxyDev &g = gals[I.galaxy];
+	if (g.likefunc != LIKE_T) return;
+	const long n = (long)g.W * g.H;
+	const double N = g.n_region;
+	double a = 0;
+	for (long o = threadIdx.x; o < n; o += blockDim.x)
+		if (g.region[o]) a += (double)I.w[o];
+	const double mean = block_sum(a, sh) / N;
+	a = 0;
+	for (long o = threadIdx.x; o < n; o += blockDim.x)
+		if (g.region[o]) {
+			const double d = (double)I.w[o] - mean;
+			a += d * d;
+		}
+	const double var = block_sum(a, sh) / (N - 1);
+	double dof = 2 * var / (var - 1);
+	bool clamped = false;
+	if (N < 2 || dof != dof) dof = HUGE_VAL;
+	else if (!(dof > 1)) {
+		dof = 1;
+		clamped = true;
+	}
+	const bool normal = sf::is_inf(dof);
+	double sl = 0, sc = 0;
+	for (long o = threadIdx.x; o < n; o += blockDim.x)
+		if (g.region[o]) {
+			const double z = (double)I.w[o];
+			if (normal) sl += z * z;
+			else {
+				sl += log1p(z * z / dof);
+				sc += z * z / (dof * (dof + z * z));
+			}
+		}
+	sl = block_sum(sl, sh);
+	sc = block_sum(sc, sh);
+	double L;
+	if (normal) L = -N * 0.918938533204672741780329736406 - 0.5 * sl;
+	else L = N * (lgamma((dof + 1) / 2) - lgamma(dof / 2) - 0.5 * log(dof * M_PI)) - 0.5 * (dof + 1) * sl;
+	if (threadIdx.x == 0) {
+		ll[inst] = L;
+		gscale[inst] = 1;
+	}
+	if (!grad) return;
+	double kdof = 0;
+	if (!normal && !clamped) {
+		const double dLdnu = N * (0.5 * sf::digamma((dof + 1) / 2) - 0.5 * sf::digamma(dof / 2) - 0.5 / dof) - 0.5 * sl +
+		                     0.5 * (dof + 1) * sc;
+		kdof = dLdnu * (-2 / ((var - 1) * (var - 1))) * 2 / (N - 1);
+	}
+	for (long o = threadIdx.x; o < n; o += blockDim.x) {
+		if (!g.region[o]) continue;
+		const long y = o / g.W, x = o - y * g.W;
+		const bool in_calc = !g.has_calc || g.calc[(y + g.pady) * g.DW + x + g.padx];
+		const double z = (double)I.w[o];
+		const double dz = normal ? -z : -(dof + 1) * z / (dof + z * z) + kdof * (z - mean);
+		I.w[o] = in_calc ? (float)(-dz * sqrt((double)g.ivar[o])) : 0.f;
+	}
+}
+
 // ------------------------------------------------------------------ adjoint render
 
 // d(f)/d slot of one direct pixel in FP32, chain rule to the component slots,
@@ -774,25 +863,25 @@ __device__ __forceinline__ float pixel_accumulate_f32(const Comp &c, float xp, f
 // Adjoint of the directly evaluated pixels: sum_q G(q) fs d f(q) / d slot over
 // the pixels the forward render marked direct (so no classification is
 // repeated), per-thread sums in FP32 (FP64 for steep cores), the flux-scale
-// term applied once per component from sum_q G(q) fs f(q).
+// term applied once per component from sum_q G(q) fs f(q).  A block takes
+// `grp` consecutive canvas tiles of one instance (block offsets per instance
+// in grp_off), so that the per-component reduction and its atomics are paid
+// once per grp tiles.
 __global__ __launch_bounds__(256) void k_adj_render(const GalaxyDev *__restrict__ gals,
                                                     const InstDev *__restrict__ insts,
-                                                    const int *__restrict__ tile_off, int ninst,
+                                                    const int *__restrict__ grp_off, int ninst, int grp,
                                                     const Comp *__restrict__ comps, double *grad_acc)
 {
 	__shared__ double red[4][NGRAD];
 	const int b = blockIdx.x;
-	const int inst = owner_of(tile_off, ninst, b);
+	const int inst = owner_of(grp_off, ninst, b);
 	const InstDev I = insts[inst];
 	const GalaxyDev &g = gals[I.galaxy];
 	const int rows = g.conv_rows;
-	const int2 t = g.atiles[b - tile_off[inst]];
+	const int t0 = (b - grp_off[inst]) * grp, t1 = min(t0 + grp, g.n_atiles);
 	const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-	const int i = t.x * kTileW + lane;
-	const int j0 = t.y * 4 * rows + wv * rows;
 	const int wpr = (g.DW + 63) >> 6;
 	const double hx = g.sx / 2., hy = g.sy / 2.;
-	const double x = hx + i * g.sy;
 	for (int k = 0; k < I.ncomp; ++k) {
 		const Comp &c = comps[I.comp0 + k];
 		double gl[NGRAD];
@@ -800,19 +889,25 @@ __global__ __launch_bounds__(256) void k_adj_render(const GalaxyDev *__restrict_
 		for (int s = 0; s < NGRAD; ++s) gl[s] = 0;
 		double vsum = 0;
 		if (c.grad64) {
-			for (int r = 0; r < rows; ++r) {
-				const int j = j0 + r;
-				if (j >= g.DH) break;
-				const unsigned long long m = I.dmask[((long)k * g.DH + j) * wpr + t.x];
-				if (!((m >> lane) & 1ull)) continue;
-				const float Gq = I.G[i + j * g.DW];
-				if (Gq == 0.f) continue;
-				const double y = hy + j * g.sx;
-				const double xr = x - c.xc, yr = y - c.yc;
-				const double xp = xr * c.cos_ang + yr * c.sin_ang;
-				const double yp = -xr * c.sin_over + yr * c.cos_over;
-				const double wq = (double)Gq * c.flux_scale;
-				vsum += wq * sample_accumulate<double>(c, xp, yp, wq, gl);
+			for (int ti = t0; ti < t1; ++ti) {
+				const int2 t = g.atiles[ti];
+				const int i = t.x * kTileW + lane;
+				const int j0 = t.y * 4 * rows + wv * rows;
+				const double x = hx + i * g.sy;
+				for (int r = 0; r < rows; ++r) {
+					const int j = j0 + r;
+					if (j >= g.DH) break;
+					const unsigned long long m = I.dmask[((long)k * g.DH + j) * wpr + t.x];
+					if (!((m >> lane) & 1ull)) continue;
+					const float Gq = I.G[i + j * g.DW];
+					if (Gq == 0.f) continue;
+					const double y = hy + j * g.sx;
+					const double xr = x - c.xc, yr = y - c.yc;
+					const double xp = xr * c.cos_ang + yr * c.sin_ang;
+					const double yp = -xr * c.sin_over + yr * c.cos_over;
+					const double wq = (double)Gq * c.flux_scale;
+					vsum += wq * sample_accumulate<double>(c, xp, yp, wq, gl);
+				}
 			}
 		}
 		else {
@@ -821,20 +916,26 @@ __global__ __launch_bounds__(256) void k_adj_render(const GalaxyDev *__restrict_
 			for (int s = 0; s < NGRAD; ++s) gf[s] = 0.f;
 			float vf = 0.f;
 			const float fs = (float)c.flux_scale;
-			for (int r = 0; r < rows; ++r) {
-				const int j = j0 + r;
-				if (j >= g.DH) break;
-				const unsigned long long m = I.dmask[((long)k * g.DH + j) * wpr + t.x];
-				if (!m) continue;
-				if (!((m >> lane) & 1ull)) continue;
-				const float Gq = I.G[i + j * g.DW];
-				if (Gq == 0.f) continue;
-				const double y = hy + j * g.sx;
-				const double xr = x - c.xc, yr = y - c.yc;
-				const float xp = (float)(xr * c.cos_ang + yr * c.sin_ang);
-				const float yp = (float)(-xr * c.sin_over + yr * c.cos_over);
-				const float wq = Gq * fs;
-				vf += wq * pixel_accumulate_f32(c, xp, yp, wq, gf);
+			for (int ti = t0; ti < t1; ++ti) {
+				const int2 t = g.atiles[ti];
+				const int i = t.x * kTileW + lane;
+				const int j0 = t.y * 4 * rows + wv * rows;
+				const double x = hx + i * g.sy;
+				for (int r = 0; r < rows; ++r) {
+					const int j = j0 + r;
+					if (j >= g.DH) break;
+					const unsigned long long m = I.dmask[((long)k * g.DH + j) * wpr + t.x];
+					if (!m) continue;
+					if (!((m >> lane) & 1ull)) continue;
+					const float Gq = I.G[i + j * g.DW];
+					if (Gq == 0.f) continue;
+					const double y = hy + j * g.sx;
+					const double xr = x - c.xc, yr = y - c.yc;
+					const float xp = (float)(xr * c.cos_ang + yr * c.sin_ang);
+					const float yp = (float)(-xr * c.sin_over + yr * c.cos_over);
+					const float wq = Gq * fs;
+					vf += wq * pixel_accumulate_f32(c, xp, yp, wq, gf);
+				}
 			}
 #pragma unroll
 			for (int s = 0; s < NGRAD; ++s) gl[s] = (double)gf[s];
@@ -987,11 +1088,20 @@ int launch_like_finalize(const GalaxyDev *gals, const InstDev *insts, const int 
 	return (int)hipGetLastError();
 }
 
-int launch_adj_render(const GalaxyDev *gals, const InstDev *insts, const int *tile_off, int ninst, int nblocks,
+int launch_like_t(const GalaxyDev *gals, const InstDev *insts, int ninst, int grad, double *ll, double *gscale,
+                  hipStream_t st)
+{
+	if (ninst <= 0) return 0;
+	k_like_t<<<ninst, 1024, 0, st>>>(gals, insts, grad, ll, gscale);
+	return (int)hipGetLastError();
+}
+
+int launch_adj_render(const GalaxyDev *gals, const InstDev *insts, const int *grp_off, int ninst, int nblocks, int grp,
                       const Comp *comps, double *grad_acc, hipStream_t st)
 {
 	if (nblocks <= 0) return 0;
-	k_adj_render<<<nblocks, 256, 0, st>>>(gals, insts, tile_off, ninst, comps, grad_acc);
+	if (grp < 1) return (int)hipErrorInvalidValue;
+	k_adj_render<<<nblocks, 256, 0, st>>>(gals, insts, grp_off, ninst, grp, comps, grad_acc);
 	return (int)hipGetLastError();
 }
 

@@ -53,14 +53,15 @@ static void run_case(int W, int H, int K, int pad)
 	const int twn = n1 > n2 ? n1 : n2;
 	double2 *tw = dev<double2>(twn / 2);
 	for (int t = 0; t < twn / 2; ++t) tw[t] = make_double2(std::cos(2 * M_PI * t / twn), -std::sin(2 * M_PI * t / twn));
-	double2 *spec = dev<double2>((size_t)m1 * n2);
+	double2 *spec = dev<double2>((size_t)fft_pitch(n1) * n2);
 	double *src = dev<double>((size_t)n1 * n2);
 	src[0] = 1;
 	g.fft_spec = spec;
 	uint8_t *calc = dev<uint8_t>((size_t)g.DW * g.DH), *region = dev<uint8_t>((size_t)W * H);
 	std::memset(calc, 1, (size_t)g.DW * g.DH);
 	std::memset(region, 1, (size_t)W * H);
-	float *data = dev<float>((size_t)W * H), *ivar = dev<float>((size_t)W * H);
+	double *data = dev<double>((size_t)W * H);
+	float *ivar = dev<float>((size_t)W * H);
 	for (int i = 0; i < W * H; ++i) ivar[i] = 1;
 	g.calc = calc;
 	g.region = region;
@@ -73,7 +74,7 @@ static void run_case(int W, int H, int K, int pad)
 	insts[0].canvas = dev<float>((size_t)g.DW * g.DH);
 	insts[0].w = dev<float>((size_t)W * H);
 	insts[0].G = dev<float>((size_t)g.DW * g.DH);
-	insts[0].fbuf = dev<double2>((size_t)m1 * n2);
+	insts[0].fbuf = dev<double2>((size_t)fft_pitch(n1) * n2);
 	const int cb = n2 >= 4096 ? 1 : (n2 >= 2048 ? 2 : (4096 / n2 > 8 ? 8 : 4096 / n2));
 	int *off = dev<int>(2);
 	int *lk = dev<int>(2);

@@ -359,6 +359,35 @@ def test_likelihood_gradients_match_finite_differences():
             assert abs(g[k] - fd) <= 1e-6 * max(1, abs(fd)), (lf, k)
 
 
+@pytest.mark.parametrize("noise", [1.5, 0.7])
+def test_t_likelihood_gradient_matches_finite_differences(noise):
+    """The t likelihood's dof follows the sample variance of the residuals
+    (R/profitLikeModel.R:281-286): residuals 1.5 sigma wide give dof = 2v/(v-1)
+    ~ 3.6, whose dependence on every pixel enters the gradient; 0.7 sigma wide
+    clamp dof at 1, where it does not."""
+    rng = np.random.default_rng(9)
+    n = 60
+    model = rng.random(n) * 10 + 1
+    sigma = np.sqrt(model + 4)
+    data = model + noise * rng.normal(size=n) * sigma
+    z = (data - model) / sigma
+    v = np.var(z, ddof=1)
+    assert (v > 1) == (noise > 1)
+    _, g = O.loglike_image("t", data, sigma, model, want_grad=True)
+    for k in range(n):
+        h = 1e-6 * model[k]
+        mp, mm = model.copy(), model.copy()
+        mp[k] += h
+        mm[k] -= h
+        fd = (O.loglike_image("t", data, sigma, mp) - O.loglike_image("t", data, sigma, mm)) / (2 * h)
+        assert abs(g[k] - fd) <= 1e-6 * max(1, abs(fd)), k
+    # the dof term matters: dropping it moves the unclamped gradient
+    if noise > 1:
+        dof = 2 * v / (v - 1)
+        fixed = -(-(dof + 1) * z / (dof + z * z)) / sigma
+        assert np.max(np.abs(fixed - g)) > 1e-3 * np.max(np.abs(g))
+
+
 # ---------------------------------------------------------------- gradient
 
 def _small_fit(components, magzero=30.0, size=48, psf=None):
