@@ -39,8 +39,9 @@ enum {
 	PROFIT_AMD_KING = 3,
 	PROFIT_AMD_CORESERSIC = 4,
 	PROFIT_AMD_BROKENEXP = 5,
-	PROFIT_AMD_SKY = 100,
-	PROFIT_AMD_NULL = 101
+	PROFIT_AMD_SKY = 100,  /* sky.cpp: bg in shape[0] */
+	PROFIT_AMD_NULL = 101,
+	PROFIT_AMD_PSF = 102   /* psf.cpp: a PSF point source, xcen, ycen, mag */
 };
 
 /* convolvers (src/libprofit/src/profit/convolve.h:68-80) */
@@ -61,7 +62,8 @@ enum { PROFIT_AMD_LIKE_NORM = 0, PROFIT_AMD_LIKE_CHISQ = 1, PROFIT_AMD_LIKE_T = 
  * One profile with libprofit's parameter names (radial.cpp:724-752 and each
  * profile's constructor).  shape[]:
  *   sersic re, nser | moffat fwhm, con | ferrer rout, a, b | king rc, rt, a |
- *   coresersic re, rb, nser, a, b | brokenexp h1, h2, rb, a | sky bg.
+ *   coresersic re, rb, nser, a, b | brokenexp h1, h2, rb, a | sky bg | psf
+ *   (point source): xcen, ycen, mag only.
  * profit_amd_profile_defaults() fills libprofit's constructor defaults.
  */
 typedef struct profit_amd_profile {
@@ -87,6 +89,7 @@ typedef struct profit_amd_model_desc {
 	uint32_t mask_w, mask_h;
 	int32_t convolver;   /* PROFIT_AMD_CONV_* */
 	int32_t device;      /* HIP device ordinal */
+	double psf_scale_x, psf_scale_y; /* Model::set_psf_pixel_scale; 0 = libprofit's default 1 */
 } profit_amd_model_desc;
 
 const char *profit_amd_last_error(void);
@@ -135,7 +138,9 @@ int profit_amd_upsample(const double *img, uint32_t w, uint32_t h, uint32_t fact
  * and the likelihood function.  A batch is a list of instances, each a
  * galaxy plus a component count; one call evaluates logL (and its gradient)
  * of every instance in one launch sequence (a HIP graph after the first call).
- * Every component of a galaxy with a PSF is convolved.
+ * Every radial component of a galaxy with a PSF is convolved (as ProFit's R
+ * code sets); sky and point-source (PROFIT_AMD_PSF) components follow their
+ * convolve flag, as libprofit's Model::produce_image does.
  */
 typedef struct profit_amd_engine profit_amd_engine;
 
@@ -151,6 +156,7 @@ typedef struct profit_amd_galaxy_desc {
 	const double *data;       /* width*height */
 	const double *sigma;      /* width*height */
 	int32_t likefunc;         /* PROFIT_AMD_LIKE_* */
+	double psf_scale_x, psf_scale_y; /* PSF pixel scale for point sources; 0 = 1 */
 } profit_amd_galaxy_desc;
 
 int profit_amd_engine_create(int32_t device, profit_amd_engine **engine);

@@ -16,9 +16,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
-SERSIC, MOFFAT, FERRER, KING, CORESERSIC, BROKENEXP = range(6)
+SERSIC, MOFFAT, FERRER, KING, CORESERSIC, BROKENEXP, POINT, SKY = range(8)
 PROFILE_TYPES = {"sersic": SERSIC, "moffat": MOFFAT, "ferrer": FERRER, "ferrers": FERRER,
-                 "king": KING, "coresersic": CORESERSIC, "brokenexp": BROKENEXP}
+                 "king": KING, "coresersic": CORESERSIC, "brokenexp": BROKENEXP, "psf": POINT, "sky": SKY}
 SHAPE_NAMES = {
     SERSIC: ("re", "nser"),
     MOFFAT: ("fwhm", "con"),
@@ -26,6 +26,8 @@ SHAPE_NAMES = {
     KING: ("rc", "rt", "a"),
     CORESERSIC: ("re", "rb", "nser", "a", "b"),
     BROKENEXP: ("h1", "h2", "rb", "a"),
+    POINT: (),
+    SKY: ("bg",),
 }
 CONV = {"brute": 0, "brute-old": 1, "fft": 2}
 LIKE = {"norm": 0, "chisq": 1, "t": 2, "pois": 3}
@@ -49,7 +51,8 @@ class Model(C.Structure):
                 ("return_finesampled", C.c_int), ("crop", C.c_int), ("adjust_mask", C.c_int),
                 ("dry_run", C.c_int), ("psf", C.POINTER(C.c_double)), ("psf_w", C.c_uint),
                 ("psf_h", C.c_uint), ("mask", C.POINTER(C.c_ubyte)), ("mask_w", C.c_uint),
-                ("mask_h", C.c_uint), ("convolver", C.c_int), ("threads", C.c_int)]
+                ("mask_h", C.c_uint), ("convolver", C.c_int), ("threads", C.c_int),
+                ("psf_scale_x", C.c_double), ("psf_scale_y", C.c_double)]
 
 
 class Diag(C.Structure):
@@ -233,7 +236,7 @@ class ModelSpec:
 
     def __init__(self, width, height, psf=None, mask=None, magzero=0.0, scale=(1.0, 1.0),
                  finesampling=1, return_finesampled=True, crop=True, adjust_mask=True,
-                 convolver="brute", threads=1, dry_run=False):
+                 convolver="brute", threads=1, dry_run=False, psf_scale=(1.0, 1.0)):
         m = Model()
         lib().or_model_defaults(C.byref(m))
         m.width, m.height = int(width), int(height)
@@ -246,6 +249,7 @@ class ModelSpec:
         m.dry_run = int(dry_run)
         m.convolver = CONV[convolver]
         m.threads = int(threads)
+        m.psf_scale_x, m.psf_scale_y = psf_scale
         self._psf = None
         self._mask = None
         if psf is not None:

@@ -45,7 +45,9 @@ enum {
 	OR_FERRER = 2,
 	OR_KING = 3,
 	OR_CORESERSIC = 4,
-	OR_BROKENEXP = 5
+	OR_BROKENEXP = 5,
+	OR_POINT = 6,   /* "psf": PSF point source (psf.cpp), parameters xcen, ycen, mag */
+	OR_SKY = 7      /* "sky": constant background (sky.cpp), parameter bg in shape[0] */
 };
 
 /* Gradient slots per component. */
@@ -65,6 +67,7 @@ enum {
  *   king       rc, rt, a
  *   coresersic re, rb, nser, a, b
  *   brokenexp  h1, h2, rb, a
+ *   sky        bg
  */
 typedef struct or_profile {
 	int type;
@@ -91,6 +94,7 @@ typedef struct or_model {
 	unsigned mask_w, mask_h;
 	int convolver;
 	int threads;
+	double psf_scale_x, psf_scale_y; /* Model::set_psf_pixel_scale (default 1, 1) */
 } or_model;
 
 void or_model_defaults(or_model *m);

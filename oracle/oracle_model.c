@@ -72,6 +72,8 @@ void or_profile_defaults(int type, or_profile *p)
 	case OR_KING: p->shape[0] = 1; p->shape[1] = 3; p->shape[2] = 2; break;
 	case OR_CORESERSIC: p->shape[0] = 1; p->shape[1] = 1; p->shape[2] = 4; p->shape[3] = 1; p->shape[4] = 1; break;
 	case OR_BROKENEXP: p->shape[0] = 1; p->shape[1] = 1; p->shape[2] = 1; p->shape[3] = 1; break;
+	case OR_POINT: p->mag = 0; break; /* psf.cpp:147-156 */
+	case OR_SKY: break;               /* sky.cpp:65-71: bg = 0 */
 	}
 }
 
@@ -85,6 +87,11 @@ typedef struct comp {
 	double img_sx, img_sy; /* Model image pixel scale (Sersic centre rule) */
 	int sersic_kind;
 	double dlnfs[OR_NGRAD];/* d ln(flux_scale) / d slot */
+	/* point sources: the model's normalised PSF, its pixel scale and the padding offset */
+	const double *psf;
+	unsigned kw, kh, off_x, off_y;
+	double psf_sx, psf_sy;
+	double inv_fs2;        /* sky: 1 / finesampling^2 */
 } comp;
 
 static int almost_equals(double x, double y) { return fabs(x - y) < 1e-10; }
@@ -194,6 +201,7 @@ static double brokenexp_integrand(double r, void *ctx)
 static int validate(const or_profile *p)
 {
 	const double *s = p->shape;
+	if (p->type == OR_POINT || p->type == OR_SKY) return 0; /* psf.cpp:40-45 is checked in analyze_inputs */
 	if (p->axrat <= 0) return fail("axrat <= 0, must have axrat > 0");
 	if (p->axrat > 1) return fail("axrat > 1, must have axrat <= 1");
 	if (p->box <= -2) return fail("box <= -2, must have box > -2");
@@ -257,6 +265,15 @@ static void setup_comp(comp *c, const or_profile *p, double magzero, double sx, 
 {
 	memset(c, 0, sizeof(*c));
 	c->p = *p;
+	c->off_x = off_x;
+	c->off_y = off_y;
+	if (p->type == OR_POINT) {
+		/* psf.cpp:66: scale = 10^(-0.4 (mag - magzero)) */
+		c->flux_scale = pow(10, -0.4 * (p->mag - magzero));
+		c->dlnfs[OR_G_MAG] = -0.4 * log(10.0);
+		return;
+	}
+	if (p->type == OR_SKY) return; /* bg / fs^2 is set by prepare() */
 	c->img_sx = img_sx;
 	c->img_sy = img_sy;
 	const double *s = p->shape;
@@ -674,10 +691,107 @@ static double subsample_pixel(walk *w, double x0, double x1, double y0, double y
  * function accumulates sum_q G[q] * d(flux_scale * pixel_value_q)/d slot
  * into gsum[OR_NGRAD].
  */
+/* bind() of psf.cpp:48-56: floor of the value, clamped to [0, max] */
+static unsigned psf_bind(double v, unsigned max)
+{
+	double f = floor(v);
+	if (f < 0) return 0;
+	if (f > (double)max) return max;
+	return (unsigned)f;
+}
+
+/*
+ * PsfProfile::evaluate (psf.cpp:58-145): each canvas pixel of the PSF's
+ * footprint receives the PSF pixels it overlaps, weighted by the overlap
+ * area.  The reference's index bounds are kept as written (the x end of the
+ * footprint uses offset.y, the second x index is clamped by the PSF height,
+ * the first y index by its width); a flat PSF index beyond the array, which
+ * only those bounds can produce for a non-square PSF, reads 0.  With G the
+ * pass accumulates sum_q G(q) d(scale val(q))/d(xcen, ycen, mag) instead: the
+ * overlap lengths are piecewise linear in the centre, with derivative
+ * [psf_x + psx < x + sx] - [psf_x > x] along x.
+ */
+static void render_point(const comp *c, double *img, unsigned W, unsigned H, double sx, double sy,
+                         const double *G, double *gsum)
+{
+	const double psx = c->psf_sx, psy = c->psf_sy;
+	const unsigned kw = c->kw, kh = c->kh;
+	const double scale = c->flux_scale;
+	const double origin_x = c->p.xcen + c->off_x * sx - kw * psx / 2;
+	const double end_x = c->p.xcen + c->off_y * sy + kw * psx / 2;
+	const double origin_y = c->p.ycen + c->off_x * sx - kh * psy / 2;
+	const double end_y = c->p.ycen + c->off_y * sy + kh * psy / 2;
+	const unsigned x0 = psf_bind(origin_x / sx, W - 1), y0 = psf_bind(origin_y / sy, H - 1);
+	const unsigned x1 = psf_bind(end_x / sx, W - 1), y1 = psf_bind(end_y / sy, H - 1);
+	double g[3] = {0, 0, 0};
+	for (unsigned pix_y = y0; pix_y <= y1; pix_y++) {
+		const double y = pix_y * sy;
+		for (unsigned pix_x = x0; pix_x <= x1; pix_x++) {
+			const double x = pix_x * sx;
+			const unsigned qx0 = psf_bind(floor((x - origin_x) / psx), kw - 1);
+			const unsigned qx1 = psf_bind(floor((x - origin_x + sx) / psx), kh - 1);
+			const unsigned qy0 = psf_bind(floor((y - origin_y) / psy), kw - 1);
+			const unsigned qy1 = psf_bind(floor((y - origin_y + sy) / psy), kh - 1);
+			double val = 0, dvx = 0, dvy = 0;
+			for (unsigned qy = qy0; qy <= qy1; qy++) {
+				const double psf_y = qy * psy + origin_y;
+				const double iy = fmin(y + sy, psf_y + psy) - fmax(y, psf_y);
+				const double diy = (psf_y + psy < y + sy ? 1.0 : 0.0) - (psf_y > y ? 1.0 : 0.0);
+				for (unsigned qx = qx0; qx <= qx1; qx++) {
+					const double psf_x = qx * psx + origin_x;
+					const double ix = fmin(x + sx, psf_x + psx) - fmax(x, psf_x);
+					const double dix = (psf_x + psx < x + sx ? 1.0 : 0.0) - (psf_x > x ? 1.0 : 0.0);
+					const size_t k = qx + (size_t)qy * kw;
+					const double w = k < (size_t)kw * kh ? c->psf[k] : 0.0;
+					val += w * (ix * iy) / (psx * psy);
+					dvx += w * (dix * iy) / (psx * psy);
+					dvy += w * (ix * diy) / (psx * psy);
+				}
+			}
+			const size_t idx = pix_x + (size_t)pix_y * W;
+			if (G) {
+				g[0] += G[idx] * scale * dvx;
+				g[1] += G[idx] * scale * dvy;
+				g[2] += G[idx] * scale * val;
+			}
+			else {
+				img[idx] += val * scale;
+			}
+		}
+	}
+	if (G && gsum) {
+		gsum[OR_G_XCEN] += g[0];
+		gsum[OR_G_YCEN] += g[1];
+		gsum[OR_G_MAG] += g[2] * c->dlnfs[OR_G_MAG];
+	}
+}
+
+/* SkyProfile::evaluate (sky.cpp:47-63): bg / fs^2 on every pixel of the calc
+ * mask; with G, sum_q G(q) / fs^2 into the bg slot */
+static void render_sky(const comp *c, double *img, const unsigned char *mask, unsigned W, unsigned H,
+                       const double *G, double *gsum)
+{
+	double gs = 0;
+	for (size_t idx = 0; idx < (size_t)W * H; idx++) {
+		if (mask && !mask[idx]) continue;
+		if (G) gs += G[idx];
+		else img[idx] += c->flux_scale;
+	}
+	if (G && gsum) gsum[OR_G_S0] += gs * c->inv_fs2;
+}
+
 static void render_comp(const comp *c, double *img, const unsigned char *mask, unsigned W,
                         unsigned H, double sx, double sy, int threads, or_diag *diag, int plane,
                         tape *tp, int tape_mode, const double *G, double *gsum)
 {
+	if (c->p.type == OR_POINT) {
+		render_point(c, img, W, H, sx, sy, G, gsum);
+		return;
+	}
+	if (c->p.type == OR_SKY) {
+		render_sky(c, img, mask, W, H, G, gsum);
+		return;
+	}
 	double half_xbin = sx / 2.;
 	double half_ybin = sy / 2.;
 	double flux_scale = c->flux_scale;
@@ -1103,6 +1217,8 @@ void or_model_defaults(or_model *m)
 	m->adjust_mask = 1;
 	m->convolver = OR_CONV_BRUTE;
 	m->threads = 1;
+	m->psf_scale_x = 1;
+	m->psf_scale_y = 1;
 }
 
 static int analyze_inputs(const or_model *m, const or_profile *profiles, int nprof, analysis *an)
@@ -1119,8 +1235,11 @@ static int analyze_inputs(const or_model *m, const or_profile *profiles, int npr
 		if (profiles[k].convolve) an->convolution_required = 1;
 	int has_psf = m->psf && m->psf_w > 0 && m->psf_h > 0;
 	if (an->convolution_required && !has_psf) return fail("No psf provided but profile(s) requested convolution");
-	for (int k = 0; k < nprof; k++)
+	for (int k = 0; k < nprof; k++) {
 		if (validate(&profiles[k])) return -1;
+		if (profiles[k].type == OR_POINT && !has_psf)
+			return fail("No psf present in the model, cannot produce a psf profile");
+	}
 	analyze_expansion(m->width, m->height, has_mask ? m->mask : NULL, m->mask_w, m->mask_h,
 	                  has_psf ? m->psf_w : 0, has_psf ? m->psf_h : 0, m->finesampling, an, m->adjust_mask);
 	return 0;
@@ -1188,6 +1307,21 @@ static int prepare(const or_model *m, const or_profile *profiles, int nprof, eva
 		setup_comp(&e->comps[k], &profiles[k], m->magzero, e->psx, e->psy, m->scale_x, m->scale_y,
 		           e->an.pad_x, e->an.pad_y);
 	if (m->psf && m->psf_w > 0 && m->psf_h > 0) e->psf = normalized_psf(m);
+	for (int k = 0; k < nprof; k++) {
+		comp *c = &e->comps[k];
+		if (c->p.type == OR_POINT) {
+			c->psf = e->psf;
+			c->kw = m->psf_w;
+			c->kh = m->psf_h;
+			c->psf_sx = m->psf_scale_x;
+			c->psf_sy = m->psf_scale_y;
+		}
+		else if (c->p.type == OR_SKY) {
+			/* sky.cpp:42-45: bg = requested_bg / (finesampling * finesampling) */
+			c->flux_scale = c->p.shape[0] / (double)(m->finesampling * m->finesampling);
+			c->inv_fs2 = 1.0 / (double)(m->finesampling * m->finesampling);
+		}
+	}
 	return 0;
 }
 
@@ -1515,8 +1649,8 @@ static int check_grad_support(const or_fit *fit, const or_profile *profiles, int
 	if (fit->model.finesampling != 1) return fail("gradient supports finesampling = 1 only");
 	for (int k = 0; k < nprof; k++) {
 		int t = profiles[k].type;
-		if (!(t == OR_SERSIC || t == OR_MOFFAT || t == OR_FERRER || t == OR_KING))
-			return fail("gradient supports sersic, moffat, ferrer and king profiles");
+		if (!(t == OR_SERSIC || t == OR_MOFFAT || t == OR_FERRER || t == OR_KING || t == OR_POINT || t == OR_SKY))
+			return fail("gradient supports sersic, moffat, ferrer, king, psf and sky profiles");
 		if (t == OR_SERSIC && profiles[k].rescale_flux) return fail("gradient does not support rescale_flux");
 	}
 	return 0;

@@ -20,10 +20,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libprofit_amd.so")
 
 SERSIC, MOFFAT, FERRER, KING, CORESERSIC, BROKENEXP = range(6)
-SKY, NULL = 100, 101
+SKY, NULL, PSF = 100, 101, 102
 PROFILE_TYPES = {"sersic": SERSIC, "moffat": MOFFAT, "ferrer": FERRER, "ferrers": FERRER,
                  "king": KING, "coresersic": CORESERSIC, "brokenexp": BROKENEXP,
-                 "sky": SKY, "null": NULL}
+                 "sky": SKY, "null": NULL, "psf": PSF}
 SHAPE_NAMES = {
     SERSIC: ("re", "nser"),
     MOFFAT: ("fwhm", "con"),
@@ -33,6 +33,7 @@ SHAPE_NAMES = {
     BROKENEXP: ("h1", "h2", "rb", "a"),
     SKY: ("bg",),
     NULL: (),
+    PSF: (),
 }
 CONVOLVERS = {"brute-old": 0, "brute": 1, "fft": 3, "null": 4, "brute-bf16x3": 16}
 LIKE = {"norm": 0, "chisq": 1, "t": 2, "pois": 3}
@@ -60,7 +61,8 @@ class ModelDesc(C.Structure):
                 ("return_finesampled", C.c_int32), ("crop", C.c_int32), ("adjust_mask", C.c_int32),
                 ("dry_run", C.c_int32), ("psf", C.POINTER(C.c_double)), ("psf_w", C.c_uint32),
                 ("psf_h", C.c_uint32), ("mask", C.POINTER(C.c_uint8)), ("mask_w", C.c_uint32),
-                ("mask_h", C.c_uint32), ("convolver", C.c_int32), ("device", C.c_int32)]
+                ("mask_h", C.c_uint32), ("convolver", C.c_int32), ("device", C.c_int32),
+                ("psf_scale_x", C.c_double), ("psf_scale_y", C.c_double)]
 
 
 class GalaxyDesc(C.Structure):
@@ -70,7 +72,7 @@ class GalaxyDesc(C.Structure):
                 ("calcregion", C.POINTER(C.c_uint8)), ("calc_w", C.c_uint32),
                 ("calc_h", C.c_uint32), ("region", C.POINTER(C.c_uint8)),
                 ("data", C.POINTER(C.c_double)), ("sigma", C.POINTER(C.c_double)),
-                ("likefunc", C.c_int32)]
+                ("likefunc", C.c_int32), ("psf_scale_x", C.c_double), ("psf_scale_y", C.c_double)]
 
 
 # every symbol include/profit_amd.h declares: (name, restype, argtypes)
@@ -181,8 +183,9 @@ def profile_array(profiles):
 
 
 def _model_desc(width, height, psf, mask, magzero, scale, finesampling, return_finesampled, crop,
-                adjust_mask, convolver, device, dry_run, keep):
+                adjust_mask, convolver, device, dry_run, keep, psf_scale=(1.0, 1.0)):
     d = ModelDesc()
+    d.psf_scale_x, d.psf_scale_y = psf_scale
     d.width, d.height = int(width), int(height)
     d.scale_x, d.scale_y = scale
     d.magzero = magzero
@@ -208,11 +211,11 @@ def _model_desc(width, height, psf, mask, magzero, scale, finesampling, return_f
 
 def make_model(width, height, profiles, psf=None, mask=None, magzero=0.0, scale=(1.0, 1.0),
                finesampling=1, return_finesampled=True, crop=True, adjust_mask=True,
-               convolver="brute", device=0, dry_run=False):
+               convolver="brute", device=0, dry_run=False, psf_scale=(1.0, 1.0)):
     """Model::evaluate on the GPU.  Returns dict(image=[y, x] array, offset=(x, y))."""
     keep = []
     d = _model_desc(width, height, psf, mask, magzero, scale, finesampling, return_finesampled,
-                    crop, adjust_mask, convolver, device, dry_run, keep)
+                    crop, adjust_mask, convolver, device, dry_run, keep, psf_scale)
     arr = profile_array(profiles)
     L = lib()
     dw, dh = C.c_uint32(), C.c_uint32()
@@ -301,10 +304,11 @@ class Engine:
             pass
 
     def add_galaxy(self, width, height, data, sigma, psf=None, calcregion=None, region=None,
-                   likefunc="chisq", magzero=0.0, scale=(1.0, 1.0)):
+                   likefunc="chisq", magzero=0.0, scale=(1.0, 1.0), psf_scale=(1.0, 1.0)):
         g = GalaxyDesc()
         g.width, g.height = int(width), int(height)
         g.scale_x, g.scale_y = scale
+        g.psf_scale_x, g.psf_scale_y = psf_scale
         g.magzero = magzero
         keep = []
         if psf is not None:

@@ -48,6 +48,7 @@ void build_model(profit::Model &m, const profit_amd_model_desc *d, const profit_
 		m.set_psf(profit::Image(p, d->psf_w, d->psf_h));
 	}
 	if (d->mask && d->mask_w > 0 && d->mask_h > 0) m.set_mask(profit::Mask(d->mask, d->mask_w, d->mask_h));
+	m.set_psf_pixel_scale({d->psf_scale_x > 0 ? d->psf_scale_x : 1.0, d->psf_scale_y > 0 ? d->psf_scale_y : 1.0});
 	switch (d->convolver) {
 	case PROFIT_AMD_CONV_BRUTE_OLD: m.set_convolver(profit::create_convolver(profit::BRUTE_OLD)); break;
 	case PROFIT_AMD_CONV_BRUTE: m.set_convolver(profit::create_convolver(profit::BRUTE)); break;
@@ -67,6 +68,13 @@ void build_model(profit::Model &m, const profit_amd_model_desc *d, const profit_
 		else if (pp.type == PROFIT_AMD_SKY) {
 			p = m.add_profile("sky");
 			p->parameter("bg", pp.shape[0]);
+			p->parameter("convolve", pp.convolve != 0);
+		}
+		else if (pp.type == PROFIT_AMD_PSF) {
+			p = m.add_profile("psf");
+			p->parameter("xcen", pp.xcen);
+			p->parameter("ycen", pp.ycen);
+			p->parameter("mag", pp.mag);
 			p->parameter("convolve", pp.convolve != 0);
 		}
 		else if (pp.type == PROFIT_AMD_NULL) {
@@ -96,11 +104,8 @@ int profit_amd_profile_defaults(int32_t type, profit_amd_profile *p)
 	return guarded([&] {
 		if (!p) throw std::invalid_argument("null profile pointer");
 		profit_amd::ProfileParams q;
-		profit_amd::default_params(type >= 0 && type < 6 ? type : 0, &q);
-		q.type = type;
+		profit_amd::default_params(type, &q);
 		std::memcpy(p, &q, sizeof(q));
-		if (type == PROFIT_AMD_SKY || type == PROFIT_AMD_NULL)
-			for (double &s : p->shape) s = 0;
 	});
 }
 
@@ -256,6 +261,8 @@ int profit_amd_engine_add_galaxy(profit_amd_engine *engine, const profit_amd_gal
 		s.cw = (int)g->calc_w;
 		s.ch = (int)g->calc_h;
 		s.region = g->region;
+		s.psf_scale_x = g->psf_scale_x > 0 ? g->psf_scale_x : 1.0;
+		s.psf_scale_y = g->psf_scale_y > 0 ? g->psf_scale_y : 1.0;
 		s.data = g->data;
 		s.sigma = g->sigma;
 		s.likefunc = g->likefunc;

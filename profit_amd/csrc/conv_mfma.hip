@@ -212,8 +212,9 @@ __global__ __launch_bounds__(256) void k_conv_mfma(const GalaxyDev *__restrict__
 					const long o = (long)yy * OW + xx;
 					float wvv = 0.f;
 					if (g.region[o]) {
-						const bool in_calc = !g.has_calc || g.calc[(long)(yy + g.pady) * g.DW + xx + g.padx];
-						const double m = in_calc ? accd[q][r] : 0.0;
+						const long qc = (long)(yy + g.pady) * g.DW + xx + g.padx;
+						const bool in_calc = !g.has_calc || g.calc[qc];
+						const double m = in_calc ? accd[q][r] + (I.has_aux ? (double)I.aux[qc] : 0.0) : 0.0;
 						const double d = g.data[o];
 						if (pois) {
 							const double mm = s * m, dd = s * d;
@@ -252,6 +253,7 @@ __global__ __launch_bounds__(256) void k_conv_mfma(const GalaxyDev *__restrict__
 					const long o = (long)yy * OW + xx;
 					float v = (float)accd[q][r];
 					if (MODE == CONV_MODEL && g.has_calc && !g.calc[o]) v = 0.f;
+					if (MODE == CONV_MODEL && I.has_aux) v += I.aux[o];
 					I.G[o] = v;
 				}
 			}

@@ -63,13 +63,19 @@ struct GalaxyDev {
 	const uint4 *mfrag_adj;   // and of the adjoint weights
 	const int2 *mtiles_like;  // 128 x 128 tiles on the image holding region pixels
 	const int2 *mtiles_canvas;// 128 x 128 tiles on the canvas holding calc pixels
+	// the model's normalised PSF for point sources (null without a PSF), its
+	// size and pixel scale (Model::set_psf_pixel_scale, default 1 x 1)
+	const float *psf_pt;
+	int32_t pkx, pky;
+	double psf_sx, psf_sy;
 	const int2 *rtiles;    // render / adjoint-render tiles (64 x 4) on the canvas
 	const int2 *ctiles;    // forward convolution tiles on the image
 	const int2 *atiles;    // adjoint convolution tiles on the canvas
 };
 
 struct InstDev {
-	int32_t galaxy, comp0, ncomp, reserved;
+	int32_t galaxy, comp0, ncomp;
+	int32_t has_aux;       // the likelihood path draws unconvolved point sources / sky into aux
 	float *canvas;         // DW*DH rendered (to be convolved) image
 	float *w;              // W*H  d logL / d model (up to a per-instance scalar)
 	float *G;              // DW*DH  correlated gradient image / convolved model

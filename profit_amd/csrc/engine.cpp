@@ -2,6 +2,8 @@
 // batched launch sequences and their hipGraph replay.
 #include "engine.h"
 
+#include <cstddef>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -429,7 +431,8 @@ int Engine::add_galaxy(const GalaxySpec &s)
 	g.img_sy = s.scale_y;
 	g.sx = s.scale_x / s.finesampling;
 	g.sy = s.scale_y / s.finesampling;
-	const bool conv = s.psf && s.Kx > 0 && s.Ky > 0;
+	const bool have_psf = s.psf && s.Kx > 0 && s.Ky > 0;
+	const bool conv = have_psf && s.psf_convolves;
 	// without a PSF the likelihood path convolves with a 1 x 1 unit kernel,
 	// which copies the canvas exactly
 	g.has_psf = conv ? 1 : 0;
@@ -453,6 +456,22 @@ int Engine::add_galaxy(const GalaxySpec &s)
 	g.has_calc = s.calc != nullptr;
 	g.render_fp64 = render_fp64_ ? 1 : 0;
 	const size_t npix = (size_t)s.W * s.H, ndraw = (size_t)g.DW * g.DH;
+	if (have_psf) {
+		// the model's PSF for point sources (psf.cpp:58-145 reads model.psf, normalised by set_psf)
+		std::vector<double> p(s.psf, s.psf + (size_t)s.Kx * s.Ky);
+		double sum = 0;
+		for (double v : p) sum += v;
+		std::vector<float> pt(p.size());
+		for (size_t k = 0; k < p.size(); ++k) pt[k] = (float)((s.normalise_psf && sum > 0) ? p[k] / sum : p[k]);
+		bufs.psf_pt.alloc(pt.size() * sizeof(float));
+		hip_check(hipMemcpy(bufs.psf_pt.as<float>(), pt.data(), pt.size() * 4, hipMemcpyHostToDevice), "psf");
+		g.psf_pt = bufs.psf_pt.as<float>();
+		g.pkx = s.Kx;
+		g.pky = s.Ky;
+	}
+	g.psf_sx = s.psf_scale_x;
+	g.psf_sy = s.psf_scale_y;
+	if (!(g.psf_sx > 0 && g.psf_sy > 0)) throw std::invalid_argument("PSF pixel scale must be positive");
 	{
 		std::vector<double> p(1, 1.0);
 		if (conv) {
@@ -714,6 +733,8 @@ void Engine::set_batch(const std::vector<int> &inst_galaxy, const std::vector<in
 	}
 	n_ct_blocks_ = ct_off_[n];
 	n_at_blocks_ = at_off_[n];
+	inst_aux_.assign(n, 0);
+	any_nonradial_ = false;
 	// brute-force model / adjoint tiles and the FFT passes, per instance
 	bt_off_.assign(n + 1, 0);
 	ml_off_.assign(n + 1, 0);
@@ -852,6 +873,9 @@ void Engine::record(bool grad)
 		hip_check(launch_adj_render(gals, insts, d_ag_off_.as<int>(), n, n_ag_blocks_, adj_grp_, d_comps_.as<Comp>(),
 		                            d_grad_acc_.as<double>(), st),
 		          "adjoint render");
+		if (any_nonradial_)
+			hip_check(launch_adj_nonradial(gals, insts, n, d_comps_.as<Comp>(), d_grad_acc_.as<double>(), st),
+			          "adjoint point sources and sky");
 		mark(7);
 		hip_check(launch_subsample(true, subsample_blocks_, gals, insts, d_comps_.as<Comp>(), work_queue(), 0, d_grad_acc_.as<double>(), st),
 		          "adjoint subsample");
@@ -879,6 +903,40 @@ void Engine::check_setup_errors()
 			os << "component " << c << ": " << setup_error_message(err[c]);
 			throw std::invalid_argument(os.str());
 		}
+}
+
+// Point sources and sky among the current parameters (h_params_): whether
+// any is present (the adjoint launches k_adj_nonradial; a change re-records
+// the graphs) and, per instance, whether one is drawn without convolution,
+// which puts the aux canvas in use (InstDev::has_aux, read by the kernels).
+// With enable = false every instance's flag is cleared (the model-canvas path
+// draws its aux canvas itself).  Returns whether anything changed.
+bool Engine::sync_aux_flags(bool enable)
+{
+	const ProfileParams *p = h_params_.as<ProfileParams>();
+	bool any = false, changed = false;
+	for (int i = 0; i < ninst(); ++i) {
+		int aux = 0;
+		for (int k = 0; k < inst_ncomp_[i]; ++k) {
+			const ProfileParams &q = p[inst_comp0_[i] + k];
+			if (is_radial(q.type)) continue;
+			any = true;
+			if (!q.convolve) aux = 1;
+		}
+		if (!enable) aux = 0;
+		if (aux != inst_aux_[i]) {
+			inst_aux_[i] = aux;
+			const size_t off = (size_t)i * sizeof(InstDev) + offsetof(InstDev, has_aux);
+			hip_check(hipMemcpy(d_insts_.as<char>() + off, &aux, sizeof(int), hipMemcpyHostToDevice), "insts");
+			changed = true;
+		}
+	}
+	if (any != any_nonradial_) {
+		any_nonradial_ = any;
+		invalidate_graphs(graph_exec_);
+		changed = true;
+	}
+	return changed;
 }
 
 void Engine::run(bool grad, bool use_graph)
@@ -933,6 +991,7 @@ void Engine::evaluate(const ProfileParams *params, double *ll, double *grad, boo
 {
 	if (ninst() == 0) return;
 	std::memcpy(h_params_.as<ProfileParams>(), params, ncomp_total_ * sizeof(ProfileParams));
+	sync_aux_flags(true);
 	run(grad != nullptr, use_graph);
 	check_setup_errors();
 	if (ll) std::memcpy(ll, h_ll_.as<double>(), ninst() * sizeof(double));
@@ -948,6 +1007,7 @@ void Engine::diagnostics(const ProfileParams *params, int *cls, int *nodes)
 	const int nc = ncomp_total_;
 	DevBuf dcls(plane * nc * sizeof(int)), dnodes(plane * nc * sizeof(int));
 	std::memcpy(h_params_.as<ProfileParams>(), params, nc * sizeof(ProfileParams));
+	sync_aux_flags(false);
 	hipStream_t st = stream_;
 	const GalaxyDev *gals = d_gals_.as<GalaxyDev>();
 	const InstDev *insts = d_insts_.as<InstDev>();
@@ -981,6 +1041,7 @@ void Engine::model_images(const ProfileParams *params, float *out)
 	hip_check(hipSetDevice(device_), "hipSetDevice");
 	const int n = ninst(), nc = ncomp_total_;
 	std::memcpy(h_params_.as<ProfileParams>(), params, nc * sizeof(ProfileParams));
+	sync_aux_flags(true);
 	hipStream_t st = stream_;
 	const GalaxyDev *gals = d_gals_.as<GalaxyDev>();
 	const InstDev *insts = d_insts_.as<InstDev>();
@@ -1024,6 +1085,7 @@ void Engine::model_canvas(const ProfileParams *params, float *out)
 	hip_check(hipSetDevice(device_), "hipSetDevice");
 	const int nc = ncomp_total_;
 	std::memcpy(h_params_.as<ProfileParams>(), params, nc * sizeof(ProfileParams));
+	sync_aux_flags(false);
 	hipStream_t st = stream_;
 	const GalaxyDev *gals = d_gals_.as<GalaxyDev>();
 	const InstDev *insts = d_insts_.as<InstDev>();

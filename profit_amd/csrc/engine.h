@@ -28,6 +28,8 @@ int launch_conv(int mode, int rows, int nblocks, int kx_max, int kc, const Galax
                 const int *tile_off, int ninst, const int *lk_base, double *partial, hipStream_t st);
 int launch_like_finalize(const GalaxyDev *gals, const InstDev *insts, const int *lk_off, int ninst,
                          const double *partial, double *ll, double *gscale, hipStream_t st);
+int launch_adj_nonradial(const GalaxyDev *gals, const InstDev *insts, int ninst, const Comp *comps, double *grad_acc,
+                         hipStream_t st);
 int launch_like_t(const GalaxyDev *gals, const InstDev *insts, int ninst, int grad, double *ll, double *gscale,
                   hipStream_t st);
 int launch_adj_render(const GalaxyDev *gals, const InstDev *insts, const int *grp_off, int ninst, int nblocks, int grp,
@@ -131,6 +133,8 @@ struct GalaxySpec {
 	int likefunc = LIKE_CHISQ;
 	int finesampling = 1;                // W, H are then the fine grid; profiles use scale/finesampling
 	bool normalise_psf = true;           // Model::set_psf normalises; a bare Convolver does not
+	bool psf_convolves = true;           // false: the PSF only draws point sources (no padding)
+	double psf_scale_x = 1, psf_scale_y = 1;  // Model::set_psf_pixel_scale
 };
 
 // Accumulated per-stage device time (HIP events on the engine stream) over the
@@ -203,7 +207,7 @@ class Engine {
   private:
 	struct GalaxyBuffers {
 		DevBuf psf_fwd, psf_adj, calc, region, data, ivar, rtiles, ctiles, atiles, fft_spec;
-		DevBuf mfrag_fwd, mfrag_adj, mtiles_like, mtiles_canvas;
+		DevBuf mfrag_fwd, mfrag_adj, mtiles_like, mtiles_canvas, psf_pt;
 		std::vector<int2> ct_host, at_host;
 		std::vector<double> psf64;  // normalised (or raw) PSF, Ky rows of Kx
 	};
@@ -252,6 +256,11 @@ class Engine {
 	std::vector<int> inst_galaxy_, inst_ncomp_, inst_comp0_;
 	int ncomp_total_ = 0;
 	bool any_t_ = false;  // an instance of the batch uses the t likelihood
+	// point sources and sky of the current parameters: any at all (adjoint
+	// pass), and per instance whether one is drawn without convolution
+	bool any_nonradial_ = false;
+	std::vector<int> inst_aux_;
+	bool sync_aux_flags(bool enable);
 	std::vector<int> ct_off_, at_off_, ag_off_;
 	int n_ct_blocks_ = 0, n_at_blocks_ = 0, n_ag_blocks_ = 0;
 	int adj_grp_ = 1;  // canvas tiles per block of the adjoint render

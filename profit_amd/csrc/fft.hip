@@ -337,9 +337,10 @@ __global__ __launch_bounds__(256) void k_fft_rows_inv(const GalaxyDev *__restric
 			const long o = (long)(r + row) * g.W + x;
 			float wv = 0.f;
 			if (g.region[o]) {
-				const bool in_calc = !g.has_calc || g.calc[(long)(y + row) * g.DW + x + g.padx];
+				const long qc = (long)(y + row) * g.DW + x + g.padx;
+				const bool in_calc = !g.has_calc || g.calc[qc];
 				const double2 z = sh[bitrev(x + g.padx, L)];
-				const double m = in_calc ? (row ? z.y : z.x) * scale : 0.0;
+				const double m = in_calc ? (row ? z.y : z.x) * scale + (I.has_aux ? (double)I.aux[qc] : 0.0) : 0.0;
 				const double d = g.data[o];
 				if (pois) {
 					const double mm = s * m, dd = s * d;
@@ -377,6 +378,7 @@ __global__ __launch_bounds__(256) void k_fft_rows_inv(const GalaxyDev *__restric
 		double v = (row ? z.y : z.x) * scale;
 		const long o = (long)(y + row) * g.DW + x;
 		if (mode == 1 && g.has_calc && !g.calc[o]) v = 0;
+		if (mode == 1 && I.has_aux) v += (double)I.aux[o];
 		I.G[o] = (float)v;
 	}
 }

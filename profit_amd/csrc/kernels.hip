@@ -53,7 +53,8 @@ __global__ void k_setup(const ProfileParams *__restrict__ params, const int *__r
 	const GalaxyDev &g = gals[insts[inst].galaxy];
 	sf::QuadState qs;
 	Comp out;
-	const int e = setup_component(params[c], g.magzero, g.sx, g.sy, g.img_sx, g.img_sy, g.padx, g.pady, &qs, &out);
+	int e = setup_component(params[c], g.magzero, g.sx, g.sy, g.img_sx, g.img_sy, g.padx, g.pady, &qs, &out);
+	if (e == SE_OK && params[c].type == POINTSRC && !g.psf_pt) e = SE_NO_PSF;
 	out.instance = inst;
 	comps[c] = out;
 	errors[c] = e;
@@ -117,9 +118,81 @@ __device__ __forceinline__ void append_entry(bool take, int lane, QEntry *list, 
 	}
 }
 
+// ------------------------------------------------------------------ point sources
+
+// bind() of psf.cpp:48-56: the floor of a value clamped to [0, max]
+__device__ __forceinline__ int psf_bind(double v, int max)
+{
+	const double f = floor(v);
+	if (f < 0) return 0;
+	if (f > (double)max) return max;
+	return (int)f;
+}
+
+// Footprint of a point source on the canvas (psf.cpp:79-92): the PSF's
+// origin and the clamped pixel range it covers.  The x end uses offset.y and
+// the y origin offset.x, as the reference does (Comp::s[0..1], xc, yc).
+struct PointFoot {
+	double ox, oy;
+	int x0, x1, y0, y1;
+};
+
+__device__ __forceinline__ PointFoot point_footprint(const Comp &c, const GalaxyDev &g)
+{
+	PointFoot f;
+	f.ox = c.xc - g.pkx * g.psf_sx / 2;
+	f.oy = c.yc - g.pky * g.psf_sy / 2;
+	const double ex = c.s[0] + g.pkx * g.psf_sx / 2, ey = c.s[1] + g.pky * g.psf_sy / 2;
+	f.x0 = psf_bind(f.ox / g.sx, g.DW - 1);
+	f.y0 = psf_bind(f.oy / g.sy, g.DH - 1);
+	f.x1 = psf_bind(ex / g.sx, g.DW - 1);
+	f.y1 = psf_bind(ey / g.sy, g.DH - 1);
+	return f;
+}
+
+// psf.cpp:94-141 at canvas pixel (px, py) of the footprint, before the flux
+// scale: the PSF pixels the image pixel overlaps, weighted by the overlap
+// area (the reference's index clamps kept; a flat index past the PSF array,
+// reachable only through them for a non-square PSF, reads 0).  dvx, dvy: the
+// derivatives along xcen and ycen, from overlap lengths that are piecewise
+// linear in the centre.
+__device__ double point_value(const GalaxyDev &g, const PointFoot &f, int px, int py, double *dvx, double *dvy)
+{
+	const double sx = g.sx, sy = g.sy, psx = g.psf_sx, psy = g.psf_sy;
+	const int kw = g.pkx, kh = g.pky;
+	const double x = px * sx, y = py * sy;
+	const int qx0 = psf_bind(floor((x - f.ox) / psx), kw - 1);
+	const int qx1 = psf_bind(floor((x - f.ox + sx) / psx), kh - 1);
+	const int qy0 = psf_bind(floor((y - f.oy) / psy), kw - 1);
+	const int qy1 = psf_bind(floor((y - f.oy + sy) / psy), kh - 1);
+	const long nk = (long)kw * kh;
+	double val = 0, gx = 0, gy = 0;
+	for (int qy = qy0; qy <= qy1; ++qy) {
+		const double psf_y = qy * psy + f.oy;
+		const double iy = fmin(y + sy, psf_y + psy) - fmax(y, psf_y);
+		const double diy = (psf_y + psy < y + sy ? 1.0 : 0.0) - (psf_y > y ? 1.0 : 0.0);
+		for (int qx = qx0; qx <= qx1; ++qx) {
+			const double psf_x = qx * psx + f.ox;
+			const double ix = fmin(x + sx, psf_x + psx) - fmax(x, psf_x);
+			const double dix = (psf_x + psx < x + sx ? 1.0 : 0.0) - (psf_x > x ? 1.0 : 0.0);
+			const long k = qx + (long)qy * kw;
+			const double w = k < nk ? (double)g.psf_pt[k] : 0.0;
+			val += w * (ix * iy) / (psx * psy);
+			gx += w * (dix * iy) / (psx * psy);
+			gy += w * (ix * diy) / (psx * psy);
+		}
+	}
+	*dvx = gx;
+	*dvy = gy;
+	return val;
+}
+
 // Tiles are 64 x (4*rows) pixels; thread (tx, ty) covers column tx and rows
 // ty*rows .. ty*rows+rows-1.  filter: -1 every component, 0/1 only those whose
-// convolve flag equals it.  target: 0 canvas, 1 the aux canvas.
+// convolve flag equals it.  target: 0 canvas, 1 the aux canvas.  With filter
+// -1, point sources and sky drawn without convolution go to the aux canvas
+// when the instance has one in use (InstDev::has_aux); radial profiles are
+// always convolved there.
 __global__ __launch_bounds__(256) void k_render(const GalaxyDev *__restrict__ gals, const InstDev *__restrict__ insts,
                                                 const int *__restrict__ tile_off, int ninst,
                                                 const Comp *__restrict__ comps, WorkQueue wq, int filter, int target,
@@ -144,10 +217,29 @@ __global__ __launch_bounds__(256) void k_render(const GalaxyDev *__restrict__ ga
 		// radial.cpp:313-314: the x centre uses scale.y and the y centre scale.x
 		const double y = hy + j * g.sx;
 		const double x = hx + i * g.sy;
-		float acc = 0.f;
+		float acc = 0.f, aux_acc = 0.f;
 		for (int k = 0; k < I.ncomp; ++k) {
 			const Comp &c = comps[I.comp0 + k];
 			if (filter >= 0 && c.convolve != filter) continue;
+			if (!is_radial(c.type)) {
+				// sky.cpp:47-63 (calc pixels only) and psf.cpp:58-145 (mask ignored)
+				float v = 0.f;
+				if (c.type == SKY) {
+					if (active) v = (float)c.flux_scale;
+				}
+				else if (c.type == POINTSRC && inb) {
+					const PointFoot f = point_footprint(c, g);
+					if (i >= f.x0 && i <= f.x1 && j >= f.y0 && j <= f.y1) {
+						double dvx, dvy;
+						v = (float)(point_value(g, f, i, j, &dvx, &dvy) * c.flux_scale);
+					}
+				}
+				if (filter < 0 && !c.convolve && I.has_aux) aux_acc += v;
+				else acc += v;
+				if (diag_cls && inb) diag_cls[(long)(I.comp0 + k) * g.DW * g.DH + pix] = 0;
+				if (I.dmask && lane == 0 && j < g.DH) I.dmask[((long)k * g.DH + j) * ((g.DW + 63) >> 6) + t.x] = 0ull;
+				continue;
+			}
 			bool sub = false, heavy = false;
 			int cls = 0;
 			if (active) {
@@ -172,6 +264,7 @@ __global__ __launch_bounds__(256) void k_render(const GalaxyDev *__restrict__ ga
 			append_entry(heavy, lane, wq.heavy, wq.count + 1, wq.hcap, QEntry{I.comp0 + k, pix});
 		}
 		if (inb) out[pix] = acc;
+		if (inb && filter < 0 && I.has_aux) I.aux[pix] = aux_acc;
 	}
 }
 
@@ -663,8 +756,9 @@ __global__ __launch_bounds__(256) void k_conv(const GalaxyDev *__restrict__ gals
 				if (g.region[o]) {
 					// the model is zero outside the calc region (libprofit masks the
 					// convolution output there); such pixels do not feed the adjoint
-					const bool in_calc = !g.has_calc || g.calc[(long)(oyy + g.pady) * g.DW + oxx + g.padx];
-					const double m = in_calc ? acc[r] : 0.0;
+					const long q = (long)(oyy + g.pady) * g.DW + oxx + g.padx;
+					const bool in_calc = !g.has_calc || g.calc[q];
+					const double m = in_calc ? acc[r] + (I.has_aux ? (double)I.aux[q] : 0.0) : 0.0;
 					const double d = g.data[o];
 					if (pois) {
 						const double mm = s * m, dd = s * d;
@@ -699,6 +793,7 @@ __global__ __launch_bounds__(256) void k_conv(const GalaxyDev *__restrict__ gals
 				const long o = (long)oyy * OW + oxx;
 				float v = (float)acc[r];
 				if (MODE == CONV_MODEL && g.has_calc && !g.calc[o]) v = 0.f;
+				if (MODE == CONV_MODEL && I.has_aux) v += I.aux[o];
 				out[o] = v;
 			}
 		}
@@ -749,6 +844,17 @@ __global__ void k_like_finalize(const GalaxyDev *__restrict__ gals, const InstDe
 	}
 	ll[i] = L;
 	gscale[i] = gsc;
+}
+
+// Adjoint weight of canvas pixel (x, y) for a component drawn into the
+// convolved canvas (G) or into the aux canvas, whose pixels enter the model
+// image unconvolved (w at the image pixel, zero on the padding).
+__device__ __forceinline__ double adjoint_weight(const GalaxyDev &g, const InstDev &I, bool via_aux, int x, int y)
+{
+	if (!via_aux) return (double)I.G[(long)y * g.DW + x];
+	const int xi = x - g.padx, yi = y - g.pady;
+	if (xi < 0 || yi < 0 || xi >= g.W || yi >= g.H) return 0.0;
+	return (double)I.w[(long)yi * g.W + xi];
 }
 
 // Sum over the block (blockDim.x a multiple of 64, at most 1024) in a fixed
@@ -957,6 +1063,62 @@ __global__ __launch_bounds__(256) void k_adj_render(const GalaxyDev *__restrict_
 	}
 }
 
+// Gradient of the point sources and the sky (one block per instance): the
+// adjoint weight of canvas pixel q is G(q) for a convolved component and
+// w(q - padding) on the image for one drawn without convolution.
+//   point source: sum_q W(q) fs d val(q) / d(xcen, ycen) and fs val(q) d ln fs / d mag
+//   sky:          sum over calc pixels of W(q) / fs^2 into the bg slot
+__global__ __launch_bounds__(256) void k_adj_nonradial(const GalaxyDev *__restrict__ gals,
+                                                       const InstDev *__restrict__ insts,
+                                                       const Comp *__restrict__ comps, double *grad_acc)
+{
+	__shared__ double sh[4];
+	const int inst = blockIdx.x;
+	const InstDev I = insts[inst];
+	const GalaxyDev &g = gals[I.galaxy];
+	for (int k = 0; k < I.ncomp; ++k) {
+		const Comp &c = comps[I.comp0 + k];
+		if (c.type != SKY && c.type != POINTSRC) continue;
+		const bool via_aux = !c.convolve && I.has_aux;
+		double a0 = 0, a1 = 0, a2 = 0;
+		if (c.type == SKY) {
+			const long n = (long)g.DW * g.DH;
+			for (long o = threadIdx.x; o < n; o += blockDim.x) {
+				if (g.has_calc && !g.calc[o]) continue;
+				const int y = (int)(o / g.DW), x = (int)(o - (long)y * g.DW);
+				a0 += adjoint_weight(g, I, via_aux, x, y);
+			}
+		}
+		else {
+			const PointFoot f = point_footprint(c, g);
+			const int fw = f.x1 - f.x0 + 1, fh = f.y1 - f.y0 + 1;
+			for (int t = threadIdx.x; t < fw * fh; t += blockDim.x) {
+				const int y = f.y0 + t / fw, x = f.x0 + t % fw;
+				const double wq = adjoint_weight(g, I, via_aux, x, y);
+				if (wq == 0) continue;
+				double dvx, dvy;
+				const double v = point_value(g, f, x, y, &dvx, &dvy);
+				a0 += wq * dvx;
+				a1 += wq * dvy;
+				a2 += wq * v;
+			}
+		}
+		a0 = block_sum(a0, sh);
+		a1 = block_sum(a1, sh);
+		a2 = block_sum(a2, sh);
+		if (threadIdx.x == 0) {
+			double *gr = grad_acc + (long)(I.comp0 + k) * NGRAD;
+			if (c.type == SKY) gr[G_S0] += a0 * c.s[2];
+			else {
+				gr[G_XCEN] += c.flux_scale * a0;
+				gr[G_YCEN] += c.flux_scale * a1;
+				gr[G_MAG] += c.flux_scale * a2 * c.dlnfs[G_MAG];
+			}
+		}
+		__syncthreads();
+	}
+}
+
 __global__ void k_grad_finalize(const Comp *__restrict__ comps, const double *__restrict__ gscale,
                                 const double *__restrict__ grad_acc, double *grad_out, int ncomp)
 {
@@ -1085,6 +1247,14 @@ int launch_like_finalize(const GalaxyDev *gals, const InstDev *insts, const int 
 {
 	if (ninst <= 0) return 0;
 	k_like_finalize<<<(ninst + 63) / 64, 64, 0, st>>>(gals, insts, lk_off, ninst, partial, ll, gscale);
+	return (int)hipGetLastError();
+}
+
+int launch_adj_nonradial(const GalaxyDev *gals, const InstDev *insts, int ninst, const Comp *comps, double *grad_acc,
+                         hipStream_t st)
+{
+	if (ninst <= 0) return 0;
+	k_adj_nonradial<<<ninst, 256, 0, st>>>(gals, insts, comps, grad_acc);
 	return (int)hipGetLastError();
 }
 

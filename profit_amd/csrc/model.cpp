@@ -138,20 +138,28 @@ static const char *kShapeNames[6][5] = {{"re", "nser", nullptr, nullptr, nullptr
                                         {"re", "rb", "nser", "a", "b"},
                                         {"h1", "h2", "rb", "a", nullptr}};
 
-Profile::Profile(const std::string &name, int kind, int radial_type) : name_(name), kind_(kind)
+Profile::Profile(const std::string &name, int kind, int type) : name_(name), kind_(kind)
 {
-	profit_amd::default_params(radial_type < 0 ? 0 : radial_type, &params_);
-	params_.type = radial_type;
+	profit_amd::default_params(type, &params_);
 }
 
 bool Profile::set_double(const std::string &n, double v)
 {
 	if (kind_ == SKY) {
+		// sky.cpp:65-71
 		if (n == "bg") {
-			bg_ = v;
+			params_.shape[0] = v;
 			return true;
 		}
 		return false;
+	}
+	if (kind_ == POINT) {
+		// psf.cpp:147-156
+		if (n == "xcen") params_.xcen = v;
+		else if (n == "ycen") params_.ycen = v;
+		else if (n == "mag") params_.mag = v;
+		else return false;
+		return true;
 	}
 	if (kind_ != RADIAL) return false;
 	auto &p = params_;
@@ -267,10 +275,8 @@ void Profile::parameter(const std::string &spec)
 	if (parsed && set_double(n, d)) return;
 	if (!parsed) {
 		profit_amd::ProfileParams probe = params_;
-		double bgprobe = bg_;
 		if (set_double(n, 0.0)) {
 			params_ = probe;
-			bg_ = bgprobe;
 			throw parse_error("double");
 		}
 	}
@@ -345,8 +351,9 @@ ConvolverPtr create_convolver(const std::string &type, const ConvolverCreationPr
 ProfilePtr Model::add_profile(const std::string &n)
 {
 	ProfilePtr p;
-	if (n == "null") p = std::make_shared<Profile>(n, Profile::NULLPROF, -1);
-	else if (n == "sky") p = std::make_shared<Profile>(n, Profile::SKY, -1);
+	if (n == "null") p = std::make_shared<Profile>(n, Profile::NULLPROF, profit_amd::NULLPROF);
+	else if (n == "sky") p = std::make_shared<Profile>(n, Profile::SKY, profit_amd::SKY);
+	else if (n == "psf") p = std::make_shared<Profile>(n, Profile::POINT, profit_amd::POINTSRC);
 	else if (n == "sersic") p = std::make_shared<Profile>(n, Profile::RADIAL, profit_amd::SERSIC);
 	else if (n == "moffat") p = std::make_shared<Profile>(n, Profile::RADIAL, profit_amd::MOFFAT);
 	else if (n == "ferrer" || n == "ferrers") p = std::make_shared<Profile>(n, Profile::RADIAL, profit_amd::FERRER);
@@ -392,7 +399,12 @@ Model::Analysis Model::analyze_inputs() const
 	Analysis a;
 	a.convolution_required = std::any_of(profiles_.begin(), profiles_.end(), [](const ProfilePtr &p) { return p->do_convolve(); });
 	if (a.convolution_required && !psf_) throw invalid_parameter("No psf provided but profile(s) requested convolution");
-	for (auto &p : profiles_) p->validate();
+	for (auto &p : profiles_) {
+		p->validate();
+		// PsfProfile::validate (psf.cpp:40-45)
+		if (p->kind() == Profile::POINT && !psf_)
+			throw invalid_parameter("No psf present in the model, cannot produce a psf profile");
+	}
 	analyze_expansion(requested_dimensions_, mask_, psf_, finesampling_, a, adjust_mask_);
 	return a;
 }
@@ -448,10 +460,17 @@ void Model::evaluate(Image &image, Point &offset_out)
 	}
 	const bool has_mask = (bool)*draw_mask;
 
-	std::vector<profit_amd::ProfileParams> radial;
-	for (auto &p : profiles_)
-		if (p->kind() == Profile::RADIAL) radial.push_back(p->params());
-	if (!radial.empty()) {
+	// every profile but "null" is drawn by the engine: radial profiles, the sky
+	// (sky.cpp:47-63) and point sources (psf.cpp:58-145), each into the
+	// convolved or the unconvolved canvas by its convolve flag
+	std::vector<profit_amd::ProfileParams> drawn;
+	bool any_point = false;
+	for (auto &p : profiles_) {
+		if (p->kind() == Profile::NULLPROF) continue;
+		drawn.push_back(p->params());
+		any_point |= p->kind() == Profile::POINT;
+	}
+	if (!drawn.empty()) {
 		profit_amd::GalaxySpec gs;
 		gs.W = (int)(requested_dimensions_.x * finesampling_);
 		gs.H = (int)(requested_dimensions_.y * finesampling_);
@@ -459,11 +478,14 @@ void Model::evaluate(Image &image, Point &offset_out)
 		gs.scale_y = scale_.y;
 		gs.magzero = magzero_;
 		gs.finesampling = (int)finesampling_;
-		if (a.convolution_required) {
+		if (a.convolution_required || any_point) {
 			gs.psf = psf_.data();
 			gs.Kx = (int)psf_.getWidth();
 			gs.Ky = (int)psf_.getHeight();
+			gs.psf_convolves = a.convolution_required;
 		}
+		gs.psf_scale_x = psf_scale_.x;
+		gs.psf_scale_y = psf_scale_.y;
 		if (has_mask) {
 			gs.calc = draw_mask->bytes();
 			gs.cw = (int)draw_mask->getWidth();
@@ -473,25 +495,17 @@ void Model::evaluate(Image &image, Point &offset_out)
 		if (convolver_ && convolver_->type() == FFT) eng.set_conv_method(profit_amd::CK_FFT);
 		if (convolver_ && convolver_->type() == BRUTE_BF16X3) eng.set_conv_method(profit_amd::CK_MFMA);
 		eng.add_galaxy(gs);
-		eng.set_batch({0}, {(int)radial.size()});
+		eng.set_batch({0}, {(int)drawn.size()});
 		const profit_amd::GalaxyDev &g = eng.galaxy(0);
 		if ((unsigned)g.DW != a.drawing_dims.x || (unsigned)g.DH != a.drawing_dims.y)
 			throw std::logic_error("engine drawing dimensions differ from the model analysis");
 		std::vector<float> canvas((size_t)g.DW * g.DH);
 		try {
-			eng.model_canvas(radial.data(), canvas.data());
+			eng.model_canvas(drawn.data(), canvas.data());
 		} catch (const std::invalid_argument &e) {
 			throw invalid_parameter(e.what());
 		}
 		for (size_t k = 0; k < canvas.size(); ++k) image[k] += canvas[k];
-	}
-	// SkyProfile (sky.cpp:40-63): bg / fs^2 on every calculated pixel
-	for (auto &p : profiles_) {
-		if (p->kind() != Profile::SKY) continue;
-		if (p->do_convolve()) throw invalid_parameter("a convolved sky profile is not supported");
-		const double bg = p->sky_bg() / (finesampling_ * finesampling_);
-		for (size_t k = 0; k < image.size(); ++k)
-			if (!has_mask || (*draw_mask)[k]) image[k] += bg;
 	}
 
 	Point offset{0, 0};

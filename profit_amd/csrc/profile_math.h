@@ -25,7 +25,14 @@ namespace profit_amd {
 // hot-path profile math: always inlined into the render and walker kernels
 #define PA_HDI __host__ __device__ __forceinline__
 
-enum ProfileType : int32_t { SERSIC = 0, MOFFAT = 1, FERRER = 2, KING = 3, CORESERSIC = 4, BROKENEXP = 5 };
+enum ProfileType : int32_t {
+	SERSIC = 0, MOFFAT = 1, FERRER = 2, KING = 3, CORESERSIC = 4, BROKENEXP = 5,
+	SKY = 100,      // libprofit "sky": a constant background (sky.cpp), bg in shape[0]
+	NULLPROF = 101, // libprofit "null": draws nothing
+	POINTSRC = 102  // libprofit "psf": a PSF point source (psf.cpp), parameters xcen, ycen, mag
+};
+
+PA_HD bool is_radial(int type) { return type >= SERSIC && type <= BROKENEXP; }
 
 // gradient slots per component
 enum GradSlot : int { G_XCEN = 0, G_YCEN, G_MAG, G_ANG, G_AXRAT, G_BOX, G_S0, G_S1, G_S2, G_S3, G_S4, NGRAD };
@@ -69,7 +76,8 @@ PA_HD void default_params(int type, ProfileParams *p)
 	case KING: p->shape[0] = 1; p->shape[1] = 3; p->shape[2] = 2; break;
 	case CORESERSIC: p->shape[0] = 1; p->shape[1] = 1; p->shape[2] = 4; p->shape[3] = 1; p->shape[4] = 1; break;
 	case BROKENEXP: p->shape[0] = 1; p->shape[1] = 1; p->shape[2] = 1; p->shape[3] = 1; break;
-	default: break;
+	case POINTSRC: p->mag = 0; break;  // psf.cpp:147-156
+	default: break;                    // sky.cpp:65-71: bg = 0
 	}
 }
 
@@ -101,7 +109,7 @@ enum SetupError : int32_t {
 	SE_OK = 0,
 	SE_AXRAT_LE0, SE_AXRAT_GT1, SE_BOX, SE_RE, SE_NSER, SE_FWHM, SE_CON, SE_ROUT, SE_FA, SE_FB,
 	SE_RC, SE_RT, SE_KA, SE_CS_RE, SE_CS_RB, SE_CS_NSER, SE_CS_A, SE_CS_B, SE_H1, SE_H2, SE_BE_RB,
-	SE_TYPE, SE_MAXREC, SE_RESOLUTION
+	SE_TYPE, SE_MAXREC, SE_RESOLUTION, SE_NO_PSF
 };
 
 inline const char *setup_error_message(int e)
@@ -128,9 +136,10 @@ inline const char *setup_error_message(int e)
 	case SE_H1: return "h1 <= 0, must have h1 > 0";
 	case SE_H2: return "h2 <= 0, must have h2 > 0";
 	case SE_BE_RB: return "rb <= 0, must have rb > 0";
-	case SE_TYPE: return "unknown radial profile type";
+	case SE_TYPE: return "unknown profile type";
 	case SE_MAXREC: return "max_recursions above 30 is not supported by the GPU recursion walker";
 	case SE_RESOLUTION: return "resolution above 16 is not supported by the GPU recursion walker";
+	case SE_NO_PSF: return "No psf present in the model, cannot produce a psf profile";
 	default: return "ok";
 	}
 }
@@ -138,6 +147,9 @@ inline const char *setup_error_message(int e)
 PA_HD int validate_params(const ProfileParams &p)
 {
 	const double *s = p.shape;
+	// PsfProfile::validate only asks for a PSF (checked against the galaxy);
+	// SkyProfile::validate checks nothing (psf.cpp:40-45, sky.cpp:36-39)
+	if (p.type == POINTSRC || p.type == SKY || p.type == NULLPROF) return SE_OK;
 	if (p.axrat <= 0) return SE_AXRAT_LE0;
 	if (p.axrat > 1) return SE_AXRAT_GT1;
 	if (p.box <= -2) return SE_BOX;
@@ -518,6 +530,33 @@ PA_HD int setup_component(const ProfileParams &p, double magzero, double sx, dou
 	if (verr != SE_OK) return verr;
 	const double *s = p.shape;
 	const double pi = 3.14159265358979323846;
+	if (!is_radial(p.type)) {
+		// point source and sky: no radial set-up.  xc, yc hold the footprint
+		// origins' centres, which libprofit shifts by offset.x * scale.x on
+		// both axes (psf.cpp:79-82); s[0], s[1] the footprint ends' centres,
+		// shifted by offset.y * scale.y; s[2] = 1 / fs^2 for the sky.
+		*c = Comp{};
+		c->type = p.type;
+		c->convolve = p.convolve;
+		c->grad_ok = 1;
+		c->axrat = 1;
+		c->xc = p.xcen + offx * sx;
+		c->yc = p.ycen + offx * sx;
+		c->s[0] = p.xcen + offy * sy;
+		c->s[1] = p.ycen + offy * sy;
+		for (int i = 0; i < NGRAD; ++i) c->dlnfs[i] = 0;
+		if (p.type == POINTSRC) {
+			c->flux_scale = ::pow(10.0, -0.4 * (p.mag - magzero));  // psf.cpp:66
+			c->dlnfs[G_MAG] = -0.4 * ::log(10.0);
+		}
+		else if (p.type == SKY) {
+			// sky.cpp:42-45: bg / (finesampling * finesampling)
+			const double fs = ::round(img_sx / sx);
+			c->s[2] = 1.0 / (fs * fs);
+			c->flux_scale = s[0] * c->s[2];
+		}
+		return SE_OK;
+	}
 	c->type = p.type;
 	c->rough = p.rough;
 	c->adjust = p.adjust;

@@ -123,7 +123,7 @@ class Model;
 
 class Profile {
   public:
-	Profile(const std::string &name, int kind, int radial_type);
+	Profile(const std::string &name, int kind, int type);
 	virtual ~Profile() = default;
 	const std::string &get_name() const { return name_; }
 	void parameter(const std::string &name, bool value);
@@ -134,11 +134,11 @@ class Profile {
 	void validate() const;
 
 	// engine-facing state
-	enum Kind { RADIAL = 0, SKY = 1, NULLPROF = 2 };
+	enum Kind { RADIAL = 0, SKY = 1, NULLPROF = 2, POINT = 3 };
 	int kind() const { return kind_; }
 	const profit_amd::ProfileParams &params() const { return params_; }
 	profit_amd::ProfileParams &params() { return params_; }
-	double sky_bg() const { return bg_; }
+	double sky_bg() const { return params_.shape[0]; }
 
   private:
 	bool set_double(const std::string &name, double v);
@@ -147,7 +147,6 @@ class Profile {
 	std::string name_;
 	int kind_;
 	profit_amd::ProfileParams params_{};
-	double bg_ = 0;
 	bool convolve_ = false;
 };
 using ProfilePtr = std::shared_ptr<Profile>;
