@@ -39,3 +39,29 @@ def rel_err(test, ref, floor_frac=1e-6):
     peak = np.abs(ref).max()
     sel = np.abs(ref) > floor_frac * peak
     return float(np.max(np.abs(test[sel] - ref[sel]) / np.abs(ref[sel]))), int(sel.sum())
+
+
+class OracleEvaluator:
+    """The FP64 oracle behind profit_amd.fit.FitData's evaluator interface
+    (tests only): logL and its fixed-tree analytic gradient per batch member,
+    with the calc region Model::adjust makes."""
+
+    def __init__(self, data):
+        h, w = data.image.shape
+        self.calcregion = None
+        if data.psf is not None:
+            self.calcregion = O.adjust_mask(data.region.astype(np.uint8), (w, h), data.psf, data.finesample)
+        spec = O.ModelSpec(w, h, psf=data.psf, mask=self.calcregion, adjust_mask=False, magzero=data.magzero,
+                           threads=8)
+        self.fit = O.FitSpec(spec, data.image, data.sigma, data.region.astype(np.uint8), likefunc=data.like_func)
+
+    @staticmethod
+    def _profiles(cl):
+        return [O.make_profile(kind, convolve=conv, **d) for kind, conv, d in cl]
+
+    def like(self, comp_lists):
+        return np.array([O.loglike(self.fit, self._profiles(cl)) for cl in comp_lists])
+
+    def like_grad(self, comp_lists):
+        out = [O.loglike_grad(self.fit, self._profiles(cl)) for cl in comp_lists]
+        return np.array([o[0] for o in out]), np.stack([o[1] for o in out])
