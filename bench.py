@@ -32,6 +32,8 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 PEAK_BF16_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
+# the 14-parameter gradient of the bulge+disk metric: 7 slots per component
+METRIC_SLOTS = ("xcen", "ycen", "mag", "ang", "axrat", "s0", "s1")
 
 
 def walkers(base, n, seed=11):
@@ -123,6 +125,10 @@ def main():
                          "the FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--conv", choices=("auto", "brute", "fft", "brute-bf16x3"), default="auto",
                     help="PSF convolution method (auto: the autotuner's choice among brute and fft)")
+    ap.add_argument("--grad-slots", choices=("metric", "all"), default="metric",
+                    help="gradient computed per component: 'metric' = the 7 parameters of the 14-parameter "
+                         "bulge+disk gradient (xcen, ycen, mag, re, nser, ang, axrat); 'all' adds box and "
+                         "the unused shape slots")
     args = ap.parse_args()
     if args.calibrate_copy:
         import torch
@@ -154,6 +160,8 @@ def main():
     eng.set_tile_rows(args.tile_rows)
     eng.set_render_fp64(args.render_fp64)
     eng.set_conv_method("brute" if args.conv == "auto" else args.conv)
+    slots = METRIC_SLOTS if args.grad_slots == "metric" else None
+    eng.set_grad_slots(slots)
     eng.set_batch([g] * B, [2] * B)
     sets = walkers(wl["components"], B)
     params = gpu_params(P, sets)
@@ -193,6 +201,7 @@ def main():
                          likefunc="chisq", magzero=30.0)
     eng1.set_render_fp64(args.render_fp64)
     eng1.set_conv_method(conv_used)
+    eng1.set_grad_slots(slots)
     eng1.set_batch([g1], [2])
     fd_value = None
     if args.fd_evals > 0:
@@ -280,8 +289,9 @@ def main():
         "config": {
             "workload": "400x400 bulge+disk (Sersic n=4 re=3 mag=17 axrat=0.9; Sersic n=1 re=15 mag=16 "
                         "axrat=0.4 ang=60) convolved with a 51x51 Moffat PSF (FWHM 4, con 3); chi^2 over an "
-                        "elliptical region (70% of pixels); magzero 30; adjoint gradient of 8 parameters per "
-                        "component",
+                        "elliptical region (70% of pixels); magzero 30; adjoint gradient of "
+                        + ("the 7 metric parameters (xcen, ycen, mag, re, nser, ang, axrat)" if args.grad_slots ==
+                           "metric" else "all 11 slots") + " per component",
             "batch": B,
             "drawing_canvas": "450x450",
             "tile_rows": args.tile_rows,

@@ -176,6 +176,13 @@ int profit_amd_engine_set_tile_rows(profit_amd_engine *engine, int32_t rows);
  * per-sample derivatives are FP32 (FP64 for steep cores). */
 int profit_amd_engine_set_render_fp64(profit_amd_engine *engine, int32_t on);
 
+/* Gradient slots profit_amd_engine_like_grad computes: bit s of `mask` asks
+ * for slot s (0 xcen .. 10 shape[4]; default all).  Slots left out come back
+ * as 0; leaving out the box slot (bit 5) spares the per-sample logarithms of
+ * d / d box.  No counterpart in the reference, whose R layer has no gradient
+ * (R/profitLikeModel.R:244-342); it serves callers that fit a subset. */
+int profit_amd_engine_set_grad_slots(profit_amd_engine *engine, uint32_t mask);
+
 /* Convolution method of the engine's galaxies with a PSF, the choice that
  * Model::set_convolver (src/libprofit/src/profit/model.h:173-176) makes with a
  * BruteForceConvolver or an FFTConvolver (convolve.h:120-209):

@@ -101,6 +101,7 @@ SYMBOLS = [
     ("profit_amd_engine_set_batch", C.c_int, [C.c_void_p, _P(C.c_int32), _P(C.c_int32), C.c_int32]),
     ("profit_amd_engine_set_tile_rows", C.c_int, [C.c_void_p, C.c_int32]),
     ("profit_amd_engine_set_render_fp64", C.c_int, [C.c_void_p, C.c_int32]),
+    ("profit_amd_engine_set_grad_slots", C.c_int, [C.c_void_p, C.c_uint32]),
     ("profit_amd_engine_set_conv_method", C.c_int, [C.c_void_p, C.c_int32]),
     ("profit_amd_engine_autotune", C.c_int, [C.c_void_p, _P(Profile), C.c_int32, C.c_int32, _P(C.c_int32),
                                              _P(C.c_double)]),
@@ -356,6 +357,17 @@ class Engine:
 
     def set_render_fp64(self, on=True):
         _check(lib().profit_amd_engine_set_render_fp64(self._h, int(on)))
+
+    def set_grad_slots(self, slots=None):
+        """Gradient slots to compute (names from GRAD_SLOTS or indices; None =
+        all); the others come back as 0."""
+        if slots is None:
+            mask = (1 << NGRAD) - 1
+        else:
+            mask = 0
+            for s in slots:
+                mask |= 1 << (GRAD_SLOTS.index(s) if isinstance(s, str) else int(s))
+        _check(lib().profit_amd_engine_set_grad_slots(self._h, mask))
 
     def set_conv_method(self, method="brute"):
         """'brute' (tiled FP32 brute force), 'fft' (FP64 FFT) or 'brute-bf16x3' (matrix

@@ -292,6 +292,14 @@ int profit_amd_engine_set_render_fp64(profit_amd_engine *engine, int32_t on)
 	return guarded([&] { engine->eng.set_render_fp64(on != 0); });
 }
 
+int profit_amd_engine_set_grad_slots(profit_amd_engine *engine, uint32_t mask)
+{
+	return guarded([&] {
+		if (!engine) throw std::invalid_argument("null engine");
+		engine->eng.set_grad_slots(mask);
+	});
+}
+
 int profit_amd_engine_set_conv_method(profit_amd_engine *engine, int32_t convolver)
 {
 	return guarded([&] {

@@ -42,6 +42,7 @@ struct GalaxyDev {
 	int32_t n_rtiles, n_ctiles, n_atiles;
 	int32_t conv_rows;     // rows per thread of the convolution tiles
 	int32_t render_fp64;   // profile values (direct pixels, recursion leaves) in FP64 instead of FP32
+	int32_t grad_box;      // the adjoint computes d / d box (the engine's gradient-slot mask)
 	double sx, sy;         // profile pixel scale
 	double img_sx, img_sy; // image pixel scale
 	double magzero;

@@ -414,6 +414,18 @@ void Engine::set_render_fp64(bool on)
 	if (!inst_galaxy_.empty()) set_batch(inst_galaxy_, inst_ncomp_);
 }
 
+void Engine::set_grad_slots(unsigned mask)
+{
+	mask &= (1u << NGRAD) - 1u;
+	if (mask == grad_slots_) return;
+	const bool box = (mask >> G_BOX) & 1u;
+	grad_slots_ = mask;
+	for (auto &g : gal_host_) g.grad_box = box ? 1 : 0;
+	gals_dirty_ = true;
+	invalidate_graphs(graph_exec_);
+	if (!inst_galaxy_.empty()) set_batch(inst_galaxy_, inst_ncomp_);
+}
+
 int Engine::add_galaxy(const GalaxySpec &s)
 {
 	if (s.W <= 0 || s.H <= 0) throw std::invalid_argument("Model's requested dimensions are 0");
@@ -455,6 +467,7 @@ int Engine::add_galaxy(const GalaxySpec &s)
 	}
 	g.has_calc = s.calc != nullptr;
 	g.render_fp64 = render_fp64_ ? 1 : 0;
+	g.grad_box = (grad_slots_ >> G_BOX) & 1u;
 	const size_t npix = (size_t)s.W * s.H, ndraw = (size_t)g.DW * g.DH;
 	if (have_psf) {
 		// the model's PSF for point sources (psf.cpp:58-145 reads model.psf, normalised by set_psf)
@@ -881,7 +894,7 @@ void Engine::record(bool grad)
 		          "adjoint subsample");
 		mark(8);
 		hip_check(launch_grad_finalize(d_comps_.as<Comp>(), d_gscale_.as<double>(), d_grad_acc_.as<double>(),
-		                               d_grad_out_.as<double>(), nc, st),
+		                               d_grad_out_.as<double>(), nc, grad_slots_, st),
 		          "grad");
 		mark(9);
 		hip_check(hipMemcpyAsync(h_grad_.as<double>(), d_grad_out_.as<double>(), (size_t)nc * NGRAD * sizeof(double),

@@ -35,7 +35,7 @@ int launch_like_t(const GalaxyDev *gals, const InstDev *insts, int ninst, int gr
 int launch_adj_render(const GalaxyDev *gals, const InstDev *insts, const int *grp_off, int ninst, int nblocks, int grp,
                       const Comp *comps, double *grad_acc, hipStream_t st);
 int launch_grad_finalize(const Comp *comps, const double *gscale, const double *grad_acc, double *grad_out,
-                         int ncomp, hipStream_t st);
+                         int ncomp, unsigned slot_mask, hipStream_t st);
 int launch_dilate(const uint8_t *in, uint8_t *tmp, uint8_t *out, int w, int h, int padx, int pady, hipStream_t st);
 // FFT path (fft.hip)
 int launch_fft_rows_fwd(int nblocks, int nmax, const GalaxyDev *gals, const InstDev *insts, const int *row_off,
@@ -193,6 +193,10 @@ class Engine {
 	void set_conv_rows(int r);
 	// profile values (direct pixels, recursion leaves) in FP32 (default) or FP64
 	void set_render_fp64(bool on);
+	// gradient slots computed (bit s = slot s); the others come back as 0.
+	// Leaving out the box slot spares the per-sample logarithms of d / d box.
+	void set_grad_slots(unsigned mask);
+	unsigned grad_slots() const { return grad_slots_; }
 	bool render_fp64() const { return render_fp64_; }
 	void set_timing(bool on);
 	void reset_stats() { stats_ = EngineStats{}; }
@@ -244,6 +248,7 @@ class Engine {
 	int mfma_pitch_ = 0, mfma_kch_ = 0;
 	int fft_cb_ = 8;
 	bool render_fp64_ = false;
+	unsigned grad_slots_ = (1u << NGRAD) - 1u;
 	int kc_max_ = 64;
 	bool timing_ = false;
 	EngineStats stats_;

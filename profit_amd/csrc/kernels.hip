@@ -55,6 +55,7 @@ __global__ void k_setup(const ProfileParams *__restrict__ params, const int *__r
 	Comp out;
 	int e = setup_component(params[c], g.magzero, g.sx, g.sy, g.img_sx, g.img_sy, g.padx, g.pady, &qs, &out);
 	if (e == SE_OK && params[c].type == POINTSRC && !g.psf_pt) e = SE_NO_PSF;
+	out.grad_box = g.grad_box;
 	out.instance = inst;
 	comps[c] = out;
 	errors[c] = e;
@@ -1120,12 +1121,12 @@ __global__ __launch_bounds__(256) void k_adj_nonradial(const GalaxyDev *__restri
 }
 
 __global__ void k_grad_finalize(const Comp *__restrict__ comps, const double *__restrict__ gscale,
-                                const double *__restrict__ grad_acc, double *grad_out, int ncomp)
+                                const double *__restrict__ grad_acc, double *grad_out, int ncomp, unsigned slot_mask)
 {
 	const int idx = blockIdx.x * blockDim.x + threadIdx.x;
 	if (idx >= ncomp * NGRAD) return;
-	const int c = idx / NGRAD;
-	grad_out[idx] = gscale[comps[c].instance] * grad_acc[idx];
+	const int c = idx / NGRAD, s = idx - c * NGRAD;
+	grad_out[idx] = ((slot_mask >> s) & 1u) ? gscale[comps[c].instance] * grad_acc[idx] : 0.0;
 }
 
 // ------------------------------------------------------------------ masks
@@ -1276,11 +1277,11 @@ int launch_adj_render(const GalaxyDev *gals, const InstDev *insts, const int *gr
 }
 
 int launch_grad_finalize(const Comp *comps, const double *gscale, const double *grad_acc, double *grad_out,
-                         int ncomp, hipStream_t st)
+                         int ncomp, unsigned slot_mask, hipStream_t st)
 {
 	if (ncomp <= 0) return 0;
 	const int n = ncomp * NGRAD;
-	k_grad_finalize<<<(n + 255) / 256, 256, 0, st>>>(comps, gscale, grad_acc, grad_out, ncomp);
+	k_grad_finalize<<<(n + 255) / 256, 256, 0, st>>>(comps, gscale, grad_acc, grad_out, ncomp, slot_mask);
 	return (int)hipGetLastError();
 }
 

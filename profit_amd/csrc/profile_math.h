@@ -90,7 +90,7 @@ struct Comp {
 	int32_t rough, centre_rule, boxy, has_rmax;
 	int32_t convolve, instance, grad_ok, adjust;
 	int32_t grad64;  // per-sample adjoint derivatives in FP64 (steep cores)
-	int32_t pad_;
+	int32_t grad_box;  // the box derivative is wanted (it costs logarithms per sample)
 	double xc, yc;                    // centre after the PSF-padding shift
 	double cos_ang, sin_ang, cos_over, sin_over;
 	double rmax_r, rsw_r;             // rscale_max*rscale, rscale_switch*rscale
@@ -99,6 +99,7 @@ struct Comp {
 	double box, rscale, bn, axrat;
 	double s[5];
 	double dbn;                       // d b_n / d nser (Sersic, core-Sersic)
+	double gk;                        // Moffat: d ln rscale / d con
 	double dlnfs[NGRAD];              // d ln(flux_scale) / d slot
 	double lumtot, rbox, ie;
 	double rscale_switch, rscale_max; // adjusted values
@@ -348,6 +349,7 @@ PA_HDI void dr_terms(const Comp &c, T x, T y, T r, T *drx, T *dry, T *drB)
 		*drx = ax > T(0) ? rp * ::pow(ax, B - T(1)) * (x > T(0) ? T(1) : T(-1)) : T(0);
 		*dry = ay > T(0) ? rp * ::pow(ay, B - T(1)) * (y > T(0) ? T(1) : T(-1)) : T(0);
 	}
+	if (!c.grad_box) return;
 	const T px = ::pow(ax, B), py = ::pow(ay, B);
 	const T S = px + py;
 	const T Sl = (ax > T(0) ? px * ::log(ax) : T(0)) + (ay > T(0) ? py * ::log(ay) : T(0));
@@ -376,20 +378,37 @@ PA_HDI void sample_grad(const Comp &c, T x, T y, T *o)
 	for (int i = 0; i < 9; ++i) o[i] = T(0);
 	switch (c.type) {
 	case SERSIC: {
+		// f = exp(-bn (base^(1/(n B)) - 1)), base = |x/re|^B + |y/re|^B, B = box + 2.
+		// Without box the powers are squares, and f follows from the same
+		// logarithm as the derivatives: one log and two exps per sample
+		// (plus two logs when the box derivative is wanted).
 		const T re = T(c.s[0]), n = T(c.s[1]), bn = T(c.bn), B = T(c.box) + T(2);
 		const T u = ::fabs(x / re), v = ::fabs(y / re);
-		const T pu = ::pow(u, B), pv = ::pow(v, B);
+		T pu, pv;
+		if (c.boxy) {
+			pu = ::pow(u, B);
+			pv = ::pow(v, B);
+		}
+		else {
+			pu = u * u;
+			pv = v * v;
+		}
 		const T base = pu + pv;
-		const T f = sersic_at<T>(c, x, y);
-		o[0] = f;
-		if (!(base > T(0))) return;
+		if (!(base > T(0))) {
+			o[0] = ::exp(bn);
+			return;
+		}
 		const T lb = ::log(base);
 		const T rf = ::exp(lb / (n * B));
+		const T f = ::exp(-bn * (rf - T(1)));
+		o[0] = f;
 		const T k = -bn * f * rf / (n * B * base);  // df/dbase
 		o[1] = u > T(0) ? k * B * pu / u / re * (x > T(0) ? T(1) : T(-1)) : T(0);
 		o[2] = v > T(0) ? k * B * pv / v / re * (y > T(0) ? T(1) : T(-1)) : T(0);
-		const T dbase_dB = (u > T(0) ? pu * ::log(u) : T(0)) + (v > T(0) ? pv * ::log(v) : T(0));
-		o[3] = -bn * f * rf * (dbase_dB / (n * B * base) - lb / (n * B * B));
+		if (c.grad_box) {
+			const T dbase_dB = (u > T(0) ? pu * ::log(u) : T(0)) + (v > T(0) ? pv * ::log(v) : T(0));
+			o[3] = -bn * f * rf * (dbase_dB / (n * B * base) - lb / (n * B * B));
+		}
 		o[4] = bn * f * rf / (n * re);
 		o[5] = f * (-T(c.dbn) * (rf - T(1)) + bn * rf * lb / (n * n * B));
 		return;
@@ -406,10 +425,8 @@ PA_HDI void sample_grad(const Comp &c, T x, T y, T *o)
 		o[2] = dfdr * dry;
 		o[3] = dfdr * drB;
 		const T dfdrs = -dfdr * r / rs;
-		const T tt = ::pow(T(2), T(1) / con);
-		const T dlnrs_dcon = T(0.5) * tt * T(0.69314718055994530942) / (con * con * (tt - T(1)));
 		o[4] = dfdrs * rs / T(c.s[0]);
-		o[5] = -f * ::log(q) + dfdrs * rs * dlnrs_dcon;
+		o[5] = -f * ::log(q) + dfdrs * rs * T(c.gk);
 		return;
 	}
 	case FERRER: {
@@ -571,11 +588,17 @@ PA_HD int setup_component(const ProfileParams &p, double magzero, double sx, dou
 	// slope, Sersic n > 4) cancel over many large samples, so their per-sample
 	// derivatives are taken in FP64
 	c->grad64 = (p.type == CORESERSIC && s[4] != 0) || (p.type == SERSIC && s[1] > 4);
-	c->pad_ = 0;
+	c->grad_box = 1;
 	for (int i = 0; i < 5; ++i) c->s[i] = s[i];
 	for (int i = 0; i < NGRAD; ++i) c->dlnfs[i] = 0;
 	c->bn = 0;
 	c->dbn = 0;
+	c->gk = 0;
+	if (p.type == MOFFAT) {
+		// rscale = fwhm / (2 sqrt(2^(1/con) - 1)): d ln rscale / d con
+		const double tt = ::pow(2.0, 1 / s[1]);
+		c->gk = 0.5 * tt * 0.69314718055994530942 / (s[1] * s[1] * (tt - 1));
+	}
 
 	if (p.type == SERSIC || p.type == CORESERSIC) c->bn = sf::qgamma(0.5, 2 * (p.type == SERSIC ? s[1] : s[2]));
 	switch (p.type) {

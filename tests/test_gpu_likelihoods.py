@@ -89,3 +89,24 @@ def test_t_data_keep_dof_unclamped(galaxies):
     z = (g["data"] - O.model_evaluate(g["fit"].spec, oracle_profiles(COMPS))["image"]) / g["sigma"]
     v = np.var(z[g["region"] == 1], ddof=1)
     assert 1.5 < v < 3.0  # dof = 2v / (v - 1) between 3 and 6: unclamped
+
+
+def test_gradient_slot_mask(galaxies):
+    """Slots left out of the mask come back as 0 and the others are unchanged
+    (leaving out the box slot skips its per-sample logarithms)."""
+    g = galaxies["chisq"]
+    eng = P.Engine()
+    gid = eng.add_galaxy(SIZE, SIZE, g["data"], g["sigma"], psf=g["psf"], calcregion=g["calc"], region=g["region"],
+                         magzero=MAGZERO)
+    eng.set_batch([gid], [len(COMPS)])
+    params = gpu_profiles(_perturbed())
+    ll, full = eng.like_grad(params)
+    eng.set_grad_slots(["xcen", "ycen", "mag", "ang", "axrat", "s0", "s1"])
+    ll2, part = eng.like_grad(params)
+    assert ll2[0] == ll[0]
+    keep = [0, 1, 2, 3, 4, 6, 7]
+    assert np.all(part[:, [5, 8, 9, 10]] == 0)
+    assert np.allclose(part[:, keep], full[:, keep], rtol=1e-9, atol=0)
+    eng.set_grad_slots(None)
+    _, again = eng.like_grad(params)
+    assert np.allclose(again, full, rtol=1e-9, atol=0)
