@@ -232,10 +232,12 @@ def main():
     render_bytes = B * dw * dh * 5.0
     kernels = {
         "render": ("k_render (classification + direct pixels)", "hbm", render_bytes, PEAK_HBM_GBS, "GB/s"),
-        "subsample": ("k_subsample<false> (recursion walker)", "hbm", render_bytes, PEAK_HBM_GBS, "GB/s"),
-        "adjoint_render": ("k_adj_render (adjoint of direct pixels)", "hbm", render_bytes, PEAK_HBM_GBS, "GB/s"),
-        "adjoint_subsample": ("k_subsample<true> (adjoint recursion walker)", "hbm", render_bytes, PEAK_HBM_GBS,
-                              "GB/s"),
+        "subsample": ("k_subsample<false, 0, 0> (recursion walker, Sersic)", "hbm", render_bytes, PEAK_HBM_GBS,
+                      "GB/s"),
+        "adjoint_render": ("k_adj_render<0, *> (adjoint of direct pixels, Sersic)", "hbm", render_bytes,
+                           PEAK_HBM_GBS, "GB/s"),
+        "adjoint_subsample": ("k_subsample<true, -1, 2> (adjoint recursion walker)", "hbm", render_bytes,
+                              PEAK_HBM_GBS, "GB/s"),
     }
     if conv_used == "fft":
         fft_flops = B * 3 * 5.0 * n_fft * np.log2(n_fft)
@@ -259,9 +261,13 @@ def main():
     if os.path.exists(tfile) and conv_used == "fft":
         with open(tfile) as f:
             tk = json.load(f)["kernels"]
-        key = kname.split(" ")[0]
+        key = kname.split(" (")[0]
         if key in tk:
             traffic = tk[key]["hbm_bytes_per_launch"]
+        elif key.startswith("FFT"):
+            # the stage's three kernels: rows forward, columns, rows inverse
+            parts = [k for k in ("k_fft_rows_fwd", "k_fft_cols", "k_fft_rows_inv") if k in tk]
+            traffic = sum(tk[k]["hbm_bytes_per_launch"] for k in parts) if len(parts) == 3 else None
     kms = stage_ms[dom]
     achieved = work / (kms * 1e-3) / (1e9 if unit == "GB/s" else 1e12)
     stage_per_step = {k: v / args.steps for k, v in sm.items()}

@@ -101,7 +101,12 @@ struct WorkQueue {
 	int *count;
 	int lcap, hcap;
 };
-constexpr int kQueueCounters = 8;
+// count[0], count[1]: lengths of the light and heavy lists; then a (heavy,
+// light) pair of hand-out counters per walker launch: the forward walk of
+// each profile family, then the adjoint walk
+constexpr int kQueueCounters = 16;
+__host__ __device__ constexpr int fwd_counter(int type) { return 2 + 2 * type; }
+constexpr int kAdjCounter = 14;
 constexpr int kGrab = 8;  // ordinary entries handed to a wavefront at a time
 
 

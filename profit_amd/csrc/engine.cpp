@@ -869,8 +869,7 @@ void Engine::record(bool grad)
 	                        work_queue(), -1, 0, nullptr, st),
 	          "render");
 	mark(2);
-	hip_check(launch_subsample(false, subsample_blocks_, gals, insts, d_comps_.as<Comp>(), work_queue(), 0, nullptr, st),
-	          "subsample");
+	launch_walkers_fwd(0, st);
 	mark(3);
 	conv_forward(0, st);
 	mark(4);
@@ -883,15 +882,19 @@ void Engine::record(bool grad)
 	if (grad) {
 		conv_adjoint(st);
 		mark(6);
-		hip_check(launch_adj_render(gals, insts, d_ag_off_.as<int>(), n, n_ag_blocks_, adj_grp_, d_comps_.as<Comp>(),
-		                            d_grad_acc_.as<double>(), st),
-		          "adjoint render");
+		for (int key = 0; key < 12; ++key)
+			if ((grad_keys_ >> key) & 1u)
+				hip_check(launch_adj_render(key >> 1, key & 1, gals, insts, d_ag_off_.as<int>(), n, n_ag_blocks_, adj_grp_,
+				                            d_comps_.as<Comp>(), d_grad_acc_.as<double>(), st),
+				          "adjoint render");
 		if (any_nonradial_)
 			hip_check(launch_adj_nonradial(gals, insts, n, d_comps_.as<Comp>(), d_grad_acc_.as<double>(), st),
 			          "adjoint point sources and sky");
 		mark(7);
-		hip_check(launch_subsample(true, subsample_blocks_, gals, insts, d_comps_.as<Comp>(), work_queue(), 0, d_grad_acc_.as<double>(), st),
-		          "adjoint subsample");
+		if (radial_types_)
+			hip_check(launch_subsample_adj(subsample_blocks_, gals, insts, d_comps_.as<Comp>(), work_queue(),
+			                               d_grad_acc_.as<double>(), st),
+			          "adjoint subsample");
 		mark(8);
 		hip_check(launch_grad_finalize(d_comps_.as<Comp>(), d_gscale_.as<double>(), d_grad_acc_.as<double>(),
 		                               d_grad_out_.as<double>(), nc, grad_slots_, st),
@@ -928,11 +931,16 @@ bool Engine::sync_aux_flags(bool enable)
 {
 	const ProfileParams *p = h_params_.as<ProfileParams>();
 	bool any = false, changed = false;
+	unsigned types = 0, keys = 0;
 	for (int i = 0; i < ninst(); ++i) {
 		int aux = 0;
 		for (int k = 0; k < inst_ncomp_[i]; ++k) {
 			const ProfileParams &q = p[inst_comp0_[i] + k];
-			if (is_radial(q.type)) continue;
+			if (is_radial(q.type)) {
+				types |= 1u << q.type;
+				keys |= 1u << (2 * q.type + (wants_grad64(q) ? 1 : 0));
+				continue;
+			}
 			any = true;
 			if (!q.convolve) aux = 1;
 		}
@@ -944,12 +952,24 @@ bool Engine::sync_aux_flags(bool enable)
 			changed = true;
 		}
 	}
-	if (any != any_nonradial_) {
+	if (any != any_nonradial_ || types != radial_types_ || keys != grad_keys_) {
 		any_nonradial_ = any;
+		radial_types_ = types;
+		grad_keys_ = keys;
 		invalidate_graphs(graph_exec_);
 		changed = true;
 	}
 	return changed;
+}
+
+// the forward walk of every radial family present
+void Engine::launch_walkers_fwd(int split, hipStream_t st)
+{
+	for (int t = 0; t < 6; ++t)
+		if ((radial_types_ >> t) & 1u)
+			hip_check(launch_subsample_fwd(t, subsample_blocks_, d_gals_.as<GalaxyDev>(), d_insts_.as<InstDev>(),
+			                               d_comps_.as<Comp>(), work_queue(), split, st),
+			          "subsample");
 }
 
 void Engine::run(bool grad, bool use_graph)
@@ -1069,8 +1089,7 @@ void Engine::model_images(const ProfileParams *params, float *out)
 		hip_check(launch_render(gals, insts, d_at_off_.as<int>(), n, n_at_blocks_, d_comps_.as<Comp>(),
 		                        work_queue(), -1, 0, nullptr, st),
 		          "render");
-		hip_check(launch_subsample(false, subsample_blocks_, gals, insts, d_comps_.as<Comp>(), work_queue(), 0, nullptr, st),
-		          "subsample");
+		launch_walkers_fwd(0, st);
 		conv_forward(1, st);
 		hip_check(hipMemcpyAsync(h_err_.as<int>(), d_err_.as<int>(), nc * sizeof(int), hipMemcpyDeviceToHost, st), "err");
 		hip_check(hipMemcpyAsync(h_qcount_.as<int>(), d_qcount_.as<int>(), 2 * sizeof(int), hipMemcpyDeviceToHost, st),
@@ -1120,8 +1139,7 @@ void Engine::model_canvas(const ProfileParams *params, float *out)
 			hip_check(launch_render(gals, insts, d_at_off_.as<int>(), 1, n_at_blocks_, d_comps_.as<Comp>(),
 			                        work_queue(), 0, 1, nullptr, st),
 			          "render");
-		hip_check(launch_subsample(false, subsample_blocks_, gals, insts, d_comps_.as<Comp>(), work_queue(), conv ? 1 : 0, nullptr, st),
-		          "subsample");
+		launch_walkers_fwd(conv ? 1 : 0, st);
 		if (conv) conv_forward(1, st);
 		hip_check(hipMemcpyAsync(h_err_.as<int>(), d_err_.as<int>(), nc * sizeof(int), hipMemcpyDeviceToHost, st), "err");
 		hip_check(hipMemcpyAsync(h_qcount_.as<int>(), d_qcount_.as<int>(), 2 * sizeof(int), hipMemcpyDeviceToHost, st),

@@ -20,8 +20,13 @@ int launch_zero_d(double *p, long n, hipStream_t st);
 int launch_zero_i(int *p, long n, hipStream_t st);
 int launch_render(const GalaxyDev *gals, const InstDev *insts, const int *tile_off, int ninst, int nblocks,
                   const Comp *comps, WorkQueue wq, int filter, int target, int *diag_cls, hipStream_t st);
-int launch_subsample(bool grad, int nblocks, const GalaxyDev *gals, const InstDev *insts, const Comp *comps,
-                     WorkQueue wq, int split, double *grad_acc, hipStream_t st);
+// the forward walker and the adjoint render, one launch per radial family
+// (and per derivative precision in the adjoint render: 1 = FP64, steep cores
+// only); the adjoint walker, one launch for all
+int launch_subsample_fwd(int type, int nblocks, const GalaxyDev *gals, const InstDev *insts, const Comp *comps,
+                         WorkQueue wq, int split, hipStream_t st);
+int launch_subsample_adj(int nblocks, const GalaxyDev *gals, const InstDev *insts, const Comp *comps, WorkQueue wq,
+                         double *grad_acc, hipStream_t st);
 int launch_subsample_nodes(int nblocks, const GalaxyDev *gals, const InstDev *insts, const Comp *comps,
                            WorkQueue wq, int *nodes, hipStream_t st);
 int launch_conv(int mode, int rows, int nblocks, int kx_max, int kc, const GalaxyDev *gals, const InstDev *insts,
@@ -32,8 +37,8 @@ int launch_adj_nonradial(const GalaxyDev *gals, const InstDev *insts, int ninst,
                          hipStream_t st);
 int launch_like_t(const GalaxyDev *gals, const InstDev *insts, int ninst, int grad, double *ll, double *gscale,
                   hipStream_t st);
-int launch_adj_render(const GalaxyDev *gals, const InstDev *insts, const int *grp_off, int ninst, int nblocks, int grp,
-                      const Comp *comps, double *grad_acc, hipStream_t st);
+int launch_adj_render(int type, int prec, const GalaxyDev *gals, const InstDev *insts, const int *grp_off, int ninst,
+                      int nblocks, int grp, const Comp *comps, double *grad_acc, hipStream_t st);
 int launch_grad_finalize(const Comp *comps, const double *gscale, const double *grad_acc, double *grad_out,
                          int ncomp, unsigned slot_mask, hipStream_t st);
 int launch_dilate(const uint8_t *in, uint8_t *tmp, uint8_t *out, int w, int h, int padx, int pady, hipStream_t st);
@@ -265,6 +270,10 @@ class Engine {
 	// pass), and per instance whether one is drawn without convolution
 	bool any_nonradial_ = false;
 	std::vector<int> inst_aux_;
+	// radial families present (bit t) and the adjoint's (family, precision)
+	// keys present (bit 2t + prec), which decide the walker launches
+	unsigned radial_types_ = 0, grad_keys_ = 0;
+	void launch_walkers_fwd(int split, hipStream_t st);
 	bool sync_aux_flags(bool enable);
 	std::vector<int> ct_off_, at_off_, ag_off_;
 	int n_ct_blocks_ = 0, n_at_blocks_ = 0, n_ag_blocks_ = 0;

@@ -19,27 +19,9 @@
 // and of the recursion rounds exactly as libprofit's own build does.
 #include <hip/hip_runtime.h>
 
-#include "device.h"
+#include "kernel_common.h"
 
 namespace profit_amd {
-
-__device__ inline int owner_of(const int *__restrict__ off, int n, int b)
-{
-	int lo = 0, hi = n - 1;
-	while (lo < hi) {
-		const int mid = (lo + hi + 1) >> 1;
-		if (off[mid] <= b) lo = mid;
-		else hi = mid - 1;
-	}
-	return lo;
-}
-
-__device__ inline double wave_sum(double v)
-{
-#pragma unroll
-	for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-	return v;
-}
 
 // ------------------------------------------------------------------ set-up
 
@@ -96,13 +78,6 @@ __device__ inline PixelClass classify(const Comp &c, double x, double y)
 }
 
 // ------------------------------------------------------------------ forward render
-
-// sersic.cpp:331-347: pixels within one image pixel of the centre of an
-// nser > 1 Sersic profile are sub-sampled at resolution 8, depth 10
-__device__ __forceinline__ bool centre_pixel(const Comp &c, double x, double y)
-{
-	return c.centre_rule && fabs(x - c.xc) < c.img_sx && fabs(y - c.yc) < c.img_sy;
-}
 
 // wave-aggregated append of the lanes with `take` to a work list
 __device__ __forceinline__ void append_entry(bool take, int lane, QEntry *list, int *count, int cap, QEntry e)
@@ -269,277 +244,7 @@ __global__ __launch_bounds__(256) void k_render(const GalaxyDev *__restrict__ ga
 	}
 }
 
-// ------------------------------------------------------------------ recursion walker
-
-struct WalkLevel {
-	double x0, x1, y0, y1, weight;
-	unsigned long long mask[4];
-};
-
-__device__ __forceinline__ double cell_centre(double lo, double half, int idx)
-{
-	// the same sequence of additions as libprofit's loops (radial.cpp:86-106)
-	double v = lo;
-	for (int t = 0; t < idx; ++t) {
-		v += half;
-		v += half;
-	}
-	return v + half;
-}
-
-// d(f)/d slot of one sample in precision T, chain rule to the component
-// slots, accumulated with weight wgt (FP64) into gs; returns f.
-template <class T>
-__device__ __forceinline__ double sample_accumulate(const Comp &c, double xp, double yp, double wgt, double *gs)
-{
-	T o[9];
-	const T fxp = (T)xp, fyp = (T)yp;
-	sample_grad<T>(c, fxp, fyp, o);
-	const T q = (T)c.axrat;
-	const T ca = (T)c.cos_ang, sa = (T)c.sin_ang;
-	const T fx = o[1], fy = o[2];
-	gs[G_XCEN] += wgt * (double)(fx * -ca + fy * (sa / q));
-	gs[G_YCEN] += wgt * (double)(fx * -sa + fy * (-ca / q));
-	gs[G_ANG] += wgt * (double)((fx * (q * fyp) + fy * (-fxp / q)) * (T(3.14159265358979323846) / T(180)));
-	gs[G_AXRAT] += wgt * (double)(fy * (-fyp / q));
-	gs[G_BOX] += wgt * (double)o[3];
-#pragma unroll
-	for (int s = 0; s < 5; ++s) gs[G_S0 + s] += wgt * (double)o[4 + s];
-	return (double)o[0];
-}
-
-__device__ __forceinline__ double sample_accumulate_any(const Comp &c, double xp, double yp, double wgt, double *gs)
-{
-	if (c.grad64) return sample_accumulate<double>(c, xp, yp, wgt, gs);
-	return sample_accumulate<float>(c, xp, yp, wgt, gs);
-}
-
-// libprofit's refinement test |test - val| > acc * val (radial.cpp:92-100).
-// Evaluated in FP32 first; only when the FP32 margin lies within a guard band
-// of 1e-3 * val around the threshold (far wider than the FP32 evaluation
-// error) is it re-evaluated in FP64, so the decision equals the FP64 one.
-// Returns the decision; *v64 receives the FP64 value when it was computed.
-__device__ __forceinline__ bool refine_cell(const Comp &c, double xp, double yp, double dyp, bool recurse, float v32,
-                                   double *v64, bool *have64)
-{
-	*have64 = false;
-	if (!recurse) return false;
-	const float t32 = eval_at<float>(c, (float)fabs(xp), (float)(fabs(yp) + fabs(dyp)));
-	const float margin = fabsf(t32 - v32) - (float)c.acc * v32;
-	if (v32 > 1e-30f && isfinite(margin) && fabsf(margin) > 1e-3f * v32) return margin > 0.f;
-	const double v = eval_at<double>(c, xp, yp);
-	const double t = eval_at<double>(c, fabs(xp), fabs(yp) + fabs(dyp));
-	*v64 = v;
-	*have64 = true;
-	return fabs(t - v) > c.acc * v;
-}
-
-// Walks the recursion tree of one pixel.  Every lane returns its share of
-// sum_s weight_s * f(s) (FP64 sample values) or, when kGrad, of the
-// derivative sums (FP32 per-sample derivatives); `scale` multiplies the
-// weights (G(q) * flux_scale in the adjoint, 1 otherwise).
-template <bool kGrad>
-__device__ __forceinline__ void walk_pixel(const Comp &c, WalkLevel *L, int lane, double x0, double x1, double y0, double y1, int res,
-                           int maxrec, double scale, bool fp64, double &sum, double *gs)
-{
-	const int ncell = res * res;
-	const int nchunk = (ncell + 63) >> 6;
-	const double inv_cells = 1.0 / (double)ncell;
-	int level = 0;
-	double nx0 = x0, nx1 = x1, ny0 = y0, ny1 = y1;
-	double weight = inv_cells;
-	for (;;) {
-		const double xbin = (nx1 - nx0) / res;
-		const double ybin = (ny1 - ny0) / res;
-		const double hx = xbin / 2., hy = ybin / 2.;
-		const bool recurse = res > 1 && level < maxrec;
-		const double dyp = -xbin * c.sin_over + ybin * c.cos_over;
-		unsigned long long m[4] = {0ull, 0ull, 0ull, 0ull};
-#pragma unroll
-		for (int k = 0; k < 4; ++k) {
-			if (k >= nchunk) break;
-			const int cell = k * 64 + lane;
-			bool fail = false;
-			if (cell < ncell) {
-				const int ci = cell / res, cj = cell - ci * res;
-				const double x = cell_centre(nx0, hx, ci);
-				const double y = cell_centre(ny0, hy, cj);
-				const double xr = x - c.xc, yr = y - c.yc;
-				const double xp = xr * c.cos_ang + yr * c.sin_ang;
-				const double yp = -xr * c.sin_over + yr * c.cos_over;
-				const float v32 = eval_at<float>(c, (float)xp, (float)yp);
-				double v64 = 0;
-				bool have64 = false;
-				fail = refine_cell(c, xp, yp, dyp, recurse, v32, &v64, &have64);
-				if (!fail) {
-					if (kGrad) {
-						// value (for the flux-scale derivatives) and derivatives in FP32
-						const double f = sample_accumulate_any(c, xp, yp, weight * scale, gs);
-						sum += f * (weight * scale);
-					}
-					else {
-						// leaf values in FP64 (default) or the FP32 value already at hand
-						if (!have64) v64 = fp64 ? eval_at<double>(c, xp, yp) : (double)v32;
-						sum += v64 * (weight * scale);
-					}
-				}
-			}
-			m[k] = __ballot(fail);
-		}
-		if (lane == 0) {
-			L[level].x0 = nx0;
-			L[level].x1 = nx1;
-			L[level].y0 = ny0;
-			L[level].y1 = ny1;
-			L[level].weight = weight;
-#pragma unroll
-			for (int k = 0; k < 4; ++k) L[level].mask[k] = m[k];
-		}
-		__builtin_amdgcn_wave_barrier();
-		// next node: lowest pending cell of the deepest level that has one
-		int lv = level;
-		int cell = -1;
-		for (;;) {
-			int found = -1;
-#pragma unroll
-			for (int k = 0; k < 4; ++k) {
-				const unsigned long long mk = L[lv].mask[k];
-				if (found < 0 && mk) {
-					found = k * 64 + (__ffsll((long long)mk) - 1);
-				}
-			}
-			if (found >= 0) {
-				cell = found;
-				break;
-			}
-			if (lv == 0) break;
-			--lv;
-		}
-		if (cell < 0) break;
-		__builtin_amdgcn_wave_barrier();
-		if (lane == 0) L[lv].mask[cell >> 6] &= ~(1ull << (cell & 63));
-		const double px0 = L[lv].x0, px1 = L[lv].x1, py0 = L[lv].y0, py1 = L[lv].y1;
-		const double pxbin = (px1 - px0) / res, pybin = (py1 - py0) / res;
-		const double phx = pxbin / 2., phy = pybin / 2.;
-		const int ci = cell / res, cj = cell - ci * res;
-		const double cx = cell_centre(px0, phx, ci);
-		const double cy = cell_centre(py0, phy, cj);
-		nx0 = cx - phx;
-		nx1 = cx + phx;
-		ny0 = cy - phy;
-		ny1 = cy + phy;
-		weight = L[lv].weight * inv_cells;
-		level = lv + 1;
-		__builtin_amdgcn_wave_barrier();
-	}
-}
-
-// One wavefront per queued pixel.  kGrad=false: adds flux_scale * value to the
-// canvas.  kGrad=true: accumulates G(q) * d(flux_scale * value)/d slot; per-lane
-// sums are carried across consecutive entries of the same component and flushed
-// with one set of atomics per run.
-template <bool kGrad>
-__global__ __launch_bounds__(256) void k_subsample(const GalaxyDev *__restrict__ gals,
-                                                   const InstDev *__restrict__ insts,
-                                                   const Comp *__restrict__ comps, WorkQueue wq, int counter,
-                                                   int split, double *grad_acc)
-{
-	__shared__ WalkLevel levels[4][kMaxLevels];
-	// the wavefront index is uniform: make that visible to the compiler so queue
-	// entries and component fields are read through the scalar cache
-	const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-	WalkLevel *L = levels[wv];
-	const int nl = min(wq.count[0], wq.lcap), nh = min(wq.count[1], wq.hcap);
-	double gs[NGRAD];
-#pragma unroll
-	for (int s = 0; s < NGRAD; ++s) gs[s] = 0;
-	double vsum = 0;
-	int cur = -1;
-	// dynamic hand-out: the deep centre trees one at a time first, then the
-	// other entries in grabs of kGrab, so no wave is left with a long tail
-	bool heavy_left = nh > 0;
-	for (;;) {
-		int base = 0, take = 0;
-		const QEntry *list;
-		{
-			int h = nh;
-			if (heavy_left) {
-				if (lane == 0) h = atomicAdd(&wq.count[counter], 1);
-				h = __shfl(h, 0);
-			}
-			if (h < nh) {
-				base = h;
-				take = 1;
-				list = wq.heavy;
-			}
-			else {
-				heavy_left = false;
-				int l = 0;
-				if (lane == 0) l = atomicAdd(&wq.count[counter + 1], kGrab);
-				l = __shfl(l, 0);
-				if (l >= nl) break;
-				base = l;
-				take = min(kGrab, nl - l);
-				list = wq.light;
-			}
-		}
-		for (int e = base; e < base + take; ++e) {
-			const QEntry q = list[e];
-			const Comp &c = comps[q.comp];
-			if (kGrad && q.comp != cur) {
-				if (cur >= 0) {
-					const Comp &pc = comps[cur];
-					const double tv = wave_sum(vsum);
-#pragma unroll
-					for (int s = 0; s < NGRAD; ++s) {
-						const double t = wave_sum(gs[s]) + pc.dlnfs[s] * tv;
-						if (lane == 0 && t != 0) atomicAdd(&grad_acc[(long)cur * NGRAD + s], t);
-						gs[s] = 0;
-					}
-					vsum = 0;
-				}
-				cur = q.comp;
-			}
-			const InstDev &I = insts[c.instance];
-			const GalaxyDev &g = gals[I.galaxy];
-			double scale = 1.0;
-			if (kGrad) {
-				const float Gq = I.G[q.pix];
-				if (Gq == 0.f) continue;
-				scale = (double)Gq * c.flux_scale;
-			}
-			const int i = q.pix % g.DW, j = q.pix / g.DW;
-			const double hx = g.sx / 2., hy = g.sy / 2.;
-			const double y = hy + j * g.sx;
-			const double x = hx + i * g.sy;
-			int res = c.res, maxrec = c.maxrec;
-			if (centre_pixel(c, x, y)) {
-				res = 8;
-				maxrec = 10;
-			}
-			double s_local = 0;
-			walk_pixel<kGrad>(c, L, lane, x - hx, x + hx, y - hy, y + hy, res, maxrec, scale, g.render_fp64 != 0, s_local,
-			                  gs);
-			if (kGrad) {
-				vsum += s_local;
-			}
-			else {
-				const double tot = wave_sum(s_local);
-				float *dst = (split && !c.convolve) ? I.aux : I.canvas;
-				if (lane == 0) atomicAdd(&dst[q.pix], (float)(c.flux_scale * tot));
-			}
-		}
-	}
-	if (kGrad && cur >= 0) {
-		const Comp &pc = comps[cur];
-		const double tv = wave_sum(vsum);
-#pragma unroll
-		for (int s = 0; s < NGRAD; ++s) {
-			const double t = wave_sum(gs[s]) + pc.dlnfs[s] * tv;
-			if (lane == 0 && t != 0) atomicAdd(&grad_acc[(long)cur * NGRAD + s], t);
-		}
-	}
-}
+// ------------------------------------------------------------------ recursion diagnostics
 
 // Counts the nodes visited by the recursion of each queued pixel (diagnostics
 // for decision parity with libprofit).
@@ -945,125 +650,6 @@ __global__ __launch_bounds__(1024) void k_like_t(const GalaxyDev *__restrict__ g
 	}
 }
 
-// ------------------------------------------------------------------ adjoint render
-
-// d(f)/d slot of one direct pixel in FP32, chain rule to the component slots,
-// accumulated with weight wgt into FP32 per-thread sums (a thread's few
-// pixels); returns f.
-__device__ __forceinline__ float pixel_accumulate_f32(const Comp &c, float xp, float yp, float wgt, float *gs)
-{
-	float o[9];
-	sample_grad<float>(c, xp, yp, o);
-	const float q = (float)c.axrat;
-	const float ca = (float)c.cos_ang, sa = (float)c.sin_ang;
-	const float fx = o[1], fy = o[2];
-	gs[G_XCEN] += wgt * (fx * -ca + fy * (sa / q));
-	gs[G_YCEN] += wgt * (fx * -sa + fy * (-ca / q));
-	gs[G_ANG] += wgt * ((fx * (q * yp) + fy * (-xp / q)) * (3.14159265358979323846f / 180.f));
-	gs[G_AXRAT] += wgt * (fy * (-yp / q));
-	gs[G_BOX] += wgt * o[3];
-#pragma unroll
-	for (int s = 0; s < 5; ++s) gs[G_S0 + s] += wgt * o[4 + s];
-	return o[0];
-}
-
-// Adjoint of the directly evaluated pixels: sum_q G(q) fs d f(q) / d slot over
-// the pixels the forward render marked direct (so no classification is
-// repeated), per-thread sums in FP32 (FP64 for steep cores), the flux-scale
-// term applied once per component from sum_q G(q) fs f(q).  A block takes
-// `grp` consecutive canvas tiles of one instance (block offsets per instance
-// in grp_off), so that the per-component reduction and its atomics are paid
-// once per grp tiles.
-__global__ __launch_bounds__(256) void k_adj_render(const GalaxyDev *__restrict__ gals,
-                                                    const InstDev *__restrict__ insts,
-                                                    const int *__restrict__ grp_off, int ninst, int grp,
-                                                    const Comp *__restrict__ comps, double *grad_acc)
-{
-	__shared__ double red[4][NGRAD];
-	const int b = blockIdx.x;
-	const int inst = owner_of(grp_off, ninst, b);
-	const InstDev I = insts[inst];
-	const GalaxyDev &g = gals[I.galaxy];
-	const int rows = g.conv_rows;
-	const int t0 = (b - grp_off[inst]) * grp, t1 = min(t0 + grp, g.n_atiles);
-	const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-	const int wpr = (g.DW + 63) >> 6;
-	const double hx = g.sx / 2., hy = g.sy / 2.;
-	for (int k = 0; k < I.ncomp; ++k) {
-		const Comp &c = comps[I.comp0 + k];
-		double gl[NGRAD];
-#pragma unroll
-		for (int s = 0; s < NGRAD; ++s) gl[s] = 0;
-		double vsum = 0;
-		if (c.grad64) {
-			for (int ti = t0; ti < t1; ++ti) {
-				const int2 t = g.atiles[ti];
-				const int i = t.x * kTileW + lane;
-				const int j0 = t.y * 4 * rows + wv * rows;
-				const double x = hx + i * g.sy;
-				for (int r = 0; r < rows; ++r) {
-					const int j = j0 + r;
-					if (j >= g.DH) break;
-					const unsigned long long m = I.dmask[((long)k * g.DH + j) * wpr + t.x];
-					if (!((m >> lane) & 1ull)) continue;
-					const float Gq = I.G[i + j * g.DW];
-					if (Gq == 0.f) continue;
-					const double y = hy + j * g.sx;
-					const double xr = x - c.xc, yr = y - c.yc;
-					const double xp = xr * c.cos_ang + yr * c.sin_ang;
-					const double yp = -xr * c.sin_over + yr * c.cos_over;
-					const double wq = (double)Gq * c.flux_scale;
-					vsum += wq * sample_accumulate<double>(c, xp, yp, wq, gl);
-				}
-			}
-		}
-		else {
-			float gf[NGRAD];
-#pragma unroll
-			for (int s = 0; s < NGRAD; ++s) gf[s] = 0.f;
-			float vf = 0.f;
-			const float fs = (float)c.flux_scale;
-			for (int ti = t0; ti < t1; ++ti) {
-				const int2 t = g.atiles[ti];
-				const int i = t.x * kTileW + lane;
-				const int j0 = t.y * 4 * rows + wv * rows;
-				const double x = hx + i * g.sy;
-				for (int r = 0; r < rows; ++r) {
-					const int j = j0 + r;
-					if (j >= g.DH) break;
-					const unsigned long long m = I.dmask[((long)k * g.DH + j) * wpr + t.x];
-					if (!m) continue;
-					if (!((m >> lane) & 1ull)) continue;
-					const float Gq = I.G[i + j * g.DW];
-					if (Gq == 0.f) continue;
-					const double y = hy + j * g.sx;
-					const double xr = x - c.xc, yr = y - c.yc;
-					const float xp = (float)(xr * c.cos_ang + yr * c.sin_ang);
-					const float yp = (float)(-xr * c.sin_over + yr * c.cos_over);
-					const float wq = Gq * fs;
-					vf += wq * pixel_accumulate_f32(c, xp, yp, wq, gf);
-				}
-			}
-#pragma unroll
-			for (int s = 0; s < NGRAD; ++s) gl[s] = (double)gf[s];
-			vsum = (double)vf;
-		}
-		// flux-scale dependence: d fs / d slot = fs * dlnfs
-#pragma unroll
-		for (int s = 0; s < NGRAD; ++s) {
-			const double v = wave_sum(gl[s] + vsum * c.dlnfs[s]);
-			if (lane == 0) red[wv][s] = v;
-		}
-		__syncthreads();
-		if (threadIdx.x < NGRAD) {
-			const int s = threadIdx.x;
-			const double v = (red[0][s] + red[1][s]) + (red[2][s] + red[3][s]);
-			if (v != 0) atomicAdd(&grad_acc[(long)(I.comp0 + k) * NGRAD + s], v);
-		}
-		__syncthreads();
-	}
-}
-
 // Gradient of the point sources and the sky (one block per instance): the
 // adjoint weight of canvas pixel q is G(q) for a convolved component and
 // w(q - padding) on the image for one drawn without convolution.
@@ -1193,15 +779,6 @@ int launch_render(const GalaxyDev *gals, const InstDev *insts, const int *tile_o
 	return (int)hipGetLastError();
 }
 
-int launch_subsample(bool grad, int nblocks, const GalaxyDev *gals, const InstDev *insts, const Comp *comps,
-                     WorkQueue wq, int split, double *grad_acc, hipStream_t st)
-{
-	if (nblocks <= 0) return 0;
-	if (grad) k_subsample<true><<<nblocks, 256, 0, st>>>(gals, insts, comps, wq, 4, split, grad_acc);
-	else k_subsample<false><<<nblocks, 256, 0, st>>>(gals, insts, comps, wq, 2, split, grad_acc);
-	return (int)hipGetLastError();
-}
-
 int launch_subsample_nodes(int nblocks, const GalaxyDev *gals, const InstDev *insts, const Comp *comps,
                            WorkQueue wq, int *nodes, hipStream_t st)
 {
@@ -1264,15 +841,6 @@ int launch_like_t(const GalaxyDev *gals, const InstDev *insts, int ninst, int gr
 {
 	if (ninst <= 0) return 0;
 	k_like_t<<<ninst, 1024, 0, st>>>(gals, insts, grad, ll, gscale);
-	return (int)hipGetLastError();
-}
-
-int launch_adj_render(const GalaxyDev *gals, const InstDev *insts, const int *grp_off, int ninst, int nblocks, int grp,
-                      const Comp *comps, double *grad_acc, hipStream_t st)
-{
-	if (nblocks <= 0) return 0;
-	if (grp < 1) return (int)hipErrorInvalidValue;
-	k_adj_render<<<nblocks, 256, 0, st>>>(gals, insts, grp_off, ninst, grp, comps, grad_acc);
 	return (int)hipGetLastError();
 }
 

@@ -34,6 +34,13 @@ enum ProfileType : int32_t {
 
 PA_HD bool is_radial(int type) { return type >= SERSIC && type <= BROKENEXP; }
 
+// shape parameters of a radial family (all five when the family is not known
+// at compile time)
+__host__ __device__ constexpr int n_shape(int type)
+{
+	return type == SERSIC || type == MOFFAT ? 2 : type == FERRER || type == KING ? 3 : type == BROKENEXP ? 4 : 5;
+}
+
 // gradient slots per component
 enum GradSlot : int { G_XCEN = 0, G_YCEN, G_MAG, G_ANG, G_AXRAT, G_BOX, G_S0, G_S1, G_S2, G_S3, G_S4, NGRAD };
 
@@ -306,10 +313,12 @@ PA_HDI T brokenexp_dlog(T r, T h1, T h2, T rb, T a, T *ds)
 	return -T(1) / h1 + expo * sig;
 }
 
-template <class T>
+// TYPE >= 0 compiles the evaluation for that profile family only (the
+// kernels specialised per family); -1 dispatches on c.type at run time.
+template <class T, int TYPE = -1>
 PA_HDI T eval_at(const Comp &c, T x, T y)
 {
-	switch (c.type) {
+	switch (TYPE >= 0 ? TYPE : c.type) {
 	case SERSIC: return sersic_at<T>(c, x, y);
 	case MOFFAT: {
 		const T rf = boxy_r<T>(c, x, y) / T(c.rscale);
@@ -372,11 +381,11 @@ PA_HDI T king_dcore(T r, T rc, T rt, T a, int which)
 }
 
 // o[0] = f, o[1] = df/dx_prof, o[2] = df/dy_prof, o[3] = df/dbox, o[4..8] = df/dshape
-template <class T>
+template <class T, int TYPE = -1>
 PA_HDI void sample_grad(const Comp &c, T x, T y, T *o)
 {
 	for (int i = 0; i < 9; ++i) o[i] = T(0);
-	switch (c.type) {
+	switch (TYPE >= 0 ? TYPE : c.type) {
 	case SERSIC: {
 		// f = exp(-bn (base^(1/(n B)) - 1)), base = |x/re|^B + |y/re|^B, B = box + 2.
 		// Without box the powers are squares, and f follows from the same
@@ -537,6 +546,14 @@ struct BrokenExpIntegrand {
 
 PA_HD bool almost_equal(double x, double y) { return ::fabs(x - y) < 1e-10; }
 
+// steep cores: the centroid derivatives of a cusp (core-Sersic with an inner
+// slope, Sersic n > 4) cancel over many large samples, so their per-sample
+// adjoint derivatives are taken in FP64
+PA_HD bool wants_grad64(const ProfileParams &p)
+{
+	return (p.type == CORESERSIC && p.shape[4] != 0) || (p.type == SERSIC && p.shape[1] > 4);
+}
+
 // initial_calculations() + the evaluate()-time centre shift, in FP64.
 // offx/offy: PSF padding (drawing-canvas shift); sx/sy: profile pixel scale
 // (image scale / finesampling); img_sx/img_sy: image pixel scale.
@@ -584,10 +601,7 @@ PA_HD int setup_component(const ProfileParams &p, double magzero, double sx, dou
 	c->img_sx = img_sx;
 	c->img_sy = img_sy;
 	c->grad_ok = !(p.type == SERSIC && p.rescale_flux);
-	// steep cores: the centroid derivatives of a cusp (core-Sersic with an inner
-	// slope, Sersic n > 4) cancel over many large samples, so their per-sample
-	// derivatives are taken in FP64
-	c->grad64 = (p.type == CORESERSIC && s[4] != 0) || (p.type == SERSIC && s[1] > 4);
+	c->grad64 = wants_grad64(p);
 	c->grad_box = 1;
 	for (int i = 0; i < 5; ++i) c->s[i] = s[i];
 	for (int i = 0; i < NGRAD; ++i) c->dlnfs[i] = 0;

@@ -65,3 +65,12 @@ class OracleEvaluator:
     def like_grad(self, comp_lists):
         out = [O.loglike_grad(self.fit, self._profiles(cl)) for cl in comp_lists]
         return np.array([o[0] for o in out]), np.stack([o[1] for o in out])
+
+
+def fp32_logl_floor(likefunc, data, sigma, model, region=None):
+    """The largest logL change that rounding each pixel of the model image to
+    FP32 can cause (sum over the likelihood pixels of |dlogL/dm| |m| 2^-24):
+    the precision floor of a model held in FP32, as the GPU canvas is."""
+    _, dll = O.loglike_image(likefunc, data, sigma, model, region, want_grad=True)
+    m = np.asarray(model, dtype=np.float64).ravel()
+    return float(np.sum(np.abs(dll * m)) * 2.0 ** -24)

@@ -1,16 +1,22 @@
 """Heterogeneous batch (config 4 draws): galaxies of different sizes, PSFs,
 component counts and profile types evaluated in one launch sequence, each
 instance checked against the FP64 oracle: model image (per-pixel relative
-error <= 1e-5 above 1e-6 of the peak), logL (<= 1e-3 absolute) and the
-adjoint gradient against the oracle's central differences on the frozen
-recursion tree (h = 1e-6 |theta|, 1e-8 |theta| for the centroid)."""
+error <= 1e-5 above 1e-6 of the peak), logL and the adjoint gradient against
+the oracle's central differences on the frozen recursion tree (h = 1e-6
+|theta|, 1e-8 |theta| for the centroid).
+
+logL gate: 1e-3 absolute, or twice the FP32 floor of the model image when
+that is larger (fp32_logl_floor: the logL change that rounding every model
+pixel to FP32 alone can cause).  At magzero 30 a compact, mis-fitted source
+(galaxy 8 of these draws at the walker positions: a 1 px Moffat peaking at
+2.3e4 counts, logL -561) has a floor of 1.2e-3; the GPU canvas is FP32."""
 import numpy as np
 import pytest
 
 import oracle as O
 import profit_amd as P
 from profit_amd import workloads as W
-from tests.helpers import gpu_profiles, oracle_profiles, rel_err
+from tests.helpers import fp32_logl_floor, gpu_profiles, oracle_profiles, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -84,7 +90,10 @@ def test_heterogeneous_batch_gradients(batch):
         steps = np.full((len(s), O.NGRAD), 1e-6)
         steps[:, :2] = 1e-8
         oll, ograd = O.loglike_grad_fd(f, oracle_profiles(s), rel_step=steps)
-        assert abs(ll[i] - oll) <= 1e-3, i
+        _, om = O.loglike(f, oracle_profiles(s), want_model=True)
+        gate = max(1e-3, 2 * fp32_logl_floor("chisq", g["data"], g["sigma"], om, g["region"]))
+        print(f"[batch] galaxy {i}: logL {ll[i]:.6f} vs {oll:.6f} (gate {gate:.2e})", flush=True)
+        assert abs(ll[i] - oll) <= gate, i
         for k, c in enumerate(s):
             slots = [0, 1, 2, 3, 4] + [6 + j for j in range(SHAPES[c["type"]])]
             o = ograd[k, slots]
