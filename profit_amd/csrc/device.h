@@ -28,6 +28,10 @@ enum ConvKind { CK_VALU = 0, CK_FFT = 1, CK_MFMA = 2 };
 // eight adjacent entries of a row fill one cache line.
 __host__ __device__ constexpr int fft_pitch(int n1) { return ((n1 / 2 + 1) + 7) & ~7; }
 
+// Row pairs per block of the FFT row passes (one pair per wavefront while
+// the pairs and the twiddles fit the LDS).
+__host__ __device__ constexpr int fft_pairs_per_block(int n1) { return n1 <= 1024 ? 4 : (n1 <= 2048 ? 2 : 1); }
+
 // One galaxy: image geometry, PSF, masks and data.  Several instances (parameter
 // sets) may share a galaxy.
 struct GalaxyDev {

@@ -783,15 +783,18 @@ void Engine::set_batch(const std::vector<int> &inst_galaxy, const std::vector<in
 		bt_off_[i + 1] = bt_off_[i] + (g.conv_kind == CK_VALU ? g.n_atiles : 0);
 		ml_off_[i + 1] = ml_off_[i] + (g.conv_kind == CK_MFMA ? g.n_mtl : 0);
 		mc_off_[i + 1] = mc_off_[i] + (g.conv_kind == CK_MFMA ? g.n_mtc : 0);
+		// row passes take fft_pairs_per_block pairs of rows per block;
+		// columns span the half-spectrum
+		const int rpb = f ? 2 * fft_pairs_per_block(g.fft_n1) : 2;
+		auto row_blocks = [rpb](int rows) { return (rows + rpb - 1) / rpb; };
 		lk_off_[i + 1] = lk_off_[i] + (g.conv_kind == CK_VALU   ? g.n_ctiles
 		                               : g.conv_kind == CK_MFMA ? g.n_mtl
-		                                                        : (g.H + 1) / 2);
-		// row passes take two rows per block; columns span the half-spectrum
-		rf0_off_[i + 1] = rf0_off_[i] + (f ? (g.DH + 1) / 2 : 0);
-		rf1_off_[i + 1] = rf1_off_[i] + (f ? (g.H + 1) / 2 : 0);
+		                                                        : row_blocks(g.H));
+		rf0_off_[i + 1] = rf0_off_[i] + (f ? row_blocks(g.DH) : 0);
+		rf1_off_[i + 1] = rf1_off_[i] + (f ? row_blocks(g.H) : 0);
 		col_off_[i + 1] = col_off_[i] + (f ? (g.fft_n1 / 2 + 1 + fft_cb_ - 1) / fft_cb_ : 0);
-		ri0_off_[i + 1] = ri0_off_[i] + (f ? (g.H + 1) / 2 : 0);
-		ri1_off_[i + 1] = ri1_off_[i] + (f ? (g.DH + 1) / 2 : 0);
+		ri0_off_[i + 1] = ri0_off_[i] + (f ? row_blocks(g.H) : 0);
+		ri1_off_[i + 1] = ri1_off_[i] + (f ? row_blocks(g.DH) : 0);
 	}
 	n_bt_blocks_ = bt_off_[n];
 	// the adjoint render gives each thread about 32 canvas pixels of a component

@@ -15,10 +15,14 @@
 //                   likelihood, masked model image, or the adjoint image G
 // Every transform runs in LDS: decimation in time from bit-reversed input for
 // the forward transforms, decimation in frequency to bit-reversed output for
-// the inverse ones, three radix-2 stages per pass in registers, twiddles
-// staged in LDS.  FP64 keeps the error
-// relative to the image maximum near 1e-15, so faint pixels keep their
-// relative accuracy.
+// the inverse ones, three radix-2 stages per pass in registers.  Two layouts
+// keep the LDS banks free of conflicts: element e of a sequence sits at
+// swz(e), an XOR of its low bits with the next ones, so the strided accesses
+// of every pass fall on distinct banks; and the twiddles are staged one table
+// per stage, in the order a stage reads them.  The row passes run one pair of
+// rows per wavefront, several pairs per block.  FP64 keeps the error relative
+// to the image maximum near 1e-15, so faint pixels keep their relative
+// accuracy.
 #include <hip/hip_runtime.h>
 
 #include "device.h"
@@ -50,34 +54,48 @@ __device__ inline double2 cmulc(double2 a, double2 b)  // a * conj(b)
 	return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
 }
 
-// twiddles of an n-point transform in LDS: w[t] = exp(-2 pi i t / n), t < n/2,
-// taken from the global table of the largest transform (twn points)
-__device__ inline void stage_twiddles(double2 *w, int n, const double2 *__restrict__ tw, int twn)
+// LDS position of element e of a sequence: bits 0-2 ^= bits 3-5 and bit 3 ^=
+// bit 6, a bijection on every power-of-two range.  A pass reads elements at
+// spacing 1, 8 or 64 (for 2^9 points); after the swizzle the 16 lanes that
+// share a bank cycle address distinct 16-byte slots in each case.
+__device__ __forceinline__ int swz(int e) { return e ^ ((e >> 3) & 7) ^ (((e >> 6) & 1) << 3); }
+
+// Twiddles of an n-point transform in LDS, one table per radix-2 stage S =
+// 1..log2(n): wt[2^(S-1) - 1 + k] = exp(-2 pi i k / 2^S), k < 2^(S-1) (n - 1
+// entries), taken from the global table of the largest transform (twn points).
+__device__ inline void stage_twiddles(double2 *wt, int n, int logn, const double2 *__restrict__ tw, int twn)
 {
 	const int step = twn / n;
-	for (int t = threadIdx.x; t < n / 2; t += blockDim.x) w[t] = tw[t * step];
+	for (int i = threadIdx.x; i < n - 1; i += blockDim.x) {
+		const int S = 32 - __clz(i + 1);  // floor(log2(i + 1)) + 1
+		const int k = i + 1 - (1 << (S - 1));
+		wt[i] = tw[(k << (logn - S)) * step];
+	}
 }
 
+__device__ __forceinline__ double2 twiddle(const double2 *wt, int S, int k) { return wt[(1 << (S - 1)) - 1 + k]; }
+
 // In-place forward DIT of `ncol` sequences of length n (sequence c at
-// data + c*stride), bit-reversed input, natural-order output.  The log2(n)
-// radix-2 stages run in passes of up to three: each thread holds the eight
-// (four, two) elements that the pass's stages combine in registers, so LDS is
-// read and written once per pass instead of once per stage.  Stage S
-// combines i and i + 2^(S-1) with twiddle w[(i mod 2^(S-1)) * n / 2^S].
+// data + c*stride, element e at swz(e)), bit-reversed input, natural-order
+// output.  The log2(n) radix-2 stages run in passes of up to three: each
+// thread holds the eight (four, two) elements that the pass's stages combine
+// in registers, so LDS is read and written once per pass instead of once per
+// stage.  Stage S combines i and i + 2^(S-1) with twiddle
+// exp(-2 pi i (i mod 2^(S-1)) / 2^S).
 template <int K>
-__device__ __forceinline__ void dit_pass(double2 *data, int logn, int ncol, int stride, int s, const double2 *w)
+__device__ __forceinline__ void dit_pass(double2 *data, int logn, int ncol, int stride, int s, const double2 *wt)
 {
 	constexpr int R = 1 << K;
 	const int sub = 1 << s;                     // 2^s: element spacing inside a group
-	const int groups_per_col = 1 << (logn - K);  // n / R
-	const int total = ncol * groups_per_col;
+	const int total = ncol << (logn - K);       // groups: n / R per sequence
 	for (int gi = threadIdx.x; gi < total; gi += blockDim.x) {
 		const int c = gi >> (logn - K), g = gi - (c << (logn - K));
 		const int lo = g & (sub - 1), hi = g >> s;
-		double2 *base = data + c * stride + (hi << (s + K)) + lo;
+		double2 *col = data + c * stride;
+		const int e0 = (hi << (s + K)) + lo;
 		double2 x[R];
 #pragma unroll
-		for (int j = 0; j < R; ++j) x[j] = base[j << s];
+		for (int j = 0; j < R; ++j) x[j] = col[swz(e0 + (j << s))];
 #pragma unroll
 		for (int t = 1; t <= K; ++t) {
 			const int h = 1 << (t - 1);
@@ -86,46 +104,46 @@ __device__ __forceinline__ void dit_pass(double2 *data, int logn, int ncol, int 
 			for (int j = 0; j < R; ++j) {
 				if (j & h) continue;
 				const int kS = ((j & (h - 1)) << s) + lo;  // index within the stage's half block
-				const double2 u = x[j], v = cmul(x[j + h], w[kS << (logn - S)]);
+				const double2 u = x[j], v = cmul(x[j + h], twiddle(wt, S, kS));
 				x[j] = cadd(u, v);
 				x[j + h] = csub(u, v);
 			}
 		}
 #pragma unroll
-		for (int j = 0; j < R; ++j) base[j << s] = x[j];
+		for (int j = 0; j < R; ++j) col[swz(e0 + (j << s))] = x[j];
 	}
 	__syncthreads();
 }
 
-__device__ inline void fft_dit(double2 *data, int n, int logn, int ncol, int stride, const double2 *w)
+__device__ inline void fft_dit(double2 *data, int logn, int ncol, int stride, const double2 *wt)
 {
-	(void)n;
 	int s = 0;
-	for (; s + 3 <= logn; s += 3) dit_pass<3>(data, logn, ncol, stride, s, w);
-	if (logn - s == 2) dit_pass<2>(data, logn, ncol, stride, s, w);
-	else if (logn - s == 1) dit_pass<1>(data, logn, ncol, stride, s, w);
+	for (; s + 3 <= logn; s += 3) dit_pass<3>(data, logn, ncol, stride, s, wt);
+	if (logn - s == 2) dit_pass<2>(data, logn, ncol, stride, s, wt);
+	else if (logn - s == 1) dit_pass<1>(data, logn, ncol, stride, s, wt);
 }
 
 // In-place inverse (conjugate twiddles, unscaled) DIF: natural-order input,
-// bit-reversed output.  Stage S (from log2(n) down to 1) maps (u, v) at i and
-// i + 2^(S-1) to (u + v, (u - v) conj w[(i mod 2^(S-1)) * n / 2^S]); passes of
-// up to three stages as in the forward transform.
+// bit-reversed output, element e at swz(e).  Stage S (from log2(n) down to 1)
+// maps (u, v) at i and i + 2^(S-1) to (u + v, (u - v) conj w); passes of up
+// to three stages as in the forward transform.
 template <int K>
-__device__ __forceinline__ void dif_pass(double2 *data, int logn, int ncol, int stride, int top, const double2 *w)
+__device__ __forceinline__ void dif_pass(double2 *data, int logn, int ncol, int stride, int top, const double2 *wt)
 {
 	// stages top, top-1, ..., top-K+1 combine indices differing in bits
 	// top-1 .. top-K, i.e. spacing 2^(top-K)
 	constexpr int R = 1 << K;
 	const int s = top - K;
 	const int sub = 1 << s;
-	const int total = ncol * (1 << (logn - K));
+	const int total = ncol << (logn - K);
 	for (int gi = threadIdx.x; gi < total; gi += blockDim.x) {
 		const int c = gi >> (logn - K), g = gi - (c << (logn - K));
 		const int lo = g & (sub - 1), hi = g >> s;
-		double2 *base = data + c * stride + (hi << top) + lo;
+		double2 *col = data + c * stride;
+		const int e0 = (hi << top) + lo;
 		double2 x[R];
 #pragma unroll
-		for (int j = 0; j < R; ++j) x[j] = base[j << s];
+		for (int j = 0; j < R; ++j) x[j] = col[swz(e0 + (j << s))];
 #pragma unroll
 		for (int t = K; t >= 1; --t) {
 			const int h = 1 << (t - 1);
@@ -136,56 +154,63 @@ __device__ __forceinline__ void dif_pass(double2 *data, int logn, int ncol, int 
 				const int kS = ((j & (h - 1)) << s) + lo;
 				const double2 u = x[j], v = x[j + h];
 				x[j] = cadd(u, v);
-				x[j + h] = cmul(csub(u, v), conjg(w[kS << (logn - S)]));
+				x[j + h] = cmul(csub(u, v), conjg(twiddle(wt, S, kS)));
 			}
 		}
 #pragma unroll
-		for (int j = 0; j < R; ++j) base[j << s] = x[j];
+		for (int j = 0; j < R; ++j) col[swz(e0 + (j << s))] = x[j];
 	}
 	__syncthreads();
 }
 
-__device__ inline void ifft_dif(double2 *data, int n, int logn, int ncol, int stride, const double2 *w)
+__device__ inline void ifft_dif(double2 *data, int logn, int ncol, int stride, const double2 *wt)
 {
-	(void)n;
 	int top = logn;
-	for (; top - 3 >= 0; top -= 3) dif_pass<3>(data, logn, ncol, stride, top, w);
-	if (top == 2) dif_pass<2>(data, logn, ncol, stride, top, w);
-	else if (top == 1) dif_pass<1>(data, logn, ncol, stride, top, w);
+	for (; top - 3 >= 0; top -= 3) dif_pass<3>(data, logn, ncol, stride, top, wt);
+	if (top == 2) dif_pass<2>(data, logn, ncol, stride, top, wt);
+	else if (top == 1) dif_pass<1>(data, logn, ncol, stride, top, wt);
 }
 
 }  // namespace
 
-// Forward transform of one pair of real rows (a = src[r], b = src[r + 1] when
-// `second`), columns [x0, x0 + nx) of the frame holding data, into the
-// half-spectrum rows `outa` and `outa + pitch`:
-// A = (Z[k] + conj Z[N-k]) / 2 and B = (Z[k] - conj Z[N-k]) / 2i, k = 0..N/2.
+// Forward transforms of `np` pairs of real rows, pair p at rows 2p, 2p + 1 of
+// `src` (row 2p + 1 only below nrows; pairs from pmax on are not loaded or
+// stored), columns [x0, x0 + nx) of the frame holding data, into the
+// half-spectrum rows 2p, 2p + 1 of `out` (pitch P):
+// A = (Z[k] + conj Z[N-k]) / 2 and B = (Z[k] - conj Z[N-k]) / 2i, k = 0..N/2,
+// for z = a + i b.  Sequence p sits at sh + p * N.
 template <class T>
-__device__ __forceinline__ void rows_fwd_pair(double2 *sh, const double2 *w, int N, int L, const T *src, long ld,
-                                              int x0, int nx, bool second, double2 *outa, int pitch)
+__device__ __forceinline__ void rows_fwd_pairs(double2 *sh, const double2 *wt, int N, int L, int np, const T *src,
+                                               long ld, int x0, int nx, int nrows, int pmax, double2 *out, int P)
 {
 	const int M = N / 2 + 1;
-	for (int t = threadIdx.x; t < N; t += blockDim.x) {
+	for (int t = threadIdx.x; t < np * N; t += blockDim.x) {
+		const int p = t >> L, k = t - (p << L);
 		double a = 0, bv = 0;
-		const int xi = t - x0;
-		if (xi >= 0 && xi < nx) {
-			a = (double)src[xi];
-			if (second) bv = (double)src[ld + xi];
+		const int xi = k - x0;
+		if (p < pmax && xi >= 0 && xi < nx) {
+			const T *row = src + (long)(2 * p) * ld;
+			a = (double)row[xi];
+			if (2 * p + 1 < nrows) bv = (double)row[ld + xi];
 		}
-		sh[bitrev(t, L)] = make_double2(a, bv);
+		sh[(p << L) + swz(bitrev(k, L))] = make_double2(a, bv);
 	}
 	__syncthreads();
-	fft_dit(sh, N, L, 1, N, w);
-	double2 *outb = outa + pitch;
-	for (int k = threadIdx.x; k < M; k += blockDim.x) {
-		const double2 z = sh[k], zc = conjg(sh[(N - k) & (N - 1)]);
+	fft_dit(sh, L, np, N, wt);
+	for (int t = threadIdx.x; t < np * M; t += blockDim.x) {
+		const int p = t / M, k = t - p * M;
+		if (p >= pmax) continue;
+		const double2 *z0 = sh + (p << L);
+		const double2 z = z0[swz(k)], zc = conjg(z0[swz((N - k) & (N - 1))]);
+		double2 *outa = out + (long)(2 * p) * P;
 		outa[k] = make_double2(0.5 * (z.x + zc.x), 0.5 * (z.y + zc.y));
 		// (z - zc) / 2i = (-i / 2) (z - zc)
-		if (second) outb[k] = make_double2(0.5 * (z.y - zc.y), -0.5 * (z.x - zc.x));
+		if (2 * p + 1 < nrows) outa[P + k] = make_double2(0.5 * (z.y - zc.y), -0.5 * (z.x - zc.x));
 	}
 }
 
-// Forward rows, two input rows per block packed as z = a + i b:
+// Forward rows, fft_pairs_per_block(N1) pairs of input rows per block, each
+// pair packed as z = a + i b:
 //   mode 0: canvas rows (DW x DH) into frame rows [0, DH)
 //   mode 1: w rows (W x H) at column padx into frame rows [pady, pady + H)
 __global__ __launch_bounds__(256) void k_fft_rows_fwd(const GalaxyDev *__restrict__ gals, const InstDev *__restrict__ insts,
@@ -197,31 +222,35 @@ __global__ __launch_bounds__(256) void k_fft_rows_fwd(const GalaxyDev *__restric
 	const int inst = owner(row_off, ninst, b);
 	const InstDev &I = insts[inst];
 	const GalaxyDev &g = gals[I.galaxy];
-	const int N = g.fft_n1, L = g.fft_l1, P = fft_pitch(N);
-	double2 *w = sh + N;
-	stage_twiddles(w, N, tw, twn);
-	const int r = 2 * (b - row_off[inst]);  // first input row of the pair
+	const int N = g.fft_n1, L = g.fft_l1, P = fft_pitch(N), np = fft_pairs_per_block(N);
+	double2 *wt = sh + np * N;
+	stage_twiddles(wt, N, L, tw, twn);
+	const int p0 = (b - row_off[inst]) * np;  // first pair of the block
 	// straight-line selects of the mode's geometry
 	const bool adj = mode == 1;
 	const int nrows = adj ? g.H : g.DH;
 	const int width = adj ? g.W : g.DW;
 	const int x0 = adj ? g.padx : 0;
 	const int y0 = adj ? g.pady : 0;
-	const float *src = (adj ? I.w : I.canvas) + (long)r * width;
-	rows_fwd_pair<float>(sh, w, N, L, src, width, x0, width, r + 1 < nrows, I.fbuf + (long)(y0 + r) * P, P);
+	const int r0 = 2 * p0;
+	const float *src = (adj ? I.w : I.canvas) + (long)r0 * width;
+	const int pmax = (nrows - r0 + 1) / 2;
+	rows_fwd_pairs<float>(sh, wt, N, L, np, src, width, x0, width, nrows - r0, pmax, I.fbuf + (long)(y0 + r0) * P, P);
 }
 
 // Forward rows of a galaxy's real FP64 PSF frame (N1 x N2, `src64`) into the
-// half-spectrum frame `dst64`, two rows per block.
+// half-spectrum frame `dst64`, fft_pairs_per_block(N1) pairs of rows per block.
 __global__ __launch_bounds__(256) void k_fft_psf_rows(const GalaxyDev *__restrict__ g, const double2 *__restrict__ tw,
                                                       int twn, const double *__restrict__ src64, double2 *dst64)
 {
 	extern __shared__ double2 sh[];
-	const int N = g->fft_n1, L = g->fft_l1, P = fft_pitch(N);
-	double2 *w = sh + N;
-	stage_twiddles(w, N, tw, twn);
-	const int r = 2 * blockIdx.x;
-	rows_fwd_pair<double>(sh, w, N, L, src64 + (long)r * N, N, 0, N, r + 1 < g->fft_n2, dst64 + (long)r * P, P);
+	const int N = g->fft_n1, L = g->fft_l1, P = fft_pitch(N), np = fft_pairs_per_block(N);
+	double2 *wt = sh + np * N;
+	stage_twiddles(wt, N, L, tw, twn);
+	const int r0 = 2 * np * blockIdx.x;
+	const int nrows = g->fft_n2 - r0;
+	rows_fwd_pairs<double>(sh, wt, N, L, np, src64 + (long)r0 * N, N, 0, N, nrows, (nrows + 1) / 2,
+	                       dst64 + (long)r0 * P, P);
 }
 
 // Column pass over CB adjacent half-spectrum columns of one instance.  Frame
@@ -242,8 +271,8 @@ __global__ __launch_bounds__(256) void k_fft_cols(const GalaxyDev *__restrict__ 
 	const GalaxyDev &g = gals[I.galaxy];
 	const int M = g.fft_n1 / 2 + 1, P = fft_pitch(g.fft_n1), N = g.fft_n2, L = g.fft_l2;
 	const int S = N + 1;
-	double2 *w = sh + cb * S;
-	stage_twiddles(w, N, tw, twn);
+	double2 *wt = sh + cb * S;
+	stage_twiddles(wt, N, L, tw, twn);
 	const int c0 = (b - col_off[inst]) * cb;
 	double2 *frame = (op == 0) ? dst64 : I.fbuf;
 	// rows holding data: 0 canvas rows [0, DH); 1 image rows [pady, pady + H); 2 all rows
@@ -253,14 +282,14 @@ __global__ __launch_bounds__(256) void k_fft_cols(const GalaxyDev *__restrict__ 
 		const int y = t / cb, c = t - y * cb;
 		double2 v = make_double2(0.0, 0.0);
 		if (y >= in_lo && y < in_hi && c0 + c < M) v = frame[(long)y * P + c0 + c];
-		sh[c * S + bitrev(y, L)] = v;
+		sh[c * S + swz(bitrev(y, L))] = v;
 	}
 	__syncthreads();
-	fft_dit(sh, N, L, cb, S, w);
+	fft_dit(sh, L, cb, S, wt);
 	if (op == 0) {
 		for (int t = threadIdx.x; t < cb * N; t += blockDim.x) {
 			const int y = t / cb, c = t - y * cb;
-			if (c0 + c < M) frame[(long)y * P + c0 + c] = sh[c * S + y];
+			if (c0 + c < M) frame[(long)y * P + c0 + c] = sh[c * S + swz(y)];
 		}
 		return;
 	}
@@ -268,28 +297,28 @@ __global__ __launch_bounds__(256) void k_fft_cols(const GalaxyDev *__restrict__ 
 		const int y = t / cb, c = t - y * cb;
 		if (c0 + c < M) {
 			const double2 s = g.fft_spec[(long)y * P + c0 + c];
-			const double2 v = sh[c * S + y];
-			sh[c * S + y] = (op == 1) ? cmul(v, s) : cmulc(v, s);
+			const double2 v = sh[c * S + swz(y)];
+			sh[c * S + swz(y)] = (op == 1) ? cmul(v, s) : cmulc(v, s);
 		}
 	}
 	__syncthreads();
-	ifft_dif(sh, N, L, cb, S, w);
+	ifft_dif(sh, L, cb, S, wt);
 	// rows written back: 0 image rows [pady, pady + H); 1 canvas rows [0, DH)
 	const int out_lo = out_mode == 0 ? g.pady : 0;
 	const int out_hi = out_mode == 0 ? g.pady + g.H : g.DH;
 	for (int t = threadIdx.x; t < cb * N; t += blockDim.x) {
 		const int y = t / cb, c = t - y * cb;
-		if (y >= out_lo && y < out_hi && c0 + c < M) frame[(long)y * P + c0 + c] = sh[c * S + bitrev(y, L)];
+		if (y >= out_lo && y < out_hi && c0 + c < M) frame[(long)y * P + c0 + c] = sh[c * S + swz(bitrev(y, L))];
 	}
 }
 
-// Inverse rows, two frame rows (half-spectra A, B) per block packed as
-// Z = A + i B over the full length (Z[k] = conj A[N-k] + i conj B[N-k] above
-// N/2); the real part of the inverse is the first row, the imaginary part the
-// second.  Epilogues:
-//   mode 0 (likelihood): image rows r, r + 1 (frame rows pady + r, ...); model
-//     = value / (N1 N2) at column padx + x; likelihood terms into one partial
-//     per block, w written
+// Inverse rows, fft_pairs_per_block(N1) pairs of frame rows per block, each
+// pair (half-spectra A, B) packed as Z = A + i B over the full length
+// (Z[k] = conj A[N-k] + i conj B[N-k] above N/2); the real part of the
+// inverse is the first row, the imaginary part the second.  Epilogues:
+//   mode 0 (likelihood): image rows (frame rows pady + r); model = value /
+//     (N1 N2) at column padx + x; likelihood terms into one partial per
+//     block, w written
 //   mode 1 (model image): canvas rows; G = model masked by calc
 //   mode 2 (adjoint): canvas rows; G = value / (N1 N2)
 __global__ __launch_bounds__(256) void k_fft_rows_inv(const GalaxyDev *__restrict__ gals, const InstDev *__restrict__ insts,
@@ -302,44 +331,51 @@ __global__ __launch_bounds__(256) void k_fft_rows_inv(const GalaxyDev *__restric
 	const int inst = owner(row_off, ninst, b);
 	const InstDev &I = insts[inst];
 	const GalaxyDev &g = gals[I.galaxy];
-	const int N = g.fft_n1, L = g.fft_l1, M = N / 2 + 1;
-	double2 *w = sh + N;
-	stage_twiddles(w, N, tw, twn);
-	const int p = b - row_off[inst];
-	const int r = 2 * p;  // first output row of the pair (image rows in mode 0)
+	const int N = g.fft_n1, L = g.fft_l1, M = N / 2 + 1, P = fft_pitch(N), np = fft_pairs_per_block(N);
+	double2 *wt = sh + np * N;
+	stage_twiddles(wt, N, L, tw, twn);
+	const int blk = b - row_off[inst];
+	const int r0 = 2 * np * blk;  // first output row of the block (image rows in mode 0)
 	const int nrows = mode == 0 ? g.H : g.DH;
-	const int y = mode == 0 ? r + g.pady : r;
-	const bool second = r + 1 < nrows;
-	const double2 *ra = I.fbuf + (long)y * fft_pitch(N);
-	const double2 *rb = ra + fft_pitch(N);
-	for (int k = threadIdx.x; k < N; k += blockDim.x) {
-		double2 z;
-		if (k < M) {
-			const double2 a = ra[k], bv = second ? rb[k] : make_double2(0.0, 0.0);
-			z = make_double2(a.x - bv.y, a.y + bv.x);  // A + i B
+	const int y0 = mode == 0 ? r0 + g.pady : r0;  // its frame row
+	for (int t = threadIdx.x; t < np * N; t += blockDim.x) {
+		const int p = t >> L, k = t - (p << L);
+		const int r = r0 + 2 * p;
+		double2 z = make_double2(0.0, 0.0);
+		if (r < nrows) {
+			const bool second = r + 1 < nrows;
+			const double2 *ra = I.fbuf + (long)(y0 + 2 * p) * P;
+			const double2 *rb = ra + P;
+			if (k < M) {
+				const double2 a = ra[k], bv = second ? rb[k] : make_double2(0.0, 0.0);
+				z = make_double2(a.x - bv.y, a.y + bv.x);  // A + i B
+			}
+			else {
+				const double2 a = conjg(ra[N - k]), bv = second ? conjg(rb[N - k]) : make_double2(0.0, 0.0);
+				z = make_double2(a.x - bv.y, a.y + bv.x);
+			}
 		}
-		else {
-			const double2 a = conjg(ra[N - k]), bv = second ? conjg(rb[N - k]) : make_double2(0.0, 0.0);
-			z = make_double2(a.x - bv.y, a.y + bv.x);
-		}
-		sh[k] = z;
+		sh[(p << L) + swz(k)] = z;
 	}
 	__syncthreads();
-	ifft_dif(sh, N, L, 1, N, w);
+	ifft_dif(sh, L, np, N, wt);
 	const double scale = 1.0 / ((double)g.fft_n1 * (double)g.fft_n2);
 	if (mode == 0) {
 		double part = 0;
 		const bool pois = g.likefunc == LIKE_POIS, tl = g.likefunc == LIKE_T;
 		const double s = g.pois_scale;
-		for (int t = threadIdx.x; t < 2 * g.W; t += blockDim.x) {
-			const int row = t >= g.W ? 1 : 0, x = t - row * g.W;
-			if (row == 1 && !second) continue;
-			const long o = (long)(r + row) * g.W + x;
+		const int W2 = 2 * g.W;
+		for (int t = threadIdx.x; t < np * W2; t += blockDim.x) {
+			const int p = t / W2, rem = t - p * W2;
+			const int row = rem >= g.W ? 1 : 0, x = rem - row * g.W;
+			const int r = r0 + 2 * p + row;
+			if (r >= nrows) continue;
+			const long o = (long)r * g.W + x;
 			float wv = 0.f;
 			if (g.region[o]) {
-				const long qc = (long)(y + row) * g.DW + x + g.padx;
+				const long qc = (long)(y0 + 2 * p + row) * g.DW + x + g.padx;
 				const bool in_calc = !g.has_calc || g.calc[qc];
-				const double2 z = sh[bitrev(x + g.padx, L)];
+				const double2 z = sh[(p << L) + swz(bitrev(x + g.padx, L))];
 				const double m = in_calc ? (row ? z.y : z.x) * scale + (I.has_aux ? (double)I.aux[qc] : 0.0) : 0.0;
 				const double d = g.data[o];
 				if (pois) {
@@ -367,16 +403,19 @@ __global__ __launch_bounds__(256) void k_fft_rows_inv(const GalaxyDev *__restric
 		if (threadIdx.x == 0) {
 			double t = 0;
 			for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t += red[k];
-			partial[lk_base[inst] + p] = t;
+			partial[lk_base[inst] + blk] = t;
 		}
 		return;
 	}
-	for (int t = threadIdx.x; t < 2 * g.DW; t += blockDim.x) {
-		const int row = t >= g.DW ? 1 : 0, x = t - row * g.DW;
-		if (row == 1 && !second) continue;
-		const double2 z = sh[bitrev(x, L)];
+	const int D2 = 2 * g.DW;
+	for (int t = threadIdx.x; t < np * D2; t += blockDim.x) {
+		const int p = t / D2, rem = t - p * D2;
+		const int row = rem >= g.DW ? 1 : 0, x = rem - row * g.DW;
+		const int r = r0 + 2 * p + row;
+		if (r >= nrows) continue;
+		const double2 z = sh[(p << L) + swz(bitrev(x, L))];
 		double v = (row ? z.y : z.x) * scale;
-		const long o = (long)(y + row) * g.DW + x;
+		const long o = (long)r * g.DW + x;
 		if (mode == 1 && g.has_calc && !g.calc[o]) v = 0;
 		if (mode == 1 && I.has_aux) v += (double)I.aux[o];
 		I.G[o] = (float)v;
@@ -404,8 +443,9 @@ static size_t lds_rows_fwd[64], lds_psf_rows[64], lds_cols[64], lds_rows_inv[64]
 
 static int cols_per_block(int n2) { return n2 >= 4096 ? 1 : (n2 >= 2048 ? 2 : (4096 / n2 > 8 ? 8 : 4096 / n2)); }
 
-// LDS of a row pass: the row plus half as many twiddles
-static size_t row_lds(int nmax) { return sizeof(double2) * (size_t)(nmax + nmax / 2); }
+// LDS of a row pass: the block's pairs and the per-stage twiddles (their
+// total grows with N, so the largest frame sets it)
+static size_t row_lds(int nmax) { return sizeof(double2) * (size_t)(fft_pairs_per_block(nmax) + 1) * nmax; }
 
 int launch_fft_rows_fwd(int nblocks, int nmax, const GalaxyDev *gals, const InstDev *insts, const int *row_off,
                         int ninst, int mode, const double2 *tw, int twn, hipStream_t st)
@@ -425,7 +465,8 @@ int launch_fft_psf_rows(int n1, int n2, const GalaxyDev *gal, const double2 *tw,
 	const size_t lds = row_lds(n1);
 	if (lds > 160 * 1024 || n2 < 2) return (int)hipErrorInvalidValue;
 	if (int e = ensure_lds((const void *)k_fft_psf_rows, lds, lds_psf_rows)) return e;
-	k_fft_psf_rows<<<n2 / 2, 256, lds, st>>>(gal, tw, twn, src64, dst64);
+	const int np = fft_pairs_per_block(n1);
+	k_fft_psf_rows<<<(n2 / 2 + np - 1) / np, 256, lds, st>>>(gal, tw, twn, src64, dst64);
 	return (int)hipGetLastError();
 }
 
@@ -434,7 +475,7 @@ int launch_fft_cols(int nblocks, int n2max, int cb, const GalaxyDev *gals, const
                     hipStream_t st)
 {
 	if (nblocks <= 0) return 0;
-	const size_t lds = sizeof(double2) * ((size_t)cb * (n2max + 1) + n2max / 2);
+	const size_t lds = sizeof(double2) * ((size_t)cb * (n2max + 1) + n2max);
 	if (lds > 160 * 1024) return (int)hipErrorInvalidValue;
 	if (int e = ensure_lds((const void *)k_fft_cols, lds, lds_cols)) return e;
 	k_fft_cols<<<nblocks, 256, lds, st>>>(gals, insts, col_off, ninst, cb, op, in_mode, out_mode, tw, twn, dst64);

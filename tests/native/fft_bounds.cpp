@@ -2,14 +2,27 @@
 // geometries with every device buffer at its exact size, so that
 // AddressSanitizer reports any access outside one.  The host-side numbers
 // (frame sizes, blocks per pass, LDS sizes) are computed the way the engine
-// computes them.  Prints "ok <geometry>" per case.
+// computes them; the dynamic LDS past the size the launcher asks for is
+// poisoned, so the kernels' LDS indexing is checked too.  Prints
+// "ok <geometry>" per case.
+#include <sanitizer/asan_interface.h>
+
 #include <cstdio>
 #include <cstdlib>
 
 #include "fft_kernels.inc"  // fft.hip up to its launchers
 
 namespace profit_amd {
-double2 sh[1 << 17];  // the kernels' dynamic LDS (blocks run one at a time)
+constexpr size_t kLdsMax = 1 << 17;
+double2 sh[kLdsMax];  // the kernels' dynamic LDS (blocks run one at a time)
+}
+
+// the launcher's dynamic LDS size in double2 entries: accesses beyond it fault
+static void set_lds(size_t n)
+{
+	ASAN_UNPOISON_MEMORY_REGION(profit_amd::sh, sizeof(profit_amd::sh));
+	if (n > profit_amd::kLdsMax) std::abort();
+	ASAN_POISON_MEMORY_REGION(profit_amd::sh + n, (profit_amd::kLdsMax - n) * sizeof(double2));
 }
 
 using namespace profit_amd;
@@ -77,23 +90,30 @@ static void run_case(int W, int H, int K, int pad)
 	insts[0].fbuf = dev<double2>((size_t)fft_pitch(n1) * n2);
 	const int cb = n2 >= 4096 ? 1 : (n2 >= 2048 ? 2 : (4096 / n2 > 8 ? 8 : 4096 / n2));
 	int *off = dev<int>(2);
+	const int np = fft_pairs_per_block(n1), rpb = 2 * np;
+	auto row_blocks = [rpb](int rows) { return (rows + rpb - 1) / rpb; };
+	const size_t row_lds = (size_t)(np + 1) * n1, col_lds = (size_t)cb * (n2 + 1) + n2;
 	int *lk = dev<int>(2);
-	double *partial = dev<double>((H + 1) / 2);
-	lk[1] = (H + 1) / 2;
+	double *partial = dev<double>(row_blocks(H));
+	lk[1] = row_blocks(H);
 	auto rows_fwd = [&](int mode, int nblocks) {
 		off[1] = nblocks;
+		set_lds(row_lds);
 		hip_cpu::launch(nblocks, 256, [&] { k_fft_rows_fwd(gals, insts, off, 1, mode, tw, twn); });
 	};
 	auto cols = [&](int op, int in_mode, int out_mode, double2 *d64) {
 		const int nb = (m1 + cb - 1) / cb;
 		off[1] = nb;
+		set_lds(col_lds);
 		hip_cpu::launch(nb, 256, [&] { k_fft_cols(gals, insts, off, 1, cb, op, in_mode, out_mode, tw, twn, d64); });
 	};
 	auto rows_inv = [&](int mode, int nblocks) {
 		off[1] = nblocks;
+		set_lds(row_lds);
 		hip_cpu::launch(nblocks, 256, [&] { k_fft_rows_inv(gals, insts, off, 1, mode, tw, twn, lk, partial); });
 	};
-	hip_cpu::launch(n2 / 2, 256, [&] { k_fft_psf_rows(gals, tw, twn, src, spec); });
+	set_lds(row_lds);
+	hip_cpu::launch((n2 / 2 + np - 1) / np, 256, [&] { k_fft_psf_rows(gals, tw, twn, src, spec); });
 	cols(0, 2, 0, spec);
 	// the PSF frame is a unit impulse at the origin, so every convolution is
 	// the identity: model = canvas, adjoint = w placed at (padx, pady)
@@ -105,20 +125,20 @@ static void run_case(int W, int H, int K, int pad)
 	for (int i = 0; i < g.DW * g.DH; ++i) insts[0].canvas[i] = rnd();
 	for (int i = 0; i < W * H; ++i) insts[0].w[i] = rnd();
 	double err = 0;
-	rows_fwd(0, (g.DH + 1) / 2);
+	rows_fwd(0, row_blocks(g.DH));
 	cols(1, 0, 1, nullptr);
-	rows_inv(1, (g.DH + 1) / 2);
+	rows_inv(1, row_blocks(g.DH));
 	for (int i = 0; i < g.DW * g.DH; ++i) err = std::fmax(err, std::fabs(insts[0].G[i] - insts[0].canvas[i]));
-	rows_fwd(0, (g.DH + 1) / 2);
+	rows_fwd(0, row_blocks(g.DH));
 	cols(1, 0, 0, nullptr);
-	rows_inv(0, (H + 1) / 2);
+	rows_inv(0, row_blocks(H));
 	for (int y = 0; y < H; ++y)
 		for (int x = 0; x < W; ++x)
 			err = std::fmax(err, std::fabs(insts[0].w[y * W + x] + insts[0].canvas[(y + pad) * g.DW + x + pad]));
 	for (int i = 0; i < W * H; ++i) insts[0].w[i] = rnd();
-	rows_fwd(1, (H + 1) / 2);
+	rows_fwd(1, row_blocks(H));
 	cols(2, 1, 1, nullptr);
-	rows_inv(2, (g.DH + 1) / 2);
+	rows_inv(2, row_blocks(g.DH));
 	for (int y = 0; y < g.DH; ++y)
 		for (int x = 0; x < g.DW; ++x) {
 			const bool in = y >= pad && y < pad + H && x >= pad && x < pad + W;
@@ -135,5 +155,9 @@ int main()
 	run_case(33, 130, 9, 0);
 	run_case(64, 40, 25, 12);
 	run_case(31, 17, 7, 3);
+	// frames of 2048 and 4096 columns: two pairs, one pair of rows per block
+	run_case(1100, 11, 5, 0);
+	run_case(2100, 5, 3, 1);
+	set_lds(profit_amd::kLdsMax);
 	return 0;
 }

@@ -25,8 +25,8 @@
 struct dim3 {
 	unsigned x = 0, y = 0, z = 0;
 };
-inline thread_local dim3 threadIdx;
-inline dim3 blockIdx, blockDim, gridDim;
+inline thread_local dim3 threadIdx, blockIdx;
+inline dim3 blockDim, gridDim;
 
 struct double2 {
 	double x, y;
@@ -51,23 +51,26 @@ namespace hip_cpu {
 inline std::barrier<> *block_barrier = nullptr;
 inline double xchg_d[1024];
 
-// runs `kernel` over `grid` blocks of `block` threads, blocks one after another
+// runs `kernel` over `grid` blocks of `block` threads, blocks one after
+// another; the same `block` host threads run every block, with a barrier
+// between blocks
 inline void launch(unsigned grid, unsigned block, const std::function<void()> &kernel)
 {
 	gridDim = dim3{grid, 1, 1};
 	blockDim = dim3{block, 1, 1};
-	for (unsigned b = 0; b < grid; ++b) {
-		blockIdx = dim3{b, 0, 0};
-		std::barrier<> bar((std::ptrdiff_t)block);
-		block_barrier = &bar;
-		std::vector<std::thread> ts;
-		for (unsigned t = 0; t < block; ++t)
-			ts.emplace_back([&, t] {
-				threadIdx = dim3{t, 0, 0};
+	std::barrier<> bar((std::ptrdiff_t)block);
+	block_barrier = &bar;
+	std::vector<std::thread> ts;
+	for (unsigned t = 0; t < block; ++t)
+		ts.emplace_back([&, t] {
+			threadIdx = dim3{t, 0, 0};
+			for (unsigned b = 0; b < grid; ++b) {
+				blockIdx = dim3{b, 0, 0};
 				kernel();
-			});
-		for (auto &t : ts) t.join();
-	}
+				bar.arrive_and_wait();
+			}
+		});
+	for (auto &t : ts) t.join();
 }
 }  // namespace hip_cpu
 
@@ -77,6 +80,12 @@ inline unsigned __brev(unsigned v)
 	unsigned r = 0;
 	for (int i = 0; i < 32; ++i) r |= ((v >> i) & 1u) << (31 - i);
 	return r;
+}
+inline int __clz(int v)
+{
+	int n = 0;
+	for (int i = 31; i >= 0 && !(((unsigned)v >> i) & 1u); --i) ++n;
+	return n;
 }
 inline double __shfl_xor(double v, int mask)
 {
