@@ -32,6 +32,15 @@ __host__ __device__ constexpr int fft_pitch(int n1) { return ((n1 / 2 + 1) + 7) 
 // the pairs and the twiddles fit the LDS).
 __host__ __device__ constexpr int fft_pairs_per_block(int n1) { return n1 <= 1024 ? 4 : (n1 <= 2048 ? 2 : 1); }
 
+// One image pixel's likelihood inputs, fetched with one 16-byte load: the
+// data (FP64), 1/sigma^2, and flags (bit 0: in the likelihood region; bit 1:
+// its canvas pixel in the calc region).
+struct alignas(16) PixelIn {
+	double data;
+	float ivar;
+	uint32_t flags;
+};
+
 // One galaxy: image geometry, PSF, masks and data.  Several instances (parameter
 // sets) may share a galaxy.
 struct GalaxyDev {
@@ -59,6 +68,7 @@ struct GalaxyDev {
 	const uint8_t *region; // W*H likelihood mask (already intersected with calc)
 	const double *data;    // W*H, FP64: the Poisson term d log m needs the data's own precision
 	const float *ivar;     // W*H, 1/sigma^2
+	const PixelIn *pix;    // W*H: data, ivar, region and calc packed
 	int32_t fft_n1, fft_n2;   // FFT frame (powers of two) when the FFT path is on
 	int32_t fft_l1, fft_l2;   // their base-2 logarithms
 	const double2 *fft_spec;  // half-spectrum of the PSF (N2 rows at fft_pitch(N1)), wrapped about the origin

@@ -75,6 +75,10 @@ static int next_pow2(int v)
 	return n;
 }
 
+// FFT frame length for v points: a power of two, at least 8 (the row passes
+// fold the bit reversal into their first and last three radix-2 stages)
+static int fft_frame(int v) { return std::max(8, next_pow2(v)); }
+
 static int ilog2(int n)
 {
 	int l = 0;
@@ -189,7 +193,7 @@ Engine::TuneReport Engine::autotune(const ProfileParams *params, int reps, bool 
 			if (kind == CK_VALU) v = (double)(g.n_ctiles + g.n_atiles) * 256.0 * conv_rows_ * g.Kx * g.Ky;
 			else if (kind == CK_MFMA) v = (double)(g.n_mtl + g.n_mtc) * g.Ky * (g.mfma_kc + 2);
 			else {
-				const double n1 = next_pow2(g.DW + g.Kx), n2 = next_pow2(g.DH + g.Ky);
+				const double n1 = fft_frame(g.DW + g.Kx), n2 = fft_frame(g.DH + g.Ky);
 				v = n1 * n2 * std::log2(n1 * n2);
 			}
 			w[inst_galaxy_[i]] += v;
@@ -278,7 +282,7 @@ void Engine::ensure_fft_spectra()
 	int twn = 2;
 	for (const auto &g : gal_host_)
 		if (g.conv_kind == CK_FFT) {
-			const int n1 = next_pow2(g.DW + g.Kx), n2 = next_pow2(g.DH + g.Ky);
+			const int n1 = fft_frame(g.DW + g.Kx), n2 = fft_frame(g.DH + g.Ky);
 			twn = std::max(twn, std::max(n1, n2));
 		}
 	if (twn > 4096) throw std::invalid_argument("FFT frame above 4096 points per axis is not supported");
@@ -298,7 +302,7 @@ void Engine::ensure_fft_spectra()
 		GalaxyDev &g = gal_host_[gi];
 		GalaxyBuffers &b = gal_bufs_[gi];
 		if (g.conv_kind != CK_FFT || b.fft_spec.size()) continue;
-		const int n1 = next_pow2(g.DW + g.Kx), n2 = next_pow2(g.DH + g.Ky);
+		const int n1 = fft_frame(g.DW + g.Kx), n2 = fft_frame(g.DH + g.Ky);
 		g.fft_n1 = n1;
 		g.fft_n2 = n2;
 		g.fft_l1 = ilog2(n1);
@@ -551,6 +555,15 @@ int Engine::add_galaxy(const GalaxySpec &s)
 		bufs.ivar.alloc(npix * 4);
 		hip_check(hipMemcpy(bufs.data.as<double>(), d.data(), npix * sizeof(double), hipMemcpyHostToDevice), "data");
 		hip_check(hipMemcpy(bufs.ivar.as<float>(), iv.data(), npix * 4, hipMemcpyHostToDevice), "ivar");
+		std::vector<PixelIn> px(npix);
+		for (int j = 0; j < s.H; ++j)
+			for (int i = 0; i < s.W; ++i) {
+				const size_t o = (size_t)j * s.W + i;
+				const size_t qc = (size_t)(j + g.pady) * g.DW + i + g.padx;
+				px[o] = PixelIn{d[o], iv[o], (uint32_t)(region[o] | (calc[qc] << 1))};
+			}
+		bufs.pix.alloc(npix * sizeof(PixelIn));
+		hip_check(hipMemcpy(bufs.pix.as<PixelIn>(), px.data(), npix * sizeof(PixelIn), hipMemcpyHostToDevice), "pix");
 	}
 	g.psf_fwd = bufs.psf_fwd.as<float>();
 	g.psf_adj = bufs.psf_adj.as<float>();
@@ -560,6 +573,7 @@ int Engine::add_galaxy(const GalaxySpec &s)
 	g.region = bufs.region.as<uint8_t>();
 	g.data = bufs.data.as<double>();
 	g.ivar = bufs.ivar.as<float>();
+	g.pix = bufs.pix.as<PixelIn>();
 	gal_host_.push_back(g);
 	gal_bufs_.push_back(std::move(bufs));
 	build_tiles((int)gal_host_.size() - 1);

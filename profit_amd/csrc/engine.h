@@ -215,7 +215,7 @@ class Engine {
 
   private:
 	struct GalaxyBuffers {
-		DevBuf psf_fwd, psf_adj, calc, region, data, ivar, rtiles, ctiles, atiles, fft_spec;
+		DevBuf psf_fwd, psf_adj, calc, region, data, ivar, pix, rtiles, ctiles, atiles, fft_spec;
 		DevBuf mfrag_fwd, mfrag_adj, mtiles_like, mtiles_canvas, psf_pt;
 		std::vector<int2> ct_host, at_host;
 		std::vector<double> psf64;  // normalised (or raw) PSF, Ky rows of Kx

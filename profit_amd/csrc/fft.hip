@@ -58,6 +58,14 @@ __device__ inline double2 cmulc(double2 a, double2 b)  // a * conj(b)
 // bit 6, a bijection on every power-of-two range.  A pass reads elements at
 // spacing 1, 8 or 64 (for 2^9 points); after the swizzle the 16 lanes that
 // share a bank cycle address distinct 16-byte slots in each case.
+// Global loads a thread issues before it waits on the first one: each load
+// loop gathers kBatch elements into registers, then stores them to LDS.
+constexpr int kBatch = 8;
+
+// Threads of a column-pass block: its LDS tile limits a CU to two blocks, so
+// the block is wide enough to keep four waves per SIMD loading.
+constexpr int kColThreads = 512;
+
 __device__ __forceinline__ int swz(int e) { return e ^ ((e >> 3) & 7) ^ (((e >> 6) & 1) << 3); }
 
 // Twiddles of an n-point transform in LDS, one table per radix-2 stage S =
@@ -75,13 +83,54 @@ __device__ inline void stage_twiddles(double2 *wt, int n, int logn, const double
 
 __device__ __forceinline__ double2 twiddle(const double2 *wt, int S, int k) { return wt[(1 << (S - 1)) - 1 + k]; }
 
-// In-place forward DIT of `ncol` sequences of length n (sequence c at
-// data + c*stride, element e at swz(e)), bit-reversed input, natural-order
-// output.  The log2(n) radix-2 stages run in passes of up to three: each
-// thread holds the eight (four, two) elements that the pass's stages combine
-// in registers, so LDS is read and written once per pass instead of once per
-// stage.  Stage S combines i and i + 2^(S-1) with twiddle
-// exp(-2 pi i (i mod 2^(S-1)) / 2^S).
+// Radix-2 stages s+1 .. s+K of a forward DIT on the 2^K elements a thread
+// holds in registers: x[j] is element e0 + (j << s), lo = e0 mod 2^s.  Stage
+// S combines i and i + 2^(S-1) with twiddle exp(-2 pi i (i mod 2^(S-1)) / 2^S).
+template <int K>
+__device__ __forceinline__ void dit_regs(double2 *x, int s, int lo, const double2 *wt)
+{
+	constexpr int R = 1 << K;
+#pragma unroll
+	for (int t = 1; t <= K; ++t) {
+		const int h = 1 << (t - 1);
+		const int S = s + t;
+#pragma unroll
+		for (int j = 0; j < R; ++j) {
+			if (j & h) continue;
+			const int kS = ((j & (h - 1)) << s) + lo;  // index within the stage's half block
+			const double2 u = x[j], v = cmul(x[j + h], twiddle(wt, S, kS));
+			x[j] = cadd(u, v);
+			x[j + h] = csub(u, v);
+		}
+	}
+}
+
+// Radix-2 stages s+K down to s+1 of an inverse DIF (conjugate twiddles,
+// unscaled) in registers: stage S maps (u, v) at i and i + 2^(S-1) to
+// (u + v, (u - v) conj w).
+template <int K>
+__device__ __forceinline__ void dif_regs(double2 *x, int s, int lo, const double2 *wt)
+{
+	constexpr int R = 1 << K;
+#pragma unroll
+	for (int t = K; t >= 1; --t) {
+		const int h = 1 << (t - 1);
+		const int S = s + t;
+#pragma unroll
+		for (int j = 0; j < R; ++j) {
+			if (j & h) continue;
+			const int kS = ((j & (h - 1)) << s) + lo;
+			const double2 u = x[j], v = x[j + h];
+			x[j] = cadd(u, v);
+			x[j + h] = cmul(csub(u, v), conjg(twiddle(wt, S, kS)));
+		}
+	}
+}
+
+// One pass of K radix-2 stages (s+1 .. s+K) over `ncol` sequences of length
+// n in LDS (sequence c at data + c*stride, element e at swz(e)): each thread
+// loads the 2^K elements a group combines, runs the stages in registers and
+// stores them back, so LDS is read and written once per pass.
 template <int K>
 __device__ __forceinline__ void dit_pass(double2 *data, int logn, int ncol, int stride, int s, const double2 *wt)
 {
@@ -96,42 +145,55 @@ __device__ __forceinline__ void dit_pass(double2 *data, int logn, int ncol, int 
 		double2 x[R];
 #pragma unroll
 		for (int j = 0; j < R; ++j) x[j] = col[swz(e0 + (j << s))];
-#pragma unroll
-		for (int t = 1; t <= K; ++t) {
-			const int h = 1 << (t - 1);
-			const int S = s + t;
-#pragma unroll
-			for (int j = 0; j < R; ++j) {
-				if (j & h) continue;
-				const int kS = ((j & (h - 1)) << s) + lo;  // index within the stage's half block
-				const double2 u = x[j], v = cmul(x[j + h], twiddle(wt, S, kS));
-				x[j] = cadd(u, v);
-				x[j + h] = csub(u, v);
-			}
-		}
+		dit_regs<K>(x, s, lo, wt);
 #pragma unroll
 		for (int j = 0; j < R; ++j) col[swz(e0 + (j << s))] = x[j];
 	}
 	__syncthreads();
 }
 
-__device__ inline void fft_dit(double2 *data, int logn, int ncol, int stride, const double2 *wt)
+// In-place forward DIT, bit-reversed input, natural-order output: passes of
+// three stages from stage 1 up, the remaining one or two stages last.
+__device__ inline void fft_dit(double2 *data, int logn, int ncol, int stride, const double2 *wt, int s = 0)
 {
-	int s = 0;
 	for (; s + 3 <= logn; s += 3) dit_pass<3>(data, logn, ncol, stride, s, wt);
 	if (logn - s == 2) dit_pass<2>(data, logn, ncol, stride, s, wt);
 	else if (logn - s == 1) dit_pass<1>(data, logn, ncol, stride, s, wt);
 }
 
-// In-place inverse (conjugate twiddles, unscaled) DIF: natural-order input,
-// bit-reversed output, element e at swz(e).  Stage S (from log2(n) down to 1)
-// maps (u, v) at i and i + 2^(S-1) to (u + v, (u - v) conj w); passes of up
-// to three stages as in the forward transform.
+// Position j of a three-bit reversal.
+__device__ __forceinline__ int rev3(int j) { return ((j & 1) << 2) | (j & 2) | (j >> 2); }
+
+// Forward DIT of `nseq` sequences of length n = 2^logn >= 8 whose first pass
+// (stages 1-3) reads its inputs through `load(c, i)` (element i of sequence
+// c, natural order) instead of LDS: group g combines the elements at
+// positions 8g + j of the bit-reversed sequence, inputs
+// rev3(j) * n/8 + bitrev(g), so no bit-reversed scatter goes through LDS.
+// Sequence c ends at data + (c << logn) in natural order.
+template <class F>
+__device__ __forceinline__ void fft_dit_gather(double2 *data, int logn, int nseq, const double2 *wt, F load)
+{
+	const int lg = logn - 3;
+	for (int gi = threadIdx.x; gi < (nseq << lg); gi += blockDim.x) {
+		const int c = gi >> lg, g = gi - (c << lg);
+		const int bg = lg ? (int)bitrev(g, lg) : 0;
+		double2 x[8];
+#pragma unroll
+		for (int j = 0; j < 8; ++j) x[j] = load(c, (rev3(j) << lg) + bg);
+		dit_regs<3>(x, 0, 0, wt);
+		double2 *col = data + (c << logn);
+#pragma unroll
+		for (int j = 0; j < 8; ++j) col[swz(8 * g + j)] = x[j];
+	}
+	__syncthreads();
+	fft_dit(data, logn, nseq, 1 << logn, wt, 3);
+}
+
+// One inverse DIF pass of K stages, top down to top-K+1 (indices differing in
+// bits top-1 .. top-K, spacing 2^(top-K)).
 template <int K>
 __device__ __forceinline__ void dif_pass(double2 *data, int logn, int ncol, int stride, int top, const double2 *wt)
 {
-	// stages top, top-1, ..., top-K+1 combine indices differing in bits
-	// top-1 .. top-K, i.e. spacing 2^(top-K)
 	constexpr int R = 1 << K;
 	const int s = top - K;
 	const int sub = 1 << s;
@@ -144,31 +206,50 @@ __device__ __forceinline__ void dif_pass(double2 *data, int logn, int ncol, int 
 		double2 x[R];
 #pragma unroll
 		for (int j = 0; j < R; ++j) x[j] = col[swz(e0 + (j << s))];
-#pragma unroll
-		for (int t = K; t >= 1; --t) {
-			const int h = 1 << (t - 1);
-			const int S = s + t;
-#pragma unroll
-			for (int j = 0; j < R; ++j) {
-				if (j & h) continue;
-				const int kS = ((j & (h - 1)) << s) + lo;
-				const double2 u = x[j], v = x[j + h];
-				x[j] = cadd(u, v);
-				x[j + h] = cmul(csub(u, v), conjg(twiddle(wt, S, kS)));
-			}
-		}
+		dif_regs<K>(x, s, lo, wt);
 #pragma unroll
 		for (int j = 0; j < R; ++j) col[swz(e0 + (j << s))] = x[j];
 	}
 	__syncthreads();
 }
 
+// In-place inverse DIF, natural-order input, bit-reversed output.
 __device__ inline void ifft_dif(double2 *data, int logn, int ncol, int stride, const double2 *wt)
 {
 	int top = logn;
 	for (; top - 3 >= 0; top -= 3) dif_pass<3>(data, logn, ncol, stride, top, wt);
 	if (top == 2) dif_pass<2>(data, logn, ncol, stride, top, wt);
 	else if (top == 1) dif_pass<1>(data, logn, ncol, stride, top, wt);
+}
+
+// Inverse DIF of `nseq` sequences of length n = 2^logn >= 8 (sequence c at
+// data + (c << logn)) whose last pass (stages 3-1) hands its results to
+// `emit(c, g, bg, x)` instead of LDS: x[j] is output rev3(j) * n/8 + bg,
+// bg = bitrev(g).  The remaining one or two stages run first.
+template <class F>
+__device__ __forceinline__ void ifft_dif_scatter(double2 *data, int logn, int nseq, const double2 *wt, F emit)
+{
+	const int stride = 1 << logn;
+	int top = logn;
+	if (logn % 3 == 2) {
+		dif_pass<2>(data, logn, nseq, stride, top, wt);
+		top -= 2;
+	}
+	else if (logn % 3 == 1) {
+		dif_pass<1>(data, logn, nseq, stride, top, wt);
+		top -= 1;
+	}
+	for (; top > 3; top -= 3) dif_pass<3>(data, logn, nseq, stride, top, wt);
+	const int lg = logn - 3;
+	for (int gi = threadIdx.x; gi < (nseq << lg); gi += blockDim.x) {
+		const int c = gi >> lg, g = gi - (c << lg);
+		const double2 *col = data + (c << logn);
+		double2 x[8];
+#pragma unroll
+		for (int j = 0; j < 8; ++j) x[j] = col[swz(8 * g + j)];
+		dif_regs<3>(x, 0, 0, wt);
+		emit(c, g, lg ? (int)bitrev(g, lg) : 0, x);
+	}
 }
 
 }  // namespace
@@ -184,19 +265,15 @@ __device__ __forceinline__ void rows_fwd_pairs(double2 *sh, const double2 *wt, i
                                                long ld, int x0, int nx, int nrows, int pmax, double2 *out, int P)
 {
 	const int M = N / 2 + 1;
-	for (int t = threadIdx.x; t < np * N; t += blockDim.x) {
-		const int p = t >> L, k = t - (p << L);
-		double a = 0, bv = 0;
+	__syncthreads();  // the twiddle tables are staged
+	fft_dit_gather(sh, L, np, wt, [&](int p, int k) {
 		const int xi = k - x0;
-		if (p < pmax && xi >= 0 && xi < nx) {
-			const T *row = src + (long)(2 * p) * ld;
-			a = (double)row[xi];
-			if (2 * p + 1 < nrows) bv = (double)row[ld + xi];
-		}
-		sh[(p << L) + swz(bitrev(k, L))] = make_double2(a, bv);
-	}
-	__syncthreads();
-	fft_dit(sh, L, np, N, wt);
+		const bool ok = p < pmax && xi >= 0 && xi < nx;
+		const bool okb = ok && 2 * p + 1 < nrows;
+		const long o = ok ? (long)(2 * p) * ld + xi : 0;
+		const double a = (double)src[o], bv = (double)src[okb ? o + ld : 0];
+		return make_double2(ok ? a : 0.0, okb ? bv : 0.0);
+	});
 	for (int t = threadIdx.x; t < np * M; t += blockDim.x) {
 		const int p = t / M, k = t - p * M;
 		if (p >= pmax) continue;
@@ -259,7 +336,7 @@ __global__ __launch_bounds__(256) void k_fft_psf_rows(const GalaxyDev *__restric
 // the PSF spectrum; op 2: with its conjugate.  The inverse column transform
 // then leaves rows [out_lo, out_hi) in the frame.  Global accesses run along
 // rows (CB adjacent entries); columns sit in LDS at a stride of N2 + 1.
-__global__ __launch_bounds__(256) void k_fft_cols(const GalaxyDev *__restrict__ gals, const InstDev *__restrict__ insts,
+__global__ __launch_bounds__(kColThreads) void k_fft_cols(const GalaxyDev *__restrict__ gals, const InstDev *__restrict__ insts,
                                                   const int *__restrict__ col_off, int ninst, int cb, int op,
                                                   int in_mode, int out_mode, const double2 *__restrict__ tw, int twn,
                                                   double2 *dst64)
@@ -278,27 +355,50 @@ __global__ __launch_bounds__(256) void k_fft_cols(const GalaxyDev *__restrict__ 
 	// rows holding data: 0 canvas rows [0, DH); 1 image rows [pady, pady + H); 2 all rows
 	const int in_lo = in_mode == 1 ? g.pady : 0;
 	const int in_hi = in_mode == 0 ? g.DH : (in_mode == 1 ? g.pady + g.H : N);
-	for (int t = threadIdx.x; t < cb * N; t += blockDim.x) {
-		const int y = t / cb, c = t - y * cb;
-		double2 v = make_double2(0.0, 0.0);
-		if (y >= in_lo && y < in_hi && c0 + c < M) v = frame[(long)y * P + c0 + c];
-		sh[c * S + swz(bitrev(y, L))] = v;
+	// cb is a power of two: entry t of the tile is row t >> lcb, column t & (cb - 1)
+	const int lcb = __ffs(cb) - 1, total = cb << L;
+	for (int base = threadIdx.x; base < total; base += kBatch * blockDim.x) {
+		double2 v[kBatch];
+#pragma unroll
+		for (int u = 0; u < kBatch; ++u) {
+			const int t = base + u * blockDim.x;
+			const int y = t >> lcb, c = t & (cb - 1);
+			const bool ok = t < total && y >= in_lo && y < in_hi && c0 + c < M;
+			const double2 x = frame[ok ? (long)y * P + c0 + c : 0];
+			v[u] = ok ? x : make_double2(0.0, 0.0);
+		}
+#pragma unroll
+		for (int u = 0; u < kBatch; ++u) {
+			const int t = base + u * blockDim.x;
+			if (t < total) sh[(t & (cb - 1)) * S + swz(bitrev(t >> lcb, L))] = v[u];
+		}
 	}
 	__syncthreads();
 	fft_dit(sh, L, cb, S, wt);
 	if (op == 0) {
-		for (int t = threadIdx.x; t < cb * N; t += blockDim.x) {
-			const int y = t / cb, c = t - y * cb;
+		for (int t = threadIdx.x; t < total; t += blockDim.x) {
+			const int y = t >> lcb, c = t & (cb - 1);
 			if (c0 + c < M) frame[(long)y * P + c0 + c] = sh[c * S + swz(y)];
 		}
 		return;
 	}
-	for (int t = threadIdx.x; t < cb * N; t += blockDim.x) {
-		const int y = t / cb, c = t - y * cb;
-		if (c0 + c < M) {
-			const double2 s = g.fft_spec[(long)y * P + c0 + c];
-			const double2 v = sh[c * S + swz(y)];
-			sh[c * S + swz(y)] = (op == 1) ? cmul(v, s) : cmulc(v, s);
+	for (int base = threadIdx.x; base < total; base += kBatch * blockDim.x) {
+		double2 sp[kBatch];
+#pragma unroll
+		for (int u = 0; u < kBatch; ++u) {
+			const int t = base + u * blockDim.x;
+			const int y = t >> lcb, c = t & (cb - 1);
+			const bool ok = t < total && c0 + c < M;
+			sp[u] = g.fft_spec[ok ? (long)y * P + c0 + c : 0];
+		}
+#pragma unroll
+		for (int u = 0; u < kBatch; ++u) {
+			const int t = base + u * blockDim.x;
+			const int y = t >> lcb, c = t & (cb - 1);
+			if (t < total && c0 + c < M) {
+				const double2 v = sh[c * S + swz(y)];
+				sh[c * S + swz(y)] = (op == 1) ? cmul(v, sp[u]) : cmulc(v, sp[u]);
+			}
 		}
 	}
 	__syncthreads();
@@ -306,8 +406,8 @@ __global__ __launch_bounds__(256) void k_fft_cols(const GalaxyDev *__restrict__ 
 	// rows written back: 0 image rows [pady, pady + H); 1 canvas rows [0, DH)
 	const int out_lo = out_mode == 0 ? g.pady : 0;
 	const int out_hi = out_mode == 0 ? g.pady + g.H : g.DH;
-	for (int t = threadIdx.x; t < cb * N; t += blockDim.x) {
-		const int y = t / cb, c = t - y * cb;
+	for (int t = threadIdx.x; t < total; t += blockDim.x) {
+		const int y = t >> lcb, c = t & (cb - 1);
 		if (y >= out_lo && y < out_hi && c0 + c < M) frame[(long)y * P + c0 + c] = sh[c * S + swz(bitrev(y, L))];
 	}
 }
@@ -321,11 +421,13 @@ __global__ __launch_bounds__(256) void k_fft_cols(const GalaxyDev *__restrict__ 
 //     block, w written
 //   mode 1 (model image): canvas rows; G = model masked by calc
 //   mode 2 (adjoint): canvas rows; G = value / (N1 N2)
+template <int MODE>
 __global__ __launch_bounds__(256) void k_fft_rows_inv(const GalaxyDev *__restrict__ gals, const InstDev *__restrict__ insts,
-                                                      const int *__restrict__ row_off, int ninst, int mode,
+                                                      const int *__restrict__ row_off, int ninst,
                                                       const double2 *__restrict__ tw, int twn,
                                                       const int *__restrict__ lk_base, double *partial)
 {
+	constexpr int mode = MODE;
 	extern __shared__ double2 sh[];
 	const int b = blockIdx.x;
 	const int inst = owner(row_off, ninst, b);
@@ -338,63 +440,85 @@ __global__ __launch_bounds__(256) void k_fft_rows_inv(const GalaxyDev *__restric
 	const int r0 = 2 * np * blk;  // first output row of the block (image rows in mode 0)
 	const int nrows = mode == 0 ? g.H : g.DH;
 	const int y0 = mode == 0 ? r0 + g.pady : r0;  // its frame row
-	for (int t = threadIdx.x; t < np * N; t += blockDim.x) {
-		const int p = t >> L, k = t - (p << L);
-		const int r = r0 + 2 * p;
-		double2 z = make_double2(0.0, 0.0);
-		if (r < nrows) {
-			const bool second = r + 1 < nrows;
-			const double2 *ra = I.fbuf + (long)(y0 + 2 * p) * P;
-			const double2 *rb = ra + P;
-			if (k < M) {
-				const double2 a = ra[k], bv = second ? rb[k] : make_double2(0.0, 0.0);
-				z = make_double2(a.x - bv.y, a.y + bv.x);  // A + i B
-			}
-			else {
-				const double2 a = conjg(ra[N - k]), bv = second ? conjg(rb[N - k]) : make_double2(0.0, 0.0);
-				z = make_double2(a.x - bv.y, a.y + bv.x);
+	const int total = np * N;
+	const double2 *rows = I.fbuf + (long)y0 * P;
+	for (int base = threadIdx.x; base < total; base += kBatch * blockDim.x) {
+		double2 va[kBatch], vb[kBatch];
+#pragma unroll
+		for (int u = 0; u < kBatch; ++u) {
+			const int t = base + u * blockDim.x;
+			const int p = t >> L, k = t - (p << L);
+			const int r = r0 + 2 * p;
+			const bool ok = t < total && r < nrows, okb = ok && r + 1 < nrows;
+			// above N/2 the entries are conjugates of the ones at N - k
+			const long o = (long)(2 * p) * P + (k < M ? k : N - k);
+			va[u] = rows[ok ? o : 0];
+			vb[u] = rows[okb ? o + P : 0];
+			if (!ok) va[u] = make_double2(0.0, 0.0);
+			if (!okb) vb[u] = make_double2(0.0, 0.0);
+		}
+#pragma unroll
+		for (int u = 0; u < kBatch; ++u) {
+			const int t = base + u * blockDim.x;
+			if (t < total) {
+				const int p = t >> L, k = t - (p << L);
+				const double2 a = k < M ? va[u] : conjg(va[u]), bv = k < M ? vb[u] : conjg(vb[u]);
+				sh[(p << L) + swz(k)] = make_double2(a.x - bv.y, a.y + bv.x);  // A + i B
 			}
 		}
-		sh[(p << L) + swz(k)] = z;
 	}
 	__syncthreads();
-	ifft_dif(sh, L, np, N, wt);
 	const double scale = 1.0 / ((double)g.fft_n1 * (double)g.fft_n2);
-	if (mode == 0) {
+	const int lg = L - 3;
+	if constexpr (mode == 0) {
 		double part = 0;
 		const bool pois = g.likefunc == LIKE_POIS, tl = g.likefunc == LIKE_T;
 		const double s = g.pois_scale;
-		const int W2 = 2 * g.W;
-		for (int t = threadIdx.x; t < np * W2; t += blockDim.x) {
-			const int p = t / W2, rem = t - p * W2;
-			const int row = rem >= g.W ? 1 : 0, x = rem - row * g.W;
-			const int r = r0 + 2 * p + row;
-			if (r >= nrows) continue;
-			const long o = (long)r * g.W + x;
-			float wv = 0.f;
-			if (g.region[o]) {
-				const long qc = (long)(y0 + 2 * p + row) * g.DW + x + g.padx;
-				const bool in_calc = !g.has_calc || g.calc[qc];
-				const double2 z = sh[(p << L) + swz(bitrev(x + g.padx, L))];
-				const double m = in_calc ? (row ? z.y : z.x) * scale + (I.has_aux ? (double)I.aux[qc] : 0.0) : 0.0;
-				const double d = g.data[o];
-				if (pois) {
-					const double mm = s * m, dd = s * d;
-					part += mm - dd * log(mm);
-					wv = (float)(-s * (1.0 - d / m));
+		// x[j] is frame column rev3(j) * N/8 + bg of rows 2p (real part) and
+		// 2p + 1 (imaginary part); image column = frame column - padx
+		ifft_dif_scatter(sh, L, np, wt, [&](int p, int, int bg, const double2 *x) {
+#pragma unroll
+			for (int half = 0; half < 4; ++half) {
+				const int row = half >> 1, j0 = (half & 1) * 4;
+				const int r = r0 + 2 * p + row;
+				// the pixels' inputs for four columns first, then the terms
+				PixelIn q[8];
+				double ax[8];
+#pragma unroll
+				for (int j = j0; j < j0 + 4; ++j) {
+					const int xi = (rev3(j) << lg) + bg - g.padx;
+					const bool ok = r < nrows && xi >= 0 && xi < g.W;
+					q[j] = g.pix[ok ? (long)r * g.W + xi : 0];
+					ax[j] = I.has_aux && ok ? (double)I.aux[(long)(y0 + 2 * p + row) * g.DW + xi + g.padx] : 0.0;
 				}
-				else {
-					const double diff = d - m;
-					const double iv = g.ivar[o];
-					part += diff * diff * iv;
-					wv = (float)(diff * iv);
-					// t: the standardised residual, for k_like_t
-					if (tl) wv = (float)(diff * sqrt(iv));
+#pragma unroll
+				for (int j = j0; j < j0 + 4; ++j) {
+					const int xi = (rev3(j) << lg) + bg - g.padx;
+					if (r >= nrows || xi < 0 || xi >= g.W) continue;
+					float wv = 0.f;
+					if (q[j].flags & 1u) {
+						const bool in_calc = (q[j].flags & 2u) != 0;
+						const double m = in_calc ? (row ? x[j].y : x[j].x) * scale + ax[j] : 0.0;
+						const double d = q[j].data;
+						if (pois) {
+							const double mm = s * m, dd = s * d;
+							part += mm - dd * log(mm);
+							wv = (float)(-s * (1.0 - d / m));
+						}
+						else {
+							const double diff = d - m;
+							const double iv = q[j].ivar;
+							part += diff * diff * iv;
+							wv = (float)(diff * iv);
+							// t: the standardised residual, for k_like_t
+							if (tl) wv = (float)(diff * sqrt(iv));
+						}
+						if (!in_calc && !tl) wv = 0.f;
+					}
+					I.w[(long)r * g.W + xi] = wv;
 				}
-				if (!in_calc && !tl) wv = 0.f;
 			}
-			I.w[o] = wv;
-		}
+		});
 		__shared__ double red[4];
 #pragma unroll
 		for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
@@ -407,19 +531,22 @@ __global__ __launch_bounds__(256) void k_fft_rows_inv(const GalaxyDev *__restric
 		}
 		return;
 	}
-	const int D2 = 2 * g.DW;
-	for (int t = threadIdx.x; t < np * D2; t += blockDim.x) {
-		const int p = t / D2, rem = t - p * D2;
-		const int row = rem >= g.DW ? 1 : 0, x = rem - row * g.DW;
-		const int r = r0 + 2 * p + row;
-		if (r >= nrows) continue;
-		const double2 z = sh[(p << L) + swz(bitrev(x, L))];
-		double v = (row ? z.y : z.x) * scale;
-		const long o = (long)r * g.DW + x;
-		if (mode == 1 && g.has_calc && !g.calc[o]) v = 0;
-		if (mode == 1 && I.has_aux) v += (double)I.aux[o];
-		I.G[o] = (float)v;
-	}
+	ifft_dif_scatter(sh, L, np, wt, [&](int p, int, int bg, const double2 *x) {
+#pragma unroll
+		for (int row = 0; row < 2; ++row) {
+			const int r = r0 + 2 * p + row;
+#pragma unroll
+			for (int j = 0; j < 8; ++j) {
+				const int xc = (rev3(j) << lg) + bg;
+				if (r >= nrows || xc >= g.DW) continue;
+				const long o = (long)r * g.DW + xc;
+				double v = (row ? x[j].y : x[j].x) * scale;
+				if (mode == 1 && g.has_calc && !g.calc[o]) v = 0;
+				if (mode == 1 && I.has_aux) v += (double)I.aux[o];
+				I.G[o] = (float)v;
+			}
+		}
+	});
 }
 
 // ------------------------------------------------------------------ launchers
@@ -439,7 +566,7 @@ static int ensure_lds(const void *fn, size_t lds, size_t *done)
 	done[dev] = lds;
 	return 0;
 }
-static size_t lds_rows_fwd[64], lds_psf_rows[64], lds_cols[64], lds_rows_inv[64];
+static size_t lds_rows_fwd[64], lds_psf_rows[64], lds_cols[64], lds_rows_inv[64], lds_rows_inv1[64], lds_rows_inv2[64];
 
 static int cols_per_block(int n2) { return n2 >= 4096 ? 1 : (n2 >= 2048 ? 2 : (4096 / n2 > 8 ? 8 : 4096 / n2)); }
 
@@ -478,7 +605,7 @@ int launch_fft_cols(int nblocks, int n2max, int cb, const GalaxyDev *gals, const
 	const size_t lds = sizeof(double2) * ((size_t)cb * (n2max + 1) + n2max);
 	if (lds > 160 * 1024) return (int)hipErrorInvalidValue;
 	if (int e = ensure_lds((const void *)k_fft_cols, lds, lds_cols)) return e;
-	k_fft_cols<<<nblocks, 256, lds, st>>>(gals, insts, col_off, ninst, cb, op, in_mode, out_mode, tw, twn, dst64);
+	k_fft_cols<<<nblocks, kColThreads, lds, st>>>(gals, insts, col_off, ninst, cb, op, in_mode, out_mode, tw, twn, dst64);
 	return (int)hipGetLastError();
 }
 
@@ -487,10 +614,21 @@ int launch_fft_rows_inv(int nblocks, int nmax, const GalaxyDev *gals, const Inst
                         hipStream_t st)
 {
 	if (nblocks <= 0) return 0;
+	if (mode < 0 || mode > 2) return (int)hipErrorInvalidValue;
 	const size_t lds = row_lds(nmax);
 	if (lds > 160 * 1024) return (int)hipErrorInvalidValue;
-	if (int e = ensure_lds((const void *)k_fft_rows_inv, lds, lds_rows_inv)) return e;
-	k_fft_rows_inv<<<nblocks, 256, lds, st>>>(gals, insts, row_off, ninst, mode, tw, twn, lk_base, partial);
+	if (mode == 0) {
+		if (int e = ensure_lds((const void *)k_fft_rows_inv<0>, lds, lds_rows_inv)) return e;
+		k_fft_rows_inv<0><<<nblocks, 256, lds, st>>>(gals, insts, row_off, ninst, tw, twn, lk_base, partial);
+	}
+	else if (mode == 1) {
+		if (int e = ensure_lds((const void *)k_fft_rows_inv<1>, lds, lds_rows_inv1)) return e;
+		k_fft_rows_inv<1><<<nblocks, 256, lds, st>>>(gals, insts, row_off, ninst, tw, twn, lk_base, partial);
+	}
+	else {
+		if (int e = ensure_lds((const void *)k_fft_rows_inv<2>, lds, lds_rows_inv2)) return e;
+		k_fft_rows_inv<2><<<nblocks, 256, lds, st>>>(gals, insts, row_off, ninst, tw, twn, lk_base, partial);
+	}
 	return (int)hipGetLastError();
 }
 

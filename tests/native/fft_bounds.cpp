@@ -7,6 +7,7 @@
 // "ok <geometry>" per case.
 #include <sanitizer/asan_interface.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 
@@ -58,7 +59,7 @@ static void run_case(int W, int H, int K, int pad)
 	g.has_calc = 1;
 	g.likefunc = LIKE_CHISQ;
 	g.pois_scale = 1;
-	const int n1 = next_pow2(g.DW + K), n2 = next_pow2(g.DH + K), m1 = n1 / 2 + 1;
+	const int n1 = std::max(8, next_pow2(g.DW + K)), n2 = std::max(8, next_pow2(g.DH + K)), m1 = n1 / 2 + 1;
 	g.fft_n1 = n1;
 	g.fft_n2 = n2;
 	g.fft_l1 = ilog2(n1);
@@ -80,6 +81,12 @@ static void run_case(int W, int H, int K, int pad)
 	g.region = region;
 	g.data = data;
 	g.ivar = ivar;
+	PixelIn *pix = dev<PixelIn>((size_t)W * H);
+	for (int y = 0; y < H; ++y)
+		for (int x = 0; x < W; ++x)
+			pix[y * W + x] = PixelIn{data[y * W + x], ivar[y * W + x],
+			                         (uint32_t)(region[y * W + x] | (calc[(y + pad) * g.DW + x + pad] << 1))};
+	g.pix = pix;
 	GalaxyDev *gals = dev<GalaxyDev>(1);
 	gals[0] = g;
 	InstDev *insts = dev<InstDev>(1);
@@ -105,12 +112,16 @@ static void run_case(int W, int H, int K, int pad)
 		const int nb = (m1 + cb - 1) / cb;
 		off[1] = nb;
 		set_lds(col_lds);
-		hip_cpu::launch(nb, 256, [&] { k_fft_cols(gals, insts, off, 1, cb, op, in_mode, out_mode, tw, twn, d64); });
+		hip_cpu::launch(nb, kColThreads, [&] { k_fft_cols(gals, insts, off, 1, cb, op, in_mode, out_mode, tw, twn, d64); });
 	};
 	auto rows_inv = [&](int mode, int nblocks) {
 		off[1] = nblocks;
 		set_lds(row_lds);
-		hip_cpu::launch(nblocks, 256, [&] { k_fft_rows_inv(gals, insts, off, 1, mode, tw, twn, lk, partial); });
+		hip_cpu::launch(nblocks, 256, [&] {
+			if (mode == 0) k_fft_rows_inv<0>(gals, insts, off, 1, tw, twn, lk, partial);
+			else if (mode == 1) k_fft_rows_inv<1>(gals, insts, off, 1, tw, twn, lk, partial);
+			else k_fft_rows_inv<2>(gals, insts, off, 1, tw, twn, lk, partial);
+		});
 	};
 	set_lds(row_lds);
 	hip_cpu::launch((n2 / 2 + np - 1) / np, 256, [&] { k_fft_psf_rows(gals, tw, twn, src, spec); });
@@ -155,6 +166,9 @@ int main()
 	run_case(33, 130, 9, 0);
 	run_case(64, 40, 25, 12);
 	run_case(31, 17, 7, 3);
+	// the smallest frames: 8 and 16 points per axis
+	run_case(3, 2, 1, 0);
+	run_case(9, 4, 3, 1);
 	// frames of 2048 and 4096 columns: two pairs, one pair of rows per block
 	run_case(1100, 11, 5, 0);
 	run_case(2100, 5, 3, 1);

@@ -87,6 +87,7 @@ inline int __clz(int v)
 	for (int i = 31; i >= 0 && !(((unsigned)v >> i) & 1u); --i) ++n;
 	return n;
 }
+inline int __ffs(int v) { return __builtin_ffs(v); }
 inline double __shfl_xor(double v, int mask)
 {
 	const unsigned t = threadIdx.x;
