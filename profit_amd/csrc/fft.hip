@@ -63,7 +63,8 @@ __device__ inline double2 cmulc(double2 a, double2 b)  // a * conj(b)
 constexpr int kBatch = 8;
 
 // Threads of a column-pass block: its LDS tile limits a CU to two blocks, so
-// the block is wide enough to keep four waves per SIMD loading.
+// the block is wide enough to keep four waves per SIMD loading (and its
+// registers are held to the 128 per lane that four waves allow).
 constexpr int kColThreads = 512;
 
 __device__ __forceinline__ int swz(int e) { return e ^ ((e >> 3) & 7) ^ (((e >> 6) & 1) << 3); }
@@ -335,8 +336,8 @@ __global__ __launch_bounds__(256) void k_fft_psf_rows(const GalaxyDev *__restric
 // only (all N2 rows of `dst64` written: the PSF spectrum); op 1: product with
 // the PSF spectrum; op 2: with its conjugate.  The inverse column transform
 // then leaves rows [out_lo, out_hi) in the frame.  Global accesses run along
-// rows (CB adjacent entries); columns sit in LDS at a stride of N2 + 1.
-__global__ __launch_bounds__(kColThreads) void k_fft_cols(const GalaxyDev *__restrict__ gals, const InstDev *__restrict__ insts,
+// rows (CB adjacent entries); columns sit in LDS at a stride of N2 + 2.
+__global__ __launch_bounds__(kColThreads, 4) void k_fft_cols(const GalaxyDev *__restrict__ gals, const InstDev *__restrict__ insts,
                                                   const int *__restrict__ col_off, int ninst, int cb, int op,
                                                   int in_mode, int out_mode, const double2 *__restrict__ tw, int twn,
                                                   double2 *dst64)
@@ -347,7 +348,9 @@ __global__ __launch_bounds__(kColThreads) void k_fft_cols(const GalaxyDev *__res
 	const InstDev &I = insts[inst];
 	const GalaxyDev &g = gals[I.galaxy];
 	const int M = g.fft_n1 / 2 + 1, P = fft_pitch(g.fft_n1), N = g.fft_n2, L = g.fft_l2;
-	const int S = N + 1;
+	// column stride: 2 mod 16 entries keeps the columns-fastest pass on
+	// distinct LDS banks
+	const int S = N + 2;
 	double2 *wt = sh + cb * S;
 	stage_twiddles(wt, N, L, tw, twn);
 	const int c0 = (b - col_off[inst]) * cb;
@@ -374,35 +377,47 @@ __global__ __launch_bounds__(kColThreads) void k_fft_cols(const GalaxyDev *__res
 		}
 	}
 	__syncthreads();
-	fft_dit(sh, L, cb, S, wt);
 	if (op == 0) {
+		fft_dit(sh, L, cb, S, wt);
 		for (int t = threadIdx.x; t < total; t += blockDim.x) {
 			const int y = t >> lcb, c = t & (cb - 1);
 			if (c0 + c < M) frame[(long)y * P + c0 + c] = sh[c * S + swz(y)];
 		}
 		return;
 	}
-	for (int base = threadIdx.x; base < total; base += kBatch * blockDim.x) {
-		double2 sp[kBatch];
+	// forward DIT with the one or two odd stages first, then passes of three;
+	// its last pass (stages L-2 .. L) and the first inverse pass (stages L ..
+	// L-2) combine the same eight elements, so they run in one step around the
+	// product with the PSF spectrum
+	int s = L % 3;
+	if (s == 2) dit_pass<2>(sh, L, cb, S, 0, wt);
+	else if (s == 1) dit_pass<1>(sh, L, cb, S, 0, wt);
+	for (; s + 3 < L; s += 3) dit_pass<3>(sh, L, cb, S, s, wt);
+	const int lg = L - 3;
+	const double2 *spec = g.fft_spec;
+	for (int gi = threadIdx.x; gi < (cb << lg); gi += blockDim.x) {
+		// columns fastest: a wave reads whole 128-byte runs of spectrum rows
+		const int c = gi & (cb - 1), lo = gi >> lcb;
+		const bool ok = c0 + c < M;
+		double2 *col = sh + c * S;
+		double2 x[8], sp[8];
 #pragma unroll
-		for (int u = 0; u < kBatch; ++u) {
-			const int t = base + u * blockDim.x;
-			const int y = t >> lcb, c = t & (cb - 1);
-			const bool ok = t < total && c0 + c < M;
-			sp[u] = g.fft_spec[ok ? (long)y * P + c0 + c : 0];
+		for (int j = 0; j < 8; ++j) {
+			sp[j] = spec[ok ? (long)(lo + (j << lg)) * P + c0 + c : 0];
+			x[j] = col[swz(lo + (j << lg))];
 		}
+		dit_regs<3>(x, lg, lo, wt);
 #pragma unroll
-		for (int u = 0; u < kBatch; ++u) {
-			const int t = base + u * blockDim.x;
-			const int y = t >> lcb, c = t & (cb - 1);
-			if (t < total && c0 + c < M) {
-				const double2 v = sh[c * S + swz(y)];
-				sh[c * S + swz(y)] = (op == 1) ? cmul(v, sp[u]) : cmulc(v, sp[u]);
-			}
-		}
+		for (int j = 0; j < 8; ++j) x[j] = (op == 1) ? cmul(x[j], sp[j]) : cmulc(x[j], sp[j]);
+		dif_regs<3>(x, lg, lo, wt);
+#pragma unroll
+		for (int j = 0; j < 8; ++j) col[swz(lo + (j << lg))] = x[j];
 	}
 	__syncthreads();
-	ifft_dif(sh, L, cb, S, wt);
+	int top = lg;
+	for (; top - 3 >= 0; top -= 3) dif_pass<3>(sh, L, cb, S, top, wt);
+	if (top == 2) dif_pass<2>(sh, L, cb, S, top, wt);
+	else if (top == 1) dif_pass<1>(sh, L, cb, S, top, wt);
 	// rows written back: 0 image rows [pady, pady + H); 1 canvas rows [0, DH)
 	const int out_lo = out_mode == 0 ? g.pady : 0;
 	const int out_hi = out_mode == 0 ? g.pady + g.H : g.DH;
@@ -602,7 +617,7 @@ int launch_fft_cols(int nblocks, int n2max, int cb, const GalaxyDev *gals, const
                     hipStream_t st)
 {
 	if (nblocks <= 0) return 0;
-	const size_t lds = sizeof(double2) * ((size_t)cb * (n2max + 1) + n2max);
+	const size_t lds = sizeof(double2) * ((size_t)cb * (n2max + 2) + n2max);
 	if (lds > 160 * 1024) return (int)hipErrorInvalidValue;
 	if (int e = ensure_lds((const void *)k_fft_cols, lds, lds_cols)) return e;
 	k_fft_cols<<<nblocks, kColThreads, lds, st>>>(gals, insts, col_off, ninst, cb, op, in_mode, out_mode, tw, twn, dst64);
