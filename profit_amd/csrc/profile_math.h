@@ -199,6 +199,16 @@ PA_HD int validate_params(const ProfileParams &p)
 
 // ------------------------------------------------------------------ evaluate_at
 
+// FP32 log2 and exp2 on the transcendental unit (v_log_f32, v_exp_f32: about
+// 1 ulp for normal arguments).  The FP32 profile values and derivatives need
+// a few parts in 1e7; subnormal logarithm arguments take the library path.
+__device__ __forceinline__ float log2_fast(float x)
+{
+	return x >= 1.17549435e-38f ? __builtin_amdgcn_logf(x) : ::log2f(x);
+}
+__device__ __forceinline__ float exp2_fast(float x) { return __builtin_amdgcn_exp2f(x); }
+constexpr double kLog2e = 1.4426950408889634, kLn2 = 0.6931471805599453;
+
 template <class T>
 PA_HDI T boxy_r(const Comp &c, T x, T y)
 {
@@ -223,7 +233,10 @@ PA_HDI T sersic_at(const Comp &c, T x, T y)
 		case K_FOUR: rf = ::sqrt(::sqrt(base)); break;
 		case K_EIGHT: rf = ::sqrt(::sqrt(::sqrt(base))); break;
 		case K_SIXTEEN: rf = ::sqrt(::sqrt(::sqrt(::sqrt(base)))); break;
-		default: rf = ::pow(base, T(1) / (nser * B)); break;
+		default:
+			if constexpr (sizeof(T) == 4) rf = exp2_fast(log2_fast(base) / (nser * B));
+			else rf = ::pow(base, T(1) / (nser * B));
+			break;
 		}
 	}
 	else {
@@ -236,9 +249,14 @@ PA_HDI T sersic_at(const Comp &c, T x, T y)
 		case K_FOUR: rf = ::sqrt(::sqrt(::sqrt(base))); break;
 		case K_EIGHT: rf = ::sqrt(::sqrt(::sqrt(::sqrt(base)))); break;
 		case K_SIXTEEN: rf = ::sqrt(::sqrt(::sqrt(::sqrt(::sqrt(base))))); break;
-		default: rf = ::pow(::sqrt(base), T(1) / nser); break;
+		default:
+			// base^(1/(2n)) = 2^(log2(base) / (2n)) in FP32
+			if constexpr (sizeof(T) == 4) rf = exp2_fast(log2_fast(base) * (T(0.5) / nser));
+			else rf = ::pow(::sqrt(base), T(1) / nser);
+			break;
 		}
 	}
+	if constexpr (sizeof(T) == 4) return exp2_fast(-bn * (rf - T(1)) * T(kLog2e));
 	return ::exp(-bn * (rf - T(1)));
 }
 
@@ -407,9 +425,18 @@ PA_HDI void sample_grad(const Comp &c, T x, T y, T *o)
 			o[0] = ::exp(bn);
 			return;
 		}
-		const T lb = ::log(base);
-		const T rf = ::exp(lb / (n * B));
-		const T f = ::exp(-bn * (rf - T(1)));
+		T lb, rf, f;
+		if constexpr (sizeof(T) == 4) {
+			const T l2 = log2_fast(base);
+			rf = exp2_fast(l2 / (n * B));
+			f = exp2_fast(-bn * (rf - T(1)) * T(kLog2e));
+			lb = l2 * T(kLn2);
+		}
+		else {
+			lb = ::log(base);
+			rf = ::exp(lb / (n * B));
+			f = ::exp(-bn * (rf - T(1)));
+		}
 		o[0] = f;
 		const T k = -bn * f * rf / (n * B * base);  // df/dbase
 		o[1] = u > T(0) ? k * B * pu / u / re * (x > T(0) ? T(1) : T(-1)) : T(0);
