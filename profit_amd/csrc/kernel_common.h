@@ -64,13 +64,15 @@ __device__ __forceinline__ double sample_accumulate(const Comp &c, double xp, do
 	T o[9];
 	const T fxp = (T)xp, fyp = (T)yp;
 	sample_grad<T, TYPE>(c, fxp, fyp, o);
-	const T q = (T)c.axrat;
+	// the rotation's chain rule with the per-component sin/q, cos/q and 1/q
+	const T q = (T)c.axrat, iq = (T)c.inv_axrat;
 	const T ca = (T)c.cos_ang, sa = (T)c.sin_ang;
+	const T sq = (T)c.sin_over, cq = (T)c.cos_over;
 	const T fx = o[1], fy = o[2];
-	gs[G_XCEN] += wgt * (double)(fx * -ca + fy * (sa / q));
-	gs[G_YCEN] += wgt * (double)(fx * -sa + fy * (-ca / q));
-	gs[G_ANG] += wgt * (double)((fx * (q * fyp) + fy * (-fxp / q)) * (T(3.14159265358979323846) / T(180)));
-	gs[G_AXRAT] += wgt * (double)(fy * (-fyp / q));
+	gs[G_XCEN] += wgt * (double)(fx * -ca + fy * sq);
+	gs[G_YCEN] += wgt * (double)(fx * -sa + fy * -cq);
+	gs[G_ANG] += wgt * (double)((fx * (q * fyp) + fy * (-fxp * iq)) * (T(3.14159265358979323846) / T(180)));
+	gs[G_AXRAT] += wgt * (double)(fy * (-fyp * iq));
 	if (c.grad_box) gs[G_BOX] += wgt * (double)o[3];
 #pragma unroll
 	for (int s = 0; s < n_shape(TYPE); ++s) gs[G_S0 + s] += wgt * (double)o[4 + s];
@@ -325,13 +327,15 @@ __device__ __forceinline__ float pixel_accumulate_f32(const Comp &c, float xp, f
 {
 	float o[9];
 	sample_grad<float, TYPE>(c, xp, yp, o);
-	const float q = (float)c.axrat;
+	// the rotation's chain rule with the per-component sin/q, cos/q and 1/q
+	const float q = (float)c.axrat, iq = (float)c.inv_axrat;
 	const float ca = (float)c.cos_ang, sa = (float)c.sin_ang;
+	const float sq = (float)c.sin_over, cq = (float)c.cos_over;
 	const float fx = o[1], fy = o[2];
-	gs[G_XCEN] += wgt * (fx * -ca + fy * (sa / q));
-	gs[G_YCEN] += wgt * (fx * -sa + fy * (-ca / q));
-	gs[G_ANG] += wgt * ((fx * (q * yp) + fy * (-xp / q)) * (3.14159265358979323846f / 180.f));
-	gs[G_AXRAT] += wgt * (fy * (-yp / q));
+	gs[G_XCEN] += wgt * (fx * -ca + fy * sq);
+	gs[G_YCEN] += wgt * (fx * -sa + fy * -cq);
+	gs[G_ANG] += wgt * ((fx * (q * yp) + fy * (-xp * iq)) * (3.14159265358979323846f / 180.f));
+	gs[G_AXRAT] += wgt * (fy * (-yp * iq));
 	if (c.grad_box) gs[G_BOX] += wgt * o[3];
 #pragma unroll
 	for (int s = 0; s < n_shape(TYPE); ++s) gs[G_S0 + s] += wgt * o[4 + s];

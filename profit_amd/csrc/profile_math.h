@@ -110,6 +110,10 @@ struct Comp {
 	double dlnfs[NGRAD];              // d ln(flux_scale) / d slot
 	double lumtot, rbox, ie;
 	double rscale_switch, rscale_max; // adjusted values
+	// Sersic: 1/re, 1/re^2, 1/n, 1/(n B) and 1/(2n) (B = box + 2), so the
+	// FP32 values and the derivatives multiply instead of dividing
+	double inv_re, inv_re2, inv_n, inv_nB, inv_2n;
+	double inv_axrat;
 };
 
 // validation error codes (radial.cpp:204-214 and each profile's validate())
@@ -234,13 +238,14 @@ PA_HDI T sersic_at(const Comp &c, T x, T y)
 		case K_EIGHT: rf = ::sqrt(::sqrt(::sqrt(base))); break;
 		case K_SIXTEEN: rf = ::sqrt(::sqrt(::sqrt(::sqrt(base)))); break;
 		default:
-			if constexpr (sizeof(T) == 4) rf = exp2_fast(log2_fast(base) / (nser * B));
+			if constexpr (sizeof(T) == 4) rf = exp2_fast(log2_fast(base) * T(c.inv_nB));
 			else rf = ::pow(base, T(1) / (nser * B));
 			break;
 		}
 	}
 	else {
-		base = (x * x + y * y) / (re * re);
+		if constexpr (sizeof(T) == 4) base = (x * x + y * y) * T(c.inv_re2);
+		else base = (x * x + y * y) / (re * re);
 		switch (c.kind) {
 		case K_HALF: rf = base; break;
 		case K_ONE: rf = ::sqrt(base); break;
@@ -251,7 +256,7 @@ PA_HDI T sersic_at(const Comp &c, T x, T y)
 		case K_SIXTEEN: rf = ::sqrt(::sqrt(::sqrt(::sqrt(::sqrt(base))))); break;
 		default:
 			// base^(1/(2n)) = 2^(log2(base) / (2n)) in FP32
-			if constexpr (sizeof(T) == 4) rf = exp2_fast(log2_fast(base) * (T(0.5) / nser));
+			if constexpr (sizeof(T) == 4) rf = exp2_fast(log2_fast(base) * T(c.inv_2n));
 			else rf = ::pow(::sqrt(base), T(1) / nser);
 			break;
 		}
@@ -409,8 +414,9 @@ PA_HDI void sample_grad(const Comp &c, T x, T y, T *o)
 		// Without box the powers are squares, and f follows from the same
 		// logarithm as the derivatives: one log and two exps per sample
 		// (plus two logs when the box derivative is wanted).
-		const T re = T(c.s[0]), n = T(c.s[1]), bn = T(c.bn), B = T(c.box) + T(2);
-		const T u = ::fabs(x / re), v = ::fabs(y / re);
+		const T bn = T(c.bn), B = T(c.box) + T(2);
+		const T ire = T(c.inv_re), in = T(c.inv_n), inB = T(c.inv_nB);
+		const T u = ::fabs(x * ire), v = ::fabs(y * ire);
 		T pu, pv;
 		if (c.boxy) {
 			pu = ::pow(u, B);
@@ -428,25 +434,28 @@ PA_HDI void sample_grad(const Comp &c, T x, T y, T *o)
 		T lb, rf, f;
 		if constexpr (sizeof(T) == 4) {
 			const T l2 = log2_fast(base);
-			rf = exp2_fast(l2 / (n * B));
+			rf = exp2_fast(l2 * inB);
 			f = exp2_fast(-bn * (rf - T(1)) * T(kLog2e));
 			lb = l2 * T(kLn2);
 		}
 		else {
 			lb = ::log(base);
-			rf = ::exp(lb / (n * B));
+			rf = ::exp(lb * inB);
 			f = ::exp(-bn * (rf - T(1)));
 		}
 		o[0] = f;
-		const T k = -bn * f * rf / (n * B * base);  // df/dbase
-		o[1] = u > T(0) ? k * B * pu / u / re * (x > T(0) ? T(1) : T(-1)) : T(0);
-		o[2] = v > T(0) ? k * B * pv / v / re * (y > T(0) ? T(1) : T(-1)) : T(0);
+		const T k = -bn * f * rf * inB / base;  // df/dbase
+		// d base / d x_prof = B pu / u / re (2 u / re without box)
+		const T bx = c.boxy ? (u > T(0) ? B * pu / u : T(0)) : T(2) * u;
+		const T by = c.boxy ? (v > T(0) ? B * pv / v : T(0)) : T(2) * v;
+		o[1] = k * bx * ire * (x > T(0) ? T(1) : T(-1));
+		o[2] = k * by * ire * (y > T(0) ? T(1) : T(-1));
 		if (c.grad_box) {
 			const T dbase_dB = (u > T(0) ? pu * ::log(u) : T(0)) + (v > T(0) ? pv * ::log(v) : T(0));
-			o[3] = -bn * f * rf * (dbase_dB / (n * B * base) - lb / (n * B * B));
+			o[3] = -bn * f * rf * (dbase_dB * inB / base - lb * inB / B);
 		}
-		o[4] = bn * f * rf / (n * re);
-		o[5] = f * (-T(c.dbn) * (rf - T(1)) + bn * rf * lb / (n * n * B));
+		o[4] = bn * f * rf * in * ire;
+		o[5] = f * (-T(c.dbn) * (rf - T(1)) + bn * rf * lb * in * inB);
 		return;
 	}
 	case MOFFAT: {
@@ -642,6 +651,14 @@ PA_HD int setup_component(const ProfileParams &p, double magzero, double sx, dou
 	}
 
 	if (p.type == SERSIC || p.type == CORESERSIC) c->bn = sf::qgamma(0.5, 2 * (p.type == SERSIC ? s[1] : s[2]));
+	c->inv_re = c->inv_re2 = c->inv_n = c->inv_nB = c->inv_2n = 0;
+	if (p.type == SERSIC) {
+		c->inv_re = 1 / s[0];
+		c->inv_re2 = 1 / (s[0] * s[0]);
+		c->inv_n = 1 / s[1];
+		c->inv_nB = 1 / (s[1] * (p.box + 2));
+		c->inv_2n = 0.5 / s[1];
+	}
 	switch (p.type) {
 	case SERSIC: c->rscale = s[0]; break;
 	case MOFFAT: c->rscale = s[0] / (2 * ::sqrt(::pow(2.0, 1 / s[1]) - 1)); break;
@@ -743,6 +760,7 @@ PA_HD int setup_component(const ProfileParams &p, double magzero, double sx, dou
 	c->sin_ang = ::sin(angrad);
 	c->cos_over = c->cos_ang / p.axrat;
 	c->sin_over = c->sin_ang / p.axrat;
+	c->inv_axrat = 1 / p.axrat;
 
 	double flux_scale = sx * sy * ie;
 	if (p.type == SERSIC && p.rescale_flux) {

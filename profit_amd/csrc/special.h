@@ -51,10 +51,12 @@ PA_HD double digamma(double x)
 	return acc + ::log(x) - 0.5 / x - series;
 }
 
-PA_HD double log_kernel(double a, double x) { return a * ::log(x) - x - lgam(a); }
+// log(x^a e^-x / Gamma(a)); lga = lgam(a), computed once by the callers
+// that evaluate many x for one a
+PA_HD double log_kernel(double a, double x, double lga) { return a * ::log(x) - x - lga; }
 
 // regularised lower incomplete gamma by series
-PA_HD double p_series(double a, double x)
+PA_HD double p_series(double a, double x, double lga)
 {
 	double term = 1 / a, sum = term, an = a;
 	for (int k = 0; k < 2000; ++k) {
@@ -63,11 +65,11 @@ PA_HD double p_series(double a, double x)
 		sum += term;
 		if (::fabs(term) <= ::fabs(sum) * 5e-18) break;
 	}
-	return sum * ::exp(log_kernel(a, x));
+	return sum * ::exp(log_kernel(a, x, lga));
 }
 
 // regularised upper incomplete gamma by continued fraction
-PA_HD double q_cfrac(double a, double x)
+PA_HD double q_cfrac(double a, double x, double lga)
 {
 	const double tiny = 1e-300;
 	double b = x + 1 - a;
@@ -86,20 +88,21 @@ PA_HD double q_cfrac(double a, double x)
 		h *= delta;
 		if (::fabs(delta - 1) <= 5e-18) break;
 	}
-	return ::exp(log_kernel(a, x)) * h;
+	return ::exp(log_kernel(a, x, lga)) * h;
 }
 
-PA_HD double pgamma(double x, double a)
+PA_HD double pgamma(double x, double a, double lga)
 {
 	if (!(x > 0)) return 0;
 	if (is_inf(x)) return 1;
-	return x < a + 1 ? p_series(a, x) : 1 - q_cfrac(a, x);
+	return x < a + 1 ? p_series(a, x, lga) : 1 - q_cfrac(a, x, lga);
 }
+PA_HD double pgamma(double x, double a) { return pgamma(x, a, lgam(a)); }
 
-PA_HD double qgamma_upper(double x, double a)
+PA_HD double qgamma_upper(double x, double a, double lga)
 {
 	if (!(x > 0)) return 1;
-	return x < a + 1 ? 1 - p_series(a, x) : q_cfrac(a, x);
+	return x < a + 1 ? 1 - p_series(a, x, lga) : q_cfrac(a, x, lga);
 }
 
 // quantile of the unit-scale gamma distribution
@@ -123,13 +126,14 @@ PA_HD double qgamma(double p, double a)
 		if (!(x > 0)) x = 1e-300;
 	}
 	const bool upper = p >= 0.9;
+	const double lga = lgam(a);
 	double lo = 0, hi = INFINITY;
 	for (int it = 0; it < 100; ++it) {
-		const double f = upper ? (1 - p) - qgamma_upper(x, a) : pgamma(x, a) - p;
+		const double f = upper ? (1 - p) - qgamma_upper(x, a, lga) : pgamma(x, a, lga) - p;
 		if (f == 0) break;
 		if (f < 0) lo = x;
 		else hi = x;
-		const double dens = ::exp(log_kernel(a, x)) / x;
+		const double dens = ::exp(log_kernel(a, x, lga)) / x;
 		if (!(dens > 0)) break;
 		const double newton = f / dens;
 		const double curv = (a - 1) / x - 1;
@@ -144,17 +148,23 @@ PA_HD double qgamma(double p, double a)
 	return x;
 }
 
-// dP(a,x)/da from the term-wise derivative of P = e^-x sum x^(a+k)/Gamma(a+k+1)
+// dP(a,x)/da from the term-wise derivative of P = e^-x sum x^(a+k)/Gamma(a+k+1):
+// term k is x^(a+k) e^-x / Gamma(a+k+1) * (ln x - psi(a+k+1)), both factors
+// carried from k to k+1 by Gamma(z+1) = z Gamma(z) and psi(z+1) = psi(z) + 1/z
 PA_HD double dpgamma_da(double x, double a)
 {
 	if (!(x > 0)) return 0;
 	const double lx = ::log(x);
+	double e = ::exp(a * lx - x - lgam(a + 1));
+	double psi = digamma(a + 1);
 	double sum = 0;
 	for (int k = 0; k < 4000; ++k) {
-		const double ak = a + k + 1;
-		const double term = ::exp((a + k) * lx - x - lgam(ak)) * (lx - digamma(ak));
+		const double term = e * (lx - psi);
 		sum += term;
 		if (k > x && ::fabs(term) <= 1e-18 * ::fabs(sum)) break;
+		const double z = a + k + 1;
+		e *= x / z;
+		psi += 1 / z;
 	}
 	return sum;
 }
