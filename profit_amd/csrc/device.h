@@ -114,6 +114,9 @@ struct WorkQueue {
 	QEntry *light, *heavy;
 	int *count;
 	int lcap, hcap;
+	// per entry: 1 when the forward walk refined no cell of the root (the
+	// tree is the root alone), so the adjoint walk skips the refinement tests
+	uint8_t *lroot, *hroot;
 };
 // count[0], count[1]: lengths of the light and heavy lists; then a (heavy,
 // light) pair of hand-out counters per walker launch: the forward walk of

@@ -670,17 +670,20 @@ void Engine::ensure_queue(long need, long hneed)
 	if (need > qcap_) {
 		qcap_ = need;
 		d_queue_.alloc((size_t)qcap_ * sizeof(QEntry));
+		d_qroot_.alloc((size_t)qcap_);
 	}
 	if (hneed > hcap_) {
 		hcap_ = hneed;
 		d_hqueue_.alloc((size_t)hcap_ * sizeof(QEntry));
+		d_hroot_.alloc((size_t)hcap_);
 	}
 	invalidate_graphs(graph_exec_);
 }
 
 WorkQueue Engine::work_queue()
 {
-	return WorkQueue{d_queue_.as<QEntry>(), d_hqueue_.as<QEntry>(), d_qcount_.as<int>(), (int)qcap_, (int)hcap_};
+	return WorkQueue{d_queue_.as<QEntry>(), d_hqueue_.as<QEntry>(), d_qcount_.as<int>(), (int)qcap_, (int)hcap_,
+	                 d_qroot_.as<uint8_t>(), d_hroot_.as<uint8_t>()};
 }
 
 // whether the last evaluation's work lists fitted

@@ -108,7 +108,8 @@ __device__ __forceinline__ bool refine_cell(const Comp &c, double xp, double yp,
 // otherwise); `fp64`: forward leaf values in FP64.
 template <bool kGrad, int TYPE, int PREC>
 __device__ __forceinline__ void walk_pixel(const Comp &c, WalkLevel *L, int lane, double x0, double x1, double y0, double y1, int res,
-                           int maxrec, double scale, bool fp64, double &sum, double *gs)
+                           int maxrec, double scale, bool fp64, double &sum, double *gs, bool root_only,
+                           uint8_t *root_flag)
 {
 	const int ncell = res * res;
 	const int nchunk = (ncell + 63) >> 6;
@@ -120,7 +121,9 @@ __device__ __forceinline__ void walk_pixel(const Comp &c, WalkLevel *L, int lane
 		const double xbin = (nx1 - nx0) / res;
 		const double ybin = (ny1 - ny0) / res;
 		const double hx = xbin / 2., hy = ybin / 2.;
-		const bool recurse = res > 1 && level < maxrec;
+		// root_only (adjoint): the forward walk found no cell of the root to
+		// refine, so every cell is a leaf and no test is repeated
+		const bool recurse = res > 1 && level < maxrec && !root_only;
 		const double dyp = -xbin * c.sin_over + ybin * c.cos_over;
 		unsigned long long m[4] = {0ull, 0ull, 0ull, 0ull};
 #pragma unroll
@@ -135,7 +138,8 @@ __device__ __forceinline__ void walk_pixel(const Comp &c, WalkLevel *L, int lane
 				const double xr = x - c.xc, yr = y - c.yc;
 				const double xp = xr * c.cos_ang + yr * c.sin_ang;
 				const double yp = -xr * c.sin_over + yr * c.cos_over;
-				const float v32 = eval_at<float, TYPE>(c, (float)xp, (float)yp);
+				// the adjoint needs the FP32 value only for the refinement test
+				const float v32 = (kGrad && !recurse) ? 0.f : eval_at<float, TYPE>(c, (float)xp, (float)yp);
 				double v64 = 0;
 				bool have64 = false;
 				fail = refine_cell<TYPE>(c, xp, yp, dyp, recurse, v32, &v64, &have64);
@@ -156,6 +160,7 @@ __device__ __forceinline__ void walk_pixel(const Comp &c, WalkLevel *L, int lane
 			}
 			m[k] = __ballot(fail);
 		}
+		if (!kGrad && level == 0 && lane == 0) *root_flag = (m[0] | m[1] | m[2] | m[3]) == 0ull;
 		if (lane == 0) {
 			L[level].x0 = nx0;
 			L[level].x1 = nx1;
@@ -236,6 +241,7 @@ __global__ __launch_bounds__(256) void k_subsample(const GalaxyDev *__restrict__
 	for (;;) {
 		int base = 0, take = 0;
 		const QEntry *list;
+		uint8_t *roots;
 		{
 			int h = nh;
 			if (heavy_left) {
@@ -246,6 +252,7 @@ __global__ __launch_bounds__(256) void k_subsample(const GalaxyDev *__restrict__
 				base = h;
 				take = 1;
 				list = wq.heavy;
+				roots = wq.hroot;
 			}
 			else {
 				heavy_left = false;
@@ -256,6 +263,7 @@ __global__ __launch_bounds__(256) void k_subsample(const GalaxyDev *__restrict__
 				base = l;
 				take = min(kGrab, nl - l);
 				list = wq.light;
+				roots = wq.lroot;
 			}
 		}
 		for (int e = base; e < base + take; ++e) {
@@ -295,7 +303,7 @@ __global__ __launch_bounds__(256) void k_subsample(const GalaxyDev *__restrict__
 			}
 			double s_local = 0;
 			walk_pixel<kGrad, TYPE, PREC>(c, L, lane, x - hx, x + hx, y - hy, y + hy, res, maxrec, scale,
-			                              g.render_fp64 != 0, s_local, gs);
+			                              g.render_fp64 != 0, s_local, gs, kGrad && roots[e] != 0, roots + e);
 			if (kGrad) {
 				vsum += s_local;
 			}
