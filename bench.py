@@ -241,9 +241,9 @@ def main():
     }
     if conv_used == "fft":
         fft_flops = B * 3 * 5.0 * n_fft * np.log2(n_fft)
-        kernels["conv_like"] = ("FFT convolution + likelihood (k_fft_rows_fwd, k_fft_cols, k_fft_rows_inv)", "mfma",
-                                fft_flops, PEAK_FP64_TFLOPS, "TFLOP/s")
-        kernels["adjoint_conv"] = ("FFT adjoint correlation (k_fft_rows_fwd, k_fft_cols, k_fft_rows_inv)", "mfma",
+        kernels["conv_like"] = ("FFT convolution + likelihood (k_fft_rows_fwd, k_fft_cols, k_fft_rows_inv<0>)",
+                                "mfma", fft_flops, PEAK_FP64_TFLOPS, "TFLOP/s")
+        kernels["adjoint_conv"] = ("FFT adjoint correlation (k_fft_rows_fwd, k_fft_cols, k_fft_rows_inv<2>)", "mfma",
                                    fft_flops, PEAK_FP64_TFLOPS, "TFLOP/s")
     else:
         peak = PEAK_BF16_TFLOPS if conv_used == "brute-bf16x3" else PEAK_FP32_TFLOPS
@@ -266,7 +266,9 @@ def main():
             traffic = tk[key]["hbm_bytes_per_launch"]
         elif key.startswith("FFT"):
             # the stage's three kernels: rows forward, columns, rows inverse
-            parts = [k for k in ("k_fft_rows_fwd", "k_fft_cols", "k_fft_rows_inv") if k in tk]
+            # (likelihood epilogue k_fft_rows_inv<0>, adjoint image <2>)
+            inv = "k_fft_rows_inv<0>" if dom == "conv_like" else "k_fft_rows_inv<2>"
+            parts = [k for k in ("k_fft_rows_fwd", "k_fft_cols", inv) if k in tk]
             traffic = sum(tk[k]["hbm_bytes_per_launch"] for k in parts) if len(parts) == 3 else None
     kms = stage_ms[dom]
     achieved = work / (kms * 1e-3) / (1e9 if unit == "GB/s" else 1e12)

@@ -402,13 +402,16 @@ __global__ __launch_bounds__(kColThreads, 4) void k_fft_cols(const GalaxyDev *__
 		double2 *col = sh + c * S;
 		double2 x[8], sp[8];
 #pragma unroll
-		for (int j = 0; j < 8; ++j) {
-			sp[j] = spec[ok ? (long)(lo + (j << lg)) * P + c0 + c : 0];
-			x[j] = col[swz(lo + (j << lg))];
-		}
+		for (int j = 0; j < 8; ++j) x[j] = col[swz(lo + (j << lg))];
 		dit_regs<3>(x, lg, lo, wt);
+		// spectrum entries four at a time (the registers of four waves per SIMD)
 #pragma unroll
-		for (int j = 0; j < 8; ++j) x[j] = (op == 1) ? cmul(x[j], sp[j]) : cmulc(x[j], sp[j]);
+		for (int h = 0; h < 8; h += 4) {
+#pragma unroll
+			for (int j = h; j < h + 4; ++j) sp[j] = spec[ok ? (long)(lo + (j << lg)) * P + c0 + c : 0];
+#pragma unroll
+			for (int j = h; j < h + 4; ++j) x[j] = (op == 1) ? cmul(x[j], sp[j]) : cmulc(x[j], sp[j]);
+		}
 		dif_regs<3>(x, lg, lo, wt);
 #pragma unroll
 		for (int j = 0; j < 8; ++j) col[swz(lo + (j << lg))] = x[j];
