@@ -266,8 +266,25 @@ __global__ __launch_bounds__(256) void k_subsample(const GalaxyDev *__restrict__
 				roots = wq.lroot;
 			}
 		}
+		// the grab's entries (and in the adjoint their G(q) and root flags)
+		// fetched at once, lane u holding entry base + u, so the per-entry
+		// chain of dependent loads is paid once per grab
+		int qc = 0, qp = 0, qroot = 0;
+		float qg = 0.f;
+		if (lane < take) {
+			const QEntry qe = list[base + lane];
+			qc = qe.comp;
+			qp = qe.pix;
+			if (kGrad) {
+				qg = insts[comps[qc].instance].G[qp];
+				qroot = roots[base + lane];
+			}
+		}
 		for (int e = base; e < base + take; ++e) {
-			const QEntry q = list[e];
+			const int u = e - base;
+			QEntry q;
+			q.comp = __builtin_amdgcn_readlane(qc, u);
+			q.pix = __builtin_amdgcn_readlane(qp, u);
 			const Comp &c = comps[q.comp];
 			if ((TYPE >= 0 && c.type != TYPE) || (kGrad && PREC != 2 && c.grad64 != PREC)) continue;
 			if (kGrad && q.comp != cur) {
@@ -288,7 +305,7 @@ __global__ __launch_bounds__(256) void k_subsample(const GalaxyDev *__restrict__
 			const GalaxyDev &g = gals[I.galaxy];
 			double scale = 1.0;
 			if (kGrad) {
-				const float Gq = I.G[q.pix];
+				const float Gq = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, qg), u));
 				if (Gq == 0.f) continue;
 				scale = (double)Gq * c.flux_scale;
 			}
@@ -303,7 +320,8 @@ __global__ __launch_bounds__(256) void k_subsample(const GalaxyDev *__restrict__
 			}
 			double s_local = 0;
 			walk_pixel<kGrad, TYPE, PREC>(c, L, lane, x - hx, x + hx, y - hy, y + hy, res, maxrec, scale,
-			                              g.render_fp64 != 0, s_local, gs, kGrad && roots[e] != 0, roots + e);
+			                              g.render_fp64 != 0, s_local, gs,
+			                              kGrad && __builtin_amdgcn_readlane(qroot, u) != 0, roots + e);
 			if (kGrad) {
 				vsum += s_local;
 			}
