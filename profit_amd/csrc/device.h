@@ -124,7 +124,7 @@ struct WorkQueue {
 constexpr int kQueueCounters = 16;
 __host__ __device__ constexpr int fwd_counter(int type) { return 2 + 2 * type; }
 constexpr int kAdjCounter = 14;
-constexpr int kGrab = 8;  // ordinary entries handed to a wavefront at a time
+constexpr int kGrab = 16;  // ordinary entries handed to a wavefront at a time
 
 
 }  // namespace profit_amd
