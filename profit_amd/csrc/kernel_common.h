@@ -370,13 +370,14 @@ __device__ __forceinline__ float pixel_accumulate_f32(const Comp &c, float xp, f
 
 // Adjoint of the directly evaluated pixels: sum_q G(q) fs d f(q) / d slot over
 // the pixels the forward render marked direct (so no classification is
-// repeated), per-thread sums in FP32 (FP64 for steep cores), the flux-scale
-// term applied once per component from sum_q G(q) fs f(q).  A block takes
-// `grp` consecutive canvas tiles of one instance (block offsets per instance
-// in grp_off), so that the per-component reduction and its atomics are paid
-// once per grp tiles.  Compiled per profile family TYPE and derivative
-// precision G64 (steep cores); components of other kinds are left to the
-// launch made for them.
+// repeated), FP32 per-pixel derivatives, per-thread sums in FP32 (in FP64 for
+// steep cores, G64: their centroid and shape terms cancel over many pixels,
+// which FP32 sums do not survive), the flux-scale term applied once per
+// component from sum_q G(q) fs f(q).  A block takes `grp` consecutive canvas
+// tiles of one instance (block offsets per instance in grp_off), so that the
+// per-component reduction and its atomics are paid once per grp tiles.
+// Compiled per profile family TYPE and G64; components of other kinds are
+// left to the launch made for them.
 template <int TYPE, bool G64>
 __global__ __launch_bounds__(256) void k_adj_render(const GalaxyDev *__restrict__ gals,
                                                     const InstDev *__restrict__ insts,
@@ -418,7 +419,7 @@ __global__ __launch_bounds__(256) void k_adj_render(const GalaxyDev *__restrict_
 					const double xp = xr * c.cos_ang + yr * c.sin_ang;
 					const double yp = -xr * c.sin_over + yr * c.cos_over;
 					const double wq = (double)Gq * c.flux_scale;
-					vsum += wq * sample_accumulate<double, TYPE>(c, xp, yp, wq, gl);
+					vsum += wq * sample_accumulate<float, TYPE>(c, xp, yp, wq, gl);
 				}
 			}
 		}

@@ -2,7 +2,7 @@
 // (k_subsample<true>, one kernel for every family, which measured faster on
 // the 400x400 bulge+disk batch than one kernel per family: 3.81 against
 // 4.16 ms) and the adjoint of the directly evaluated pixels (k_adj_render,
-// one kernel per radial family and derivative precision, FP64 only for the
+// one kernel per radial family and sum precision, FP64 sums only for the
 // steep-core families Sersic and core-Sersic); see kernel_common.h.
 #include "kernel_common.h"
 
