@@ -102,10 +102,13 @@ __device__ __forceinline__ bool refine_cell(const Comp &c, double xp, double yp,
 
 // Walks the recursion tree of one pixel of a TYPE profile (-1: any family,
 // dispatched at run time).  Every lane returns its share of sum_s weight_s *
-// f(s) or, when kGrad, of the derivative sums (per-sample derivatives in FP64
-// when PREC is 1, FP32 when 0, the component's choice when 2);
+// f(s) or, when kGrad, of the derivative sums (FP32 per-sample derivatives,
+// FP64 sums: steep cores keep the gradient margins of FP64 derivatives, and
+// the walker then holds 2.0 instead of 2.3 ms on the bulge+disk batch);
 // `scale` multiplies the weights (G(q) * flux_scale in the adjoint, 1
-// otherwise); `fp64`: forward leaf values in FP64.
+// otherwise); `fp64`: forward leaf values in FP64.  PREC selects, in
+// k_subsample, the components a launch takes by their steep-core flag (2:
+// all).
 template <bool kGrad, int TYPE, int PREC>
 __device__ __forceinline__ void walk_pixel(const Comp &c, WalkLevel *L, int lane, double x0, double x1, double y0, double y1, int res,
                            int maxrec, double scale, bool fp64, double &sum, double *gs, bool root_only,
@@ -146,9 +149,7 @@ __device__ __forceinline__ void walk_pixel(const Comp &c, WalkLevel *L, int lane
 				if (!fail) {
 					if (kGrad) {
 						// value (for the flux-scale derivatives) and derivatives
-						const bool p64 = PREC == 2 ? c.grad64 != 0 : PREC == 1;
-						const double f = p64 ? sample_accumulate<double, TYPE>(c, xp, yp, weight * scale, gs)
-						                     : sample_accumulate<float, TYPE>(c, xp, yp, weight * scale, gs);
+						const double f = sample_accumulate<float, TYPE>(c, xp, yp, weight * scale, gs);
 						sum += f * (weight * scale);
 					}
 					else {
