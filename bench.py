@@ -236,7 +236,7 @@ def main():
                       "GB/s"),
         "adjoint_render": ("k_adj_render<0, *> (adjoint of direct pixels, Sersic)", "hbm", render_bytes,
                            PEAK_HBM_GBS, "GB/s"),
-        "adjoint_subsample": ("k_subsample<true, -1, 2> (adjoint recursion walker)", "hbm", render_bytes,
+        "adjoint_subsample": ("k_subsample<true, 0, 2> (adjoint recursion walker, Sersic)", "hbm", render_bytes,
                               PEAK_HBM_GBS, "GB/s"),
     }
     if conv_used == "fft":

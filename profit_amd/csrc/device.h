@@ -120,10 +120,10 @@ struct WorkQueue {
 };
 // count[0], count[1]: lengths of the light and heavy lists; then a (heavy,
 // light) pair of hand-out counters per walker launch: the forward walk of
-// each profile family, then the adjoint walk
-constexpr int kQueueCounters = 16;
+// each profile family, then the adjoint walk of each family
+constexpr int kQueueCounters = 26;
 __host__ __device__ constexpr int fwd_counter(int type) { return 2 + 2 * type; }
-constexpr int kAdjCounter = 14;
+__host__ __device__ constexpr int adj_counter(int type) { return 14 + 2 * type; }
 constexpr int kGrab = 16;  // ordinary entries handed to a wavefront at a time
 
 

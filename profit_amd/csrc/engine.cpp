@@ -911,10 +911,11 @@ void Engine::record(bool grad)
 			hip_check(launch_adj_nonradial(gals, insts, n, d_comps_.as<Comp>(), d_grad_acc_.as<double>(), st),
 			          "adjoint point sources and sky");
 		mark(7);
-		if (radial_types_)
-			hip_check(launch_subsample_adj(subsample_blocks_, gals, insts, d_comps_.as<Comp>(), work_queue(),
-			                               d_grad_acc_.as<double>(), st),
-			          "adjoint subsample");
+		for (int t = 0; t < 6; ++t)
+			if ((radial_types_ >> t) & 1u)
+				hip_check(launch_subsample_adj(t, subsample_blocks_, gals, insts, d_comps_.as<Comp>(), work_queue(),
+				                               d_grad_acc_.as<double>(), st),
+				          "adjoint subsample");
 		mark(8);
 		hip_check(launch_grad_finalize(d_comps_.as<Comp>(), d_gscale_.as<double>(), d_grad_acc_.as<double>(),
 		                               d_grad_out_.as<double>(), nc, grad_slots_, st),

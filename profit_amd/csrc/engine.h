@@ -25,8 +25,8 @@ int launch_render(const GalaxyDev *gals, const InstDev *insts, const int *tile_o
 // only); the adjoint walker, one launch for all
 int launch_subsample_fwd(int type, int nblocks, const GalaxyDev *gals, const InstDev *insts, const Comp *comps,
                          WorkQueue wq, int split, hipStream_t st);
-int launch_subsample_adj(int nblocks, const GalaxyDev *gals, const InstDev *insts, const Comp *comps, WorkQueue wq,
-                         double *grad_acc, hipStream_t st);
+int launch_subsample_adj(int type, int nblocks, const GalaxyDev *gals, const InstDev *insts, const Comp *comps,
+                         WorkQueue wq, double *grad_acc, hipStream_t st);
 int launch_subsample_nodes(int nblocks, const GalaxyDev *gals, const InstDev *insts, const Comp *comps,
                            WorkQueue wq, int *nodes, hipStream_t st);
 int launch_conv(int mode, int rows, int nblocks, int kx_max, int kc, const GalaxyDev *gals, const InstDev *insts,
