@@ -231,7 +231,8 @@ def main():
     n_fft = n1 * n2
     render_bytes = B * dw * dh * 5.0
     kernels = {
-        "render": ("k_render (classification + direct pixels)", "hbm", render_bytes, PEAK_HBM_GBS, "GB/s"),
+        "render": ("k_render<0> (classification + direct pixels, Sersic)", "hbm", render_bytes, PEAK_HBM_GBS,
+                   "GB/s"),
         "subsample": ("k_subsample<false, 0, 0> (recursion walker, Sersic)", "hbm", render_bytes, PEAK_HBM_GBS,
                       "GB/s"),
         "adjoint_render": ("k_adj_render<0, *> (adjoint of direct pixels, Sersic)", "hbm", render_bytes,

@@ -19,7 +19,7 @@ int launch_setup(const ProfileParams *params, const int *comp_inst, const InstDe
 int launch_zero_d(double *p, long n, hipStream_t st);
 int launch_zero_i(int *p, long n, hipStream_t st);
 int launch_render(const GalaxyDev *gals, const InstDev *insts, const int *tile_off, int ninst, int nblocks,
-                  const Comp *comps, WorkQueue wq, int filter, int target, int *diag_cls, hipStream_t st);
+                  const Comp *comps, WorkQueue wq, int filter, int target, int *diag_cls, hipStream_t st, int type = -1);
 // the forward walker and the adjoint render, one launch per radial family
 // (and per derivative precision in the adjoint render: 1 = FP64, steep cores
 // only); the adjoint walker, one launch for all

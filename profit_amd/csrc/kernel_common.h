@@ -217,10 +217,12 @@ __device__ __forceinline__ void walk_pixel(const Comp &c, WalkLevel *L, int lane
 // value)/d slot; per-lane sums are carried across consecutive entries of the
 // same component and flushed with one set of atomics per run.  `counter`:
 // the pair of hand-out counters of this launch (heavy, light).  Held to the
-// registers of three waves per SIMD (the Sersic adjoint needs 172 unbounded
-// and spills 4 at 168; three waves took it from 1.82 to 1.43 ms).
+// registers of three waves per SIMD in the adjoint (the Sersic one needs 172
+// unbounded and spills 4 at 168; three waves took it from 1.82 to 1.43 ms)
+// and four in the forward walk (Sersic 144 -> 128 with 13 spills, 1.29 ->
+// 1.21 ms).
 template <bool kGrad, int TYPE, int PREC>
-__global__ __launch_bounds__(256, 3) void k_subsample(const GalaxyDev *__restrict__ gals,
+__global__ __launch_bounds__(256, kGrad ? 3 : 4) void k_subsample(const GalaxyDev *__restrict__ gals,
                                                    const InstDev *__restrict__ insts,
                                                    const Comp *__restrict__ comps, WorkQueue wq, int counter,
                                                    int split, double *grad_acc)

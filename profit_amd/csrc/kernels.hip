@@ -169,6 +169,7 @@ __device__ double point_value(const GalaxyDev &g, const PointFoot &f, int px, in
 // -1, point sources and sky drawn without convolution go to the aux canvas
 // when the instance has one in use (InstDev::has_aux); radial profiles are
 // always convolved there.
+template <int TYPE>
 __global__ __launch_bounds__(256) void k_render(const GalaxyDev *__restrict__ gals, const InstDev *__restrict__ insts,
                                                 const int *__restrict__ tile_off, int ninst,
                                                 const Comp *__restrict__ comps, WorkQueue wq, int filter, int target,
@@ -197,7 +198,7 @@ __global__ __launch_bounds__(256) void k_render(const GalaxyDev *__restrict__ ga
 		for (int k = 0; k < I.ncomp; ++k) {
 			const Comp &c = comps[I.comp0 + k];
 			if (filter >= 0 && c.convolve != filter) continue;
-			if (!is_radial(c.type)) {
+			if (TYPE < 0 && !is_radial(c.type)) {
 				// sky.cpp:47-63 (calc pixels only) and psf.cpp:58-145 (mask ignored)
 				float v = 0.f;
 				if (c.type == SKY) {
@@ -222,8 +223,8 @@ __global__ __launch_bounds__(256) void k_render(const GalaxyDev *__restrict__ ga
 				const PixelClass pc = classify(c, x, y);
 				cls = pc.cls;
 				if (pc.cls == 2) {
-					if (g.render_fp64) acc += (float)(c.flux_scale * eval_at<double>(c, pc.xp, pc.yp));
-					else acc += (float)c.flux_scale * eval_at<float>(c, (float)pc.xp, (float)pc.yp);
+					if (g.render_fp64) acc += (float)(c.flux_scale * eval_at<double, TYPE>(c, pc.xp, pc.yp));
+					else acc += (float)c.flux_scale * eval_at<float, TYPE>(c, (float)pc.xp, (float)pc.yp);
 				}
 				else if (pc.cls == 3) {
 					// the deep centre-pixel trees (sersic.cpp:331-347) go to their own list
@@ -772,10 +773,15 @@ int launch_zero_i(int *p, long n, hipStream_t st)
 }
 
 int launch_render(const GalaxyDev *gals, const InstDev *insts, const int *tile_off, int ninst, int nblocks,
-                  const Comp *comps, WorkQueue wq, int filter, int target, int *diag_cls, hipStream_t st)
+                  const Comp *comps, WorkQueue wq, int filter, int target, int *diag_cls, hipStream_t st, int type)
 {
 	if (nblocks <= 0) return 0;
-	k_render<<<nblocks, 256, 0, st>>>(gals, insts, tile_off, ninst, comps, wq, filter, target, diag_cls);
+	// type >= 0: every component of the batch is of that radial family (the
+	// kernel then holds that family's code alone); -1 any mix
+	if (type == SERSIC)
+		k_render<SERSIC><<<nblocks, 256, 0, st>>>(gals, insts, tile_off, ninst, comps, wq, filter, target, diag_cls);
+	else
+		k_render<-1><<<nblocks, 256, 0, st>>>(gals, insts, tile_off, ninst, comps, wq, filter, target, diag_cls);
 	return (int)hipGetLastError();
 }
 
