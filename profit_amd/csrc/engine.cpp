@@ -885,8 +885,10 @@ void Engine::record(bool grad)
 	hip_check(launch_zero_i(d_qcount_.as<int>(), kQueueCounters, st), "zero");
 	if (grad) hip_check(launch_zero_d(d_grad_acc_.as<double>(), (long)nc * NGRAD, st), "zero");
 	mark(1);
-	// a batch of Sersic components alone takes the render compiled for them
-	const int rtype = (!any_nonradial_ && radial_types_ == (1u << SERSIC)) ? SERSIC : -1;
+	// a batch of one radial family alone takes the render compiled for it
+	int rtype = -1;
+	for (int t = 0; t < 6; ++t)
+		if (!any_nonradial_ && radial_types_ == (1u << t)) rtype = t;
 	hip_check(launch_render(gals, insts, d_at_off_.as<int>(), n, n_at_blocks_, d_comps_.as<Comp>(),
 	                        work_queue(), -1, 0, nullptr, st, rtype),
 	          "render");

@@ -778,10 +778,17 @@ int launch_render(const GalaxyDev *gals, const InstDev *insts, const int *tile_o
 	if (nblocks <= 0) return 0;
 	// type >= 0: every component of the batch is of that radial family (the
 	// kernel then holds that family's code alone); -1 any mix
-	if (type == SERSIC)
-		k_render<SERSIC><<<nblocks, 256, 0, st>>>(gals, insts, tile_off, ninst, comps, wq, filter, target, diag_cls);
-	else
-		k_render<-1><<<nblocks, 256, 0, st>>>(gals, insts, tile_off, ninst, comps, wq, filter, target, diag_cls);
+#define PA_RENDER(T) k_render<T><<<nblocks, 256, 0, st>>>(gals, insts, tile_off, ninst, comps, wq, filter, target, diag_cls)
+	switch (type) {
+	case SERSIC: PA_RENDER(SERSIC); break;
+	case MOFFAT: PA_RENDER(MOFFAT); break;
+	case FERRER: PA_RENDER(FERRER); break;
+	case KING: PA_RENDER(KING); break;
+	case CORESERSIC: PA_RENDER(CORESERSIC); break;
+	case BROKENEXP: PA_RENDER(BROKENEXP); break;
+	default: PA_RENDER(-1); break;
+	}
+#undef PA_RENDER
 	return (int)hipGetLastError();
 }
 
