@@ -2,10 +2,11 @@
 // wave reductions, libprofit's cell positions and refinement test, and the
 // templates of the recursion walker and of the adjoint render.
 // walkers_fwd.hip compiles the forward walker once per profile family and
-// walkers_adj.hip the adjoint render once per family and derivative
-// precision: a kernel's register allocation is the maximum over the code it
-// holds, so one family per kernel lifts the forward walker from two to three
-// waves per SIMD and the adjoint render from two to seven or eight.
+// walkers_adj.hip the adjoint walker once per family and the adjoint render
+// once per family and sum precision: a kernel's register allocation is the
+// maximum over the code it holds, so one family per kernel lifts the walkers
+// from two to three or four waves per SIMD and the adjoint render from two
+// to seven or eight.
 #pragma once
 
 #include <hip/hip_runtime.h>
