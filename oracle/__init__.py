@@ -29,7 +29,7 @@ SHAPE_NAMES = {
     POINT: (),
     SKY: ("bg",),
 }
-CONV = {"brute": 0, "brute-old": 1, "fft": 2}
+CONV = {"brute": 0, "brute-old": 1, "fft": 2, "null": 3}
 LIKE = {"norm": 0, "chisq": 1, "t": 2, "pois": 3}
 NGRAD = 11
 GRAD_SLOTS = ("xcen", "ycen", "mag", "ang", "axrat", "box", "s0", "s1", "s2", "s3", "s4")

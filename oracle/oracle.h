@@ -80,7 +80,8 @@ typedef struct or_profile {
 
 void or_profile_defaults(int type, or_profile *p);
 
-enum { OR_CONV_BRUTE = 0, OR_CONV_BRUTE_OLD = 1, OR_CONV_FFT = 2 };
+/* OR_CONV_NULL: NullConvolver (convolve.cpp:521-524), the source unchanged */
+enum { OR_CONV_BRUTE = 0, OR_CONV_BRUTE_OLD = 1, OR_CONV_FFT = 2, OR_CONV_NULL = 3 };
 
 typedef struct or_model {
 	unsigned width, height;

@@ -1117,6 +1117,11 @@ static double *convolve_impl(int convolver, const double *src, unsigned sw, unsi
 	double *out = calloc((size_t)sw * sh, sizeof(double));
 	*offx = *offy = 0;
 	*rw = sw; *rh = sh;
+	if (convolver == OR_CONV_NULL) {
+		/* NullConvolver::convolve_impl (convolve.cpp:521-524) */
+		memcpy(out, src, sizeof(double) * (size_t)sw * sh);
+		return out;
+	}
 	if (convolver == OR_CONV_BRUTE_OLD) {
 		conv_brute_old(src, sw, sh, krn, kw, kh, mask, out, threads);
 		return out;
