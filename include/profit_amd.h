@@ -109,6 +109,21 @@ int profit_amd_make_model(const profit_amd_model_desc *model, const profit_amd_p
 int profit_amd_drawing_dims(const profit_amd_model_desc *model, const profit_amd_profile *profiles,
                             int32_t nprofiles, uint32_t *width, uint32_t *height);
 
+/* The dimensions of the image profit_amd_make_model returns for these inputs:
+ * the requested ones, or larger for an uncropped FFT frame (crop = 0), or the
+ * drawing dimensions for a dry run.  No counterpart in the reference, whose
+ * Model::evaluate (model.cpp:271-338) allocates its result; it lets a caller
+ * size `out` (R_profit_make_model reuses previous_image when the dimensions
+ * match, src/r_profit.cpp:662-670). */
+int profit_amd_output_dims(const profit_amd_model_desc *model, const profit_amd_profile *profiles, int32_t nprofiles,
+                           uint32_t *width, uint32_t *height);
+
+/* Releases the device set-ups (buffers, tile lists, FFT spectra) that
+ * repeated profit_amd_make_model calls with the same image geometry, PSF and
+ * calc region reuse.  Replaces R_profit_clear_cache (src/r_profit.cpp:822-825),
+ * which clears libprofit's OpenCL kernel cache. */
+int profit_amd_clear_cache(void);
+
 /* Model::adjust(mask, dims, psf, finesampling) (model.cpp:248-257); replaces
  * R_profit_adjust_mask (src/r_profit.cpp:757-772). */
 int profit_amd_adjust_mask(const uint8_t *mask, uint32_t mask_w, uint32_t mask_h, uint32_t img_w, uint32_t img_h,
@@ -167,13 +182,18 @@ int profit_amd_engine_set_batch(profit_amd_engine *engine, const int32_t *inst_g
 /* Tile height of the convolution / rendering tiles: 4*rows (rows = 2, 4, 8, 16). */
 int profit_amd_engine_set_tile_rows(profit_amd_engine *engine, int32_t rows);
 
-/* Precision of the forward model's profile values: 0 = FP32 (default), 1 =
- * FP64, for the directly evaluated pixels and the leaf samples of the
- * sub-pixel recursion alike.  Either way: classification and cell positions
- * are FP64, refinement decisions are FP32 with an FP64 re-test inside a guard
- * band (so they equal the FP64 decisions), the convolution is FP32 with FP64
- * accumulation across PSF columns, likelihood sums are FP64 and the adjoint's
- * per-sample derivatives are FP32 (FP64 for steep cores). */
+/* Precision of the forward model: 0 = FP32 profile values (default), 1 =
+ * FP64 profile values, for the directly evaluated pixels and the leaf samples
+ * of the sub-pixel recursion alike; 2 = FP64 profile values summed into an
+ * FP64 canvas that the FP64 FFT convolves (every galaxy with a PSF), so the
+ * model image is FP64 end to end: the mode for bright sources whose logL the
+ * FP32 canvas moves by more than 1e-3.  In every mode: classification is FP64,
+ * refinement decisions are FP32 with an FP64 re-test at libprofit's own cell
+ * position inside a guard band (so they equal the FP64 decisions), the brute
+ * convolution (modes 0, 1) is FP32 with FP64 accumulation across PSF columns,
+ * likelihood sums are FP64 and the adjoint's
+ * per-sample derivatives are FP32, summed in FP64 in the recursion walk and,
+ * for steep cores, in the direct pixels. */
 int profit_amd_engine_set_render_fp64(profit_amd_engine *engine, int32_t on);
 
 /* Gradient slots profit_amd_engine_like_grad computes: bit s of `mask` asks
@@ -227,6 +247,15 @@ int profit_amd_engine_model_images(profit_amd_engine *engine, const profit_amd_p
 int profit_amd_engine_diagnostics(profit_amd_engine *engine, const profit_amd_profile *params, int32_t *cls,
                                   int32_t *nodes);
 
+/* The sub-pixel settings each component of the batch is drawn with, after
+ * the adjustment RadialProfile::initial_calculations makes when adjust is on
+ * (radial.cpp:158-185, which writes them back into the profile's acc,
+ * rscale_switch, resolution and rscale_max): one value per component in
+ * each array (nullable); 0 for point sources, sky and null profiles. */
+int profit_amd_engine_adjusted_settings(profit_amd_engine *engine, const profit_amd_profile *params, int32_t *resolution,
+                                        int32_t *max_recursions, double *acc, double *rscale_switch,
+                                        double *rscale_max);
+
 /* Drawing-canvas dimensions and PSF padding of a registered galaxy. */
 int profit_amd_engine_galaxy_geometry(profit_amd_engine *engine, int32_t galaxy_id, uint32_t *draw_w,
                                       uint32_t *draw_h, uint32_t *pad_x, uint32_t *pad_y);
@@ -244,6 +273,12 @@ int profit_amd_engine_set_timing(profit_amd_engine *engine, int32_t on);
  * the batch (2 * psf_w * psf_h * output pixels summed over instances). */
 int profit_amd_engine_stats(profit_amd_engine *engine, double *stage_ms, int64_t *stage_calls, int64_t *evaluations,
                             int32_t *queue_length, double *conv_flops, double *adjoint_flops);
+/* The FFT stages kernel by kernel (HIP events between the kernels while
+ * timing is on): accumulated milliseconds of the forward rows, columns and
+ * inverse rows of the likelihood stage (kernel_ms[0..2]) and of the adjoint
+ * stage (kernel_ms[3..5]), and the number of evaluations each was timed over
+ * (calls[0..1]).  No counterpart in the reference (measurement). */
+int profit_amd_engine_fft_stats(profit_amd_engine *engine, double *kernel_ms, int64_t *calls);
 int profit_amd_engine_reset_stats(profit_amd_engine *engine);
 
 #ifdef __cplusplus

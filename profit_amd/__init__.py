@@ -85,6 +85,9 @@ SYMBOLS = [
                                         _P(C.c_uint32), _P(C.c_uint32), _P(C.c_int32), _P(C.c_int32)]),
     ("profit_amd_drawing_dims", C.c_int, [_P(ModelDesc), _P(Profile), C.c_int32, _P(C.c_uint32),
                                           _P(C.c_uint32)]),
+    ("profit_amd_output_dims", C.c_int, [_P(ModelDesc), _P(Profile), C.c_int32, _P(C.c_uint32),
+                                         _P(C.c_uint32)]),
+    ("profit_amd_clear_cache", C.c_int, []),
     ("profit_amd_adjust_mask", C.c_int, [_P(C.c_uint8), C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                          _P(C.c_double), C.c_uint32, C.c_uint32, C.c_uint32,
                                          _P(C.c_uint8), C.c_size_t, _P(C.c_uint32), _P(C.c_uint32)]),
@@ -114,6 +117,9 @@ SYMBOLS = [
     ("profit_amd_engine_set_timing", C.c_int, [C.c_void_p, C.c_int32]),
     ("profit_amd_engine_stats", C.c_int, [C.c_void_p, _P(C.c_double), _P(C.c_int64), _P(C.c_int64),
                                           _P(C.c_int32), _P(C.c_double), _P(C.c_double)]),
+    ("profit_amd_engine_adjusted_settings", C.c_int, [C.c_void_p, _P(Profile), _P(C.c_int32), _P(C.c_int32),
+                                                      _P(C.c_double), _P(C.c_double), _P(C.c_double)]),
+    ("profit_amd_engine_fft_stats", C.c_int, [C.c_void_p, _P(C.c_double), _P(C.c_int64)]),
     ("profit_amd_engine_reset_stats", C.c_int, [C.c_void_p]),
 ]
 STAGES = ("setup", "render", "subsample", "conv_like", "like_finalize", "adjoint_conv", "adjoint_render",
@@ -220,7 +226,7 @@ def make_model(width, height, profiles, psf=None, mask=None, magzero=0.0, scale=
     arr = profile_array(profiles)
     L = lib()
     dw, dh = C.c_uint32(), C.c_uint32()
-    _check(L.profit_amd_drawing_dims(C.byref(d), arr, len(profiles), C.byref(dw), C.byref(dh)))
+    _check(L.profit_amd_output_dims(C.byref(d), arr, len(profiles), C.byref(dw), C.byref(dh)))
     cap = max(1, dw.value * dh.value)
     out = np.zeros(cap)
     ow, oh, ox, oy = C.c_uint32(), C.c_uint32(), C.c_int32(), C.c_int32()
@@ -228,6 +234,11 @@ def make_model(width, height, profiles, psf=None, mask=None, magzero=0.0, scale=
                                    C.byref(oh), C.byref(ox), C.byref(oy)))
     img = out[: ow.value * oh.value].reshape(oh.value, ow.value).copy()
     return {"image": img, "offset": (ox.value, oy.value)}
+
+
+def clear_cache():
+    """Releases the device set-ups kept for repeated make_model calls."""
+    _check(lib().profit_amd_clear_cache())
 
 
 def adjust_mask(mask, img_dims, psf, finesampling=1):
@@ -356,6 +367,8 @@ class Engine:
         _check(lib().profit_amd_engine_set_tile_rows(self._h, rows))
 
     def set_render_fp64(self, on=True):
+        """Model precision: False / 0 FP32 profile values (default), True / 1
+        FP64 values, 2 FP64 values, FP64 canvas and FP64 FFT convolution."""
         _check(lib().profit_amd_engine_set_render_fp64(self._h, int(on)))
 
     def set_grad_slots(self, slots=None):
@@ -435,6 +448,33 @@ class Engine:
         return {"stage_ms": dict(zip(STAGES, ms.tolist())), "stage_calls": dict(zip(STAGES, calls.tolist())),
                 "evaluations": ev.value, "queue_length": q.value, "conv_flops": cf.value,
                 "adjoint_flops": af.value}
+
+    def adjusted_settings(self, params):
+        """Per component: resolution, max_recursions, acc, rscale_switch and
+        rscale_max as adjusted by the set-up."""
+        arr = self._params(params)
+        nc = int(sum(self.inst_ncomp))
+        res = np.zeros(max(1, nc), dtype=np.int32)
+        mr = np.zeros(max(1, nc), dtype=np.int32)
+        acc, rsw, rmax = np.zeros(max(1, nc)), np.zeros(max(1, nc)), np.zeros(max(1, nc))
+        _check(lib().profit_amd_engine_adjusted_settings(self._h, arr, res.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                         mr.ctypes.data_as(C.POINTER(C.c_int32)), _d(acc), _d(rsw),
+                                                         _d(rmax)))
+        return dict(resolution=res[:nc], max_recursions=mr[:nc], acc=acc[:nc], rscale_switch=rsw[:nc],
+                    rscale_max=rmax[:nc])
+
+    def fft_stats(self):
+        """Milliseconds per timed evaluation of each FFT kernel: forward rows,
+        columns, inverse rows of the likelihood stage and of the adjoint stage."""
+        ms = np.zeros(6)
+        calls = np.zeros(2, dtype=np.int64)
+        _check(lib().profit_amd_engine_fft_stats(self._h, _d(ms), calls.ctypes.data_as(C.POINTER(C.c_int64))))
+        names = ("rows_fwd", "cols", "rows_inv")
+        out = {}
+        for f, stage in enumerate(("like", "adj")):
+            for k, n in enumerate(names):
+                out[f"{stage}_{n}"] = ms[3 * f + k] / calls[f] if calls[f] else 0.0
+        return out
 
     def diagnostics(self, params):
         arr = self._params(params)

@@ -142,6 +142,24 @@ int profit_amd_drawing_dims(const profit_amd_model_desc *model, const profit_amd
 	});
 }
 
+int profit_amd_output_dims(const profit_amd_model_desc *model, const profit_amd_profile *profiles, int32_t nprofiles,
+                           uint32_t *width, uint32_t *height)
+{
+	return guarded([&] {
+		if (!model) throw std::invalid_argument("null model description");
+		profit::Model m(model->width, model->height);
+		build_model(m, model, profiles, nprofiles);
+		const profit::Dimensions d = m.get_output_dimensions();
+		*width = d.x;
+		*height = d.y;
+	});
+}
+
+int profit_amd_clear_cache(void)
+{
+	return guarded([&] { profit::clear_cache(); });
+}
+
 int profit_amd_adjust_mask(const uint8_t *mask, uint32_t mask_w, uint32_t mask_h, uint32_t img_w, uint32_t img_h,
                            const double *psf, uint32_t psf_w, uint32_t psf_h, uint32_t finesampling, uint8_t *out,
                            size_t out_cap, uint32_t *out_w, uint32_t *out_h)
@@ -289,7 +307,7 @@ int profit_amd_engine_set_tile_rows(profit_amd_engine *engine, int32_t rows)
 
 int profit_amd_engine_set_render_fp64(profit_amd_engine *engine, int32_t on)
 {
-	return guarded([&] { engine->eng.set_render_fp64(on != 0); });
+	return guarded([&] { engine->eng.set_render_fp64(on); });
 }
 
 int profit_amd_engine_set_grad_slots(profit_amd_engine *engine, uint32_t mask)
@@ -340,11 +358,7 @@ int profit_amd_engine_like_grad(profit_amd_engine *engine, const profit_amd_prof
 {
 	return guarded([&] {
 		if (!grad) throw std::invalid_argument("null gradient buffer");
-		const auto *p = reinterpret_cast<const profit_amd::ProfileParams *>(params);
-		for (int c = 0; c < engine->eng.ncomp(); ++c)
-			if (p[c].type == profit_amd::SERSIC && p[c].rescale_flux)
-				throw profit::invalid_parameter("the gradient does not support sersic profiles with rescale_flux");
-		engine->eng.evaluate(p, loglike, grad);
+		engine->eng.evaluate(reinterpret_cast<const profit_amd::ProfileParams *>(params), loglike, grad);
 	});
 }
 
@@ -393,6 +407,28 @@ int profit_amd_engine_stats(profit_amd_engine *engine, double *stage_ms, int64_t
 		if (queue_length) *queue_length = s.queue_count;
 		if (conv_flops) *conv_flops = engine->eng.conv_flops_forward();
 		if (adjoint_flops) *adjoint_flops = engine->eng.conv_flops_adjoint();
+	});
+}
+
+int profit_amd_engine_adjusted_settings(profit_amd_engine *engine, const profit_amd_profile *params, int32_t *resolution,
+                                        int32_t *max_recursions, double *acc, double *rscale_switch,
+                                        double *rscale_max)
+{
+	return guarded([&] {
+		engine->eng.adjusted_settings(reinterpret_cast<const profit_amd::ProfileParams *>(params), resolution,
+		                              max_recursions, acc, rscale_switch, rscale_max);
+	});
+}
+
+int profit_amd_engine_fft_stats(profit_amd_engine *engine, double *kernel_ms, int64_t *calls)
+{
+	return guarded([&] {
+		const profit_amd::EngineStats &s = engine->eng.stats();
+		for (int f = 0; f < 2; ++f) {
+			for (int k = 0; k < 3; ++k)
+				if (kernel_ms) kernel_ms[3 * f + k] = s.fft_ms[f][k];
+			if (calls) calls[f] = s.fft_calls[f];
+		}
 	});
 }
 

@@ -55,6 +55,7 @@ struct GalaxyDev {
 	int32_t n_rtiles, n_ctiles, n_atiles;
 	int32_t conv_rows;     // rows per thread of the convolution tiles
 	int32_t render_fp64;   // profile values (direct pixels, recursion leaves) in FP64 instead of FP32
+	int32_t canvas64;      // the canvas is FP64 (InstDev::canvas64) and every PSF convolution the FFT
 	int32_t grad_box;      // the adjoint computes d / d box (the engine's gradient-slot mask)
 	double sx, sy;         // profile pixel scale
 	double img_sx, img_sy; // image pixel scale
@@ -99,6 +100,7 @@ struct InstDev {
 	// per component, canvas row and 64-pixel tile column: bit set where the
 	// forward render evaluated the pixel directly (ncomp x DH x ceil(DW/64))
 	uint64_t *dmask;
+	double *canvas64;      // DW*DH FP64 canvas (GalaxyDev::canvas64), else null
 };
 
 struct QEntry {

@@ -65,6 +65,9 @@ Engine::~Engine()
 		if (g) (void)hipGraphExecDestroy(g);
 	for (auto &e : events_)
 		if (e) (void)hipEventDestroy(e);
+	for (auto &row : fft_events_)
+		for (auto &e : row)
+			if (e) (void)hipEventDestroy(e);
 	if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -89,13 +92,23 @@ static int ilog2(int n)
 void Engine::set_timing(bool on)
 {
 	timing_ = on;
-	if (on && !events_[0])
+	if (on && !events_[0]) {
 		for (auto &e : events_) hip_check(hipEventCreate(&e), "hipEventCreate");
+		for (auto &row : fft_events_)
+			for (auto &e : row) hip_check(hipEventCreate(&e), "hipEventCreate");
+	}
 }
 
 void Engine::mark(int k)
 {
 	if (timing_) hip_check(hipEventRecord(events_[k], stream_), "hipEventRecord");
+}
+
+void Engine::mark_fft(int stage, int k, hipStream_t st)
+{
+	if (!timing_) return;
+	hip_check(hipEventRecord(fft_events_[stage][k], st), "hipEventRecord");
+	if (k == 3) fft_marked_[stage] = true;
 }
 
 double Engine::conv_flops_forward() const
@@ -181,6 +194,14 @@ Engine::TuneReport Engine::autotune(const ProfileParams *params, int reps, bool 
 	if (ninst() == 0) throw std::invalid_argument("autotune needs a batch");
 	reps = std::max(1, reps);
 	TuneReport rep;
+	if (canvas64_) {
+		// an FP64 canvas has one convolver, the FP64 FFT
+		rep.ms[3] = rep.ms[5] = time_conv(params, reps);
+		rep.kinds.assign(gal_host_.size(), CK_VALU);
+		for (size_t g = 0; g < gal_host_.size(); ++g)
+			if (gal_host_[g].has_psf) rep.kinds[g] = CK_FFT;
+		return rep;
+	}
 	const int rows_cand[3] = {4, 8, 16};
 	std::vector<double> w_valu[3], w_fft, w_mfma;
 	const int ng = (int)gal_host_.size();
@@ -368,12 +389,21 @@ void Engine::conv_forward(int mode, hipStream_t st)
 		          "conv mfma");
 	}
 	if (rf0_off_[n] == 0) return;
-	hip_check(launch_fft_rows_fwd(rf0_off_[n], fft_nmax1_, gals, insts, d_rf0_off_.as<int>(), n, 0, d_tw_.as<double2>(),
-	                              twn_, st),
-	          "fft rows");
+	const bool timed = mode == 0;  // the likelihood evaluation's stage
+	if (timed) mark_fft(0, 0, st);
+	if (canvas64_)
+		hip_check(launch_fft_rows_fwd64(rf0_off_[n], fft_nmax1_, gals, insts, d_rf0_off_.as<int>(), n,
+		                                d_tw_.as<double2>(), twn_, st),
+		          "fft rows");
+	else
+		hip_check(launch_fft_rows_fwd(rf0_off_[n], fft_nmax1_, gals, insts, d_rf0_off_.as<int>(), n, 0,
+		                              d_tw_.as<double2>(), twn_, st),
+		          "fft rows");
+	if (timed) mark_fft(0, 1, st);
 	hip_check(launch_fft_cols(col_off_[n], fft_nmax2_, fft_cb_, gals, insts, d_col_off_.as<int>(), n, 1, 0,
 	                          mode == 0 ? 0 : 1, d_tw_.as<double2>(), twn_, nullptr, st),
 	          "fft cols");
+	if (timed) mark_fft(0, 2, st);
 	if (mode == 0)
 		hip_check(launch_fft_rows_inv(ri0_off_[n], fft_nmax1_, gals, insts, d_ri0_off_.as<int>(), n, 0,
 		                              d_tw_.as<double2>(), twn_, lk, partial, st),
@@ -382,6 +412,7 @@ void Engine::conv_forward(int mode, hipStream_t st)
 		hip_check(launch_fft_rows_inv(ri1_off_[n], fft_nmax1_, gals, insts, d_ri1_off_.as<int>(), n, 1,
 		                              d_tw_.as<double2>(), twn_, lk, partial, st),
 		          "fft rows inv");
+	if (timed) mark_fft(0, 3, st);
 }
 
 void Engine::conv_adjoint(hipStream_t st)
@@ -397,22 +428,32 @@ void Engine::conv_adjoint(hipStream_t st)
 	                           nullptr, st),
 	          "adjoint conv mfma");
 	if (rf1_off_[n] == 0) return;
+	mark_fft(1, 0, st);
 	hip_check(launch_fft_rows_fwd(rf1_off_[n], fft_nmax1_, gals, insts, d_rf1_off_.as<int>(), n, 1, d_tw_.as<double2>(),
 	                              twn_, st),
 	          "fft rows");
+	mark_fft(1, 1, st);
 	hip_check(launch_fft_cols(col_off_[n], fft_nmax2_, fft_cb_, gals, insts, d_col_off_.as<int>(), n, 2, 1, 1,
 	                          d_tw_.as<double2>(), twn_, nullptr, st),
 	          "fft cols");
+	mark_fft(1, 2, st);
 	hip_check(launch_fft_rows_inv(ri1_off_[n], fft_nmax1_, gals, insts, d_ri1_off_.as<int>(), n, 2, d_tw_.as<double2>(),
 	                              twn_, lk, nullptr, st),
 	          "fft rows inv");
+	mark_fft(1, 3, st);
 }
 
-void Engine::set_render_fp64(bool on)
+void Engine::set_render_fp64(int mode)
 {
-	if (on == render_fp64_) return;
+	if (mode < 0 || mode > 2) throw std::invalid_argument("render precision must be 0 (FP32), 1 (FP64 values) or 2 (FP64)");
+	const bool on = mode >= 1, c64 = mode == 2;
+	if (on == render_fp64_ && c64 == canvas64_) return;
 	render_fp64_ = on;
-	for (auto &g : gal_host_) g.render_fp64 = on ? 1 : 0;
+	canvas64_ = c64;
+	for (auto &g : gal_host_) {
+		g.render_fp64 = on ? 1 : 0;
+		g.canvas64 = c64 ? 1 : 0;
+	}
 	gals_dirty_ = true;
 	invalidate_graphs(graph_exec_);
 	if (!inst_galaxy_.empty()) set_batch(inst_galaxy_, inst_ncomp_);
@@ -471,6 +512,7 @@ int Engine::add_galaxy(const GalaxySpec &s)
 	}
 	g.has_calc = s.calc != nullptr;
 	g.render_fp64 = render_fp64_ ? 1 : 0;
+	g.canvas64 = canvas64_ ? 1 : 0;
 	g.grad_box = (grad_slots_ >> G_BOX) & 1u;
 	const size_t npix = (size_t)s.W * s.H, ndraw = (size_t)g.DW * g.DH;
 	if (have_psf) {
@@ -709,7 +751,8 @@ void Engine::set_batch(const std::vector<int> &inst_galaxy, const std::vector<in
 	kind_override_.resize(gal_host_.size(), -1);
 	for (size_t gi = 0; gi < gal_host_.size(); ++gi) {
 		GalaxyDev &g = gal_host_[gi];
-		const int k = !g.has_psf ? CK_VALU : kind_override_[gi] >= 0 ? kind_override_[gi] : conv_method_;
+		// an FP64 canvas is convolved by the FP64 FFT
+		const int k = !g.has_psf ? CK_VALU : canvas64_ ? CK_FFT : kind_override_[gi] >= 0 ? kind_override_[gi] : conv_method_;
 		if (k != g.conv_kind) {
 			g.conv_kind = k;
 			gals_dirty_ = true;
@@ -747,15 +790,18 @@ void Engine::set_batch(const std::vector<int> &inst_galaxy, const std::vector<in
 		DevBuf canvas(ndraw * 4), w(npix * 4), G(ndraw * 4), aux(ndraw * 4), fbuf;
 		if (g.conv_kind == CK_FFT) fbuf.alloc((size_t)fft_pitch(g.fft_n1) * g.fft_n2 * sizeof(double2));
 		DevBuf dmask(std::max<size_t>(8, (size_t)inst_ncomp[i] * g.DH * ((g.DW + 63) / 64) * sizeof(uint64_t)));
+		DevBuf canvas64;
+		if (canvas64_) canvas64.alloc(ndraw * sizeof(double));
 		insts[i] = InstDev{gi,          inst_comp0_[i], inst_ncomp[i],   0,
 		                   canvas.as<float>(), w.as<float>(), G.as<float>(), aux.as<float>(), fbuf.as<double2>(),
-		                   dmask.as<uint64_t>()};
+		                   dmask.as<uint64_t>(), canvas64.as<double>()};
 		inst_bufs_.push_back(std::move(canvas));
 		inst_bufs_.push_back(std::move(w));
 		inst_bufs_.push_back(std::move(G));
 		inst_bufs_.push_back(std::move(aux));
 		inst_bufs_.push_back(std::move(fbuf));
 		inst_bufs_.push_back(std::move(dmask));
+		inst_bufs_.push_back(std::move(canvas64));
 		qneed += (long)inst_ncomp[i] * (long)std::min<size_t>(ndraw, 4096);
 		// centre pixels: within one image pixel of the centre, (2 fs)^2 at most
 		const long fs = std::lround(g.img_sx / g.sx);
@@ -1036,7 +1082,17 @@ void Engine::run(bool grad, bool use_graph)
 					stats_.ms[k] += ms;
 					stats_.calls[k] += 1;
 				}
+				for (int f = 0; f < 2; ++f) {
+					if (!fft_marked_[f]) continue;
+					for (int k = 0; k < 3; ++k) {
+						float ms = 0;
+						hip_check(hipEventElapsedTime(&ms, fft_events_[f][k], fft_events_[f][k + 1]), "elapsed");
+						stats_.fft_ms[f][k] += ms;
+					}
+					stats_.fft_calls[f] += 1;
+				}
 			}
+			fft_marked_[0] = fft_marked_[1] = false;
 			stats_.evaluations += 1;
 			return;
 		}
@@ -1094,6 +1150,35 @@ void Engine::diagnostics(const ProfileParams *params, int *cls, int *nodes)
 	hip_check(hipMemcpy(nodes, dnodes.as<int>(), plane * nc * sizeof(int), hipMemcpyDeviceToHost), "nodes");
 }
 
+void Engine::adjusted_settings(const ProfileParams *params, int *res, int *maxrec, double *acc, double *rsw,
+                               double *rmax)
+{
+	hip_check(hipSetDevice(device_), "hipSetDevice");
+	const int nc = ncomp_total_;
+	if (nc == 0) return;
+	std::memcpy(h_params_.as<ProfileParams>(), params, nc * sizeof(ProfileParams));
+	hipStream_t st = stream_;
+	hip_check(hipMemcpyAsync(d_params_.as<ProfileParams>(), h_params_.as<ProfileParams>(), nc * sizeof(ProfileParams),
+	                         hipMemcpyHostToDevice, st),
+	          "params");
+	hip_check(launch_setup(d_params_.as<ProfileParams>(), d_comp_inst_.as<int>(), d_insts_.as<InstDev>(),
+	                       d_gals_.as<GalaxyDev>(), d_comps_.as<Comp>(), d_err_.as<int>(), nc, st),
+	          "setup");
+	std::vector<Comp> comps(nc);
+	hip_check(hipMemcpyAsync(comps.data(), d_comps_.as<Comp>(), nc * sizeof(Comp), hipMemcpyDeviceToHost, st), "comps");
+	hip_check(hipMemcpyAsync(h_err_.as<int>(), d_err_.as<int>(), nc * sizeof(int), hipMemcpyDeviceToHost, st), "err");
+	hip_check(hipStreamSynchronize(st), "sync");
+	check_setup_errors();
+	for (int c = 0; c < nc; ++c) {
+		const bool radial = is_radial(comps[c].type);
+		if (res) res[c] = radial ? comps[c].res : 0;
+		if (maxrec) maxrec[c] = radial ? comps[c].maxrec : 0;
+		if (acc) acc[c] = radial ? comps[c].acc : 0;
+		if (rsw) rsw[c] = radial ? comps[c].rscale_switch : 0;
+		if (rmax) rmax[c] = radial ? comps[c].rscale_max : 0;
+	}
+}
+
 void Engine::model_images(const ProfileParams *params, float *out)
 {
 	hip_check(hipSetDevice(device_), "hipSetDevice");
@@ -1136,7 +1221,7 @@ void Engine::model_images(const ProfileParams *params, float *out)
 	}
 }
 
-void Engine::model_canvas(const ProfileParams *params, float *out)
+void Engine::model_canvas(const ProfileParams *params, float *out, float *out_direct)
 {
 	if (ninst() != 1) throw std::invalid_argument("model_canvas needs a batch of one instance");
 	hip_check(hipSetDevice(device_), "hipSetDevice");
@@ -1165,7 +1250,7 @@ void Engine::model_canvas(const ProfileParams *params, float *out)
 			                        work_queue(), 0, 1, nullptr, st),
 			          "render");
 		launch_walkers_fwd(conv ? 1 : 0, st);
-		if (conv) conv_forward(1, st);
+		if (conv && !out_direct) conv_forward(1, st);
 		hip_check(hipMemcpyAsync(h_err_.as<int>(), d_err_.as<int>(), nc * sizeof(int), hipMemcpyDeviceToHost, st), "err");
 		hip_check(hipMemcpyAsync(h_qcount_.as<int>(), d_qcount_.as<int>(), 2 * sizeof(int), hipMemcpyDeviceToHost, st),
 		          "qcount");
@@ -1175,7 +1260,18 @@ void Engine::model_canvas(const ProfileParams *params, float *out)
 		grow_queue();
 	}
 	const size_t nd = (size_t)g.DW * g.DH;
-	if (conv) {
+	if (out_direct) {
+		// the two canvases before any convolution
+		if (conv) {
+			hip_check(hipMemcpy(out, inst_bufs_[0].as<float>(), nd * 4, hipMemcpyDeviceToHost), "model");
+			hip_check(hipMemcpy(out_direct, inst_bufs_[3].as<float>(), nd * 4, hipMemcpyDeviceToHost), "model");
+		}
+		else {
+			std::memset(out, 0, nd * 4);
+			hip_check(hipMemcpy(out_direct, inst_bufs_[0].as<float>(), nd * 4, hipMemcpyDeviceToHost), "model");
+		}
+	}
+	else if (conv) {
 		std::vector<float> aux(nd);
 		hip_check(hipMemcpy(out, inst_bufs_[2].as<float>(), nd * 4, hipMemcpyDeviceToHost), "model");
 		hip_check(hipMemcpy(aux.data(), inst_bufs_[3].as<float>(), nd * 4, hipMemcpyDeviceToHost), "model");

@@ -21,8 +21,8 @@ int launch_zero_i(int *p, long n, hipStream_t st);
 int launch_render(const GalaxyDev *gals, const InstDev *insts, const int *tile_off, int ninst, int nblocks,
                   const Comp *comps, WorkQueue wq, int filter, int target, int *diag_cls, hipStream_t st, int type = -1);
 // the forward walker and the adjoint render, one launch per radial family
-// (and per derivative precision in the adjoint render: 1 = FP64, steep cores
-// only); the adjoint walker, one launch for all
+// (and per sum precision in the adjoint render: 1 = FP64 per-thread sums,
+// steep cores only); the adjoint walker, one launch per family
 int launch_subsample_fwd(int type, int nblocks, const GalaxyDev *gals, const InstDev *insts, const Comp *comps,
                          WorkQueue wq, int split, hipStream_t st);
 int launch_subsample_adj(int type, int nblocks, const GalaxyDev *gals, const InstDev *insts, const Comp *comps,
@@ -45,6 +45,9 @@ int launch_dilate(const uint8_t *in, uint8_t *tmp, uint8_t *out, int w, int h, i
 // FFT path (fft.hip)
 int launch_fft_rows_fwd(int nblocks, int nmax, const GalaxyDev *gals, const InstDev *insts, const int *row_off,
                         int ninst, int mode, const double2 *tw, int twn, hipStream_t st);
+// mode 0 of launch_fft_rows_fwd from the FP64 canvas (GalaxyDev::canvas64)
+int launch_fft_rows_fwd64(int nblocks, int nmax, const GalaxyDev *gals, const InstDev *insts, const int *row_off,
+                          int ninst, const double2 *tw, int twn, hipStream_t st);
 int launch_fft_psf_rows(int n1, int n2, const GalaxyDev *gal, const double2 *tw, int twn, const double *src64,
                         double2 *dst64, hipStream_t st);
 int launch_fft_cols(int nblocks, int n2max, int cb, const GalaxyDev *gals, const InstDev *insts, const int *col_off,
@@ -151,6 +154,10 @@ struct EngineStats {
 	long calls[ST_COUNT] = {0};
 	int queue_count = 0;
 	long evaluations = 0;
+	// the FFT stages kernel by kernel: [forward, adjoint][rows forward,
+	// columns, rows inverse], over the evaluations that ran FFT galaxies
+	double fft_ms[2][3] = {{0}};
+	long fft_calls[2] = {0};
 };
 
 class Engine {
@@ -184,11 +191,18 @@ class Engine {
 	void evaluate(const ProfileParams *params, double *ll, double *grad, bool use_graph = true);
 	// rendering diagnostics for parity tests: per-component class and node maps
 	void diagnostics(const ProfileParams *params, int *cls, int *nodes);
+	// the adjusted sub-pixel settings of every component of the batch (set-up
+	// only): resolution, max_recursions, acc, rscale_switch, rscale_max
+	void adjusted_settings(const ProfileParams *params, int *res, int *maxrec, double *acc, double *rsw,
+	                       double *rmax);
 	// the convolved model of every instance (W*H floats each, region-independent)
 	void model_images(const ProfileParams *params, float *out);
 	// Model::evaluate's drawing canvas for a single instance: convolved profiles
 	// (masked by the calc region) plus the profiles drawn without convolution.
-	void model_canvas(const ProfileParams *params, float *out);
+	// With out_direct given, nothing is convolved: out receives the canvas of
+	// the profiles to be convolved and out_direct that of the others, as
+	// Model::produce_image holds them before its convolver runs.
+	void model_canvas(const ProfileParams *params, float *out, float *out_direct = nullptr);
 	// Mask::expand_by on the device
 	static void dilate(const uint8_t *in, uint8_t *out, int w, int h, int padx, int pady, int device = 0);
 
@@ -197,12 +211,15 @@ class Engine {
 	int conv_rows() const { return conv_rows_; }
 	void set_conv_rows(int r);
 	// profile values (direct pixels, recursion leaves) in FP32 (default) or FP64
-	void set_render_fp64(bool on);
+	// (1); 2: FP64 values, an FP64 canvas and the FFT for every PSF
+	// convolution, the model image FP64 end to end
+	void set_render_fp64(int mode);
 	// gradient slots computed (bit s = slot s); the others come back as 0.
 	// Leaving out the box slot spares the per-sample logarithms of d / d box.
 	void set_grad_slots(unsigned mask);
 	unsigned grad_slots() const { return grad_slots_; }
 	bool render_fp64() const { return render_fp64_; }
+	bool canvas_fp64() const { return canvas64_; }
 	void set_timing(bool on);
 	void reset_stats() { stats_ = EngineStats{}; }
 	// algorithmic work of the batch: pixels of the likelihood region and of the
@@ -253,6 +270,7 @@ class Engine {
 	int mfma_pitch_ = 0, mfma_kch_ = 0;
 	int fft_cb_ = 8;
 	bool render_fp64_ = false;
+	bool canvas64_ = false;
 	unsigned grad_slots_ = (1u << NGRAD) - 1u;
 	int kc_max_ = 64;
 	bool timing_ = false;
@@ -280,8 +298,8 @@ class Engine {
 	int adj_grp_ = 1;  // canvas tiles per block of the adjoint render
 	int kx_max_ = 0;
 	DevBuf d_insts_, d_ct_off_, d_at_off_, d_ag_off_, d_comp_inst_;
-	// per instance: canvas, w, G, aux, FFT frame, direct-pixel mask
-	static constexpr int kInstBufs = 6;
+	// per instance: canvas, w, G, aux, FFT frame, direct-pixel mask, FP64 canvas
+	static constexpr int kInstBufs = 7;
 	std::vector<DevBuf> inst_bufs_;
 	DevBuf d_params_, d_comps_, d_err_, d_queue_, d_qcount_, d_partial_, d_ll_, d_gscale_, d_grad_acc_, d_grad_out_;
 	long qcap_ = 0, hcap_ = 0;
@@ -291,6 +309,10 @@ class Engine {
 	hipGraphExec_t graph_exec_[2] = {nullptr, nullptr};
 	hipEvent_t events_[ST_COUNT + 1] = {};
 	void mark(int k);
+	// around the three FFT kernels of the forward (0) and adjoint (1) stage
+	hipEvent_t fft_events_[2][4] = {};
+	bool fft_marked_[2] = {false, false};
+	void mark_fft(int stage, int k, hipStream_t st);
 	int subsample_blocks_ = 2048;
 };
 

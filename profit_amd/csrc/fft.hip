@@ -316,6 +316,28 @@ __global__ __launch_bounds__(256) void k_fft_rows_fwd(const GalaxyDev *__restric
 	rows_fwd_pairs<float>(sh, wt, N, L, np, src, width, x0, width, nrows - r0, pmax, I.fbuf + (long)(y0 + r0) * P, P);
 }
 
+// Forward rows of the FP64 canvas (mode 0 of k_fft_rows_fwd for galaxies with
+// GalaxyDev::canvas64), a kernel of its own so that no pointer is selected on
+// a branch.
+__global__ __launch_bounds__(256) void k_fft_rows_fwd64(const GalaxyDev *__restrict__ gals,
+                                                        const InstDev *__restrict__ insts,
+                                                        const int *__restrict__ row_off, int ninst,
+                                                        const double2 *__restrict__ tw, int twn)
+{
+	extern __shared__ double2 sh[];
+	const int b = blockIdx.x;
+	const int inst = owner(row_off, ninst, b);
+	const InstDev &I = insts[inst];
+	const GalaxyDev &g = gals[I.galaxy];
+	const int N = g.fft_n1, L = g.fft_l1, P = fft_pitch(N), np = fft_pairs_per_block(N);
+	double2 *wt = sh + np * N;
+	stage_twiddles(wt, N, L, tw, twn);
+	const int r0 = 2 * (b - row_off[inst]) * np;
+	const double *src = I.canvas64 + (long)r0 * g.DW;
+	const int pmax = (g.DH - r0 + 1) / 2;
+	rows_fwd_pairs<double>(sh, wt, N, L, np, src, g.DW, 0, g.DW, g.DH - r0, pmax, I.fbuf + (long)r0 * P, P);
+}
+
 // Forward rows of a galaxy's real FP64 PSF frame (N1 x N2, `src64`) into the
 // half-spectrum frame `dst64`, fft_pairs_per_block(N1) pairs of rows per block.
 __global__ __launch_bounds__(256) void k_fft_psf_rows(const GalaxyDev *__restrict__ g, const double2 *__restrict__ tw,
@@ -584,7 +606,7 @@ static int ensure_lds(const void *fn, size_t lds, size_t *done)
 	done[dev] = lds;
 	return 0;
 }
-static size_t lds_rows_fwd[64], lds_psf_rows[64], lds_cols[64], lds_rows_inv[64], lds_rows_inv1[64], lds_rows_inv2[64];
+static size_t lds_rows_fwd[64], lds_rows_fwd64[64], lds_psf_rows[64], lds_cols[64], lds_rows_inv[64], lds_rows_inv1[64], lds_rows_inv2[64];
 
 static int cols_per_block(int n2) { return n2 >= 4096 ? 1 : (n2 >= 2048 ? 2 : (4096 / n2 > 8 ? 8 : 4096 / n2)); }
 
@@ -601,6 +623,17 @@ int launch_fft_rows_fwd(int nblocks, int nmax, const GalaxyDev *gals, const Inst
 	if (lds > 160 * 1024) return (int)hipErrorInvalidValue;
 	if (int e = ensure_lds((const void *)k_fft_rows_fwd, lds, lds_rows_fwd)) return e;
 	k_fft_rows_fwd<<<nblocks, 256, lds, st>>>(gals, insts, row_off, ninst, mode, tw, twn);
+	return (int)hipGetLastError();
+}
+
+int launch_fft_rows_fwd64(int nblocks, int nmax, const GalaxyDev *gals, const InstDev *insts, const int *row_off,
+                          int ninst, const double2 *tw, int twn, hipStream_t st)
+{
+	if (nblocks <= 0) return 0;
+	const size_t lds = row_lds(nmax);
+	if (lds > 160 * 1024) return (int)hipErrorInvalidValue;
+	if (int e = ensure_lds((const void *)k_fft_rows_fwd64, lds, lds_rows_fwd64)) return e;
+	k_fft_rows_fwd64<<<nblocks, 256, lds, st>>>(gals, insts, row_off, ninst, tw, twn);
 	return (int)hipGetLastError();
 }
 

@@ -80,44 +80,116 @@ __device__ __forceinline__ double sample_accumulate(const Comp &c, double xp, do
 	return (double)o[0];
 }
 
-// libprofit's refinement test |test - val| > acc * val (radial.cpp:92-100).
-// Evaluated in FP32 first; only when the FP32 margin lies within a guard band
-// of 1e-3 * val around the threshold (far wider than the FP32 evaluation
-// error) is it re-evaluated in FP64, so the decision equals the FP64 one.
-// Returns the decision; *v64 receives the FP64 value when it was computed.
-template <int TYPE = -1>
-__device__ __forceinline__ bool refine_cell(const Comp &c, double xp, double yp, double dyp, bool recurse, float v32,
-                                   double *v64, bool *have64)
+// Whether a test point lies within 1% of a King or Ferrer profile's cut-off
+// (rt, rout).  There the value is a difference of nearly equal terms (King:
+// A - C; Ferrer: 1 - t^(2-b)) whose FP32 relative error grows without bound,
+// and a cell across the cut can be nonzero in FP32 and zero in FP64; such
+// cells take the FP64 test.  The test point (|x|, |y| + |dy|) lies at least as
+// far out as the cell centre, so it stands for both.
+template <int TYPE>
+__device__ __forceinline__ bool near_cut(const Comp &c, float ax, float ay)
 {
-	*have64 = false;
-	if (!recurse) return false;
-	const float t32 = eval_at<float, TYPE>(c, (float)fabs(xp), (float)(fabs(yp) + fabs(dyp)));
+	if (TYPE >= 0 && TYPE != KING && TYPE != FERRER) return false;
+	const int t = TYPE >= 0 ? TYPE : c.type;
+	if (t != KING && t != FERRER) return false;
+	const float cut = (float)(t == KING ? c.s[1] : c.rscale);
+	return boxy_r<float>(c, ax, ay) > 0.99f * cut;
+}
+
+// libprofit's refinement test |test - val| > acc * val (radial.cpp:92-100),
+// evaluated in FP32 at FP32 cell positions: 1 refine, 0 leaf, 2 undecided.
+// The decision is taken only when the FP32 margin lies outside a guard band
+// of 1e-3 * val around the threshold; the FP32 evaluation errors (values and
+// positions, a few parts in 1e7 of val) are far inside it, so a decision taken
+// here equals the FP64 one.  Cells near a King or Ferrer cut-off are always
+// undecided (near_cut).  Undecided cells take refine_cell64 at libprofit's
+// own FP64 cell position.
+template <int TYPE = -1>
+__device__ __forceinline__ int refine_cell32(const Comp &c, float xp, float yp, float dyp, float v32)
+{
+	const float ax = fabsf(xp), ay = fabsf(yp) + fabsf(dyp);
+	const float t32 = eval_at<float, TYPE>(c, ax, ay);
 	const float margin = fabsf(t32 - v32) - (float)c.acc * v32;
-	if (v32 > 1e-30f && isfinite(margin) && fabsf(margin) > 1e-3f * v32) return margin > 0.f;
+	if (v32 > 1e-30f && isfinite(margin) && fabsf(margin) > 1e-3f * v32 && !near_cut<TYPE>(c, ax, ay))
+		return margin > 0.f ? 1 : 0;
+	return 2;
+}
+
+// the FP64 refinement test; *v64 receives the FP64 value
+template <int TYPE = -1>
+__device__ __forceinline__ bool refine_cell64(const Comp &c, double xp, double yp, double dyp, double *v64)
+{
 	const double v = eval_at<double, TYPE>(c, xp, yp);
 	const double t = eval_at<double, TYPE>(c, fabs(xp), fabs(yp) + fabs(dyp));
 	*v64 = v;
-	*have64 = true;
 	return fabs(t - v) > c.acc * v;
 }
 
+// Profile coordinates of cell (ci, cj) in FP32: the node's lower corner
+// relative to the component centre (ox, oy, formed once per node in FP64 and
+// rounded) plus the cell's offset.  Relative to the distance from the centre
+// the position errs by a few parts in 1e8, which moves a profile value by
+// |d ln f / d ln r| times that: far inside the refinement test's guard band.
+__device__ __forceinline__ void cell_pos32(const Comp &c, float ox, float oy, float fxbin, float fybin, int ci, int cj,
+                                           float *xp, float *yp)
+{
+	const float dx = ox + ((float)ci + 0.5f) * fxbin;
+	const float dy = oy + ((float)cj + 0.5f) * fybin;
+	*xp = dx * (float)c.cos_ang + dy * (float)c.sin_ang;
+	*yp = -dx * (float)c.sin_over + dy * (float)c.cos_over;
+}
+
+// libprofit's FP64 cell position (its sequence of additions, radial.cpp:86-106)
+// in profile coordinates
+__device__ __forceinline__ void cell_pos64(const Comp &c, double nx0, double ny0, double hx, double hy, int ci, int cj,
+                                           double *xp, double *yp)
+{
+	const double x = cell_centre(nx0, hx, ci), y = cell_centre(ny0, hy, cj);
+	const double xr = x - c.xc, yr = y - c.yc;
+	*xp = xr * c.cos_ang + yr * c.sin_ang;
+	*yp = -xr * c.sin_over + yr * c.cos_over;
+}
+
+// d(f)/d slot of one sample in FP32, chain rule to the component slots,
+// accumulated with weight wgt into FP32 sums gs; returns f.
+template <int TYPE>
+__device__ __forceinline__ float pixel_accumulate_f32(const Comp &c, float xp, float yp, float wgt, float *gs)
+{
+	float o[9];
+	sample_grad<float, TYPE>(c, xp, yp, o);
+	// the rotation's chain rule with the per-component sin/q, cos/q and 1/q
+	const float q = (float)c.axrat, iq = (float)c.inv_axrat;
+	const float ca = (float)c.cos_ang, sa = (float)c.sin_ang;
+	const float sq = (float)c.sin_over, cq = (float)c.cos_over;
+	const float fx = o[1], fy = o[2];
+	gs[G_XCEN] += wgt * (fx * -ca + fy * sq);
+	gs[G_YCEN] += wgt * (fx * -sa + fy * -cq);
+	gs[G_ANG] += wgt * ((fx * (q * yp) + fy * (-xp * iq)) * (3.14159265358979323846f / 180.f));
+	gs[G_AXRAT] += wgt * (fy * (-yp * iq));
+	if (c.grad_box) gs[G_BOX] += wgt * o[3];
+#pragma unroll
+	for (int s = 0; s < n_shape(TYPE); ++s) gs[G_S0 + s] += wgt * o[4 + s];
+	return o[0];
+}
+
 // Walks the recursion tree of one pixel of a TYPE profile (-1: any family,
-// dispatched at run time).  Every lane returns its share of sum_s weight_s *
-// f(s) or, when kGrad, of the derivative sums (FP32 per-sample derivatives,
-// FP64 sums: steep cores keep the gradient margins of FP64 derivatives, and
-// the walker then holds 2.0 instead of 2.3 ms on the bulge+disk batch);
-// `scale` multiplies the weights (G(q) * flux_scale in the adjoint, 1
-// otherwise); `fp64`: forward leaf values in FP64.  PREC selects, in
-// k_subsample, the components a launch takes by their steep-core flag (2:
-// all).
+// dispatched at run time).  Cell positions, values and refinement tests are
+// FP32; libprofit's FP64 cell position is formed only for an undecided test
+// and for FP64 leaf values (`fp64`).  The descent to a refined cell follows
+// libprofit's FP64 node bounds, so the trees are its own.  Every lane returns
+// its share of sum_s weight_s * f(s) in `sum` or, when kGrad, of the
+// derivative sums: FP32 per-sample derivatives, summed in FP64 (gs, sum) for
+// steep cores and in FP32 (gsf, sumf) otherwise.  `scale` multiplies the
+// weights (G(q) * flux_scale in the adjoint, 1 otherwise).
 template <bool kGrad, int TYPE, int PREC>
 __device__ __forceinline__ void walk_pixel(const Comp &c, WalkLevel *L, int lane, double x0, double x1, double y0, double y1, int res,
-                           int maxrec, double scale, bool fp64, double &sum, double *gs, bool root_only,
-                           uint8_t *root_flag)
+                           int maxrec, double scale, bool fp64, double &sum, double *gs, float &sumf, float *gsf,
+                           bool root_only, uint8_t *root_flag)
 {
 	const int ncell = res * res;
 	const int nchunk = (ncell + 63) >> 6;
 	const double inv_cells = 1.0 / (double)ncell;
+	const bool sums64 = !kGrad || c.grad64;
 	int level = 0;
 	double nx0 = x0, nx1 = x1, ny0 = y0, ny1 = y1;
 	double weight = inv_cells;
@@ -129,6 +201,9 @@ __device__ __forceinline__ void walk_pixel(const Comp &c, WalkLevel *L, int lane
 		// refine, so every cell is a leaf and no test is repeated
 		const bool recurse = res > 1 && level < maxrec && !root_only;
 		const double dyp = -xbin * c.sin_over + ybin * c.cos_over;
+		const float ox = (float)(nx0 - c.xc), oy = (float)(ny0 - c.yc);
+		const float fxbin = (float)xbin, fybin = (float)ybin, fdyp = (float)dyp;
+		const double w = weight * scale;
 		unsigned long long m[4] = {0ull, 0ull, 0ull, 0ull};
 #pragma unroll
 		for (int k = 0; k < 4; ++k) {
@@ -137,26 +212,45 @@ __device__ __forceinline__ void walk_pixel(const Comp &c, WalkLevel *L, int lane
 			bool fail = false;
 			if (cell < ncell) {
 				const int ci = cell / res, cj = cell - ci * res;
-				const double x = cell_centre(nx0, hx, ci);
-				const double y = cell_centre(ny0, hy, cj);
-				const double xr = x - c.xc, yr = y - c.yc;
-				const double xp = xr * c.cos_ang + yr * c.sin_ang;
-				const double yp = -xr * c.sin_over + yr * c.cos_over;
+				float xp, yp;
+				cell_pos32(c, ox, oy, fxbin, fybin, ci, cj, &xp, &yp);
 				// the adjoint needs the FP32 value only for the refinement test
-				const float v32 = (kGrad && !recurse) ? 0.f : eval_at<float, TYPE>(c, (float)xp, (float)yp);
+				const float v32 = (kGrad && !recurse) ? 0.f : eval_at<float, TYPE>(c, xp, yp);
+				int dec = recurse ? refine_cell32<TYPE>(c, xp, yp, fdyp, v32) : 0;
 				double v64 = 0;
 				bool have64 = false;
-				fail = refine_cell<TYPE>(c, xp, yp, dyp, recurse, v32, &v64, &have64);
+				if (dec == 2) {
+					double xp64, yp64;
+					cell_pos64(c, nx0, ny0, hx, hy, ci, cj, &xp64, &yp64);
+					dec = refine_cell64<TYPE>(c, xp64, yp64, dyp, &v64) ? 1 : 0;
+					have64 = true;
+				}
+				fail = dec == 1;
 				if (!fail) {
 					if (kGrad) {
 						// value (for the flux-scale derivatives) and derivatives
-						const double f = sample_accumulate<float, TYPE>(c, xp, yp, weight * scale, gs);
-						sum += f * (weight * scale);
+						if (sums64) {
+							const double f = sample_accumulate<float, TYPE>(c, (double)xp, (double)yp, w, gs);
+							sum += f * w;
+						}
+						else {
+							const float fw = (float)w;
+							sumf += fw * pixel_accumulate_f32<TYPE>(c, xp, yp, fw, gsf);
+						}
 					}
 					else {
 						// leaf values in FP64 or the FP32 value already at hand
-						if (!have64) v64 = fp64 ? eval_at<double, TYPE>(c, xp, yp) : (double)v32;
-						sum += v64 * (weight * scale);
+						if (!have64) {
+							if (fp64) {
+								double xp64, yp64;
+								cell_pos64(c, nx0, ny0, hx, hy, ci, cj, &xp64, &yp64);
+								v64 = eval_at<double, TYPE>(c, xp64, yp64);
+							}
+							else {
+								v64 = (double)v32;
+							}
+						}
+						sum += v64 * w;
 					}
 				}
 			}
@@ -216,12 +310,10 @@ __device__ __forceinline__ void walk_pixel(const Comp &c, WalkLevel *L, int lane
 // are left to the launch made for them).  kGrad=false: adds flux_scale *
 // value to the canvas.  kGrad=true: accumulates G(q) * d(flux_scale *
 // value)/d slot; per-lane sums are carried across consecutive entries of the
-// same component and flushed with one set of atomics per run.  `counter`:
-// the pair of hand-out counters of this launch (heavy, light).  Held to the
-// registers of three waves per SIMD in the adjoint (the Sersic one needs 172
-// unbounded and spills 4 at 168; three waves took it from 1.82 to 1.43 ms)
-// and four in the forward walk (Sersic 144 -> 128 with 13 spills, 1.29 ->
-// 1.21 ms).
+// same component (the FP32 ones folded into FP64 after every grab) and
+// flushed with one set of atomics per run.  `counter`: the pair of hand-out
+// counters of this launch (heavy, light).  Held to the registers of three
+// waves per SIMD in the adjoint and four in the forward walk.
 template <bool kGrad, int TYPE, int PREC>
 __global__ __launch_bounds__(256, kGrad ? 3 : 4) void k_subsample(const GalaxyDev *__restrict__ gals,
                                                    const InstDev *__restrict__ insts,
@@ -234,13 +326,40 @@ __global__ __launch_bounds__(256, kGrad ? 3 : 4) void k_subsample(const GalaxyDe
 	const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 	WalkLevel *L = levels[wv];
 	const int nl = min(wq.count[0], wq.lcap), nh = min(wq.count[1], wq.hcap);
-	double gs[NGRAD];
-#pragma unroll
-	for (int s = 0; s < NGRAD; ++s) gs[s] = 0;
 	// slots this family can have: the geometric ones and its shape parameters
 	constexpr int kSlots = G_S0 + n_shape(TYPE);
+	double gs[kSlots];
+	float gsf[NGRAD];
+#pragma unroll
+	for (int s = 0; s < kSlots; ++s) gs[s] = 0;
+#pragma unroll
+	for (int s = 0; s < NGRAD; ++s) gsf[s] = 0.f;
 	double vsum = 0;
+	float vsumf = 0.f;
 	int cur = -1;
+	// the FP32 sums into the FP64 ones
+	auto fold = [&]() {
+#pragma unroll
+		for (int s = 0; s < kSlots; ++s) {
+			gs[s] += (double)gsf[s];
+			gsf[s] = 0.f;
+		}
+		vsum += (double)vsumf;
+		vsumf = 0.f;
+	};
+	// the current component's sums to its gradient accumulators
+	auto flush = [&]() {
+		fold();
+		const Comp &pc = comps[cur];
+		const double tv = wave_sum(vsum);
+#pragma unroll
+		for (int s = 0; s < kSlots; ++s) {
+			const double t = wave_sum(gs[s]) + pc.dlnfs[s] * tv;
+			if (lane == 0 && t != 0) atomicAdd(&grad_acc[(long)cur * NGRAD + s], t);
+			gs[s] = 0;
+		}
+		vsum = 0;
+	};
 	// dynamic hand-out: the deep centre trees one at a time first, then the
 	// other entries in grabs of kGrab, so no wave is left with a long tail
 	bool heavy_left = nh > 0;
@@ -294,17 +413,7 @@ __global__ __launch_bounds__(256, kGrad ? 3 : 4) void k_subsample(const GalaxyDe
 			const Comp &c = comps[q.comp];
 			if ((TYPE >= 0 && c.type != TYPE) || (kGrad && PREC != 2 && c.grad64 != PREC)) continue;
 			if (kGrad && q.comp != cur) {
-				if (cur >= 0) {
-					const Comp &pc = comps[cur];
-					const double tv = wave_sum(vsum);
-#pragma unroll
-					for (int s = 0; s < kSlots; ++s) {
-						const double t = wave_sum(gs[s]) + pc.dlnfs[s] * tv;
-						if (lane == 0 && t != 0) atomicAdd(&grad_acc[(long)cur * NGRAD + s], t);
-						gs[s] = 0;
-					}
-					vsum = 0;
-				}
+				if (cur >= 0) flush();
 				cur = q.comp;
 			}
 			const InstDev &I = insts[c.instance];
@@ -326,53 +435,26 @@ __global__ __launch_bounds__(256, kGrad ? 3 : 4) void k_subsample(const GalaxyDe
 			}
 			double s_local = 0;
 			walk_pixel<kGrad, TYPE, PREC>(c, L, lane, x - hx, x + hx, y - hy, y + hy, res, maxrec, scale,
-			                              g.render_fp64 != 0, s_local, gs,
+			                              g.render_fp64 != 0, s_local, gs, vsumf, gsf,
 			                              kGrad && __builtin_amdgcn_readlane(qroot, u) != 0, roots + e);
 			if (kGrad) {
 				vsum += s_local;
 			}
 			else {
 				const double tot = wave_sum(s_local);
-				float *dst = (split && !c.convolve) ? I.aux : I.canvas;
-				if (lane == 0) atomicAdd(&dst[q.pix], (float)(c.flux_scale * tot));
+				const bool to_aux = split && !c.convolve;
+				if (lane == 0) {
+					if (g.canvas64 && !to_aux) atomicAdd(&I.canvas64[q.pix], c.flux_scale * tot);
+					else atomicAdd(&(to_aux ? I.aux : I.canvas)[q.pix], (float)(c.flux_scale * tot));
+				}
 			}
 		}
+		if (kGrad) fold();
 	}
-	if (kGrad && cur >= 0) {
-		const Comp &pc = comps[cur];
-		const double tv = wave_sum(vsum);
-#pragma unroll
-		for (int s = 0; s < kSlots; ++s) {
-			const double t = wave_sum(gs[s]) + pc.dlnfs[s] * tv;
-			if (lane == 0 && t != 0) atomicAdd(&grad_acc[(long)cur * NGRAD + s], t);
-		}
-	}
+	if (kGrad && cur >= 0) flush();
 }
 
 // ------------------------------------------------------------------ adjoint render
-
-// d(f)/d slot of one direct pixel in FP32, chain rule to the component slots,
-// accumulated with weight wgt into FP32 per-thread sums (a thread's few
-// pixels); returns f.
-template <int TYPE>
-__device__ __forceinline__ float pixel_accumulate_f32(const Comp &c, float xp, float yp, float wgt, float *gs)
-{
-	float o[9];
-	sample_grad<float, TYPE>(c, xp, yp, o);
-	// the rotation's chain rule with the per-component sin/q, cos/q and 1/q
-	const float q = (float)c.axrat, iq = (float)c.inv_axrat;
-	const float ca = (float)c.cos_ang, sa = (float)c.sin_ang;
-	const float sq = (float)c.sin_over, cq = (float)c.cos_over;
-	const float fx = o[1], fy = o[2];
-	gs[G_XCEN] += wgt * (fx * -ca + fy * sq);
-	gs[G_YCEN] += wgt * (fx * -sa + fy * -cq);
-	gs[G_ANG] += wgt * ((fx * (q * yp) + fy * (-xp * iq)) * (3.14159265358979323846f / 180.f));
-	gs[G_AXRAT] += wgt * (fy * (-yp * iq));
-	if (c.grad_box) gs[G_BOX] += wgt * o[3];
-#pragma unroll
-	for (int s = 0; s < n_shape(TYPE); ++s) gs[G_S0 + s] += wgt * o[4 + s];
-	return o[0];
-}
 
 // Adjoint of the directly evaluated pixels: sum_q G(q) fs d f(q) / d slot over
 // the pixels the forward render marked direct (so no classification is

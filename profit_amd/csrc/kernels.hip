@@ -195,6 +195,7 @@ __global__ __launch_bounds__(256) void k_render(const GalaxyDev *__restrict__ ga
 		const double y = hy + j * g.sx;
 		const double x = hx + i * g.sy;
 		float acc = 0.f, aux_acc = 0.f;
+		double acc64 = 0.0;  // the FP64 canvas's sum (GalaxyDev::canvas64)
 		for (int k = 0; k < I.ncomp; ++k) {
 			const Comp &c = comps[I.comp0 + k];
 			if (filter >= 0 && c.convolve != filter) continue;
@@ -212,6 +213,7 @@ __global__ __launch_bounds__(256) void k_render(const GalaxyDev *__restrict__ ga
 					}
 				}
 				if (filter < 0 && !c.convolve && I.has_aux) aux_acc += v;
+				else if (g.canvas64 && !target) acc64 += (double)v;
 				else acc += v;
 				if (diag_cls && inb) diag_cls[(long)(I.comp0 + k) * g.DW * g.DH + pix] = 0;
 				if (I.dmask && lane == 0 && j < g.DH) I.dmask[((long)k * g.DH + j) * ((g.DW + 63) >> 6) + t.x] = 0ull;
@@ -223,7 +225,8 @@ __global__ __launch_bounds__(256) void k_render(const GalaxyDev *__restrict__ ga
 				const PixelClass pc = classify(c, x, y);
 				cls = pc.cls;
 				if (pc.cls == 2) {
-					if (g.render_fp64) acc += (float)(c.flux_scale * eval_at<double, TYPE>(c, pc.xp, pc.yp));
+					if (g.canvas64 && !target) acc64 += c.flux_scale * eval_at<double, TYPE>(c, pc.xp, pc.yp);
+					else if (g.render_fp64) acc += (float)(c.flux_scale * eval_at<double, TYPE>(c, pc.xp, pc.yp));
 					else acc += (float)c.flux_scale * eval_at<float, TYPE>(c, (float)pc.xp, (float)pc.yp);
 				}
 				else if (pc.cls == 3) {
@@ -240,7 +243,10 @@ __global__ __launch_bounds__(256) void k_render(const GalaxyDev *__restrict__ ga
 			append_entry(sub, lane, wq.light, wq.count, wq.lcap, QEntry{I.comp0 + k, pix});
 			append_entry(heavy, lane, wq.heavy, wq.count + 1, wq.hcap, QEntry{I.comp0 + k, pix});
 		}
-		if (inb) out[pix] = acc;
+		if (inb) {
+			if (g.canvas64 && !target) I.canvas64[pix] = acc64;
+			else out[pix] = acc;
+		}
 		if (inb && filter < 0 && I.has_aux) I.aux[pix] = aux_acc;
 	}
 }
@@ -289,17 +295,20 @@ __global__ __launch_bounds__(256) void k_subsample_nodes(const GalaxyDev *__rest
 				const int cell = k * 64 + lane;
 				bool fail = false;
 				if (cell < ncell && recurse) {
-					// the production decision path (FP32 test with the FP64 guard band)
+					// the production decision path (the walker's FP32 test at FP32
+					// cell positions, FP64 at libprofit's position when undecided)
 					const int ci = cell / res, cj = cell - ci * res;
-					const double cx = cell_centre(nx0, chx, ci), cy = cell_centre(ny0, chy, cj);
-					const double xr = cx - c.xc, yr = cy - c.yc;
-					const double xp = xr * c.cos_ang + yr * c.sin_ang;
-					const double yp = -xr * c.sin_over + yr * c.cos_over;
 					const double dyp = -xbin * c.sin_over + ybin * c.cos_over;
-					const float v32 = eval_at<float>(c, (float)xp, (float)yp);
-					double v64;
-					bool have64;
-					fail = refine_cell(c, xp, yp, dyp, true, v32, &v64, &have64);
+					float xp, yp;
+					cell_pos32(c, (float)(nx0 - c.xc), (float)(ny0 - c.yc), (float)xbin, (float)ybin, ci, cj, &xp, &yp);
+					const float v32 = eval_at<float>(c, xp, yp);
+					int dec = refine_cell32(c, xp, yp, (float)dyp, v32);
+					if (dec == 2) {
+						double xp64, yp64, v64;
+						cell_pos64(c, nx0, ny0, chx, chy, ci, cj, &xp64, &yp64);
+						dec = refine_cell64(c, xp64, yp64, dyp, &v64) ? 1 : 0;
+					}
+					fail = dec == 1;
 				}
 				m[k] = __ballot(fail);
 			}
@@ -465,7 +474,10 @@ __global__ __launch_bounds__(256) void k_conv(const GalaxyDev *__restrict__ gals
 					// convolution output there); such pixels do not feed the adjoint
 					const long q = (long)(oyy + g.pady) * g.DW + oxx + g.padx;
 					const bool in_calc = !g.has_calc || g.calc[q];
-					const double m = in_calc ? acc[r] + (I.has_aux ? (double)I.aux[q] : 0.0) : 0.0;
+					// an FP64 canvas meets k_conv only through the unit kernel of a
+					// galaxy without a PSF (its PSF galaxies take the FFT)
+					const double conv = g.canvas64 ? I.canvas64[q] : acc[r];
+					const double m = in_calc ? conv + (I.has_aux ? (double)I.aux[q] : 0.0) : 0.0;
 					const double d = g.data[o];
 					if (pois) {
 						const double mm = s * m, dd = s * d;
@@ -499,6 +511,7 @@ __global__ __launch_bounds__(256) void k_conv(const GalaxyDev *__restrict__ gals
 			if (oxx < OW && oyy < OH) {
 				const long o = (long)oyy * OW + oxx;
 				float v = (float)acc[r];
+				if (MODE == CONV_MODEL && g.canvas64) v = (float)I.canvas64[o];
 				if (MODE == CONV_MODEL && g.has_calc && !g.calc[o]) v = 0.f;
 				if (MODE == CONV_MODEL && I.has_aux) v += I.aux[o];
 				out[o] = v;

@@ -8,8 +8,11 @@
 #include "profit.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
+#include <list>
+#include <mutex>
 #include <sstream>
 
 #include "engine.h"
@@ -294,34 +297,81 @@ void Profile::validate() const
 
 // ------------------------------------------------------------------ Convolver
 
-PointPair Convolver::padding(const Dimensions &, const Dimensions &) const { return {}; }
-
-Image Convolver::convolve_impl(const Image &src, const Image &krn, const Mask &mask)
+// FFTConvolver::offset_after_convolution (convolve.cpp:269-281): where the
+// source starts within the 2 * max(src, krn) frame of the result
+static Point fft_offset_after_convolution(const Dimensions &sd, const Dimensions &kd)
 {
-	Image out(src.getDimensions());
-	profit_amd::convolve_image(src.data(), (int)src.getWidth(), (int)src.getHeight(), krn.data(), (int)krn.getWidth(),
-	                           (int)krn.getHeight(), mask ? mask.bytes() : nullptr, out.data(), device_,
-	                           type_ == FFT ? profit_amd::CK_FFT : type_ == BRUTE_BF16X3 ? profit_amd::CK_MFMA : profit_amd::CK_VALU);
+	Point off = sd / 2;
+	if (sd.x % 2 == 0 || kd.x % 2 == 0) off.x -= 1;
+	if (sd.y % 2 == 0 || kd.y % 2 == 0) off.y -= 1;
+	return off;
+}
+
+PointPair Convolver::padding(const Dimensions &sd, const Dimensions &kd) const
+{
+	if (type_ != FFT) return {};
+	// FFTConvolver::padding (convolve.cpp:262-267)
+	const Dimensions ext = max(sd, kd) * 2;
+	const Point off = fft_offset_after_convolution(sd, kd);
+	return {off, ext - ext / 2 - off};
+}
+
+static int kernel_method(ConvolverType t)
+{
+	return t == FFT ? profit_amd::CK_FFT : t == BRUTE_BF16X3 ? profit_amd::CK_MFMA : profit_amd::CK_VALU;
+}
+
+Image Convolver::convolve_impl(const Image &src, const Image &krn, const Mask &mask, bool crop, Point &offset_out)
+{
+	if (type_ == NO_OP) return src;  // NullConvolver (convolve.cpp:521-524)
+	const Dimensions sd = src.getDimensions(), kd = krn.getDimensions();
+	if (type_ != FFT || crop) {
+		Image out(sd);
+		profit_amd::convolve_image(src.data(), (int)sd.x, (int)sd.y, krn.data(), (int)kd.x, (int)kd.y,
+		                           mask ? mask.bytes() : nullptr, out.data(), device_, kernel_method(type_));
+		return out;
+	}
+	// FFTConvolver with crop = false (convolve.cpp:283-311): the whole frame of
+	// 2 * max(src, krn), holding the linear convolution with the kernel placed at
+	// (src - krn) / 2, masked by the mask extended to the frame at the offset.
+	// The frame is the GPU convolution of the source placed at s = k0 + c in a
+	// zero image (c = kw - 1 - kw/2, the centre the same-size sum uses), whose
+	// same-size result at X is sum_i src(i) krn(X - k0 - i).
+	const Dimensions ext = max(sd, kd) * 2;
+	if (!(ext >= sd)) throw std::invalid_argument("extension not possible");
+	const Point k0 = (sd - kd) / 2;
+	const Point s{k0.x + (kd.x - 1 - kd.x / 2), k0.y + (kd.y - 1 - kd.y / 2)};
+	const Dimensions work{std::max(ext.x, s.x + sd.x), std::max(ext.y, s.y + sd.y)};
+	Image placed = src.extend(work, s);
+	Image conv(work);
+	profit_amd::convolve_image(placed.data(), (int)work.x, (int)work.y, krn.data(), (int)kd.x, (int)kd.y, nullptr,
+	                           conv.data(), device_, profit_amd::CK_FFT);
+	Image out = conv.crop(ext, {0, 0});
+	const Point off = fft_offset_after_convolution(sd, kd);
+	if (mask) {
+		const Mask em = mask.extend(ext, off);
+		for (size_t k = 0; k < out.size(); ++k)
+			if (!em[k]) out[k] = 0;
+	}
+	if (&offset_out != &NO_OFFSET) offset_out = off;
 	return out;
 }
 
-Image Convolver::convolve(const Image &src, const Image &krn, const Mask &mask, bool, Point &)
+Image Convolver::convolve(const Image &src, const Image &krn, const Mask &mask, bool crop, Point &offset_out)
 {
-	if (type_ == NO_OP) return src;
 	const Dimensions sd = src.getDimensions(), kd = krn.getDimensions();
 	if (!(sd >= kd)) {
-		// libprofit's max() takes the x extent for both axes (coordinates.h:188-192)
-		const unsigned m = std::max(sd.x, kd.x);
-		const Dimensions nd{m, m};
+		// convolve.cpp:58-74: extend to max(src, krn), whose x extent serves both axes
+		const Dimensions nd = max(sd, kd);
 		if (nd.x < sd.x || nd.y < sd.y) throw std::invalid_argument("extension not possible");
 		const Dimensions diff = nd - sd;
 		Image es = src.extend(nd, {0, 0});
 		Mask em;
 		if (mask) em = mask.extend(nd, {0, 0});
-		Image r = convolve_impl(es, krn, em);
+		Image r = convolve_impl(es, krn, em, crop, offset_out);
 		return r.crop(r.getDimensions() - diff, {0, 0});
 	}
-	return convolve_impl(src, krn, mask);
+	return convolve_impl(src, krn, mask, crop, offset_out);
 }
 
 ConvolverPtr create_convolver(ConvolverType type, const ConvolverCreationPreferences &prefs)
@@ -411,6 +461,39 @@ Model::Analysis Model::analyze_inputs() const
 
 Dimensions Model::get_drawing_dimensions() const { return analyze_inputs().drawing_dims; }
 
+// The dimensions of the image evaluate() returns: the drawing dimensions,
+// then as evaluate changes them (model.cpp:282-329): the FFT convolver's
+// uncropped frame, the crop of the PSF padding and the downsampling.
+Dimensions Model::get_output_dimensions() const
+{
+	const Analysis a = analyze_inputs();
+	Dimensions d = a.drawing_dims;
+	if (dry_run_) return d;
+	const ConvolverPtr convolver = convolver_ ? convolver_ : create_convolver(BRUTE);
+	const Dimensions kd = psf_.getDimensions();
+	if (a.convolution_required && convolver->type() == FFT && !crop_) {
+		Dimensions src = d, diff{0, 0};
+		if (!(d >= kd)) {
+			src = max(d, kd);
+			diff = src - d;
+		}
+		d = max(src, kd) * 2 - diff;
+	}
+	if (a.psf_padding) {
+		Dimensions crop_dims = a.drawing_dims - a.psf_padding * 2;
+		if (!crop_) {
+			const PointPair actual = convolver->padding(a.drawing_dims, kd);
+			const PointPair intended = convolver->padding(a.drawing_dims - a.psf_padding * 2, kd);
+			crop_dims = d - a.psf_padding * 2;
+			crop_dims -= (actual.first - intended.first) + (actual.second - intended.second);
+		}
+		d = crop_dims;
+	}
+	if (finesampling_ > 1 && !return_finesampled_)
+		d = Dimensions{(d.x + finesampling_ - 1) / finesampling_, (d.y + finesampling_ - 1) / finesampling_};
+	return d;
+}
+
 // model.cpp:230-246
 void Model::adjust_impl(Mask &mask, const Image &psf, unsigned int fs, const Analysis &a)
 {
@@ -433,6 +516,103 @@ Image Model::evaluate(Point &offset_out)
 	Image image;
 	evaluate(image, offset_out);
 	return image;
+}
+
+// ------------------------------------------------------------------ engine cache
+//
+// A Model's drawing set-up (dimensions, PSF, calc region, convolution kernel)
+// becomes one engine galaxy: its device buffers, tile lists, FFT spectrum and
+// work lists.  Repeated evaluations with the same set-up (the parameters
+// change, the geometry does not) reuse that engine instead of building it
+// again; a few set-ups are kept, the least recently used dropped first.
+// clear_cache() releases them (R_profit_clear_cache, src/r_profit.cpp:822-825).
+
+namespace {
+
+struct CachedEngine {
+	uint64_t key = 0;
+	std::unique_ptr<profit_amd::Engine> engine;
+	int ncomp = -1;
+};
+
+std::mutex g_cache_mu;
+std::list<CachedEngine> g_cache;  // most recently used first
+constexpr size_t kCacheEntries = 4;
+
+struct Fnv {
+	uint64_t h = 1469598103934665603ull;
+	void add(const void *p, size_t n)
+	{
+		const unsigned char *b = static_cast<const unsigned char *>(p);
+		for (size_t k = 0; k < n; ++k) h = (h ^ b[k]) * 1099511628211ull;
+	}
+	template <class T> void add(const T &v) { add(&v, sizeof(T)); }
+};
+
+uint64_t spec_key(const profit_amd::GalaxySpec &s, int method, int device)
+{
+	Fnv f;
+	f.add(s.W);
+	f.add(s.H);
+	f.add(s.scale_x);
+	f.add(s.scale_y);
+	f.add(s.magzero);
+	f.add(s.finesampling);
+	f.add(s.psf_convolves);
+	f.add(s.psf_scale_x);
+	f.add(s.psf_scale_y);
+	f.add(s.Kx);
+	f.add(s.Ky);
+	if (s.psf) f.add(s.psf, sizeof(double) * (size_t)s.Kx * s.Ky);
+	f.add(s.cw);
+	f.add(s.ch);
+	f.add(s.calc != nullptr);
+	if (s.calc) f.add(s.calc, (size_t)s.cw * s.ch);
+	f.add(method);
+	f.add(device);
+	return f.h;
+}
+
+// the cached engine of a set-up (built on first use), moved to the front;
+// the caller holds g_cache_mu
+profit_amd::Engine &cached_engine(const profit_amd::GalaxySpec &gs, int method, int device, int ncomp)
+{
+	const uint64_t key = spec_key(gs, method, device);
+	for (auto it = g_cache.begin(); it != g_cache.end(); ++it)
+		if (it->key == key) {
+			g_cache.splice(g_cache.begin(), g_cache, it);
+			break;
+		}
+	if (g_cache.empty() || g_cache.front().key != key) {
+		CachedEngine ce;
+		ce.key = key;
+		ce.engine.reset(new profit_amd::Engine(device));
+		ce.engine->set_conv_method(method);
+		ce.engine->add_galaxy(gs);
+		g_cache.push_front(std::move(ce));
+		while (g_cache.size() > kCacheEntries) g_cache.pop_back();
+	}
+	CachedEngine &ce = g_cache.front();
+	if (ce.ncomp != ncomp) {
+		ce.engine->set_batch({0}, {ncomp});
+		ce.ncomp = ncomp;
+	}
+	return *ce.engine;
+}
+
+}  // namespace
+
+void clear_cache()
+{
+	std::lock_guard<std::mutex> lock(g_cache_mu);
+	g_cache.clear();
+}
+
+std::map<std::string, std::shared_ptr<ProfileStats>> Model::get_stats() const
+{
+	std::map<std::string, std::shared_ptr<ProfileStats>> out;
+	for (auto &p : profiles_) out[p->get_name()] = p->stats();
+	return out;
 }
 
 // model.cpp:271-380
@@ -470,7 +650,16 @@ void Model::evaluate(Image &image, Point &offset_out)
 		drawn.push_back(p->params());
 		any_point |= p->kind() == Profile::POINT;
 	}
-	if (!drawn.empty()) {
+	// ensure_convolver (model.cpp:144-150): brute force unless one was set
+	const ConvolverPtr convolver = convolver_ ? convolver_ : create_convolver(BRUTE);
+	const ConvolverType ctype = convolver->type();
+	// produce_image step by step (the two canvases come back unconvolved, then
+	// the convolver runs) where its result is not the engine's fused
+	// convolution: the null convolver, and the FFT convolver's uncropped frame
+	const bool literal = a.convolution_required && (ctype == NO_OP || (ctype == FFT && !crop_));
+	Point offset{0, 0};
+	RadialProfileStats times;
+	if (!drawn.empty() || literal) {
 		profit_amd::GalaxySpec gs;
 		gs.W = (int)(requested_dimensions_.x * finesampling_);
 		gs.H = (int)(requested_dimensions_.y * finesampling_);
@@ -491,27 +680,56 @@ void Model::evaluate(Image &image, Point &offset_out)
 			gs.cw = (int)draw_mask->getWidth();
 			gs.ch = (int)draw_mask->getHeight();
 		}
-		profit_amd::Engine eng(device_);
-		if (convolver_ && convolver_->type() == FFT) eng.set_conv_method(profit_amd::CK_FFT);
-		if (convolver_ && convolver_->type() == BRUTE_BF16X3) eng.set_conv_method(profit_amd::CK_MFMA);
-		eng.add_galaxy(gs);
-		eng.set_batch({0}, {(int)drawn.size()});
-		const profit_amd::GalaxyDev &g = eng.galaxy(0);
-		if ((unsigned)g.DW != a.drawing_dims.x || (unsigned)g.DH != a.drawing_dims.y)
-			throw std::logic_error("engine drawing dimensions differ from the model analysis");
-		std::vector<float> canvas((size_t)g.DW * g.DH);
-		try {
-			eng.model_canvas(drawn.data(), canvas.data());
-		} catch (const std::invalid_argument &e) {
-			throw invalid_parameter(e.what());
+		const size_t nd = (size_t)a.drawing_dims.x * a.drawing_dims.y;
+		std::vector<float> canvas(nd, 0.f), direct(literal ? nd : 0, 0.f);
+		if (!drawn.empty()) {
+			const int method = literal ? profit_amd::CK_VALU : kernel_method(ctype);
+			std::lock_guard<std::mutex> lock(g_cache_mu);
+			profit_amd::Engine &eng = cached_engine(gs, method, device_, (int)drawn.size());
+			const profit_amd::GalaxyDev &g = eng.galaxy(0);
+			if ((unsigned)g.DW != a.drawing_dims.x || (unsigned)g.DH != a.drawing_dims.y)
+				throw std::logic_error("engine drawing dimensions differ from the model analysis");
+			const auto t0 = std::chrono::steady_clock::now();
+			try {
+				eng.model_canvas(drawn.data(), canvas.data(), literal ? direct.data() : nullptr);
+			} catch (const std::invalid_argument &e) {
+				throw invalid_parameter(e.what());
+			}
+			times.total = (nsecs_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+			                  std::chrono::steady_clock::now() - t0)
+			                  .count();
 		}
-		for (size_t k = 0; k < canvas.size(); ++k) image[k] += canvas[k];
+		if (!literal) {
+			for (size_t k = 0; k < nd; ++k) image[k] += canvas[k];
+		}
+		else {
+			// model.cpp:355-377: the unconvolved profiles on the canvas, then
+			// convolve, extend the canvas to the result's frame and add
+			for (size_t k = 0; k < nd; ++k) image[k] += direct[k];
+			Image to_convolve(a.drawing_dims);
+			for (size_t k = 0; k < nd; ++k) to_convolve[k] = canvas[k];
+			const Image conv = convolver->convolve(to_convolve, psf_, *draw_mask, crop_, offset);
+			if (conv.getDimensions() != a.drawing_dims) image = image.extend(conv.getDimensions(), offset);
+			for (size_t k = 0; k < image.size(); ++k) image[k] += conv[k];
+		}
 	}
+	for (auto &p : profiles_) p->set_stats(times);
 
-	Point offset{0, 0};
 	if (a.psf_padding) {
-		const Dimensions crop_dims = a.drawing_dims - a.psf_padding * 2;
-		image = image.crop(crop_dims, a.psf_padding);
+		// model.cpp:306-324
+		Point crop_offset = a.psf_padding;
+		Dimensions crop_dims = a.drawing_dims - a.psf_padding * 2;
+		if (!crop_) {
+			const PointPair actual = convolver->padding(a.drawing_dims, psf_.getDimensions());
+			const PointPair intended = convolver->padding(a.drawing_dims - a.psf_padding * 2, psf_.getDimensions());
+			const Point offset_diff = actual.first - intended.first;
+			const Dimensions dim_diff = actual.second - intended.second;
+			crop_dims = image.getDimensions() - a.psf_padding * 2;
+			crop_dims -= offset_diff + dim_diff;
+			crop_offset += offset_diff;
+			offset -= offset_diff;
+		}
+		image = image.crop(crop_dims, crop_offset);
 	}
 	if (finesampling_ > 1 && !return_finesampled_) {
 		image = image.downsample_sum(finesampling_);
@@ -530,7 +748,10 @@ void Model::evaluate(Image &image, Point &offset_out)
 void init() {}
 void finish() {}
 bool has_openmp() { return false; }
-bool has_fftw() { return false; }
+// libprofit answers whether it was built with FFTW, i.e. whether its FFT
+// convolver exists (library.h:86-88); this one always has an FFT convolver
+// (its own GPU FFT, not FFTW)
+bool has_fftw() { return true; }
 bool has_opencl() { return false; }
 std::string version() { return "1.9.3"; }
 

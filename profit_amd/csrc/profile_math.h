@@ -96,7 +96,7 @@ struct Comp {
 	int32_t type, kind, res, maxrec;
 	int32_t rough, centre_rule, boxy, has_rmax;
 	int32_t convolve, instance, grad_ok, adjust;
-	int32_t grad64;  // per-sample adjoint derivatives in FP64 (steep cores)
+	int32_t grad64;  // steep core: the direct-pixel adjoint sums its FP32 per-pixel derivatives in FP64
 	int32_t grad_box;  // the box derivative is wanted (it costs logarithms per sample)
 	double xc, yc;                    // centre after the PSF-padding shift
 	double cos_ang, sin_ang, cos_over, sin_over;
@@ -583,8 +583,8 @@ struct BrokenExpIntegrand {
 PA_HD bool almost_equal(double x, double y) { return ::fabs(x - y) < 1e-10; }
 
 // steep cores: the centroid derivatives of a cusp (core-Sersic with an inner
-// slope, Sersic n > 4) cancel over many large samples, so their per-sample
-// adjoint derivatives are taken in FP64
+// slope, Sersic n > 4) cancel over many large samples, so the direct-pixel
+// adjoint sums their FP32 per-pixel derivatives in FP64 per thread
 PA_HD bool wants_grad64(const ProfileParams &p)
 {
 	return (p.type == CORESERSIC && p.shape[4] != 0) || (p.type == SERSIC && p.shape[1] > 4);
@@ -636,7 +636,7 @@ PA_HD int setup_component(const ProfileParams &p, double magzero, double sx, dou
 	c->axrat = p.axrat;
 	c->img_sx = img_sx;
 	c->img_sy = img_sy;
-	c->grad_ok = !(p.type == SERSIC && p.rescale_flux);
+	c->grad_ok = 1;
 	c->grad64 = wants_grad64(p);
 	c->grad_box = 1;
 	for (int i = 0; i < 5; ++i) c->s[i] = s[i];
@@ -801,6 +801,20 @@ PA_HD int setup_component(const ProfileParams &p, double magzero, double sx, dou
 		const double dlnL = 1 / n + 2 * sf::digamma(2 * n) + c->dbn - 2 * ::log(bn) - 2 * n * c->dbn / bn;
 		c->dlnfs[G_S0] = -2 / s[0];
 		c->dlnfs[G_S1] = -dlnL;
+		if (p.rescale_flux && p.adjust) {
+			// rescale = 1 / P(x, 2n), x = b_n (rmax / re)^(1/n) (sersic.cpp:303-311),
+			// with the adjusted rmax held fixed (it is piecewise constant):
+			// d ln rescale = -(P'(x) dx + 2 dP/da dn) / P
+			const double x = bn * ::pow(rmax / s[0], 1 / n);
+			const double P = sf::pgamma(x, a);
+			if (x > 0 && P > 0) {
+				const double dens_x = ::exp((a - 1) * ::log(x) - x - sf::lgam(a));
+				const double dx_dre = -x / (n * s[0]);
+				const double dx_dn = x * (c->dbn / bn - ::log(rmax / s[0]) / (n * n));
+				c->dlnfs[G_S0] += -dens_x * dx_dre / P;
+				c->dlnfs[G_S1] += -(dens_x * dx_dn + 2 * sf::dpgamma_da(x, a)) / P;
+			}
+		}
 		break;
 	}
 	case MOFFAT: {

@@ -57,7 +57,15 @@ struct Dimensions {
 	Dimensions operator/(unsigned int f) const { return {x / f, y / f}; }
 	Dimensions operator+(const Dimensions &o) const { return {x + o.x, y + o.y}; }
 	Dimensions operator-(const Dimensions &o) const { return {x - o.x, y - o.y}; }
+	Dimensions &operator+=(const Dimensions &o) { return *this = *this + o; }
+	Dimensions &operator-=(const Dimensions &o) { return *this = *this - o; }
 };
+// coordinates.h:182-186: libprofit's max() takes the x extent for both axes
+inline Dimensions max(const Dimensions &a, const Dimensions &b)
+{
+	const unsigned int m = a.x > b.x ? a.x : b.x;
+	return {m, m};
+}
 using Point = Dimensions;
 using PointPair = std::pair<Point, Point>;
 
@@ -121,6 +129,24 @@ class Mask {
 
 class Model;
 
+// profile.h:47-74: evaluation statistics.  libprofit fills them per profile;
+// here every profile of a Model is drawn in one pass of the kernels, so each
+// profile reports that pass's wall time (render, sub-pixel walk and, for the
+// profiles that convolve, the convolution).
+using nsecs_t = uint64_t;
+class ProfileStats {
+  public:
+	virtual ~ProfileStats() = default;
+	nsecs_t total = 0;
+};
+struct radial_subsampling_stats {
+	nsecs_t pre_subsampling = 0, new_subsampling = 0, inital_transform = 0, final_transform = 0, total = 0;
+};
+struct RadialProfileStats : ProfileStats {
+	radial_subsampling_stats subsampling;
+	nsecs_t final_image = 0;
+};
+
 class Profile {
   public:
 	Profile(const std::string &name, int kind, int type);
@@ -132,6 +158,17 @@ class Profile {
 	void parameter(const std::string &spec);  // "name = value"
 	bool do_convolve() const;
 	void validate() const;
+	// Profile::get_stats (profile.h:198): the last evaluation's statistics
+	std::shared_ptr<ProfileStats> get_stats() const { return stats(); }
+	std::shared_ptr<ProfileStats> stats() const { return stats_; }
+	void set_stats(const RadialProfileStats &s)
+	{
+		if (kind_ == RADIAL) stats_ = std::make_shared<RadialProfileStats>(s);
+		else {
+			stats_ = std::make_shared<ProfileStats>();
+			stats_->total = s.total;
+		}
+	}
 
 	// engine-facing state
 	enum Kind { RADIAL = 0, SKY = 1, NULLPROF = 2, POINT = 3 };
@@ -148,6 +185,7 @@ class Profile {
 	int kind_;
 	profit_amd::ProfileParams params_{};
 	bool convolve_ = false;
+	std::shared_ptr<ProfileStats> stats_;
 };
 using ProfilePtr = std::shared_ptr<Profile>;
 
@@ -171,11 +209,15 @@ class Convolver {
 	// first and cropped back afterwards; masked output pixels are zero.
 	Image convolve(const Image &src, const Image &krn, const Mask &mask, bool crop = true,
 	               Point &offset_out = NO_OFFSET);
+	// Extra padding of an uncropped result (convolve.cpp:50-53, 262-267): none,
+	// except for the FFT convolver's 2 * max(src, krn) frame
 	virtual PointPair padding(const Dimensions &src_dims, const Dimensions &krn_dims) const;
 	ConvolverType type() const { return type_; }
 
   protected:
-	Image convolve_impl(const Image &src, const Image &krn, const Mask &mask);
+	// convolve_impl of each convolver; FFT with crop = false returns the whole
+	// 2 * max(src, krn) frame and its offset (convolve.cpp:283-311)
+	Image convolve_impl(const Image &src, const Image &krn, const Mask &mask, bool crop, Point &offset_out);
 	ConvolverType type_;
 	int device_;
 };
@@ -198,6 +240,12 @@ class Model {
 	Image evaluate(Point &offset_out = NO_OFFSET);
 	void evaluate(Image &image, Point &offset_out = NO_OFFSET);
 	Dimensions get_drawing_dimensions() const;
+	// dimensions of the image evaluate() returns (after the crop, an uncropped
+	// FFT frame and the downsampling); no counterpart in libprofit, whose
+	// evaluate allocates its result
+	Dimensions get_output_dimensions() const;
+	// Model::get_stats (model.cpp:382-388): per profile name
+	std::map<std::string, std::shared_ptr<ProfileStats>> get_stats() const;
 	static void adjust(Mask &mask, const Dimensions &dims, const Image &psf, unsigned int finesampling);
 
 	void set_dimensions(const Dimensions &d) { requested_dimensions_ = d; }
@@ -247,6 +295,9 @@ class Model {
 
 void init();
 void finish();
+// drops the device set-ups that repeated Model::evaluate calls reuse
+// (the role of R_profit_clear_cache, src/r_profit.cpp:822-825)
+void clear_cache();
 bool has_openmp();
 bool has_fftw();
 bool has_opencl();

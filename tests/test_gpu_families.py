@@ -2,7 +2,9 @@
 compiled for that family alone (k_render<TYPE>), which only a like/like_grad
 evaluation uses.  Each family's batch of two instances is checked against the
 FP64 oracle: logL within 1e-3 and the gradient finite and the same for the
-two identical instances (up to the order of the atomic sums)."""
+two identical instances up to the grouping of the adjoint walker's FP32
+partial sums (per grab of queue entries, whose boundaries differ between
+the two instances) and the order of the atomic sums: 1e-6 relative."""
 import numpy as np
 import pytest
 
@@ -44,4 +46,4 @@ def test_single_family_batch(kind):
     print(f"[family] {kind}: logL {ll[0]:.6f} / {ll2[0]:.6f} vs oracle {oll:.6f}", flush=True)
     assert abs(ll[0] - oll) <= 1e-3 and abs(ll[1] - oll) <= 1e-3
     assert abs(ll2[0] - oll) <= 1e-3
-    assert np.all(np.isfinite(grad)) and np.allclose(grad[0], grad[1], rtol=1e-9, atol=1e-12 * np.abs(grad).max())
+    assert np.all(np.isfinite(grad)) and np.allclose(grad[0], grad[1], rtol=1e-6, atol=1e-9 * np.abs(grad).max())

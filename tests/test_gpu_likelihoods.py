@@ -93,7 +93,9 @@ def test_t_data_keep_dof_unclamped(galaxies):
 
 def test_gradient_slot_mask(galaxies):
     """Slots left out of the mask come back as 0 and the others are unchanged
-    (leaving out the box slot skips its per-sample logarithms)."""
+    (leaving out the box slot skips its per-sample logarithms), up to the
+    grouping of the adjoint walker's FP32 partial sums, which follows the
+    sub-pixel queue's order (appended by atomics): 1e-6 relative."""
     g = galaxies["chisq"]
     eng = P.Engine()
     gid = eng.add_galaxy(SIZE, SIZE, g["data"], g["sigma"], psf=g["psf"], calcregion=g["calc"], region=g["region"],
@@ -103,10 +105,10 @@ def test_gradient_slot_mask(galaxies):
     ll, full = eng.like_grad(params)
     eng.set_grad_slots(["xcen", "ycen", "mag", "ang", "axrat", "s0", "s1"])
     ll2, part = eng.like_grad(params)
-    assert ll2[0] == ll[0]
+    assert abs(ll2[0] - ll[0]) <= 1e-9 * abs(ll[0])
     keep = [0, 1, 2, 3, 4, 6, 7]
     assert np.all(part[:, [5, 8, 9, 10]] == 0)
-    assert np.allclose(part[:, keep], full[:, keep], rtol=1e-9, atol=0)
+    assert np.allclose(part[:, keep], full[:, keep], rtol=1e-6, atol=0)
     eng.set_grad_slots(None)
     _, again = eng.like_grad(params)
-    assert np.allclose(again, full, rtol=1e-9, atol=0)
+    assert np.allclose(again, full, rtol=1e-6, atol=0)
