@@ -26,6 +26,7 @@ differentiated by central differences in parm (host arithmetic only).
 from __future__ import annotations
 
 import math
+import os
 import re
 
 import numpy as np
@@ -355,23 +356,32 @@ class FitData:
         out = ll + lp
         return out if np.ndim(parm) == 2 else float(out[0])
 
-    def like_grad(self, parm):
-        """(LP, d LP / d parm) for one parameter vector or a batch (rows)."""
+    def prepare(self, parm):
+        """profitRemakeModellist on each row of parm: (rows, remakes, the
+        evaluator's component lists)."""
         if self.constraints is not None:
             raise NotImplementedError("a constraints function has no derivative")
         parms = np.atleast_2d(np.asarray(parm, dtype=np.float64))
         rem = [self.remake(p) for p in parms]
-        flats = [r[1] for r in rem]
-        ll, g = self.evaluator.like_grad([self.component_params(f) for f in flats])
+        return parms, rem, [self.component_params(r[1]) for r in rem]
+
+    def finish_grad(self, parms, rem, ll, g):
+        """LP = LL + priors and d LP / d parm from the evaluator's logL and
+        per-component slot gradients of prepared rows."""
         ll = np.asarray(ll, dtype=np.float64)
         g = np.asarray(g, dtype=np.float64).reshape(len(parms), len(self.components), -1)
-        lp, gp = self._prior_terms(parms, flats, True)
+        lp, gp = self._prior_terms(parms, [r[1] for r in rem], True)
         dparm = np.zeros((len(parms), len(self.fit_ids)))
         for b, (_, _, J) in enumerate(rem):
             gflat = np.array([sum(g[b, ci, s] for ci, s in self.flat_map[i]) for i in range(len(self.names))])
             dparm[b] = gflat @ J
-        dparm += gp
-        out = ll + lp
+        return ll + lp, dparm + gp
+
+    def like_grad(self, parm):
+        """(LP, d LP / d parm) for one parameter vector or a batch (rows)."""
+        parms, rem, comps = self.prepare(parm)
+        ll, g = self.evaluator.like_grad(comps)
+        out, dparm = self.finish_grad(parms, rem, ll, g)
         if np.ndim(parm) == 2:
             return out, dparm
         return float(out[0]), dparm[0]
@@ -390,6 +400,98 @@ class FitData:
             return -lp, -g
         res = minimize(f, x0, jac=True, method=method, options={"maxiter": maxiter})
         return res
+
+
+class MultiFit:
+    """Several galaxies' FitData evaluated together: one engine holds every
+    galaxy and one call evaluates a parameter vector of each as one
+    heterogeneous batch (different image sizes, PSFs, likelihoods and
+    component counts)."""
+
+    def __init__(self, datas, device=0, conv_method=None, precision=0):
+        from . import Engine, adjust_mask
+        import profit_amd as P
+        self._P = P
+        self.datas = list(datas)
+        self.eng = Engine(device)
+        self.eng.set_render_fp64(precision)
+        if conv_method:
+            self.eng.set_conv_method(conv_method)
+        ids = []
+        for d in self.datas:
+            h, w = d.image.shape
+            calc = None
+            if d.psf is not None:
+                calc = adjust_mask(d.region.astype(np.uint8), (w, h), d.psf, d.finesample)
+            ids.append(self.eng.add_galaxy(w, h, d.image, d.sigma, psf=d.psf, calcregion=calc,
+                                           region=d.region.astype(np.uint8), likefunc=d.like_func,
+                                           magzero=d.magzero))
+        self.eng.set_batch(ids, [len(d.components) for d in self.datas])
+
+    def like_grad(self, parms):
+        """LP and d LP / d parm of each galaxy at its own parameter vector."""
+        prep = [d.prepare(p) for d, p in zip(self.datas, parms)]
+        profiles = [p for (_, _, comps) in prep for p in _profiles(self._P, comps[0])]
+        ll, g = self.eng.like_grad(profiles)
+        out, row = [], 0
+        for i, (d, (rows, rem, _)) in enumerate(zip(self.datas, prep)):
+            n = len(d.components)
+            lp, dp = d.finish_grad(rows, rem, ll[i:i + 1], g[row:row + n])
+            out.append((float(lp[0]), dp[0]))
+            row += n
+        return out
+
+
+def read_example_init(path):
+    """data/ExampleInit.tab: the example galaxies' initial bulge + disk
+    parameters, a whitespace table with a header of quoted column names;
+    returns {CATAID: {column: value}}."""
+    with open(path) as f:
+        header = [h.strip('"') for h in f.readline().split()]
+        rows = {}
+        for line in f:
+            v = line.split()
+            if not v:
+                continue
+            rec = dict(zip(header, (float(x) for x in v)))
+            rows[int(rec["CATAID"])] = rec
+    return rows
+
+
+def example_fit(directory, source, cataid, init, evaluator=None, device=0, conv_method=None):
+    """The reference's galaxy-fit example (vignettes/ProFit-Galaxy-Fit-Example.Rmd)
+    for one example galaxy: bulge + disk from the ExampleInit row, centred
+    on the image, sizes scaled by 0.2/0.339 for SDSS, tied centres, logged
+    sizes, Sersic index and axial ratio, its intervals and normal priors, the t
+    likelihood and magzero 0.  directory holds <source>/G<cataid>{fitim,sigma,
+    segim,psfim}.fits."""
+    from . import fits
+    base = os.path.join(directory, source, f"G{cataid}")
+    image = fits.read_image(base + "fitim.fits")
+    sigma = fits.read_image(base + "sigma.fits")
+    segim = fits.read_image(base + "segim.fits")
+    psf = fits.read_image(base + "psfim.fits")
+    row = init[int(cataid)]
+    ny, nx = image.shape
+    rs = 1.0 if source == "KiDS" else 0.2 / 0.339
+    modellist = {"sersic": {"xcen": [nx / 2, nx / 2], "ycen": [ny / 2, ny / 2],
+                            "mag": [row["sersic.mag1"], row["sersic.mag2"]],
+                            "re": [row["sersic.re1"] * rs, row["sersic.re2"] * rs],
+                            "nser": [row["sersic.nser1"], 1.0], "ang": [row["sersic.ang2"], row["sersic.ang2"]],
+                            "axrat": [1.0, row["sersic.axrat2"]], "box": [0.0, 0.0]}}
+    tofit = {"sersic": {"xcen": [True, None], "ycen": [True, None], "mag": [True, True], "re": [True, True],
+                        "nser": [True, False], "ang": [False, True], "axrat": [False, True], "box": [False, False]}}
+    tolog = {"sersic": {"xcen": [False, False], "ycen": [False, False], "mag": [False, False], "re": [True, True],
+                        "nser": [True, True], "ang": [False, False], "axrat": [True, True], "box": [False, False]}}
+    s = [2, 2, 5, 1, 1, 30, 0.3, math.inf]
+    sigmas = {"sersic": {k: [v, v] for k, v in zip(("xcen", "ycen", "mag", "re", "nser", "ang", "axrat", "box"), s)}}
+    priors = make_priors(modellist, sigmas, tolog, allowflat=True)
+    intervals = {"sersic": {"xcen": [(0, 300)] * 2, "ycen": [(0, 300)] * 2, "mag": [(10, 30)] * 2,
+                            "re": [(1, 100)] * 2, "nser": [(0.5, 20)] * 2, "ang": [(-180, 360)] * 2,
+                            "axrat": [(0.1, 1)] * 2, "box": [(-1, 1)] * 2}}
+    return FitData(image, modellist, sigma=sigma, segim=segim, psf=psf, tofit=tofit, tolog=tolog, priors=priors,
+                   intervals=intervals, magzero=0.0, like_func="t", evaluator=evaluator, device=device,
+                   conv_method=conv_method)
 
 
 def _profiles(module, comp_params):

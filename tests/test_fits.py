@@ -1,7 +1,6 @@
 """FITS reader (profit_amd.fits): round trips of every BITPIX with scaling
-and extensions, and the KiDS example cut-outs ProFit ships (one galaxy's set
-is a fixture under tests/golden/kids; the other example files are read from
-the reference tree when it is present)."""
+and extensions, and the example cut-outs ProFit ships (fixtures under
+tests/golden/examples: 10 KiDS and 10 SDSS galaxies)."""
 import glob
 import os
 
@@ -11,8 +10,8 @@ import pytest
 from profit_amd import fits
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-KIDS = os.path.join(HERE, "golden", "kids")
-REF = "/root/reference/inst/extdata"
+EXAMPLES = os.path.join(HERE, "golden", "examples")
+KIDS = os.path.join(EXAMPLES, "KiDS")
 
 
 @pytest.mark.parametrize("bitpix", [8, 16, 32, 64, -32, -64])
@@ -69,13 +68,10 @@ def test_kids_fixture():
     assert np.unravel_index(np.argmax(psf), psf.shape) == (12, 12)
 
 
-@pytest.mark.skipif(not os.path.isdir(REF), reason="reference tree absent")
 def test_every_example_image_reads():
-    paths = sorted(glob.glob(os.path.join(REF, "*", "*.fits")))
-    assert len(paths) > 100
+    paths = sorted(glob.glob(os.path.join(EXAMPLES, "*", "*.fits")))
+    assert len(paths) == 80
     for p in paths:
         hdus = fits.read_hdus(p)
-        n = sum(1 for _, d in hdus if d is not None)
-        if "Offset" not in p:
-            assert n == 1, p
-            assert np.all(np.isfinite(fits.read_image(p))), p
+        assert sum(1 for _, d in hdus if d is not None) == 1, p
+        assert np.all(np.isfinite(fits.read_image(p))), p
