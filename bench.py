@@ -4,15 +4,20 @@
 Workload (one "step"): a batch of B likelihood+gradient evaluations of the
 400x400 bulge+disk galaxy (Sersic n=4, re=3 + Sersic n=1, re=15) convolved
 with a 51x51 Moffat PSF (FWHM 4, con 3), chi^2 likelihood over an elliptical
-region of ~70% of the pixels, gradient w.r.t. the 8 parameters of each
-component.  The B parameter sets are walkers scattered around the truth.
+region of ~70% of the pixels, gradient w.r.t. the 7 parameters of each
+component that the metric names.  The B parameter sets are walkers scattered
+around the truth.
 
 Printed JSON line: value = batched-adjoint lik+grad evaluations/s; beside it
 the plain per-call GPU forward path with central finite differences (29
 forward likelihoods per lik+grad), the parity of the GPU result against the
 FP64 oracle, the roofline of the dominant kernel from HIP-event timings on the
-engine stream over the timed steps, and the FP64 CPU oracle (libprofit's CPU
-algorithm, OpenMP on the host cores) on a bounded sample.
+engine stream over the timed steps (every kernel of the step in
+roofline_by_kernel), likelihood-only and forward throughput on the other
+configurations, and the FP64 CPU oracle (libprofit's CPU algorithm) timed on
+the host cores, all of them and one, on a bounded sample.
+
+--workload c4 runs the 4096-galaxy heterogeneous batch instead.
 """
 from __future__ import annotations
 
@@ -34,6 +39,10 @@ PEAK_BF16_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
 # the 14-parameter gradient of the bulge+disk metric: 7 slots per component
 METRIC_SLOTS = ("xcen", "ycen", "mag", "ang", "axrat", "s0", "s1")
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def walkers(base, n, seed=11):
@@ -84,9 +93,21 @@ def fd_gradient_gpu(eng, P, comps, slots=("xcen", "ycen", "mag", "re", "nser", "
     return ll0, np.array(grads)
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(wl, calc, seconds, threads):
     """FP64 oracle: lik+grad by central differences (29 likelihoods), timed on
-    the host cores; plus its model, logL and analytic gradient for parity."""
+    `threads` host cores for about `seconds`; plus its model, logL, chi^2 and
+    analytic gradient for the parity report."""
     import oracle as O
     from tests.helpers import oracle_profiles
     spec = O.ModelSpec(400, 400, psf=wl["psf"], mask=calc, adjust_mask=False, magzero=wl["magzero"],
@@ -108,21 +129,192 @@ def cpu_baseline(wl, calc, seconds, threads):
     return {"value": n / el, "elapsed": el, "n": n, "ll": ll, "model": model, "grad": grad}
 
 
+def calibrate_copy():
+    """Copies 256 MiB device to device three times with a plain elementwise
+    kernel (torch's out = 1.0 * in over FP64: each launch reads and writes
+    256 MiB once): the known traffic that scales the FETCH_SIZE / WRITE_SIZE
+    counters."""
+    import torch
+    n = 32 * 1024 * 1024  # FP64 elements
+    src = torch.zeros(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_kernel": "torch elementwise mul (FP64)", "copy_bytes_read_per_launch": n * 8,
+                      "copy_bytes_written_per_launch": n * 8}))
+
+
+def sample_counts(eng1, params1, geometry, comps):
+    """Profile evaluations per instance at these parameters: the directly
+    evaluated pixels (k_render) and the cell values of the sub-pixel trees
+    (res^2 per tree node; resolution 8 at the Sersic centre pixels)."""
+    cls, nodes = eng1.diagnostics(params1)
+    adj = eng1.adjusted_settings(params1)
+    _, _, px, py = geometry
+    direct = int((cls == 2).sum())
+    cells = 0
+    for k, c in enumerate(comps):
+        sub = cls[k] == 3
+        res = np.full(cls[k].shape, int(adj["resolution"][k]))
+        if c["type"] == "sersic" and c["nser"] > 1:
+            jj, ii = np.nonzero(sub)
+            near = (np.abs(ii + 0.5 - (c["xcen"] + px)) < 1) & (np.abs(jj + 0.5 - (c["ycen"] + py)) < 1)
+            res[jj[near], ii[near]] = 8
+        cells += int((nodes[k][sub].astype(np.int64) * res[sub] ** 2).sum())
+    return direct, cells
+
+
+def fft_work(dw, dh, w, h, n1, n2):
+    """FLOPs each FFT kernel executes per instance (5 N log2 N per complex
+    transform of length N; two real rows per complex transform; the spectrum
+    product 6 FLOPs per entry), likelihood stage then adjoint stage, and the
+    stages' algorithmic HBM bytes (likelihood: canvas 4 B per canvas pixel and
+    data, 1/sigma^2, region and w: 13 B per image pixel; adjoint: w in and G
+    out, 4 B each)."""
+    row = 5.0 * n1 * np.log2(n1)
+    col = (n1 // 2 + 1) * (2 * 5.0 * n2 * np.log2(n2) + 6.0 * n2)
+    pairs = lambda r: (r + 1) // 2
+    flops = {"like_rows_fwd": pairs(dh) * row, "like_cols": col, "like_rows_inv": pairs(h) * row,
+             "adj_rows_fwd": pairs(h) * row, "adj_cols": col, "adj_rows_inv": pairs(dh) * row}
+    stage_bytes = {"conv_like": 4.0 * dw * dh + 13.0 * w * h, "adjoint_conv": 4.0 * w * h + 4.0 * dw * dh}
+    return flops, stage_bytes
+
+
+def time_calls(fn, reps):
+    fn()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    return reps / (time.perf_counter() - t0)
+
+
+def other_configs(P, W, reps):
+    """Likelihood-only (forward + likelihood) and lik+grad throughput on
+    configurations 1 and 5, and the per-call model image of Model::evaluate
+    (profit_amd_make_model, host arrays in and out, as R's profitMakeModel
+    calls it) on configurations 1 and 2."""
+    out = {}
+
+    def model_fn(w, h, psf, comps, magzero):
+        return P.make_model(w, h, gpu_params(P, [comps]), psf=psf, magzero=magzero)["image"]
+
+    # config 1: 100x100 Sersic, 25x25 Gaussian PSF, brute force, chi^2; a batch of 1024
+    wl1 = W.config1(model_fn, magzero=30.0)
+    eng = P.Engine()
+    g = eng.add_galaxy(100, 100, wl1["data"], wl1["sigma"], psf=wl1["psf"], region=wl1["region"], magzero=30.0)
+    eng.set_conv_method("brute")
+    eng.set_grad_slots(("xcen", "ycen", "mag", "ang", "axrat", "s0", "s1"))
+    B1 = 1024
+    eng.set_batch([g] * B1, [1] * B1)
+    p1 = gpu_params(P, walkers(wl1["components"], B1, seed=21))
+    like1 = time_calls(lambda: eng.like(p1), reps) * B1
+    grad1 = time_calls(lambda: eng.like_grad(p1), reps) * B1
+    eng.close()
+    out["config1"] = {"workload": "100x100 Sersic n=2.5 re=8 convolved with a 25x25 Gaussian (FWHM 3), brute "
+                                  "force, chi^2, magzero 30, batch 1024",
+                      "like_evals_per_s": round(like1, 1), "lik_grad_evals_per_s": round(grad1, 1)}
+    prof1 = gpu_params(P, [wl1["components"]])
+    mm1 = time_calls(lambda: P.make_model(100, 100, prof1, psf=wl1["psf"], magzero=30.0), reps)
+    out["config1"]["make_model_calls_per_s"] = round(mm1, 1)
+
+    # config 2 per call through the model path (the batched form is the main metric)
+    wl2 = W.config2()
+    prof2 = gpu_params(P, [wl2["components"]])
+    for conv in ("brute", "fft"):
+        mm2 = time_calls(lambda: P.make_model(400, 400, prof2, psf=wl2["psf"], magzero=30.0, convolver=conv), reps)
+        out.setdefault("config2", {})[f"make_model_calls_per_s_{conv}"] = round(mm2, 1)
+
+    # config 5: 2048x2048 field of 500 Sersics, 101x101 Moffat PSF, one instance
+    wl5 = W.config5(model_fn)
+    eng = P.Engine()
+    g = eng.add_galaxy(2048, 2048, wl5["data"], wl5["sigma"], psf=wl5["psf"], region=wl5["region"], magzero=30.0)
+    eng.set_grad_slots(("xcen", "ycen", "mag", "ang", "axrat", "s0", "s1"))
+    eng.set_batch([g], [len(wl5["components"])])
+    p5 = gpu_params(P, [wl5["components"]])
+    tune = eng.autotune(p5, reps=1)
+    like5 = time_calls(lambda: eng.like(p5), reps)
+    grad5 = time_calls(lambda: eng.like_grad(p5), reps)
+    eng.close()
+    out["config5"] = {"workload": "2048x2048, 500 Sersics (max_recursions 4), 101x101 Moffat PSF, chi^2, one "
+                                  "instance", "convolution": tune["kinds"][0],
+                      "like_evals_per_s": round(like5, 2), "lik_grad_evals_per_s": round(grad5, 2)}
+    return out
+
+
+def config4_leg(P, W, steps, warmup):
+    """The 4096-galaxy heterogeneous batch (config 4): lik+grad of every
+    instance per step, with the autotuner's per-galaxy convolution kernels."""
+    t0 = time.perf_counter()
+    gals = W.config4_galaxies(4096, seed=4)
+    for g in gals:
+        g["calc"] = P.adjust_mask(g["region"], (g["width"], g["height"]), g["psf"])
+        g["data"] = np.zeros((g["height"], g["width"]))
+        g["sigma"] = np.ones((g["height"], g["width"]))
+
+    def add(eng, g):
+        return eng.add_galaxy(g["width"], g["height"], g["data"], g["sigma"], psf=g["psf"], calcregion=g["calc"],
+                              region=g["region"], magzero=30.0)
+
+    truth = P.profile_array([p for g in gals for p in
+                             [P.make_profile(c["type"], convolve=True, **{k: v for k, v in c.items() if k != "type"})
+                              for c in g["components"]]])
+    meng = P.Engine()
+    ids = [add(meng, g) for g in gals]
+    meng.set_batch(ids, [len(g["components"]) for g in gals])
+    models = meng.model_images(truth)
+    meng.close()
+    W.config4_attach_data(gals, [m.astype(np.float64) for m in models])
+    eng = P.Engine()
+    ids = [add(eng, g) for g in gals]
+    eng.set_batch(ids, [len(g["components"]) for g in gals])
+    tune = eng.autotune(truth, reps=1)
+    setup_s = time.perf_counter() - t0
+    for _ in range(warmup):
+        eng.like_grad(truth)
+    eng.set_timing(True)
+    eng.reset_stats()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        eng.like_grad(truth)
+    t2 = time.perf_counter()
+    stats = eng.stats()
+    eng.set_timing(False)
+    eng.close()
+    n = len(gals)
+    kinds = tune["kinds"]
+    return {
+        "workload": "4096 galaxies, sides U(32, 256) px, 1-3 components (Sersic, Moffat, Ferrer, King, core-Sersic), "
+                    "own Gaussian or Moffat PSF 15-41 px, chi^2 over 0.48-axis ellipses, magzero 30",
+        "lik_grad_evals_per_s": round(n * steps / (t2 - t1), 1),
+        "ms_per_step": round((t2 - t1) * 1e3 / steps, 3),
+        "stage_ms_per_step": {k: round(v / steps, 4) for k, v in stats["stage_ms"].items()},
+        "convolution_kinds": {k: kinds.count(k) for k in sorted(set(kinds))},
+        "autotune_ms": tune["ms"],
+        "setup_s": round(setup_s, 1),
+        "components": int(sum(len(g["components"]) for g in gals)),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=1024)
+    ap.add_argument("--workload", choices=("c2", "c4"), default="c2",
+                    help="c2: the metric's 400x400 bulge+disk batch; c4: the 4096-galaxy heterogeneous batch")
     ap.add_argument("--fd-evals", type=int, default=3)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-other-configs", action="store_true",
+                    help="skip the likelihood-only / per-call figures of configurations 1, 2 and 5")
     ap.add_argument("--tile-rows", type=int, default=8)
     ap.add_argument("--render-fp64", action="store_true",
                     help="profile values (direct pixels, recursion leaves) in FP64 instead of FP32")
     ap.add_argument("--calibrate-copy", action="store_true",
-                    help="only copy 256 MiB device-to-device three times (the reference kernel for scaling "
-                         "the FETCH_SIZE / WRITE_SIZE counters) and exit")
+                    help="only copy 256 MiB device-to-device three times with the library's copy kernel (the "
+                         "reference kernel for scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--conv", choices=("auto", "brute", "fft", "brute-bf16x3"), default="auto",
                     help="PSF convolution method (auto: the autotuner's choice among brute and fft)")
     ap.add_argument("--grad-slots", choices=("metric", "all"), default="metric",
@@ -131,14 +323,7 @@ def main():
                          "the unused shape slots")
     args = ap.parse_args()
     if args.calibrate_copy:
-        import torch
-        n = 64 * 1024 * 1024
-        x = torch.ones(n, dtype=torch.float32, device="cuda")
-        y = torch.empty_like(x)
-        for _ in range(3):
-            y.copy_(x)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_per_launch": n * 4}))
+        calibrate_copy()
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU figures are not produced",
@@ -147,6 +332,15 @@ def main():
 
     import profit_amd as P
     from profit_amd import workloads as W
+
+    if args.workload == "c4":
+        r = config4_leg(P, W, args.steps, args.warmup)
+        print(json.dumps({"metric": "lik+grad evals/s, config 4 heterogeneous batch", "value": r["lik_grad_evals_per_s"],
+                          "unit": "lik+grad evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                          "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+                          "vs_baseline": None, "dtype": "f32 render and brute convolution, f64 FFT and sums",
+                          "data": "synthetic (GPU model plus seeded noise)", "config": r}))
+        return
 
     def gpu_model_fn(w, h, psf, comps, magzero):
         return P.make_model(w, h, gpu_params(P, [comps]), psf=psf, magzero=magzero)["image"]
@@ -168,13 +362,14 @@ def main():
     tune = None
     if args.conv == "auto":
         tune = eng.autotune(params, reps=2)
-        print(f"[bench] autotune: {tune['kinds'][0]} ({tune['ms']})", file=sys.stderr, flush=True)
+        log(f"autotune: {tune['kinds'][0]} ({tune['ms']})")
     conv_used = tune["kinds"][0] if tune else args.conv
-    print(f"[bench] batch {B}: warmup {args.warmup}", file=sys.stderr, flush=True)
+    log(f"batch {B}: warmup {args.warmup}")
     for _ in range(args.warmup):
         eng.like_grad(params)
 
     # timed region: eager launch sequence with HIP events on the engine stream
+    # between the stages and between the FFT kernels
     eng.set_timing(True)
     eng.reset_stats()
     t0 = time.perf_counter()
@@ -182,18 +377,18 @@ def main():
         ll, grad = eng.like_grad(params)
     t1 = time.perf_counter()
     stats = eng.stats()
+    fft_kernel_ms = eng.fft_stats()
     eng.set_timing(False)
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     value = B * args.steps / (t1 - t0)
-    print(f"[bench] adjoint: {value:.1f} lik+grad/s, {ms_per_step:.2f} ms/step", file=sys.stderr, flush=True)
+    log(f"adjoint: {value:.1f} lik+grad/s, {ms_per_step:.2f} ms/step")
+    log("stage ms per step: " + ", ".join(f"{k} {v / args.steps:.3f}" for k, v in stats["stage_ms"].items()))
+    log("FFT kernels ms per step: " + ", ".join(f"{k} {v:.3f}" for k, v in fft_kernel_ms.items()))
 
+    # likelihood only (no gradient) on the same batch
+    like_value = time_calls(lambda: eng.like(params), args.steps) * B
     # graph replay of the same batch (production path)
-    eng.like_grad(params)
-    tg0 = time.perf_counter()
-    for _ in range(args.steps):
-        eng.like_grad(params)
-    tg1 = time.perf_counter()
-    graph_value = B * args.steps / (tg1 - tg0)
+    graph_value = time_calls(lambda: eng.like_grad(params), args.steps) * B
 
     # plain per-call GPU forward path + central finite differences
     eng1 = P.Engine()
@@ -211,74 +406,86 @@ def main():
             fd_gradient_gpu(eng1, P, sets[i % B])
         tf1 = time.perf_counter()
         fd_value = args.fd_evals / (tf1 - tf0)
-    if fd_value:
-        print(f"[bench] plain forward + FD: {fd_value:.1f} lik+grad/s", file=sys.stderr, flush=True)
+        log(f"plain forward + FD: {fd_value:.1f} lik+grad/s")
 
-    # roofline of the dominant kernel (one launch = the whole batch), with the
-    # algorithmic work per unit of each kernel family:
-    #   rendering (k_render, the recursion walkers, the adjoint render):
-    #     5 B per canvas pixel (4 B FP32 write + 1 B mask read), HBM-bound
+    # per-kernel rooflines (one launch = the whole batch); algorithmic work per
+    # unit of each kernel family:
+    #   rendering (k_render, the recursion walkers, the adjoint render): 5 B per
+    #     canvas pixel (4 B FP32 write + 1 B mask read), reported against HBM;
+    #     their limit is the VALU / transcendental rate, so profile
+    #     evaluations per second are given beside it
+    #   FFT kernels: the FP64 FLOPs each executes; the stage's algorithmic bytes
     #   brute-force convolution: 2 K^2 FLOP per output pixel (FP32 vector peak)
-    #   FFT convolution (3 kernels): 3 transforms x 5 N log2 N FLOP, N = N1 N2
-    #     (FP64 vector peak)
     sm = stats["stage_ms"]
     ncalls = max(1, stats["stage_calls"]["subsample"])
     stage_ms = {k: v / ncalls for k, v in sm.items()}
-    dw, dh, _, _ = eng.geometry(g)
+    geom = eng.geometry(g)
+    dw, dh, _, _ = geom
     K = wl["psf"].shape[0]
-    n1 = 1 << int(np.ceil(np.log2(dw + K)))
-    n2 = 1 << int(np.ceil(np.log2(dh + K)))
-    n_fft = n1 * n2
+    n1 = max(8, 1 << int(np.ceil(np.log2(dw + K))))
+    n2 = max(8, 1 << int(np.ceil(np.log2(dh + K))))
     render_bytes = B * dw * dh * 5.0
-    kernels = {
-        "render": ("k_render<0> (classification + direct pixels, Sersic)", "hbm", render_bytes, PEAK_HBM_GBS,
-                   "GB/s"),
-        "subsample": ("k_subsample<false, 0, 0> (recursion walker, Sersic)", "hbm", render_bytes, PEAK_HBM_GBS,
-                      "GB/s"),
-        "adjoint_render": ("k_adj_render<0, *> (adjoint of direct pixels, Sersic)", "hbm", render_bytes,
-                           PEAK_HBM_GBS, "GB/s"),
-        "adjoint_subsample": ("k_subsample<true, 0, 2> (adjoint recursion walker, Sersic)", "hbm", render_bytes,
-                              PEAK_HBM_GBS, "GB/s"),
-    }
+    direct, cells = sample_counts(eng1, gpu_params(P, [wl["components"]]), geom, wl["components"])
+    traffic = {}
+    tfile = os.path.join(ROOT, "profiles", f"traffic_config2_batch{B}.json")
+    if os.path.exists(tfile) and conv_used == "fft" and not args.render_fp64:
+        with open(tfile) as f:
+            traffic = {k: v["hbm_bytes_per_launch"] for k, v in json.load(f)["kernels"].items()}
+    kernels = {}
+
+    def add_kernel(key, name, ms, bound, work, peak, unit, extra=None):
+        a = work / (ms * 1e-3) / (1e9 if unit == "GB/s" else 1e12) if ms > 0 else 0.0
+        e = {"kernel": name, "bound": bound, "ms": round(ms, 4), "achieved": round(a, 3), "peak": peak, "unit": unit,
+             "frac": round(a / peak, 5), "work_per_launch": work}
+        t = traffic.get(name)
+        if t:
+            e["traffic"] = t
+            e["traffic_GBs"] = round(t / (ms * 1e-3) / 1e9, 1)
+            e["traffic_frac"] = round(t / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)
+        if extra:
+            e.update(extra)
+        kernels[key] = e
+
+    add_kernel("render", "k_render<0>", stage_ms["render"], "hbm", render_bytes, PEAK_HBM_GBS, "GB/s",
+               {"profile_evals_per_s": round(B * direct / (stage_ms["render"] * 1e-3), 0),
+                "direct_pixels_per_instance": direct})
+    add_kernel("subsample", "k_subsample<false, 0, 0>", stage_ms["subsample"], "hbm", render_bytes, PEAK_HBM_GBS,
+               "GB/s", {"cell_evals_per_s": round(B * cells / (stage_ms["subsample"] * 1e-3), 0),
+                        "cells_per_instance": cells})
+    add_kernel("adjoint_render", "k_adj_render<0, *>", stage_ms["adjoint_render"], "hbm", render_bytes, PEAK_HBM_GBS,
+               "GB/s", {"derivative_evals_per_s": round(B * direct / (stage_ms["adjoint_render"] * 1e-3), 0)})
+    add_kernel("adjoint_subsample", "k_subsample<true, 0, 2>", stage_ms["adjoint_subsample"], "hbm", render_bytes,
+               PEAK_HBM_GBS, "GB/s",
+               {"cell_derivative_evals_per_s": round(B * cells / (stage_ms["adjoint_subsample"] * 1e-3), 0)})
     if conv_used == "fft":
-        fft_flops = B * 3 * 5.0 * n_fft * np.log2(n_fft)
-        kernels["conv_like"] = ("FFT convolution + likelihood (k_fft_rows_fwd, k_fft_cols, k_fft_rows_inv<0>)",
-                                "mfma", fft_flops, PEAK_FP64_TFLOPS, "TFLOP/s")
-        kernels["adjoint_conv"] = ("FFT adjoint correlation (k_fft_rows_fwd, k_fft_cols, k_fft_rows_inv<2>)", "mfma",
-                                   fft_flops, PEAK_FP64_TFLOPS, "TFLOP/s")
+        flops, stage_bytes = fft_work(dw, dh, 400, 400, n1, n2)
+        names = {"like_rows_fwd": "k_fft_rows_fwd", "like_cols": "k_fft_cols", "like_rows_inv": "k_fft_rows_inv<0>",
+                 "adj_rows_fwd": "k_fft_rows_fwd", "adj_cols": "k_fft_cols", "adj_rows_inv": "k_fft_rows_inv<2>"}
+        for key, kname in names.items():
+            ms = fft_kernel_ms[key]
+            # each kernel's limit is the FP64 frame it streams through HBM (its
+            # counter traffic beside it); its executed FP64 FLOPs for the compute view
+            add_kernel(key, kname, ms, "hbm", B * flops[key], PEAK_FP64_TFLOPS, "TFLOP/s")
+            kernels[key]["note"] = "work: executed FP64 FLOPs; bound by the frame's HBM traffic (traffic_frac)"
+        for stage in ("conv_like", "adjoint_conv"):
+            add_kernel(stage, "FFT stage (3 kernels)", stage_ms[stage], "hbm", B * stage_bytes[stage], PEAK_HBM_GBS,
+                       "GB/s")
     else:
         peak = PEAK_BF16_TFLOPS if conv_used == "brute-bf16x3" else PEAK_FP32_TFLOPS
         kname = "k_conv_mfma" if conv_used == "brute-bf16x3" else "k_conv"
-        kernels["conv_like"] = (f"{kname}<CONV_LIKE> (convolution + likelihood)", "mfma", stats["conv_flops"], peak,
-                                "TFLOP/s")
-        kernels["adjoint_conv"] = (f"{kname}<CONV_ADJ> (adjoint correlation)", "mfma", stats["adjoint_flops"], peak,
-                                   "TFLOP/s")
-    dom = max(kernels, key=lambda k: stage_ms[k])
-    kname, bound, work, peak, unit = kernels[dom]
-    # HBM bytes per launch of that kernel from the committed PMC summary of the
-    # same workload (FETCH_SIZE / WRITE_SIZE passes scaled by a copy kernel)
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_config2_batch{B}.json")
-    if os.path.exists(tfile) and conv_used == "fft":
-        with open(tfile) as f:
-            tk = json.load(f)["kernels"]
-        key = kname.split(" (")[0]
-        if key in tk:
-            traffic = tk[key]["hbm_bytes_per_launch"]
-        elif key.startswith("FFT"):
-            # the stage's three kernels: rows forward, columns, rows inverse
-            # (likelihood epilogue k_fft_rows_inv<0>, adjoint image <2>)
-            inv = "k_fft_rows_inv<0>" if dom == "conv_like" else "k_fft_rows_inv<2>"
-            parts = [k for k in ("k_fft_rows_fwd", "k_fft_cols", inv) if k in tk]
-            traffic = sum(tk[k]["hbm_bytes_per_launch"] for k in parts) if len(parts) == 3 else None
-    kms = stage_ms[dom]
-    achieved = work / (kms * 1e-3) / (1e9 if unit == "GB/s" else 1e12)
+        bound = "mfma" if conv_used == "brute-bf16x3" else "hbm"
+        add_kernel("conv_like", f"{kname}<CONV_LIKE>", stage_ms["conv_like"], bound, stats["conv_flops"], peak,
+                   "TFLOP/s")
+        add_kernel("adjoint_conv", f"{kname}<CONV_ADJ>", stage_ms["adjoint_conv"], bound, stats["adjoint_flops"], peak,
+                   "TFLOP/s")
+        for k in ("conv_like", "adjoint_conv"):
+            kernels[k]["note"] = "work: 2 K^2 FLOP per output pixel; FP32 VALU" if bound == "hbm" else \
+                "work: 2 K^2 FLOP per output pixel (three split-BF16 products issued per MAC)"
+    # the dominant single kernel of the step (stages of several kernels excluded)
+    single = [k for k in kernels if not kernels[k]["kernel"].startswith("FFT stage") and "*" not in kernels[k]["kernel"]]
+    dom = max(single, key=lambda k: kernels[k]["ms"])
+    d = kernels[dom]
     stage_per_step = {k: v / args.steps for k, v in sm.items()}
-    per_kernel = {}
-    for k, (kn, bd, wk, pk, un) in kernels.items():
-        a = wk / (stage_ms[k] * 1e-3) / (1e9 if un == "GB/s" else 1e12)
-        per_kernel[k] = {"kernel": kn, "bound": bd, "ms": round(stage_ms[k], 4), "achieved": round(a, 3),
-                         "unit": un, "frac": round(a / pk, 5)}
 
     out = {
         "metric": METRIC,
@@ -302,7 +509,7 @@ def main():
                         + ("the 7 metric parameters (xcen, ycen, mag, re, nser, ang, axrat)" if args.grad_slots ==
                            "metric" else "all 11 slots") + " per component",
             "batch": B,
-            "drawing_canvas": "450x450",
+            "drawing_canvas": f"{dw}x{dh}",
             "tile_rows": args.tile_rows,
             "profile_values": "fp64" if args.render_fp64 else "fp32",
             "convolution": conv_used,
@@ -314,25 +521,30 @@ def main():
             "ratio": round(value / fd_value, 2) if fd_value else None,
             "fd_forward_calls_per_lik_grad": 29,
         },
+        "like_only_evals_per_s": round(like_value, 1),
         "stage_ms_per_step": {k: round(v, 4) for k, v in stage_per_step.items()},
         "roofline": {
-            "bound": bound,
-            "achieved": round(achieved, 3),
-            "peak": peak,
-            "unit": unit,
-            "frac": round(achieved / peak, 5),
-            "traffic": traffic,
-            "kernel": kname,
-            "work_per_launch": work,
-            "avg_launch_ms": round(kms, 4),
+            "bound": d["bound"],
+            "achieved": d["achieved"],
+            "peak": d["peak"],
+            "unit": d["unit"],
+            "frac": d["frac"],
+            "traffic": d.get("traffic"),
+            "kernel": d["kernel"],
+            "work_per_launch": d["work_per_launch"],
+            "avg_launch_ms": d["ms"],
         },
-        "roofline_by_kernel": per_kernel,
+        "roofline_by_kernel": kernels,
     }
+    for k in ("cell_evals_per_s", "cell_derivative_evals_per_s", "profile_evals_per_s"):
+        if k in d:
+            out["roofline"][k] = d[k]
 
     # FP64 CPU oracle leg: baseline timing and parity of the GPU result
     if not args.no_cpu_baseline:
         threads = os.cpu_count() or 1
         cb = cpu_baseline(wl, calc, args.cpu_seconds, threads)
+        cb1 = cpu_baseline(wl, calc, 1.0, 1)  # one lik+grad (29 likelihoods) on one core
         eng1.set_batch([g1], [2])
         truth = gpu_params(P, [wl["components"]])
         gmodel = eng1.model_images(truth)[0]
@@ -345,10 +557,15 @@ def main():
         og = cb["grad"][:, gslots]
         gg = ggrad[:, gslots]
         grad_err = float(np.max(np.abs(gg - og) / np.maximum(np.abs(og), 1e-3 * np.abs(og).max())))
+        reg = wl["region"].astype(bool)
+        chi_gpu = float(np.sum(((wl["data"] - gmodel) / wl["sigma"])[reg] ** 2))
+        chi_ref = float(np.sum(((wl["data"] - ref) / wl["sigma"])[reg] ** 2))
         out["parity_vs_fp64_oracle"] = {
             "max_rel_pixel_err": pix_err,
             "pixels_checked": int(sel.sum()),
             "abs_logL_err": float(abs(gll[0] - cb["ll"])),
+            "abs_chisq_err": abs(chi_gpu - chi_ref),
+            "chisq": chi_ref,
             "max_rel_grad_err": grad_err,
             "gates": {"pixel": 1e-5, "logL": 1e-3},
         }
@@ -359,7 +576,14 @@ def main():
             "kind": "port",
             "sample": f"{cb['n']} lik+grad evaluation(s) of the same workload by central differences "
                       f"(29 FP64 likelihoods each) in {cb['elapsed']:.1f} s, OpenMP over pixels",
+            "one_core": {"value": round(cb1["value"], 5), "cores": 1,
+                         "sample": f"{cb1['n']} lik+grad evaluation(s) in {cb1['elapsed']:.1f} s"},
+            "cpu_model": cpu_model(),
         }
+    eng1.close()
+    eng.close()
+    if not args.no_other_configs:
+        out["other_configs"] = other_configs(P, W, reps=max(3, args.steps // 2))
     print(json.dumps(out))
 
 
