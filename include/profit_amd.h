@@ -135,6 +135,24 @@ int profit_amd_adjust_mask(const uint8_t *mask, uint32_t mask_w, uint32_t mask_h
 int profit_amd_convolve(int32_t convolver, const double *src, uint32_t src_w, uint32_t src_h, const double *krn,
                         uint32_t krn_w, uint32_t krn_h, const uint8_t *mask, double *out, int32_t device);
 
+/* Allocation-free variant of profit_amd_convolve.  No counterpart in the
+ * reference: it serves callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops).  src, krn, mask (nullable) and out
+ * are DEVICE pointers owned by the caller; the work is enqueued on `stream`
+ * (a hipStream_t; NULL = the default stream) and the call returns after
+ * enqueueing, calling no hipMalloc, hipFree or other allocating routine.  The
+ * staging lives in a fixed device array of about 8.7 MB per GPU, held until
+ * the process ends, which limits: src_w, src_h <= 4096 and src_w * src_h <=
+ * 2^20; krn_w, krn_h <= 255 and no larger than the source (the allocating
+ * entry point extends a smaller source).  Larger inputs are rejected with an
+ * error.  Convolvers: PROFIT_AMD_CONV_BRUTE and _BRUTE_OLD (the same kernel as
+ * profit_amd_convolve, bit-identical results) and _NULL (a plain copy of the
+ * source, which uses no scratch and so takes any size).  Every call on one
+ * GPU shares that array: calls that overlap on one GPU race. */
+int profit_amd_convolve_noalloc(int32_t convolver, const double *src, uint32_t src_w, uint32_t src_h, const double *krn,
+                                uint32_t krn_w, uint32_t krn_h, const uint8_t *mask, double *out, int32_t device,
+                                void *stream);
+
 /* Names of the available convolvers, comma separated (R_profit_convolvers). */
 const char *profit_amd_convolvers(void);
 

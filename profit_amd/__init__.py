@@ -93,6 +93,9 @@ SYMBOLS = [
                                          _P(C.c_uint8), C.c_size_t, _P(C.c_uint32), _P(C.c_uint32)]),
     ("profit_amd_convolve", C.c_int, [C.c_int32, _P(C.c_double), C.c_uint32, C.c_uint32, _P(C.c_double),
                                       C.c_uint32, C.c_uint32, _P(C.c_uint8), _P(C.c_double), C.c_int32]),
+    ("profit_amd_convolve_noalloc", C.c_int, [C.c_int32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
+                                              C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int32,
+                                              C.c_void_p]),
     ("profit_amd_convolvers", C.c_char_p, []),
     ("profit_amd_downsample", C.c_int, [_P(C.c_double), C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32,
                                         _P(C.c_double), C.c_size_t, _P(C.c_uint32), _P(C.c_uint32)]),

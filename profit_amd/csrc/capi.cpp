@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "engine.h"
+#include "noalloc.h"
 #include "profit.h"
 
 static_assert(sizeof(profit_amd_profile) == sizeof(profit_amd::ProfileParams), "profile layout");
@@ -197,6 +198,31 @@ int profit_amd_convolve(int32_t convolver, const double *src, uint32_t src_w, ui
 		if (mask) m = profit::Mask(mask, src_w, src_h);
 		profit::Image r = c->convolve(s, k, m);
 		std::memcpy(out, r.data(), r.size() * sizeof(double));
+	});
+}
+
+int profit_amd_convolve_noalloc(int32_t convolver, const double *src, uint32_t src_w, uint32_t src_h, const double *krn,
+                                uint32_t krn_w, uint32_t krn_h, const uint8_t *mask, double *out, int32_t device,
+                                void *stream)
+{
+	return guarded([&] {
+		int method;
+		switch (convolver) {
+		case PROFIT_AMD_CONV_BRUTE_OLD:
+		case PROFIT_AMD_CONV_BRUTE: method = profit_amd::NOALLOC_BRUTE; break;
+		case PROFIT_AMD_CONV_NULL: method = profit_amd::NOALLOC_NULL; break;
+		default: throw profit::invalid_parameter("the allocation-free convolution supports brute, brute-old and null");
+		}
+		if (src_w > (uint32_t)INT32_MAX || src_h > (uint32_t)INT32_MAX)
+			throw std::invalid_argument("source dimensions out of range");
+		if (method != profit_amd::NOALLOC_NULL &&
+		    (src_w > (uint32_t)profit_amd::kNoAllocMaxSide || src_h > (uint32_t)profit_amd::kNoAllocMaxSide))
+			throw std::invalid_argument("source too large for the allocation-free convolution: at most " +
+			                            std::to_string(profit_amd::kNoAllocMaxSide) + " pixels per side");
+		if (krn_w > (uint32_t)profit_amd::kNoAllocMaxKernel || krn_h > (uint32_t)profit_amd::kNoAllocMaxKernel)
+			throw std::invalid_argument("kernel dimensions must be 1 to " + std::to_string(profit_amd::kNoAllocMaxKernel));
+		profit_amd::convolve_noalloc(method, src, (int)src_w, (int)src_h, krn, (int)krn_w, (int)krn_h, mask, out, device,
+		                             static_cast<hipStream_t>(stream));
 	});
 }
 

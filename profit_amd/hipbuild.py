@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libprofit_amd.so")
-SOURCES = ["kernels.hip", "walkers_fwd.hip", "walkers_adj.hip", "fft.hip", "conv_mfma.hip", "engine.cpp", "model.cpp",
+SOURCES = ["kernels.hip", "walkers_fwd.hip", "walkers_adj.hip", "fft.hip", "conv_mfma.hip", "noalloc.hip", "engine.cpp", "model.cpp",
            "capi.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
