@@ -130,19 +130,31 @@ def cpu_baseline(wl, calc, seconds, threads):
 
 
 def calibrate_copy():
-    """Copies 256 MiB device to device three times with a plain elementwise
-    kernel (torch's out = 1.0 * in over FP64: each launch reads and writes
-    256 MiB once): the known traffic that scales the FETCH_SIZE / WRITE_SIZE
-    counters."""
-    import torch
-    n = 32 * 1024 * 1024  # FP64 elements
-    src = torch.zeros(n, dtype=torch.float64, device="cuda")
-    dst = torch.empty_like(src)
-    for _ in range(3):
-        torch.mul(src, 1.0, out=dst)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_kernel": "torch elementwise mul (FP64)", "copy_bytes_read_per_launch": n * 8,
-                      "copy_bytes_written_per_launch": n * 8}))
+    """Copies 256 MiB device to device three times with the library's plain
+    copy kernel (the null convolver of profit_amd_convolve_noalloc: each
+    launch reads and writes 256 MiB once): the known traffic that scales the
+    FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+    import profit_amd as P
+    hip = C.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipFree.argtypes = [C.c_void_p]
+    hip.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    w, h = 8192, 4096  # 32 Mi doubles
+    nbytes = w * h * 8
+    src, dst = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(src), nbytes) == 0 and hip.hipMalloc(C.byref(dst), nbytes) == 0
+    try:
+        assert hip.hipMemset(src, 0, nbytes) == 0
+        for _ in range(3):
+            P._check(P.lib().profit_amd_convolve_noalloc(P.CONVOLVERS["null"], src, w, h, src, 1, 1, None, dst, 0,
+                                                         None))
+        assert hip.hipDeviceSynchronize() == 0
+    finally:
+        hip.hipFree(src)
+        hip.hipFree(dst)
+    print(json.dumps({"copy_kernel": "k_noalloc_copy", "copy_bytes_read_per_launch": nbytes,
+                      "copy_bytes_written_per_launch": nbytes}))
 
 
 def sample_counts(eng1, params1, geometry, comps):
@@ -272,12 +284,15 @@ def config4_leg(P, W, steps, warmup):
     setup_s = time.perf_counter() - t0
     for _ in range(warmup):
         eng.like_grad(truth)
-    eng.set_timing(True)
-    eng.reset_stats()
     t1 = time.perf_counter()
     for _ in range(steps):
         eng.like_grad(truth)
     t2 = time.perf_counter()
+    # stage times from separate steps with per-stage events on
+    eng.set_timing(True)
+    eng.reset_stats()
+    for _ in range(steps):
+        eng.like_grad(truth)
     stats = eng.stats()
     eng.set_timing(False)
     eng.close()
