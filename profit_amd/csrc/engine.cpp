@@ -341,18 +341,21 @@ void Engine::ensure_fft_spectra()
 		hip_check(hipMemcpy(src.as<double>(), frame.data(), frame.size() * sizeof(double), hipMemcpyHostToDevice), "psf");
 		g.fft_spec = b.fft_spec.as<double2>();
 		// one-instance context pointing at this galaxy
-		DevBuf dg(sizeof(GalaxyDev)), di(sizeof(InstDev)), doff(2 * sizeof(int));
+		DevBuf dg(sizeof(GalaxyDev)), di(sizeof(InstDev));
 		InstDev I{};
 		I.galaxy = 0;
 		hip_check(hipMemcpy(dg.as<GalaxyDev>(), &g, sizeof(g), hipMemcpyHostToDevice), "gal");
 		hip_check(hipMemcpy(di.as<InstDev>(), &I, sizeof(I), hipMemcpyHostToDevice), "inst");
 		const int cb = fft_cols_per_block(n2);
-		const int off[2] = {0, (m1 + cb - 1) / cb};
-		hip_check(hipMemcpy(doff.as<int>(), off, sizeof(off), hipMemcpyHostToDevice), "off");
+		const int nblk = (m1 + cb - 1) / cb;
+		std::vector<int2> blk(nblk);
+		for (int t = 0; t < nblk; ++t) blk[t] = make_int2(0, t);
+		DevBuf dblk(nblk * sizeof(int2));
+		hip_check(hipMemcpy(dblk.as<int2>(), blk.data(), nblk * sizeof(int2), hipMemcpyHostToDevice), "blocks");
 		hip_check(launch_fft_psf_rows(n1, n2, dg.as<GalaxyDev>(), d_tw_.as<double2>(), twn_, src.as<double>(),
 		                              b.fft_spec.as<double2>(), stream_),
 		          "psf rows");
-		hip_check(launch_fft_cols(off[1], n2, cb, dg.as<GalaxyDev>(), di.as<InstDev>(), doff.as<int>(), 1, 0, 2, 0,
+		hip_check(launch_fft_cols(nblk, n2, cb, dg.as<GalaxyDev>(), di.as<InstDev>(), dblk.as<int2>(), 0, 2, 0,
 		                          d_tw_.as<double2>(), twn_, b.fft_spec.as<double2>(), stream_),
 		          "psf cols");
 		hip_check(hipStreamSynchronize(stream_), "sync");
@@ -400,16 +403,16 @@ void Engine::conv_forward(int mode, hipStream_t st)
 		                              d_tw_.as<double2>(), twn_, st),
 		          "fft rows");
 	if (timed) mark_fft(0, 1, st);
-	hip_check(launch_fft_cols(col_off_[n], fft_nmax2_, fft_cb_, gals, insts, d_col_off_.as<int>(), n, 1, 0,
+	hip_check(launch_fft_cols(col_off_[n], fft_nmax2_, fft_cb_, gals, insts, d_col_blk_.as<int2>(), 1, 0,
 	                          mode == 0 ? 0 : 1, d_tw_.as<double2>(), twn_, nullptr, st),
 	          "fft cols");
 	if (timed) mark_fft(0, 2, st);
 	if (mode == 0)
-		hip_check(launch_fft_rows_inv(ri0_off_[n], fft_nmax1_, gals, insts, d_ri0_off_.as<int>(), n, 0,
+		hip_check(launch_fft_rows_inv(ri0_off_[n], fft_nmax1_, gals, insts, d_ri0_blk_.as<int2>(), 0,
 		                              d_tw_.as<double2>(), twn_, lk, partial, st),
 		          "fft rows inv");
 	else
-		hip_check(launch_fft_rows_inv(ri1_off_[n], fft_nmax1_, gals, insts, d_ri1_off_.as<int>(), n, 1,
+		hip_check(launch_fft_rows_inv(ri1_off_[n], fft_nmax1_, gals, insts, d_ri1_blk_.as<int2>(), 1,
 		                              d_tw_.as<double2>(), twn_, lk, partial, st),
 		          "fft rows inv");
 	if (timed) mark_fft(0, 3, st);
@@ -433,11 +436,11 @@ void Engine::conv_adjoint(hipStream_t st)
 	                              twn_, st),
 	          "fft rows");
 	mark_fft(1, 1, st);
-	hip_check(launch_fft_cols(col_off_[n], fft_nmax2_, fft_cb_, gals, insts, d_col_off_.as<int>(), n, 2, 1, 1,
+	hip_check(launch_fft_cols(col_off_[n], fft_nmax2_, fft_cb_, gals, insts, d_col_blk_.as<int2>(), 2, 1, 1,
 	                          d_tw_.as<double2>(), twn_, nullptr, st),
 	          "fft cols");
 	mark_fft(1, 2, st);
-	hip_check(launch_fft_rows_inv(ri1_off_[n], fft_nmax1_, gals, insts, d_ri1_off_.as<int>(), n, 2, d_tw_.as<double2>(),
+	hip_check(launch_fft_rows_inv(ri1_off_[n], fft_nmax1_, gals, insts, d_ri1_blk_.as<int2>(), 2, d_tw_.as<double2>(),
 	                              twn_, lk, nullptr, st),
 	          "fft rows inv");
 	mark_fft(1, 3, st);
@@ -876,10 +879,34 @@ void Engine::set_batch(const std::vector<int> &inst_galaxy, const std::vector<in
 	upload_off(d_lk_off_, lk_off_);
 	upload_off(d_rf0_off_, rf0_off_);
 	upload_off(d_rf1_off_, rf1_off_);
-	upload_off(d_col_off_, col_off_);
-	upload_off(d_ri0_off_, ri0_off_);
-	upload_off(d_ri1_off_, ri1_off_);
 	upload_off(d_ag_off_, ag_off_);
+	// block tables: galaxy by galaxy (in order of first appearance), block
+	// index, then the galaxy's instances
+	{
+		std::vector<std::vector<int>> by_gal(gal_host_.size());
+		std::vector<int> gorder;
+		for (int i = 0; i < n; ++i) {
+			const int gi = inst_galaxy[i];
+			if (by_gal[gi].empty()) gorder.push_back(gi);
+			by_gal[gi].push_back(i);
+		}
+		auto table = [&](DevBuf &d, const std::vector<int> &off) {
+			std::vector<int2> t;
+			t.reserve(off[n]);
+			for (int gi : gorder) {
+				const int nb = off[by_gal[gi][0] + 1] - off[by_gal[gi][0]];
+				for (int b = 0; b < nb; ++b)
+					for (int i : by_gal[gi]) t.push_back(make_int2(i, b));
+			}
+			if ((int)t.size() != off[n]) throw std::logic_error("block table size");
+			d.alloc(std::max<size_t>(1, t.size()) * sizeof(int2));
+			if (!t.empty())
+				hip_check(hipMemcpy(d.as<int2>(), t.data(), t.size() * sizeof(int2), hipMemcpyHostToDevice), "blocks");
+		};
+		table(d_col_blk_, col_off_);
+		table(d_ri0_blk_, ri0_off_);
+		table(d_ri1_blk_, ri1_off_);
+	}
 	d_insts_.alloc(std::max(1, n) * sizeof(InstDev));
 	if (n) hip_check(hipMemcpy(d_insts_.as<InstDev>(), insts.data(), n * sizeof(InstDev), hipMemcpyHostToDevice), "insts");
 	d_ct_off_.alloc((n + 1) * sizeof(int));

@@ -50,12 +50,12 @@ int launch_fft_rows_fwd64(int nblocks, int nmax, const GalaxyDev *gals, const In
                           int ninst, const double2 *tw, int twn, hipStream_t st);
 int launch_fft_psf_rows(int n1, int n2, const GalaxyDev *gal, const double2 *tw, int twn, const double *src64,
                         double2 *dst64, hipStream_t st);
-int launch_fft_cols(int nblocks, int n2max, int cb, const GalaxyDev *gals, const InstDev *insts, const int *col_off,
-                    int ninst, int op, int in_mode, int out_mode, const double2 *tw, int twn, double2 *dst64,
-                    hipStream_t st);
-int launch_fft_rows_inv(int nblocks, int nmax, const GalaxyDev *gals, const InstDev *insts, const int *row_off,
-                        int ninst, int mode, const double2 *tw, int twn, const int *lk_base, double *partial,
-                        hipStream_t st);
+// The column pass and the inverse rows take a block table: (instance, block
+// of that instance) per launched block.
+int launch_fft_cols(int nblocks, int n2max, int cb, const GalaxyDev *gals, const InstDev *insts, const int2 *blocks,
+                    int op, int in_mode, int out_mode, const double2 *tw, int twn, double2 *dst64, hipStream_t st);
+int launch_fft_rows_inv(int nblocks, int nmax, const GalaxyDev *gals, const InstDev *insts, const int2 *blocks,
+                        int mode, const double2 *tw, int twn, const int *lk_base, double *partial, hipStream_t st);
 int fft_cols_per_block(int n2);
 // split-BF16x3 matrix-core path (conv_mfma.hip)
 int mfma_kc(int kx);
@@ -265,7 +265,11 @@ class Engine {
 	// split-BF16x3 tiles (likelihood / canvas), and the likelihood partials
 	// base of each instance
 	std::vector<int> bt_off_, rf0_off_, rf1_off_, col_off_, ri0_off_, ri1_off_, ml_off_, mc_off_, lk_off_;
-	DevBuf d_bt_off_, d_rf0_off_, d_rf1_off_, d_col_off_, d_ri0_off_, d_ri1_off_, d_ml_off_, d_mc_off_, d_lk_off_;
+	DevBuf d_bt_off_, d_rf0_off_, d_rf1_off_, d_ml_off_, d_mc_off_, d_lk_off_;
+	// block tables of the column pass and the inverse rows, ordered by galaxy,
+	// then block, then instance: the blocks in flight together share their
+	// galaxy's PSF spectrum columns (and pixel data) in the L2
+	DevBuf d_col_blk_, d_ri0_blk_, d_ri1_blk_;
 	int n_bt_blocks_ = 0;
 	int mfma_pitch_ = 0, mfma_kch_ = 0;
 	int fft_cb_ = 8;

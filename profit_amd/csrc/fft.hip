@@ -360,14 +360,13 @@ __global__ __launch_bounds__(256) void k_fft_psf_rows(const GalaxyDev *__restric
 // then leaves rows [out_lo, out_hi) in the frame.  Global accesses run along
 // rows (CB adjacent entries); columns sit in LDS at a stride of N2 + 2.
 __global__ __launch_bounds__(kColThreads, 4) void k_fft_cols(const GalaxyDev *__restrict__ gals, const InstDev *__restrict__ insts,
-                                                  const int *__restrict__ col_off, int ninst, int cb, int op,
+                                                  const int2 *__restrict__ blocks, int cb, int op,
                                                   int in_mode, int out_mode, const double2 *__restrict__ tw, int twn,
                                                   double2 *dst64)
 {
 	extern __shared__ double2 sh[];
-	const int b = blockIdx.x;
-	const int inst = owner(col_off, ninst, b);
-	const InstDev &I = insts[inst];
+	const int2 blk = blocks[blockIdx.x];
+	const InstDev &I = insts[blk.x];
 	const GalaxyDev &g = gals[I.galaxy];
 	const int M = g.fft_n1 / 2 + 1, P = fft_pitch(g.fft_n1), N = g.fft_n2, L = g.fft_l2;
 	// column stride: 2 mod 16 entries keeps the columns-fastest pass on
@@ -375,7 +374,7 @@ __global__ __launch_bounds__(kColThreads, 4) void k_fft_cols(const GalaxyDev *__
 	const int S = N + 2;
 	double2 *wt = sh + cb * S;
 	stage_twiddles(wt, N, L, tw, twn);
-	const int c0 = (b - col_off[inst]) * cb;
+	const int c0 = blk.y * cb;
 	double2 *frame = (op == 0) ? dst64 : I.fbuf;
 	// rows holding data: 0 canvas rows [0, DH); 1 image rows [pady, pady + H); 2 all rows
 	const int in_lo = in_mode == 1 ? g.pady : 0;
@@ -463,20 +462,20 @@ __global__ __launch_bounds__(kColThreads, 4) void k_fft_cols(const GalaxyDev *__
 //   mode 2 (adjoint): canvas rows; G = value / (N1 N2)
 template <int MODE>
 __global__ __launch_bounds__(256) void k_fft_rows_inv(const GalaxyDev *__restrict__ gals, const InstDev *__restrict__ insts,
-                                                      const int *__restrict__ row_off, int ninst,
+                                                      const int2 *__restrict__ blocks,
                                                       const double2 *__restrict__ tw, int twn,
                                                       const int *__restrict__ lk_base, double *partial)
 {
 	constexpr int mode = MODE;
 	extern __shared__ double2 sh[];
-	const int b = blockIdx.x;
-	const int inst = owner(row_off, ninst, b);
+	const int2 bi = blocks[blockIdx.x];
+	const int inst = bi.x;
 	const InstDev &I = insts[inst];
 	const GalaxyDev &g = gals[I.galaxy];
 	const int N = g.fft_n1, L = g.fft_l1, M = N / 2 + 1, P = fft_pitch(N), np = fft_pairs_per_block(N);
 	double2 *wt = sh + np * N;
 	stage_twiddles(wt, N, L, tw, twn);
-	const int blk = b - row_off[inst];
+	const int blk = bi.y;
 	const int r0 = 2 * np * blk;  // first output row of the block (image rows in mode 0)
 	const int nrows = mode == 0 ? g.H : g.DH;
 	const int y0 = mode == 0 ? r0 + g.pady : r0;  // its frame row
@@ -648,21 +647,19 @@ int launch_fft_psf_rows(int n1, int n2, const GalaxyDev *gal, const double2 *tw,
 	return (int)hipGetLastError();
 }
 
-int launch_fft_cols(int nblocks, int n2max, int cb, const GalaxyDev *gals, const InstDev *insts, const int *col_off,
-                    int ninst, int op, int in_mode, int out_mode, const double2 *tw, int twn, double2 *dst64,
-                    hipStream_t st)
+int launch_fft_cols(int nblocks, int n2max, int cb, const GalaxyDev *gals, const InstDev *insts, const int2 *blocks,
+                    int op, int in_mode, int out_mode, const double2 *tw, int twn, double2 *dst64, hipStream_t st)
 {
 	if (nblocks <= 0) return 0;
 	const size_t lds = sizeof(double2) * ((size_t)cb * (n2max + 2) + n2max);
 	if (lds > 160 * 1024) return (int)hipErrorInvalidValue;
 	if (int e = ensure_lds((const void *)k_fft_cols, lds, lds_cols)) return e;
-	k_fft_cols<<<nblocks, kColThreads, lds, st>>>(gals, insts, col_off, ninst, cb, op, in_mode, out_mode, tw, twn, dst64);
+	k_fft_cols<<<nblocks, kColThreads, lds, st>>>(gals, insts, blocks, cb, op, in_mode, out_mode, tw, twn, dst64);
 	return (int)hipGetLastError();
 }
 
-int launch_fft_rows_inv(int nblocks, int nmax, const GalaxyDev *gals, const InstDev *insts, const int *row_off,
-                        int ninst, int mode, const double2 *tw, int twn, const int *lk_base, double *partial,
-                        hipStream_t st)
+int launch_fft_rows_inv(int nblocks, int nmax, const GalaxyDev *gals, const InstDev *insts, const int2 *blocks,
+                        int mode, const double2 *tw, int twn, const int *lk_base, double *partial, hipStream_t st)
 {
 	if (nblocks <= 0) return 0;
 	if (mode < 0 || mode > 2) return (int)hipErrorInvalidValue;
@@ -670,15 +667,15 @@ int launch_fft_rows_inv(int nblocks, int nmax, const GalaxyDev *gals, const Inst
 	if (lds > 160 * 1024) return (int)hipErrorInvalidValue;
 	if (mode == 0) {
 		if (int e = ensure_lds((const void *)k_fft_rows_inv<0>, lds, lds_rows_inv)) return e;
-		k_fft_rows_inv<0><<<nblocks, 256, lds, st>>>(gals, insts, row_off, ninst, tw, twn, lk_base, partial);
+		k_fft_rows_inv<0><<<nblocks, 256, lds, st>>>(gals, insts, blocks, tw, twn, lk_base, partial);
 	}
 	else if (mode == 1) {
 		if (int e = ensure_lds((const void *)k_fft_rows_inv<1>, lds, lds_rows_inv1)) return e;
-		k_fft_rows_inv<1><<<nblocks, 256, lds, st>>>(gals, insts, row_off, ninst, tw, twn, lk_base, partial);
+		k_fft_rows_inv<1><<<nblocks, 256, lds, st>>>(gals, insts, blocks, tw, twn, lk_base, partial);
 	}
 	else {
 		if (int e = ensure_lds((const void *)k_fft_rows_inv<2>, lds, lds_rows_inv2)) return e;
-		k_fft_rows_inv<2><<<nblocks, 256, lds, st>>>(gals, insts, row_off, ninst, tw, twn, lk_base, partial);
+		k_fft_rows_inv<2><<<nblocks, 256, lds, st>>>(gals, insts, blocks, tw, twn, lk_base, partial);
 	}
 	return (int)hipGetLastError();
 }

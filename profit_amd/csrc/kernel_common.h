@@ -80,20 +80,27 @@ __device__ __forceinline__ double sample_accumulate(const Comp &c, double xp, do
 	return (double)o[0];
 }
 
-// Whether a test point lies within 1% of a King or Ferrer profile's cut-off
-// (rt, rout).  There the value is a difference of nearly equal terms (King:
-// A - C; Ferrer: 1 - t^(2-b)) whose FP32 relative error grows without bound,
-// and a cell across the cut can be nonzero in FP32 and zero in FP64; such
-// cells take the FP64 test.  The test point (|x|, |y| + |dy|) lies at least as
-// far out as the cell centre, so it stands for both.
+// Where a King or Ferrer cell lies against its profile's cut-off (rt, rout),
+// from the FP32 radii of the cell centre (x, y) and of the test point
+// (ax, ay), which lies at least as far out: 0 when the centre lies beyond
+// 1.01 x the cut (the value and the test are then exactly 0 in either
+// precision, so the cell is a leaf of value 0); 1 when the centre lies in
+// [0.97, 1.01] x the cut or the test point within 1% of it, where the value is
+// a difference of nearly equal terms (King: A - C; Ferrer: 1 - t^(2-b)) whose
+// FP32 relative error grows without bound and a point can fall on either side
+// of the cut (the FP64 test decides, and the leaf value is FP64); -1
+// otherwise, and for the other families.
 template <int TYPE>
-__device__ __forceinline__ bool near_cut(const Comp &c, float ax, float ay)
+__device__ __forceinline__ int cut_zone(const Comp &c, float x, float y, float ax, float ay)
 {
-	if (TYPE >= 0 && TYPE != KING && TYPE != FERRER) return false;
+	if (TYPE >= 0 && TYPE != KING && TYPE != FERRER) return -1;
 	const int t = TYPE >= 0 ? TYPE : c.type;
-	if (t != KING && t != FERRER) return false;
-	const float cut = (float)(t == KING ? c.s[1] : c.rscale);
-	return boxy_r<float>(c, ax, ay) > 0.99f * cut;
+	if (t != KING && t != FERRER) return -1;
+	const float cut = t == KING ? c.f.s[1] : c.f.rscale;
+	const float rc = boxy_r<float>(c, x, y), rt = boxy_r<float>(c, ax, ay);
+	if (rc > 1.01f * cut) return 0;
+	const bool band_c = rc >= 0.97f * cut, band_t = rt >= 0.99f * cut && rt <= 1.01f * cut;
+	return band_c || band_t ? 1 : -1;
 }
 
 // libprofit's refinement test |test - val| > acc * val (radial.cpp:92-100),
@@ -101,17 +108,19 @@ __device__ __forceinline__ bool near_cut(const Comp &c, float ax, float ay)
 // The decision is taken only when the FP32 margin lies outside a guard band
 // of 1e-3 * val around the threshold; the FP32 evaluation errors (values and
 // positions, a few parts in 1e7 of val) are far inside it, so a decision taken
-// here equals the FP64 one.  Cells near a King or Ferrer cut-off are always
-// undecided (near_cut).  Undecided cells take refine_cell64 at libprofit's
-// own FP64 cell position.
+// here equals the FP64 one.  King and Ferrer cells beyond their cut-off are
+// leaves; those near it are undecided (cut_zone).  Undecided cells take
+// refine_cell64 at libprofit's own FP64 cell position.
 template <int TYPE = -1>
 __device__ __forceinline__ int refine_cell32(const Comp &c, float xp, float yp, float dyp, float v32)
 {
 	const float ax = fabsf(xp), ay = fabsf(yp) + fabsf(dyp);
-	const float t32 = eval_at<float, TYPE>(c, ax, ay);
-	const float margin = fabsf(t32 - v32) - (float)c.acc * v32;
-	if (v32 > 1e-30f && isfinite(margin) && fabsf(margin) > 1e-3f * v32 && !near_cut<TYPE>(c, ax, ay))
-		return margin > 0.f ? 1 : 0;
+	const int zone = cut_zone<TYPE>(c, xp, yp, ax, ay);
+	if (zone == 0) return 0;
+	if (zone == 1) return 2;
+	const float t32 = eval_fast<TYPE>(c, ax, ay);
+	const float margin = fabsf(t32 - v32) - c.f.acc * v32;
+	if (v32 > 1e-30f && isfinite(margin) && fabsf(margin) > 1e-3f * v32) return margin > 0.f ? 1 : 0;
 	return 2;
 }
 
@@ -135,8 +144,8 @@ __device__ __forceinline__ void cell_pos32(const Comp &c, float ox, float oy, fl
 {
 	const float dx = ox + ((float)ci + 0.5f) * fxbin;
 	const float dy = oy + ((float)cj + 0.5f) * fybin;
-	*xp = dx * (float)c.cos_ang + dy * (float)c.sin_ang;
-	*yp = -dx * (float)c.sin_over + dy * (float)c.cos_over;
+	*xp = dx * c.f.cos_ang + dy * c.f.sin_ang;
+	*yp = -dx * c.f.sin_over + dy * c.f.cos_over;
 }
 
 // libprofit's FP64 cell position (its sequence of additions, radial.cpp:86-106)
@@ -158,9 +167,9 @@ __device__ __forceinline__ float pixel_accumulate_f32(const Comp &c, float xp, f
 	float o[9];
 	sample_grad<float, TYPE>(c, xp, yp, o);
 	// the rotation's chain rule with the per-component sin/q, cos/q and 1/q
-	const float q = (float)c.axrat, iq = (float)c.inv_axrat;
-	const float ca = (float)c.cos_ang, sa = (float)c.sin_ang;
-	const float sq = (float)c.sin_over, cq = (float)c.cos_over;
+	const float q = c.f.axrat, iq = c.f.inv_axrat;
+	const float ca = c.f.cos_ang, sa = c.f.sin_ang;
+	const float sq = c.f.sin_over, cq = c.f.cos_over;
 	const float fx = o[1], fy = o[2];
 	gs[G_XCEN] += wgt * (fx * -ca + fy * sq);
 	gs[G_YCEN] += wgt * (fx * -sa + fy * -cq);
@@ -189,6 +198,10 @@ __device__ __forceinline__ void walk_pixel(const Comp &c, WalkLevel *L, int lane
 	const int ncell = res * res;
 	const int nchunk = (ncell + 63) >> 6;
 	const double inv_cells = 1.0 / (double)ncell;
+	// cell -> (ci, cj) by an FP32 reciprocal: (cell + 0.5) / res lies at least
+	// 0.5 / res from an integer, far beyond the product's rounding, so the
+	// truncation is the exact quotient
+	const float inv_res = 1.f / (float)res;
 	const bool sums64 = !kGrad || c.grad64;
 	int level = 0;
 	double nx0 = x0, nx1 = x1, ny0 = y0, ny1 = y1;
@@ -204,18 +217,20 @@ __device__ __forceinline__ void walk_pixel(const Comp &c, WalkLevel *L, int lane
 		const float ox = (float)(nx0 - c.xc), oy = (float)(ny0 - c.yc);
 		const float fxbin = (float)xbin, fybin = (float)ybin, fdyp = (float)dyp;
 		const double w = weight * scale;
-		unsigned long long m[4] = {0ull, 0ull, 0ull, 0ull};
-#pragma unroll
-		for (int k = 0; k < 4; ++k) {
-			if (k >= nchunk) break;
+		// the cells in chunks of 64 (one per lane); lane 0 stores each chunk's
+		// refine mask with the node, and `any` gathers them.  A node with no
+		// cell to refine is not stored: the search below starts under it.
+		unsigned long long any = 0ull;
+#pragma unroll 1
+		for (int k = 0; k < nchunk; ++k) {
 			const int cell = k * 64 + lane;
 			bool fail = false;
 			if (cell < ncell) {
-				const int ci = cell / res, cj = cell - ci * res;
+				const int ci = (int)(((float)cell + 0.5f) * inv_res), cj = cell - ci * res;
 				float xp, yp;
 				cell_pos32(c, ox, oy, fxbin, fybin, ci, cj, &xp, &yp);
 				// the adjoint needs the FP32 value only for the refinement test
-				const float v32 = (kGrad && !recurse) ? 0.f : eval_at<float, TYPE>(c, xp, yp);
+				const float v32 = kGrad ? (recurse ? eval_fast<TYPE>(c, xp, yp) : 0.f) : eval_at<float, TYPE>(c, xp, yp);
 				int dec = recurse ? refine_cell32<TYPE>(c, xp, yp, fdyp, v32) : 0;
 				double v64 = 0;
 				bool have64 = false;
@@ -254,21 +269,27 @@ __device__ __forceinline__ void walk_pixel(const Comp &c, WalkLevel *L, int lane
 					}
 				}
 			}
-			m[k] = __ballot(fail);
+			const unsigned long long mk = __ballot(fail);
+			any |= mk;
+			if (lane == 0) L[level].mask[k] = mk;
 		}
-		if (!kGrad && level == 0 && lane == 0) *root_flag = (m[0] | m[1] | m[2] | m[3]) == 0ull;
-		if (lane == 0) {
-			L[level].x0 = nx0;
-			L[level].x1 = nx1;
-			L[level].y0 = ny0;
-			L[level].y1 = ny1;
-			L[level].weight = weight;
-#pragma unroll
-			for (int k = 0; k < 4; ++k) L[level].mask[k] = m[k];
+		if (!kGrad && level == 0 && lane == 0) *root_flag = any == 0ull;
+		if (any != 0ull) {
+			if (lane == 0) {
+				L[level].x0 = nx0;
+				L[level].x1 = nx1;
+				L[level].y0 = ny0;
+				L[level].y1 = ny1;
+				L[level].weight = weight;
+				for (int k = nchunk; k < 4; ++k) L[level].mask[k] = 0ull;
+			}
+		}
+		else if (level == 0) {
+			break;  // a leaf-only root: the tree is done
 		}
 		__builtin_amdgcn_wave_barrier();
 		// next node: lowest pending cell of the deepest level that has one
-		int lv = level;
+		int lv = any != 0ull ? level : level - 1;
 		int cell = -1;
 		for (;;) {
 			int found = -1;
@@ -531,9 +552,9 @@ __global__ __launch_bounds__(256) void k_adj_render(const GalaxyDev *__restrict_
 					const float Gq = I.G[i + j * g.DW];
 					if (Gq == 0.f) continue;
 					const double y = hy + j * g.sx;
-					const double xr = x - c.xc, yr = y - c.yc;
-					const float xp = (float)(xr * c.cos_ang + yr * c.sin_ang);
-					const float yp = (float)(-xr * c.sin_over + yr * c.cos_over);
+					const float xr = (float)(x - c.xc), yr = (float)(y - c.yc);
+					const float xp = xr * c.f.cos_ang + yr * c.f.sin_ang;
+					const float yp = -xr * c.f.sin_over + yr * c.f.cos_over;
 					const float wq = Gq * fs;
 					vf += wq * pixel_accumulate_f32<TYPE>(c, xp, yp, wq, gf);
 				}

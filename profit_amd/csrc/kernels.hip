@@ -77,6 +77,20 @@ __device__ inline PixelClass classify(const Comp &c, double x, double y)
 	return pc;
 }
 
+// The class of a pixel from FP32 profile coordinates (xp, yp) when r^2 lies
+// further than 1e-5 (relative) from both thresholds: the FP32 rotation errs
+// by a few parts in 1e7 of r^2, so such a class equals the FP64 one.  -1:
+// too close to a threshold, libprofit's FP64 classification decides.
+__device__ __forceinline__ int classify32(const Comp &c, float xp, float yp)
+{
+	const float r2 = xp * xp + yp * yp;
+	if (c.has_rmax && fabsf(r2 - c.f.rmax2) <= 1e-5f * c.f.rmax2) return -1;
+	if (c.has_rmax && r2 > c.f.rmax2) return 1;
+	if (c.rough) return 2;
+	if (fabsf(r2 - c.f.rsw2) <= 1e-5f * c.f.rsw2) return -1;
+	return r2 > c.f.rsw2 ? 2 : 3;
+}
+
 // ------------------------------------------------------------------ forward render
 
 // wave-aggregated append of the lanes with `take` to a work list
@@ -222,14 +236,26 @@ __global__ __launch_bounds__(256) void k_render(const GalaxyDev *__restrict__ ga
 			bool sub = false, heavy = false;
 			int cls = 0;
 			if (active) {
-				const PixelClass pc = classify(c, x, y);
-				cls = pc.cls;
-				if (pc.cls == 2) {
-					if (g.canvas64 && !target) acc64 += c.flux_scale * eval_at<double, TYPE>(c, pc.xp, pc.yp);
-					else if (g.render_fp64) acc += (float)(c.flux_scale * eval_at<double, TYPE>(c, pc.xp, pc.yp));
-					else acc += (float)c.flux_scale * eval_at<float, TYPE>(c, (float)pc.xp, (float)pc.yp);
+				// FP32 profile coordinates from the FP64 offsets; FP64 ones
+				// (libprofit's) only where the class or the value needs them
+				const float fxr = (float)(x - c.xc), fyr = (float)(y - c.yc);
+				const float fxp = fxr * c.f.cos_ang + fyr * c.f.sin_ang;
+				const float fyp = -fxr * c.f.sin_over + fyr * c.f.cos_over;
+				cls = classify32(c, fxp, fyp);
+				if (cls < 0) cls = classify(c, x, y).cls;
+				if (cls == 2) {
+					const bool c64 = g.canvas64 && !target;
+					if (c64 || g.render_fp64 || near_cut_fp64<TYPE>(c, fxp, fyp)) {
+						const PixelClass pc = classify(c, x, y);
+						const double v = c.flux_scale * eval_at<double, TYPE>(c, pc.xp, pc.yp);
+						if (c64) acc64 += v;
+						else acc += (float)v;
+					}
+					else {
+						acc += (float)c.flux_scale * eval_at<float, TYPE>(c, fxp, fyp);
+					}
 				}
-				else if (pc.cls == 3) {
+				else if (cls == 3) {
 					// the deep centre-pixel trees (sersic.cpp:331-347) go to their own list
 					heavy = centre_pixel(c, x, y);
 					sub = !heavy;

@@ -114,7 +114,27 @@ struct Comp {
 	// FP32 values and the derivatives multiply instead of dividing
 	double inv_re, inv_re2, inv_n, inv_nB, inv_2n;
 	double inv_axrat;
+	// FP32 copies of the constants the FP32 evaluations read, so that they
+	// enter FP32 arithmetic as scalar operands without conversions; aux per
+	// family: King 1/rc, 1/a and C = (1 + (rt/rc)^2)^(-1/a); Moffat and Ferrer
+	// 1/rscale; core-Sersic log2 rb, rb^a, re^-a, 1/(n a) and log2 re
+	struct F32 {
+		float cos_ang, sin_ang, cos_over, sin_over, axrat, inv_axrat;
+		float inv_re, inv_re2, inv_n, inv_nB, inv_2n, bn, dbn;
+		float acc, rscale, box, gk;
+		float rmax2, rsw2;  // (rscale_max rscale)^2, (rscale_switch rscale)^2
+		float s[5];
+		float aux[5];
+	} f;
 };
+
+// the FP32 copy of a component constant for T = float, the FP64 one otherwise
+template <class T>
+__host__ __device__ __forceinline__ T pick(double d, float f)
+{
+	if constexpr (sizeof(T) == 4) return f;
+	else return d;
+}
 
 // validation error codes (radial.cpp:204-214 and each profile's validate())
 enum SetupError : int32_t {
@@ -213,70 +233,162 @@ __device__ __forceinline__ float log2_fast(float x)
 __device__ __forceinline__ float exp2_fast(float x) { return __builtin_amdgcn_exp2f(x); }
 constexpr double kLog2e = 1.4426950408889634, kLn2 = 0.6931471805599453;
 
+// FP32 exp, log and pow to a few ulps on the transcendental unit, by
+// argument reduction: log2 x = k + log2 m with m = x 2^-k in [1/sqrt2, sqrt2),
+// whose logarithm (|log2 m| <= 1/2) the unit gives to an ulp; 2^y = 2^n
+// 2^(y - n) with n = rint(y).  The products that form y keep their rounding
+// errors (fma), so the result errs by a few ulps for any |y|, where
+// 2^(e log2 x) in plain FP32 errs by about |y| ulps.
+__device__ __forceinline__ float exp2_hl(float hi, float lo)
+{
+	const float n = rintf(hi);
+	return ldexpf(exp2_fast((hi - n) + lo), (int)n);
+}
+__device__ __forceinline__ float log2_kl(float x, float *k)
+{
+	int e;
+	float m = frexpf(x, &e);
+	if (m < 0.70710678f) {
+		m *= 2.f;
+		--e;
+	}
+	*k = (float)e;
+	return log2_fast(m);
+}
+__device__ __forceinline__ float pow_acc(float x, float e)
+{
+	float k;
+	const float l = log2_kl(x, &k);
+	const float p = e * k;
+	return exp2_hl(p, fmaf(e, l, fmaf(e, k, -p)));
+}
+__device__ __forceinline__ float exp_acc(float x)
+{
+	const float L = 1.44269502f, Llo = 1.92596299e-8f;  // log2 e = L + Llo
+	const float hi = x * L;
+	return exp2_hl(hi, fmaf(x, Llo, fmaf(x, L, -hi)));
+}
+__device__ __forceinline__ float log_acc(float x)
+{
+	float k;
+	const float l = log2_kl(x, &k);
+	// k ln 2 with ln 2 = 0.693145751953125 + 1.42860677e-6; the first product
+	// is exact (11 significant bits)
+	return fmaf(k, 0.693145751953125f, fmaf(k, 1.42860677e-6f, l * 0.693147182f));
+}
+
+// pow, exp and log in precision T: FP64 takes the library functions, FP32
+// the forms above.  The profiles' FP32 bases are never negative but for
+// rounding at a zero crossing (King's A - C, Ferrer's 1 - t^(2-b) at the
+// cut-off), so x <= 0 is taken as 0: 0 for e > 0, 1 for e = 0, inf below.
+template <class T>
+PA_HDI T pow_t(T x, T e)
+{
+	if constexpr (sizeof(T) == 4) {
+		if (x > T(0)) return e == T(0) ? T(1) : pow_acc(x, e);
+		if (x != x) return x;
+		return e > T(0) ? T(0) : (e == T(0) ? T(1) : T(INFINITY));
+	}
+	else {
+		return ::pow(x, e);
+	}
+}
+template <class T>
+PA_HDI T exp_t(T x)
+{
+	if constexpr (sizeof(T) == 4) return exp_acc(x);
+	else return ::exp(x);
+}
+template <class T>
+PA_HDI T log_t(T x)
+{
+	if constexpr (sizeof(T) == 4) {
+		if (x > T(0) && x < T(INFINITY)) return log_acc(x);
+	}
+	return ::log(x);
+}
+
 template <class T>
 PA_HDI T boxy_r(const Comp &c, T x, T y)
 {
-	if (!c.boxy) return ::sqrt(x * x + y * y);
-	const T b2 = T(c.box) + T(2);
-	return ::pow(::pow(::fabs(x), b2) + ::pow(::fabs(y), b2), T(1) / b2);
+	if constexpr (sizeof(T) == 4) {
+		// the hardware square root (about 1 ulp), not the correctly rounded one
+		if (!c.boxy) return __builtin_amdgcn_sqrtf(x * x + y * y);
+		const float b2 = c.f.box + 2.f;
+		return pow_t<float>(pow_t<float>(::fabsf(x), b2) + pow_t<float>(::fabsf(y), b2), 1.f / b2);
+	}
+	else {
+		if (!c.boxy) return ::sqrt(x * x + y * y);
+		const T b2 = pick<T>(c.box, c.f.box) + T(2);
+		return pow_t<T>(pow_t<T>(::fabs(x), b2) + pow_t<T>(::fabs(y), b2), T(1) / b2);
+	}
 }
 
 template <class T>
 PA_HDI T sersic_at(const Comp &c, T x, T y)
 {
-	const T re = T(c.s[0]), nser = T(c.s[1]), bn = T(c.bn);
-	T base, rf;
-	if (c.boxy) {
-		const T B = T(c.box) + T(2);
-		base = ::pow(::fabs(x / re), B) + ::pow(::fabs(y / re), B);
-		switch (c.kind) {
-		case K_HALF: rf = base * base; break;
-		case K_ONE: rf = base; break;
-		case K_TWO: rf = ::sqrt(base); break;
-		case K_THREE: rf = ::cbrt(base); break;
-		case K_FOUR: rf = ::sqrt(::sqrt(base)); break;
-		case K_EIGHT: rf = ::sqrt(::sqrt(::sqrt(base))); break;
-		case K_SIXTEEN: rf = ::sqrt(::sqrt(::sqrt(::sqrt(base)))); break;
-		default:
-			if constexpr (sizeof(T) == 4) rf = exp2_fast(log2_fast(base) * T(c.inv_nB));
-			else rf = ::pow(base, T(1) / (nser * B));
-			break;
+	if constexpr (sizeof(T) == 4) {
+		// base^(1/(n B)) = 2^(log2(base) / (n B)) and the exponential on the
+		// transcendental unit, for every n (the FP64 path keeps libprofit's
+		// special cases of n)
+		float rf;
+		if (c.boxy) {
+			const float B = c.f.box + 2.f;
+			const float base = pow_t<float>(::fabsf(x * c.f.inv_re), B) + pow_t<float>(::fabsf(y * c.f.inv_re), B);
+			rf = exp2_fast(log2_fast(base) * c.f.inv_nB);
 		}
+		else {
+			rf = exp2_fast(log2_fast((x * x + y * y) * c.f.inv_re2) * c.f.inv_2n);
+		}
+		return exp2_fast(-c.f.bn * (rf - 1.f) * float(kLog2e));
 	}
 	else {
-		if constexpr (sizeof(T) == 4) base = (x * x + y * y) * T(c.inv_re2);
-		else base = (x * x + y * y) / (re * re);
-		switch (c.kind) {
-		case K_HALF: rf = base; break;
-		case K_ONE: rf = ::sqrt(base); break;
-		case K_TWO: rf = ::sqrt(::sqrt(base)); break;
-		case K_THREE: rf = ::cbrt(::sqrt(base)); break;
-		case K_FOUR: rf = ::sqrt(::sqrt(::sqrt(base))); break;
-		case K_EIGHT: rf = ::sqrt(::sqrt(::sqrt(::sqrt(base)))); break;
-		case K_SIXTEEN: rf = ::sqrt(::sqrt(::sqrt(::sqrt(::sqrt(base))))); break;
-		default:
-			// base^(1/(2n)) = 2^(log2(base) / (2n)) in FP32
-			if constexpr (sizeof(T) == 4) rf = exp2_fast(log2_fast(base) * T(c.inv_2n));
-			else rf = ::pow(::sqrt(base), T(1) / nser);
-			break;
+		const T re = pick<T>(c.s[0], c.f.s[0]), nser = pick<T>(c.s[1], c.f.s[1]), bn = pick<T>(c.bn, c.f.bn);
+		T base, rf;
+		if (c.boxy) {
+			const T B = pick<T>(c.box, c.f.box) + T(2);
+			base = pow_t<T>(::fabs(x / re), B) + pow_t<T>(::fabs(y / re), B);
+			switch (c.kind) {
+			case K_HALF: rf = base * base; break;
+			case K_ONE: rf = base; break;
+			case K_TWO: rf = ::sqrt(base); break;
+			case K_THREE: rf = ::cbrt(base); break;
+			case K_FOUR: rf = ::sqrt(::sqrt(base)); break;
+			case K_EIGHT: rf = ::sqrt(::sqrt(::sqrt(base))); break;
+			case K_SIXTEEN: rf = ::sqrt(::sqrt(::sqrt(::sqrt(base)))); break;
+			default: rf = pow_t<T>(base, T(1) / (nser * B)); break;
+			}
 		}
+		else {
+			base = (x * x + y * y) / (re * re);
+			switch (c.kind) {
+			case K_HALF: rf = base; break;
+			case K_ONE: rf = ::sqrt(base); break;
+			case K_TWO: rf = ::sqrt(::sqrt(base)); break;
+			case K_THREE: rf = ::cbrt(::sqrt(base)); break;
+			case K_FOUR: rf = ::sqrt(::sqrt(::sqrt(base))); break;
+			case K_EIGHT: rf = ::sqrt(::sqrt(::sqrt(::sqrt(base)))); break;
+			case K_SIXTEEN: rf = ::sqrt(::sqrt(::sqrt(::sqrt(::sqrt(base))))); break;
+			default: rf = pow_t<T>(::sqrt(base), T(1) / nser); break;
+			}
+		}
+		return exp_t<T>(-bn * (rf - T(1)));
 	}
-	if constexpr (sizeof(T) == 4) return exp2_fast(-bn * (rf - T(1)) * T(kLog2e));
-	return ::exp(-bn * (rf - T(1)));
 }
 
 template <class T>
 PA_HDI T king_core(T r, T rc, T rt, T a)
 {
-	return ::pow(T(1) / ::pow(T(1) + ::pow(r / rc, T(2)), T(1) / a) - T(1) / ::pow(T(1) + ::pow(rt / rc, T(2)), T(1) / a), a);
+	return pow_t<T>(T(1) / pow_t<T>(T(1) + pow_t<T>(r / rc, T(2)), T(1) / a) - T(1) / pow_t<T>(T(1) + pow_t<T>(rt / rc, T(2)), T(1) / a), a);
 }
 
 template <class T>
 PA_HDI T coresersic_core(const Comp &c, T r)
 {
-	const T re = T(c.s[0]), rb = T(c.s[1]), nser = T(c.s[2]), a = T(c.s[3]), b = T(c.s[4]);
-	return ::pow(T(1) + ::pow(r / rb, -a), b / a) *
-	       ::exp(-T(c.bn) * ::pow((::pow(r, a) + ::pow(rb, a)) / ::pow(re, a), T(1) / (nser * a)));
+	const T re = pick<T>(c.s[0], c.f.s[0]), rb = pick<T>(c.s[1], c.f.s[1]), nser = pick<T>(c.s[2], c.f.s[2]);
+	const T a = pick<T>(c.s[3], c.f.s[3]), b = pick<T>(c.s[4], c.f.s[4]);
+	return pow_t<T>(T(1) + pow_t<T>(r / rb, -a), b / a) *
+	       exp_t<T>(-pick<T>(c.bn, c.f.bn) * pow_t<T>((pow_t<T>(r, a) + pow_t<T>(rb, a)) / pow_t<T>(re, a), T(1) / (nser * a)));
 }
 
 template <class T>
@@ -284,8 +396,8 @@ PA_HDI T brokenexp_core(T r, T h1, T h2, T rb, T a)
 {
 	T base = r - rb;
 	const T expo = T(1) / h1 - T(1) / h2;
-	if (a * base < T(40)) base = ::log(T(1) + ::exp(a * base)) / a;
-	return ::exp(-r / h1 + expo * base);
+	if (a * base < T(40)) base = log_t<T>(T(1) + exp_t<T>(a * base)) / a;
+	return exp_t<T>(-r / h1 + expo * base);
 }
 
 // d ln f / dr and d ln f / d shape of the core-Sersic profile
@@ -294,18 +406,18 @@ PA_HDI T brokenexp_core(T r, T h1, T h2, T rb, T a)
 template <class T>
 PA_HDI T coresersic_dlog(const Comp &c, T r, T *ds)
 {
-	const T re = T(c.s[0]), rb = T(c.s[1]), n = T(c.s[2]), a = T(c.s[3]), b = T(c.s[4]), bn = T(c.bn);
-	const T t = ::pow(r / rb, -a), Q = T(1) + t;
-	const T ra = ::pow(r, a), rba = ::pow(rb, a), sum = ra + rba;
-	const T S = sum / ::pow(re, a);
-	const T lS = ::log(S);
-	const T W = ::exp(lS / (n * a));
+	const T re = pick<T>(c.s[0], c.f.s[0]), rb = pick<T>(c.s[1], c.f.s[1]), n = pick<T>(c.s[2], c.f.s[2]), a = pick<T>(c.s[3], c.f.s[3]), b = pick<T>(c.s[4], c.f.s[4]), bn = pick<T>(c.bn, c.f.bn);
+	const T t = pow_t<T>(r / rb, -a), Q = T(1) + t;
+	const T ra = pow_t<T>(r, a), rba = pow_t<T>(rb, a), sum = ra + rba;
+	const T S = sum / pow_t<T>(re, a);
+	const T lS = log_t<T>(S);
+	const T W = exp_t<T>(lS / (n * a));
 	ds[0] = bn * W / (n * re);
 	ds[1] = b * t / (rb * Q) - bn * W * rba / (rb * n * sum);
-	ds[2] = -T(c.dbn) * W + bn * W * lS / (n * n * a);
-	const T dlnS_da = (ra * ::log(r) + rba * ::log(rb)) / sum - ::log(re);
-	ds[3] = -(b / (a * a)) * ::log(Q) - (b / a) * t * ::log(r / rb) / Q - bn * W * (dlnS_da / (n * a) - lS / (n * a * a));
-	ds[4] = ::log(Q) / a;
+	ds[2] = -pick<T>(c.dbn, c.f.dbn) * W + bn * W * lS / (n * n * a);
+	const T dlnS_da = (ra * log_t<T>(r) + rba * log_t<T>(rb)) / sum - log_t<T>(re);
+	ds[3] = -(b / (a * a)) * log_t<T>(Q) - (b / a) * t * log_t<T>(r / rb) / Q - bn * W * (dlnS_da / (n * a) - lS / (n * a * a));
+	ds[4] = log_t<T>(Q) / a;
 	return -b * t / (r * Q) - bn * W * ra / (r * n * sum);
 }
 
@@ -319,8 +431,8 @@ PA_HDI T brokenexp_dlog(T r, T h1, T h2, T rb, T a, T *ds)
 	const T expo = T(1) / h1 - T(1) / h2;
 	T B, sig, dBda;
 	if (z < T(40)) {
-		const T ez = ::exp(z);
-		B = ::log(T(1) + ez) / a;
+		const T ez = exp_t<T>(z);
+		B = log_t<T>(T(1) + ez) / a;
 		sig = ez / (T(1) + ez);
 		dBda = (r - rb) * sig / a - B / a;
 	}
@@ -336,30 +448,132 @@ PA_HDI T brokenexp_dlog(T r, T h1, T h2, T rb, T a, T *ds)
 	return -T(1) / h1 + expo * sig;
 }
 
+// The core-Sersic profile in FP32 from one logarithm of r:
+// t = (r/rb)^-a, Q = 1 + t, S = (r^a + rb^a) / re^a, W = S^(1/(n a)),
+// f = Q^(b/a) exp(-bn W).  With ds non-null it also gives d ln f / d shape
+// (ds[0..4]) and returns d ln f / dr in *dlog, as coresersic_dlog does.
+__device__ __forceinline__ float coresersic_f32(const Comp &c, float r, float *ds, float *dlog)
+{
+	const Comp::F32 &k = c.f;
+	const float re = k.s[0], rb = k.s[1], a = k.s[3], b = k.s[4], bn = k.bn;
+	const float lrb = k.aux[0], rba = k.aux[1], rea = k.aux[2], ina = k.aux[3], lre = k.aux[4];
+	const float lr = log2_fast(r);
+	const float t = exp2_fast(-a * (lr - lrb)), Q = 1.f + t, lQ = log2_fast(Q);
+	const float ra = exp2_fast(a * lr), sum = ra + rba, S = sum * rea, lS2 = log2_fast(S);
+	const float W = exp2_fast(lS2 * ina);
+	const float ia = 1.f / a;
+	const float e1 = b == 0.f ? 0.f : b * ia * lQ;
+	const float f = exp2_fast(e1 - bn * W * float(kLog2e));
+	if (ds) {
+		const float ln2 = float(kLn2), in = a * ina, lS = lS2 * ln2;
+		ds[0] = bn * W * in / re;
+		ds[1] = b * t / (rb * Q) - bn * W * rba * in / (rb * sum);
+		ds[2] = -k.dbn * W + bn * W * lS * ina * in;
+		const float dlnS_da = (ra * lr + rba * lrb) * ln2 / sum - lre * ln2;
+		ds[3] = -(b * ia * ia) * lQ * ln2 - b * ia * t * (lr - lrb) * ln2 / Q - bn * W * (dlnS_da * ina - lS * ina * ia);
+		ds[4] = lQ * ln2 * ia;
+		*dlog = -b * t / (r * Q) - bn * W * ra * in / (r * sum);
+	}
+	return f;
+}
+
+// King and Ferrer values within 3% of the cut-off (rt, rout) lose relative
+// precision in FP32 (A - C and 1 - (r/rout)^(2-b) cancel, and r itself is
+// rounded): directly evaluated pixels there take the FP64 evaluation.
+template <int TYPE = -1>
+__device__ __forceinline__ bool near_cut_fp64(const Comp &c, float xp, float yp)
+{
+	if (TYPE >= 0 && TYPE != KING && TYPE != FERRER) return false;
+	const int t = TYPE >= 0 ? TYPE : c.type;
+	if (t != KING && t != FERRER) return false;
+	const float cut = t == KING ? c.f.s[1] : c.f.rscale;
+	return boxy_r<float>(c, xp, yp) >= 0.97f * cut;
+}
+
+// The profile in FP32 with the plain transcendental forms (2^(e log2 x),
+// errors of about |e log2 x| ulps): the refinement test's second value,
+// which only has to fall on the right side of a guard band of 1e-3.
+template <int TYPE = -1>
+__device__ __forceinline__ float eval_fast(const Comp &c, float x, float y)
+{
+	const Comp::F32 &f = c.f;
+	switch (TYPE >= 0 ? TYPE : c.type) {
+	case SERSIC: return sersic_at<float>(c, x, y);
+	case MOFFAT: {
+		const float rf = boxy_r<float>(c, x, y) * f.aux[0];
+		return exp2_fast(-f.s[1] * log2_fast(1.f + rf * rf));
+	}
+	case FERRER: {
+		const float rf = boxy_r<float>(c, x, y) * f.aux[0];
+		if (!(rf < 1.f)) return 0.f;
+		const float g = 1.f - exp2_fast((2.f - f.s[2]) * log2_fast(rf));
+		return g > 0.f ? exp2_fast(f.s[1] * log2_fast(g)) : 0.f;
+	}
+	case KING: {
+		const float r = boxy_r<float>(c, x, y);
+		if (!(r < f.s[1])) return 0.f;
+		const float u = r * f.aux[0];
+		const float D = exp2_fast(-f.aux[1] * log2_fast(1.f + u * u)) - f.aux[2];
+		return D > 0.f ? exp2_fast(f.s[2] * log2_fast(D)) : 0.f;
+	}
+	case CORESERSIC: return coresersic_f32(c, boxy_r<float>(c, x, y), nullptr, nullptr);
+	case BROKENEXP: return brokenexp_core<float>(boxy_r<float>(c, x, y), f.s[0], f.s[1], f.s[2], f.s[3]);
+	default: return 0.f;
+	}
+}
+
 // TYPE >= 0 compiles the evaluation for that profile family only (the
 // kernels specialised per family); -1 dispatches on c.type at run time.
 template <class T, int TYPE = -1>
 PA_HDI T eval_at(const Comp &c, T x, T y)
 {
-	switch (TYPE >= 0 ? TYPE : c.type) {
-	case SERSIC: return sersic_at<T>(c, x, y);
-	case MOFFAT: {
-		const T rf = boxy_r<T>(c, x, y) / T(c.rscale);
-		return ::pow(T(1) + rf * rf, -T(c.s[1]));
+	if constexpr (sizeof(T) == 4) {
+		// FP32: the constants' FP32 copies, multiplications by reciprocals,
+		// and King's constant term (1 + (rt/rc)^2)^(-1/a) from the set-up
+		const Comp::F32 &f = c.f;
+		switch (TYPE >= 0 ? TYPE : c.type) {
+		case SERSIC: return sersic_at<float>(c, x, y);
+		case MOFFAT: {
+			const float rf = boxy_r<float>(c, x, y) * f.aux[0];
+			return pow_t<float>(1.f + rf * rf, -f.s[1]);
+		}
+		case FERRER: {
+			const float rf = boxy_r<float>(c, x, y) * f.aux[0];
+			if (rf < 1.f) return pow_t<float>(1.f - pow_t<float>(rf, 2.f - f.s[2]), f.s[1]);
+			return 0.f;
+		}
+		case KING: {
+			const float r = boxy_r<float>(c, x, y);
+			if (!(r < f.s[1])) return 0.f;
+			const float u = r * f.aux[0];
+			return pow_t<float>(pow_t<float>(1.f + u * u, -f.aux[1]) - f.aux[2], f.s[2]);
+		}
+		case CORESERSIC: return coresersic_core<float>(c, boxy_r<float>(c, x, y));
+		case BROKENEXP: return brokenexp_core<float>(boxy_r<float>(c, x, y), f.s[0], f.s[1], f.s[2], f.s[3]);
+		default: return 0.f;
+		}
 	}
-	case FERRER: {
-		const T rf = boxy_r<T>(c, x, y) / T(c.rscale);
-		if (rf < T(1)) return ::pow(T(1) - ::pow(rf, T(2) - T(c.s[2])), T(c.s[1]));
-		return T(0);
-	}
-	case KING: {
-		const T r = boxy_r<T>(c, x, y);
-		if (r < T(c.s[1])) return king_core<T>(r, T(c.s[0]), T(c.s[1]), T(c.s[2]));
-		return T(0);
-	}
-	case CORESERSIC: return coresersic_core<T>(c, boxy_r<T>(c, x, y));
-	case BROKENEXP: return brokenexp_core<T>(boxy_r<T>(c, x, y), T(c.s[0]), T(c.s[1]), T(c.s[2]), T(c.s[3]));
-	default: return T(0);
+	else {
+		switch (TYPE >= 0 ? TYPE : c.type) {
+		case SERSIC: return sersic_at<T>(c, x, y);
+		case MOFFAT: {
+			const T rf = boxy_r<T>(c, x, y) / pick<T>(c.rscale, c.f.rscale);
+			return pow_t<T>(T(1) + rf * rf, -pick<T>(c.s[1], c.f.s[1]));
+		}
+		case FERRER: {
+			const T rf = boxy_r<T>(c, x, y) / pick<T>(c.rscale, c.f.rscale);
+			if (rf < T(1)) return pow_t<T>(T(1) - pow_t<T>(rf, T(2) - pick<T>(c.s[2], c.f.s[2])), pick<T>(c.s[1], c.f.s[1]));
+			return T(0);
+		}
+		case KING: {
+			const T r = boxy_r<T>(c, x, y);
+			if (r < pick<T>(c.s[1], c.f.s[1])) return king_core<T>(r, pick<T>(c.s[0], c.f.s[0]), pick<T>(c.s[1], c.f.s[1]), pick<T>(c.s[2], c.f.s[2]));
+			return T(0);
+		}
+		case CORESERSIC: return coresersic_core<T>(c, boxy_r<T>(c, x, y));
+		case BROKENEXP: return brokenexp_core<T>(boxy_r<T>(c, x, y), pick<T>(c.s[0], c.f.s[0]), pick<T>(c.s[1], c.f.s[1]), pick<T>(c.s[2], c.f.s[2]), pick<T>(c.s[3], c.f.s[3]));
+		default: return T(0);
+		}
 	}
 }
 
@@ -370,36 +584,36 @@ PA_HDI void dr_terms(const Comp &c, T x, T y, T r, T *drx, T *dry, T *drB)
 {
 	*drx = *dry = *drB = T(0);
 	if (!(r > T(0))) return;
-	const T B = T(c.box) + T(2);
+	const T B = pick<T>(c.box, c.f.box) + T(2);
 	const T ax = ::fabs(x), ay = ::fabs(y);
 	if (!c.boxy) {
 		*drx = x / r;
 		*dry = y / r;
 	}
 	else {
-		const T rp = ::pow(r, T(1) - B);
-		*drx = ax > T(0) ? rp * ::pow(ax, B - T(1)) * (x > T(0) ? T(1) : T(-1)) : T(0);
-		*dry = ay > T(0) ? rp * ::pow(ay, B - T(1)) * (y > T(0) ? T(1) : T(-1)) : T(0);
+		const T rp = pow_t<T>(r, T(1) - B);
+		*drx = ax > T(0) ? rp * pow_t<T>(ax, B - T(1)) * (x > T(0) ? T(1) : T(-1)) : T(0);
+		*dry = ay > T(0) ? rp * pow_t<T>(ay, B - T(1)) * (y > T(0) ? T(1) : T(-1)) : T(0);
 	}
 	if (!c.grad_box) return;
-	const T px = ::pow(ax, B), py = ::pow(ay, B);
+	const T px = pow_t<T>(ax, B), py = pow_t<T>(ay, B);
 	const T S = px + py;
-	const T Sl = (ax > T(0) ? px * ::log(ax) : T(0)) + (ay > T(0) ? py * ::log(ay) : T(0));
-	*drB = r * (Sl / (B * S) - ::log(S) / (B * B));
+	const T Sl = (ax > T(0) ? px * log_t<T>(ax) : T(0)) + (ay > T(0) ? py * log_t<T>(ay) : T(0));
+	*drB = r * (Sl / (B * S) - log_t<T>(S) / (B * B));
 }
 
 template <class T>
 PA_HDI T king_dcore(T r, T rc, T rt, T a, int which)
 {
 	const T u2 = (r / rc) * (r / rc), w2 = (rt / rc) * (rt / rc);
-	const T A = ::pow(T(1) + u2, -T(1) / a), C = ::pow(T(1) + w2, -T(1) / a), D = A - C;
+	const T A = pow_t<T>(T(1) + u2, -T(1) / a), C = pow_t<T>(T(1) + w2, -T(1) / a), D = A - C;
 	if (!(D > T(0))) return T(0);
-	const T Da1 = a * ::pow(D, a - T(1));
+	const T Da1 = a * pow_t<T>(D, a - T(1));
 	switch (which) {
 	case -1: return Da1 * (-(T(2) * r / (a * rc * rc)) * A / (T(1) + u2));
 	case 0: return Da1 * ((T(2) * u2 / (a * rc)) * A / (T(1) + u2) - (T(2) * w2 / (a * rc)) * C / (T(1) + w2));
 	case 1: return Da1 * ((T(2) * rt / (a * rc * rc)) * C / (T(1) + w2));
-	default: return ::pow(D, a) * (::log(D) + (A * ::log(T(1) + u2) - C * ::log(T(1) + w2)) / (a * D));
+	default: return pow_t<T>(D, a) * (log_t<T>(D) + (A * log_t<T>(T(1) + u2) - C * log_t<T>(T(1) + w2)) / (a * D));
 	}
 }
 
@@ -414,13 +628,13 @@ PA_HDI void sample_grad(const Comp &c, T x, T y, T *o)
 		// Without box the powers are squares, and f follows from the same
 		// logarithm as the derivatives: one log and two exps per sample
 		// (plus two logs when the box derivative is wanted).
-		const T bn = T(c.bn), B = T(c.box) + T(2);
-		const T ire = T(c.inv_re), in = T(c.inv_n), inB = T(c.inv_nB);
+		const T bn = pick<T>(c.bn, c.f.bn), B = pick<T>(c.box, c.f.box) + T(2);
+		const T ire = pick<T>(c.inv_re, c.f.inv_re), in = pick<T>(c.inv_n, c.f.inv_n), inB = pick<T>(c.inv_nB, c.f.inv_nB);
 		const T u = ::fabs(x * ire), v = ::fabs(y * ire);
 		T pu, pv;
 		if (c.boxy) {
-			pu = ::pow(u, B);
-			pv = ::pow(v, B);
+			pu = pow_t<T>(u, B);
+			pv = pow_t<T>(v, B);
 		}
 		else {
 			pu = u * u;
@@ -428,7 +642,7 @@ PA_HDI void sample_grad(const Comp &c, T x, T y, T *o)
 		}
 		const T base = pu + pv;
 		if (!(base > T(0))) {
-			o[0] = ::exp(bn);
+			o[0] = exp_t<T>(bn);
 			return;
 		}
 		T lb, rf, f;
@@ -439,9 +653,9 @@ PA_HDI void sample_grad(const Comp &c, T x, T y, T *o)
 			lb = l2 * T(kLn2);
 		}
 		else {
-			lb = ::log(base);
-			rf = ::exp(lb * inB);
-			f = ::exp(-bn * (rf - T(1)));
+			lb = log_t<T>(base);
+			rf = exp_t<T>(lb * inB);
+			f = exp_t<T>(-bn * (rf - T(1)));
 		}
 		o[0] = f;
 		const T k = -bn * f * rf * inB / base;  // df/dbase
@@ -451,17 +665,32 @@ PA_HDI void sample_grad(const Comp &c, T x, T y, T *o)
 		o[1] = k * bx * ire * (x > T(0) ? T(1) : T(-1));
 		o[2] = k * by * ire * (y > T(0) ? T(1) : T(-1));
 		if (c.grad_box) {
-			const T dbase_dB = (u > T(0) ? pu * ::log(u) : T(0)) + (v > T(0) ? pv * ::log(v) : T(0));
+			const T dbase_dB = (u > T(0) ? pu * log_t<T>(u) : T(0)) + (v > T(0) ? pv * log_t<T>(v) : T(0));
 			o[3] = -bn * f * rf * (dbase_dB * inB / base - lb * inB / B);
 		}
 		o[4] = bn * f * rf * in * ire;
-		o[5] = f * (-T(c.dbn) * (rf - T(1)) + bn * rf * lb * in * inB);
+		o[5] = f * (-pick<T>(c.dbn, c.f.dbn) * (rf - T(1)) + bn * rf * lb * in * inB);
 		return;
 	}
 	case MOFFAT: {
-		const T r = boxy_r<T>(c, x, y), rs = T(c.rscale), con = T(c.s[1]);
+		const T r = boxy_r<T>(c, x, y), rs = pick<T>(c.rscale, c.f.rscale), con = pick<T>(c.s[1], c.f.s[1]);
 		const T t = r / rs, q = T(1) + t * t;
-		const T f = ::pow(q, -con);
+		if constexpr (sizeof(T) == 4) {
+			// one logarithm for the value and the con derivative
+			const float lq = log2_fast(q), f = exp2_fast(-con * lq);
+			o[0] = f;
+			const float dfdr = -con * f / q * 2.f * t * c.f.aux[0];
+			float drx, dry, drB;
+			dr_terms<float>(c, x, y, r, &drx, &dry, &drB);
+			o[1] = dfdr * drx;
+			o[2] = dfdr * dry;
+			o[3] = dfdr * drB;
+			const float dfdrs = -dfdr * r * c.f.aux[0];
+			o[4] = dfdrs * rs / c.f.s[0];
+			o[5] = -f * lq * float(kLn2) + dfdrs * rs * c.f.gk;
+			return;
+		}
+		const T f = pow_t<T>(q, -con);
 		o[0] = f;
 		const T dfdr = -con * f / q * T(2) * t / rs;
 		T drx, dry, drB;
@@ -470,32 +699,74 @@ PA_HDI void sample_grad(const Comp &c, T x, T y, T *o)
 		o[2] = dfdr * dry;
 		o[3] = dfdr * drB;
 		const T dfdrs = -dfdr * r / rs;
-		o[4] = dfdrs * rs / T(c.s[0]);
-		o[5] = -f * ::log(q) + dfdrs * rs * T(c.gk);
+		o[4] = dfdrs * rs / pick<T>(c.s[0], c.f.s[0]);
+		o[5] = -f * log_t<T>(q) + dfdrs * rs * pick<T>(c.gk, c.f.gk);
 		return;
 	}
 	case FERRER: {
-		const T r = boxy_r<T>(c, x, y), rout = T(c.s[0]), a = T(c.s[1]), b = T(c.s[2]);
+		const T r = boxy_r<T>(c, x, y), rout = pick<T>(c.s[0], c.f.s[0]), a = pick<T>(c.s[1], c.f.s[1]), b = pick<T>(c.s[2], c.f.s[2]);
 		const T t = r / rout;
 		if (!(t < T(1))) return;
-		const T g = T(1) - ::pow(t, T(2) - b);
-		const T f = ::pow(g, a);
+		if constexpr (sizeof(T) == 4) {
+			// t^(2-b), g^a and their logarithms shared: g^(a-1) = f / g,
+			// t^(1-b) = t^(2-b) / t
+			const float lt = log2_fast(t), tb = exp2_fast((2.f - b) * lt), g = 1.f - tb;
+			if (!(g > 0.f)) return;
+			const float lg = log2_fast(g), f = exp2_fast(a * lg);
+			o[0] = f;
+			float drx, dry, drB;
+			dr_terms<float>(c, x, y, r, &drx, &dry, &drB);
+			const float ga1 = f / g;
+			const float dfdr = t > 0.f ? a * ga1 * (-(2.f - b) * tb / t) / rout : 0.f;
+			o[1] = dfdr * drx;
+			o[2] = dfdr * dry;
+			o[3] = dfdr * drB;
+			o[4] = -dfdr * r / rout;
+			o[5] = f * lg * float(kLn2);
+			o[6] = t > 0.f ? a * ga1 * tb * lt * float(kLn2) : 0.f;
+			return;
+		}
+		const T g = T(1) - pow_t<T>(t, T(2) - b);
+		const T f = pow_t<T>(g, a);
 		o[0] = f;
 		T drx, dry, drB;
 		dr_terms<T>(c, x, y, r, &drx, &dry, &drB);
-		const T dfdr = r > T(0) ? a * ::pow(g, a - T(1)) * (-(T(2) - b) * ::pow(t, T(1) - b)) / rout : T(0);
+		const T dfdr = r > T(0) ? a * pow_t<T>(g, a - T(1)) * (-(T(2) - b) * pow_t<T>(t, T(1) - b)) / rout : T(0);
 		o[1] = dfdr * drx;
 		o[2] = dfdr * dry;
 		o[3] = dfdr * drB;
 		o[4] = -dfdr * r / rout;
-		o[5] = g > T(0) ? f * ::log(g) : T(0);
-		o[6] = t > T(0) ? a * ::pow(g, a - T(1)) * ::pow(t, T(2) - b) * ::log(t) : T(0);
+		o[5] = g > T(0) ? f * log_t<T>(g) : T(0);
+		o[6] = t > T(0) ? a * pow_t<T>(g, a - T(1)) * pow_t<T>(t, T(2) - b) * log_t<T>(t) : T(0);
 		return;
 	}
 	case KING: {
 		const T r = boxy_r<T>(c, x, y);
-		const T rc = T(c.s[0]), rt = T(c.s[1]), a = T(c.s[2]);
+		const T rc = pick<T>(c.s[0], c.f.s[0]), rt = pick<T>(c.s[1], c.f.s[1]), a = pick<T>(c.s[2], c.f.s[2]);
 		if (!(r < rt)) return;
+		if constexpr (sizeof(T) == 4) {
+			// A = (1 + u^2)^(-1/a), D = A - C, f = D^a, with the logarithms
+			// shared among the value and the four derivatives
+			const float irc = c.f.aux[0], ia = c.f.aux[1], C = c.f.aux[2];
+			const float u = r * irc, q = 1.f + u * u, lq = log2_fast(q);
+			const float A = exp2_fast(-ia * lq), D = A - C;
+			if (!(D > 0.f)) return;
+			const float lD = log2_fast(D), f = exp2_fast(a * lD);
+			o[0] = f;
+			const float Da1 = a * f / D;  // a D^(a-1)
+			const float w = rt * irc, qw = 1.f + w * w;
+			const float Aq = A / q, Cq = C / qw;
+			const float dfdr = Da1 * (-2.f * ia * irc * u) * Aq;
+			float drx, dry, drB;
+			dr_terms<float>(c, x, y, r, &drx, &dry, &drB);
+			o[1] = dfdr * drx;
+			o[2] = dfdr * dry;
+			o[3] = dfdr * drB;
+			o[4] = Da1 * 2.f * ia * irc * (u * u * Aq - w * w * Cq);
+			o[5] = Da1 * 2.f * ia * irc * w * Cq;
+			o[6] = f * float(kLn2) * (lD + (A * lq - C * log2_fast(qw)) * ia / D);
+			return;
+		}
 		o[0] = king_core<T>(r, rc, rt, a);
 		T drx, dry, drB;
 		dr_terms<T>(c, x, y, r, &drx, &dry, &drB);
@@ -508,6 +779,23 @@ PA_HDI void sample_grad(const Comp &c, T x, T y, T *o)
 	}
 	case CORESERSIC: {
 		const T r = boxy_r<T>(c, x, y);
+		if constexpr (sizeof(T) == 4) {
+			if (!(r > 0.f)) {
+				o[0] = coresersic_f32(c, r, nullptr, nullptr);
+				return;
+			}
+			float ds[5], dlog;
+			const float f = coresersic_f32(c, r, ds, &dlog);
+			o[0] = f;
+			const float dfdr = f * dlog;
+			float drx, dry, drB;
+			dr_terms<float>(c, x, y, r, &drx, &dry, &drB);
+			o[1] = dfdr * drx;
+			o[2] = dfdr * dry;
+			o[3] = dfdr * drB;
+			for (int w = 0; w < 5; ++w) o[4 + w] = f * ds[w];
+			return;
+		}
 		const T f = coresersic_core<T>(c, r);
 		o[0] = f;
 		if (!(r > T(0))) return;
@@ -523,7 +811,7 @@ PA_HDI void sample_grad(const Comp &c, T x, T y, T *o)
 	}
 	case BROKENEXP: {
 		const T r = boxy_r<T>(c, x, y);
-		const T h1 = T(c.s[0]), h2 = T(c.s[1]), rb = T(c.s[2]), a = T(c.s[3]);
+		const T h1 = pick<T>(c.s[0], c.f.s[0]), h2 = pick<T>(c.s[1], c.f.s[1]), rb = pick<T>(c.s[2], c.f.s[2]), a = pick<T>(c.s[3], c.f.s[3]);
 		const T f = brokenexp_core<T>(r, h1, h2, rb, a);
 		o[0] = f;
 		T ds[4];
@@ -859,6 +1147,43 @@ PA_HD int setup_component(const ProfileParams &p, double magzero, double sx, dou
 		break;
 	}
 	default: break;
+	}
+	Comp::F32 &f = c->f;
+	f.cos_ang = (float)c->cos_ang;
+	f.sin_ang = (float)c->sin_ang;
+	f.cos_over = (float)c->cos_over;
+	f.sin_over = (float)c->sin_over;
+	f.axrat = (float)c->axrat;
+	f.inv_axrat = (float)c->inv_axrat;
+	f.inv_re = (float)c->inv_re;
+	f.inv_re2 = (float)c->inv_re2;
+	f.inv_n = (float)c->inv_n;
+	f.inv_nB = (float)c->inv_nB;
+	f.inv_2n = (float)c->inv_2n;
+	f.bn = (float)c->bn;
+	f.dbn = (float)c->dbn;
+	f.acc = (float)c->acc;
+	f.rscale = (float)c->rscale;
+	f.box = (float)c->box;
+	f.gk = (float)c->gk;
+	f.rmax2 = (float)(c->rmax_r * c->rmax_r);
+	f.rsw2 = (float)(c->rsw_r * c->rsw_r);
+	for (int i = 0; i < 5; ++i) f.s[i] = (float)c->s[i];
+	for (int i = 0; i < 5; ++i) f.aux[i] = 0.f;
+	if (p.type == CORESERSIC) {
+		f.aux[0] = (float)::log2(s[1]);
+		f.aux[1] = (float)::pow(s[1], s[3]);
+		f.aux[2] = (float)::pow(s[0], -s[3]);
+		f.aux[3] = (float)(1 / (s[2] * s[3]));
+		f.aux[4] = (float)::log2(s[0]);
+	}
+	else if (p.type == KING) {
+		f.aux[0] = (float)(1 / s[0]);
+		f.aux[1] = (float)(1 / s[2]);
+		f.aux[2] = (float)::pow(1 + (s[1] / s[0]) * (s[1] / s[0]), -1 / s[2]);
+	}
+	else if (p.type == MOFFAT || p.type == FERRER) {
+		f.aux[0] = (float)(1 / c->rscale);
 	}
 	return SE_OK;
 }

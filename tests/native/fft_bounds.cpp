@@ -108,19 +108,25 @@ static void run_case(int W, int H, int K, int pad)
 		set_lds(row_lds);
 		hip_cpu::launch(nblocks, 256, [&] { k_fft_rows_fwd(gals, insts, off, 1, mode, tw, twn); });
 	};
+	// block tables (instance 0, block b), at their exact sizes
+	auto table = [&](int nb) {
+		int2 *t = dev<int2>((size_t)nb);
+		for (int b = 0; b < nb; ++b) t[b] = make_int2(0, b);
+		return t;
+	};
 	auto cols = [&](int op, int in_mode, int out_mode, double2 *d64) {
 		const int nb = (m1 + cb - 1) / cb;
-		off[1] = nb;
+		const int2 *blk = table(nb);
 		set_lds(col_lds);
-		hip_cpu::launch(nb, kColThreads, [&] { k_fft_cols(gals, insts, off, 1, cb, op, in_mode, out_mode, tw, twn, d64); });
+		hip_cpu::launch(nb, kColThreads, [&] { k_fft_cols(gals, insts, blk, cb, op, in_mode, out_mode, tw, twn, d64); });
 	};
 	auto rows_inv = [&](int mode, int nblocks) {
-		off[1] = nblocks;
+		const int2 *blk = table(nblocks);
 		set_lds(row_lds);
 		hip_cpu::launch(nblocks, 256, [&] {
-			if (mode == 0) k_fft_rows_inv<0>(gals, insts, off, 1, tw, twn, lk, partial);
-			else if (mode == 1) k_fft_rows_inv<1>(gals, insts, off, 1, tw, twn, lk, partial);
-			else k_fft_rows_inv<2>(gals, insts, off, 1, tw, twn, lk, partial);
+			if (mode == 0) k_fft_rows_inv<0>(gals, insts, blk, tw, twn, lk, partial);
+			else if (mode == 1) k_fft_rows_inv<1>(gals, insts, blk, tw, twn, lk, partial);
+			else k_fft_rows_inv<2>(gals, insts, blk, tw, twn, lk, partial);
 		});
 	};
 	set_lds(row_lds);
