@@ -108,24 +108,29 @@ struct QEntry {
 	int32_t pix;
 };
 
-// Sub-pixel work lists: the ordinary entries and the deep centre-pixel entries
-// (resolution 8, depth 10).  count[0], count[1]: their lengths; count[2..7]:
-// hand-out counters (heavy, light) of the forward walker, the adjoint walker
-// and the node-count walker.
+constexpr int kFamilies = 6;  // radial profile families (SERSIC .. BROKENEXP)
+
+// Sub-pixel work lists: the ordinary entries, one list per profile family
+// (so that each family's walker reads only its own entries), and the deep
+// centre-pixel entries (resolution 8, depth 10).  Family f's list is
+// light[lbase[f], lbase[f+1]).
 struct WorkQueue {
 	QEntry *light, *heavy;
 	int *count;
-	int lcap, hcap;
+	int hcap;
 	// per entry: 1 when the forward walk refined no cell of the root (the
 	// tree is the root alone), so the adjoint walk skips the refinement tests
 	uint8_t *lroot, *hroot;
+	int lbase[kFamilies + 1];
 };
-// count[0], count[1]: lengths of the light and heavy lists; then a (heavy,
-// light) pair of hand-out counters per walker launch: the forward walk of
-// each profile family, then the adjoint walk of each family
-constexpr int kQueueCounters = 26;
+// count[1]: length of the heavy list; then a (heavy, light) pair of hand-out
+// counters per walker launch: the forward walk of each profile family, then
+// the adjoint walk of each family; then the length of each family's light
+// list (count[0] is not used)
+constexpr int kQueueCounters = 32;
 __host__ __device__ constexpr int fwd_counter(int type) { return 2 + 2 * type; }
 __host__ __device__ constexpr int adj_counter(int type) { return 14 + 2 * type; }
+__host__ __device__ constexpr int light_count(int type) { return 26 + type; }
 constexpr int kGrab = 16;  // ordinary entries handed to a wavefront at a time
 
 

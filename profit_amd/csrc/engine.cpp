@@ -709,13 +709,21 @@ void Engine::upload_galaxies()
 	gals_dirty_ = false;
 }
 
-void Engine::ensure_queue(long need, long hneed)
+// need: the light-list capacity of each profile family
+void Engine::ensure_queue(const long *need, long hneed)
 {
-	if (need <= qcap_ && hneed <= hcap_) return;
-	if (need > qcap_) {
-		qcap_ = need;
-		d_queue_.alloc((size_t)qcap_ * sizeof(QEntry));
-		d_qroot_.alloc((size_t)qcap_);
+	bool grow = false;
+	for (int f = 0; f < kFamilies; ++f) grow |= need[f] > lcap_[f];
+	if (!grow && hneed <= hcap_) return;
+	if (grow) {
+		long total = 0;
+		for (int f = 0; f < kFamilies; ++f) {
+			lcap_[f] = std::max(lcap_[f], need[f]);
+			total += lcap_[f];
+		}
+		if (total > INT32_MAX) throw std::runtime_error("sub-pixel work lists beyond 2^31 entries");
+		d_queue_.alloc((size_t)total * sizeof(QEntry));
+		d_qroot_.alloc((size_t)total);
 	}
 	if (hneed > hcap_) {
 		hcap_ = hneed;
@@ -727,22 +735,32 @@ void Engine::ensure_queue(long need, long hneed)
 
 WorkQueue Engine::work_queue()
 {
-	return WorkQueue{d_queue_.as<QEntry>(), d_hqueue_.as<QEntry>(), d_qcount_.as<int>(), (int)qcap_, (int)hcap_,
-	                 d_qroot_.as<uint8_t>(), d_hroot_.as<uint8_t>()};
+	WorkQueue q{d_queue_.as<QEntry>(), d_hqueue_.as<QEntry>(), d_qcount_.as<int>(), (int)hcap_,
+	            d_qroot_.as<uint8_t>(), d_hroot_.as<uint8_t>(), {}};
+	q.lbase[0] = 0;
+	for (int f = 0; f < kFamilies; ++f) q.lbase[f + 1] = q.lbase[f] + (int)lcap_[f];
+	return q;
 }
 
 // whether the last evaluation's work lists fitted
 bool Engine::queue_ok() const
 {
 	const int *q = h_qcount_.as<int>();
-	return q[0] <= qcap_ && q[1] <= hcap_;
+	for (int f = 0; f < kFamilies; ++f)
+		if (q[light_count(f)] > lcap_[f]) return false;
+	return q[1] <= hcap_;
 }
 
 // grows the work lists to what the last evaluation needed (drops the graphs)
 void Engine::grow_queue()
 {
 	const int *q = h_qcount_.as<int>();
-	ensure_queue(std::max<long>(qcap_, (long)q[0] + q[0] / 4 + 1024), std::max<long>(hcap_, (long)q[1] + 64));
+	long need[kFamilies];
+	for (int f = 0; f < kFamilies; ++f) {
+		const long n = q[light_count(f)];
+		need[f] = n > lcap_[f] ? n + n / 4 + 1024 : lcap_[f];
+	}
+	ensure_queue(need, std::max<long>(hcap_, (long)q[1] + 64));
 }
 
 void Engine::set_batch(const std::vector<int> &inst_galaxy, const std::vector<int> &inst_ncomp)
@@ -918,6 +936,14 @@ void Engine::set_batch(const std::vector<int> &inst_galaxy, const std::vector<in
 	if (ncomp_total_)
 		hip_check(hipMemcpy(d_comp_inst_.as<int>(), comp_inst.data(), ncomp_total_ * sizeof(int), hipMemcpyHostToDevice),
 		          "comp_inst");
+	{
+		// the identity order until the first parameters give the families
+		std::vector<int> order(nc);
+		for (int c = 0; c < nc; ++c) order[c] = c;
+		d_setup_order_.alloc(nc * sizeof(int));
+		hip_check(hipMemcpy(d_setup_order_.as<int>(), order.data(), nc * sizeof(int), hipMemcpyHostToDevice), "order");
+		setup_types_.assign(nc, -1);
+	}
 	d_params_.alloc(nc * sizeof(ProfileParams));
 	d_comps_.alloc(nc * sizeof(Comp));
 	d_err_.alloc(nc * sizeof(int));
@@ -931,10 +957,14 @@ void Engine::set_batch(const std::vector<int> &inst_galaxy, const std::vector<in
 	h_ll_.alloc(std::max(1, n) * sizeof(double));
 	h_grad_.alloc((size_t)nc * NGRAD * sizeof(double));
 	h_err_.alloc(nc * sizeof(int));
-	h_qcount_.alloc(2 * sizeof(int));
-	qcap_ = 0;
+	h_qcount_.alloc(kQueueCounters * sizeof(int));
+	for (int f = 0; f < kFamilies; ++f) lcap_[f] = 0;
 	hcap_ = 0;
-	ensure_queue(qneed, hneed);
+	// each family may take a third of the estimate; a list that overflows
+	// grows and the evaluation runs again
+	long need[kFamilies];
+	for (int f = 0; f < kFamilies; ++f) need[f] = qneed / 3 + 1024;
+	ensure_queue(need, hneed);
 	int ncu = 256;
 	hipDeviceProp_t prop;
 	if (hipGetDeviceProperties(&prop, device_) == hipSuccess) ncu = prop.multiProcessorCount;
@@ -952,9 +982,12 @@ void Engine::record(bool grad)
 	                         hipMemcpyHostToDevice, st),
 	          "params");
 	mark(0);
-	hip_check(launch_setup(d_params_.as<ProfileParams>(), d_comp_inst_.as<int>(), insts, gals, d_comps_.as<Comp>(),
+	hip_check(launch_setup(d_params_.as<ProfileParams>(), d_setup_order_.as<int>(), d_comp_inst_.as<int>(), insts, gals, d_comps_.as<Comp>(),
 	                       d_err_.as<int>(), nc, st),
 	          "setup");
+	if (grad && (radial_types_ & ((1u << KING) | (1u << CORESERSIC) | (1u << BROKENEXP))))
+		hip_check(launch_setup_grad(d_setup_order_.as<int>(), d_comps_.as<Comp>(), d_err_.as<int>(), nc, st),
+		          "setup grad");
 	hip_check(launch_zero_i(d_qcount_.as<int>(), kQueueCounters, st), "zero");
 	if (grad) hip_check(launch_zero_d(d_grad_acc_.as<double>(), (long)nc * NGRAD, st), "zero");
 	mark(1);
@@ -1005,7 +1038,7 @@ void Engine::record(bool grad)
 	hip_check(hipMemcpyAsync(h_ll_.as<double>(), d_ll_.as<double>(), n * sizeof(double), hipMemcpyDeviceToHost, st),
 	          "ll copy");
 	hip_check(hipMemcpyAsync(h_err_.as<int>(), d_err_.as<int>(), nc * sizeof(int), hipMemcpyDeviceToHost, st), "err");
-	hip_check(hipMemcpyAsync(h_qcount_.as<int>(), d_qcount_.as<int>(), 2 * sizeof(int), hipMemcpyDeviceToHost, st), "qcount");
+	hip_check(hipMemcpyAsync(h_qcount_.as<int>(), d_qcount_.as<int>(), kQueueCounters * sizeof(int), hipMemcpyDeviceToHost, st), "qcount");
 }
 
 void Engine::check_setup_errors()
@@ -1028,6 +1061,24 @@ void Engine::check_setup_errors()
 bool Engine::sync_aux_flags(bool enable)
 {
 	const ProfileParams *p = h_params_.as<ProfileParams>();
+	// the set-up runs one thread per component with the components grouped
+	// by profile family, so that a wavefront runs one family's code
+	{
+		bool same = (int)setup_types_.size() >= ncomp_total_;
+		for (int c = 0; same && c < ncomp_total_; ++c) same = setup_types_[c] == p[c].type;
+		if (!same && ncomp_total_ > 0) {
+			std::vector<int> order;
+			order.reserve(ncomp_total_);
+			for (int f = 0; f <= kFamilies; ++f)
+				for (int c = 0; c < ncomp_total_; ++c)
+					if ((is_radial(p[c].type) ? p[c].type : kFamilies) == f) order.push_back(c);
+			hip_check(hipMemcpy(d_setup_order_.as<int>(), order.data(), ncomp_total_ * sizeof(int),
+			                    hipMemcpyHostToDevice),
+			          "order");
+			setup_types_.assign(ncomp_total_, 0);
+			for (int c = 0; c < ncomp_total_; ++c) setup_types_[c] = p[c].type;
+		}
+	}
 	bool any = false, changed = false;
 	unsigned types = 0, keys = 0;
 	for (int i = 0; i < ninst(); ++i) {
@@ -1099,7 +1150,12 @@ void Engine::run(bool grad, bool use_graph)
 			record(grad);
 			hip_check(hipStreamSynchronize(stream_), "sync");
 		}
-		stats_.queue_count = h_qcount_.as<int>()[0] + h_qcount_.as<int>()[1];
+		{
+			const int *q = h_qcount_.as<int>();
+			int total = q[1];
+			for (int f = 0; f < kFamilies; ++f) total += q[light_count(f)];
+			stats_.queue_count = total;
+		}
 		if (queue_ok()) {
 			if (timing_) {
 				const int last = grad ? ST_COUNT : ST_ADJ_CONV;
@@ -1156,7 +1212,7 @@ void Engine::diagnostics(const ProfileParams *params, int *cls, int *nodes)
 		hip_check(hipMemcpyAsync(d_params_.as<ProfileParams>(), h_params_.as<ProfileParams>(), nc * sizeof(ProfileParams),
 		                         hipMemcpyHostToDevice, st),
 		          "params");
-		hip_check(launch_setup(d_params_.as<ProfileParams>(), d_comp_inst_.as<int>(), insts, gals, d_comps_.as<Comp>(),
+		hip_check(launch_setup(d_params_.as<ProfileParams>(), d_setup_order_.as<int>(), d_comp_inst_.as<int>(), insts, gals, d_comps_.as<Comp>(),
 		                       d_err_.as<int>(), nc, st),
 		          "setup");
 		hip_check(launch_zero_i(d_qcount_.as<int>(), kQueueCounters, st), "zero");
@@ -1166,7 +1222,7 @@ void Engine::diagnostics(const ProfileParams *params, int *cls, int *nodes)
 		hip_check(launch_subsample_nodes(subsample_blocks_, gals, insts, d_comps_.as<Comp>(), work_queue(), dnodes.as<int>(), st),
 		          "nodes");
 		hip_check(hipMemcpyAsync(h_err_.as<int>(), d_err_.as<int>(), nc * sizeof(int), hipMemcpyDeviceToHost, st), "err");
-		hip_check(hipMemcpyAsync(h_qcount_.as<int>(), d_qcount_.as<int>(), 2 * sizeof(int), hipMemcpyDeviceToHost, st),
+		hip_check(hipMemcpyAsync(h_qcount_.as<int>(), d_qcount_.as<int>(), kQueueCounters * sizeof(int), hipMemcpyDeviceToHost, st),
 		          "qcount");
 		hip_check(hipStreamSynchronize(st), "sync");
 		check_setup_errors();
@@ -1188,7 +1244,7 @@ void Engine::adjusted_settings(const ProfileParams *params, int *res, int *maxre
 	hip_check(hipMemcpyAsync(d_params_.as<ProfileParams>(), h_params_.as<ProfileParams>(), nc * sizeof(ProfileParams),
 	                         hipMemcpyHostToDevice, st),
 	          "params");
-	hip_check(launch_setup(d_params_.as<ProfileParams>(), d_comp_inst_.as<int>(), d_insts_.as<InstDev>(),
+	hip_check(launch_setup(d_params_.as<ProfileParams>(), d_setup_order_.as<int>(), d_comp_inst_.as<int>(), d_insts_.as<InstDev>(),
 	                       d_gals_.as<GalaxyDev>(), d_comps_.as<Comp>(), d_err_.as<int>(), nc, st),
 	          "setup");
 	std::vector<Comp> comps(nc);
@@ -1219,7 +1275,7 @@ void Engine::model_images(const ProfileParams *params, float *out)
 		hip_check(hipMemcpyAsync(d_params_.as<ProfileParams>(), h_params_.as<ProfileParams>(), nc * sizeof(ProfileParams),
 		                         hipMemcpyHostToDevice, st),
 		          "params");
-		hip_check(launch_setup(d_params_.as<ProfileParams>(), d_comp_inst_.as<int>(), insts, gals, d_comps_.as<Comp>(),
+		hip_check(launch_setup(d_params_.as<ProfileParams>(), d_setup_order_.as<int>(), d_comp_inst_.as<int>(), insts, gals, d_comps_.as<Comp>(),
 		                       d_err_.as<int>(), nc, st),
 		          "setup");
 		hip_check(launch_zero_i(d_qcount_.as<int>(), kQueueCounters, st), "zero");
@@ -1229,7 +1285,7 @@ void Engine::model_images(const ProfileParams *params, float *out)
 		launch_walkers_fwd(0, st);
 		conv_forward(1, st);
 		hip_check(hipMemcpyAsync(h_err_.as<int>(), d_err_.as<int>(), nc * sizeof(int), hipMemcpyDeviceToHost, st), "err");
-		hip_check(hipMemcpyAsync(h_qcount_.as<int>(), d_qcount_.as<int>(), 2 * sizeof(int), hipMemcpyDeviceToHost, st),
+		hip_check(hipMemcpyAsync(h_qcount_.as<int>(), d_qcount_.as<int>(), kQueueCounters * sizeof(int), hipMemcpyDeviceToHost, st),
 		          "qcount");
 		hip_check(hipStreamSynchronize(st), "sync");
 		check_setup_errors();
@@ -1264,7 +1320,7 @@ void Engine::model_canvas(const ProfileParams *params, float *out, float *out_di
 		hip_check(hipMemcpyAsync(d_params_.as<ProfileParams>(), h_params_.as<ProfileParams>(), nc * sizeof(ProfileParams),
 		                         hipMemcpyHostToDevice, st),
 		          "params");
-		hip_check(launch_setup(d_params_.as<ProfileParams>(), d_comp_inst_.as<int>(), insts, gals, d_comps_.as<Comp>(),
+		hip_check(launch_setup(d_params_.as<ProfileParams>(), d_setup_order_.as<int>(), d_comp_inst_.as<int>(), insts, gals, d_comps_.as<Comp>(),
 		                       d_err_.as<int>(), nc, st),
 		          "setup");
 		hip_check(launch_zero_i(d_qcount_.as<int>(), kQueueCounters, st), "zero");
@@ -1279,7 +1335,7 @@ void Engine::model_canvas(const ProfileParams *params, float *out, float *out_di
 		launch_walkers_fwd(conv ? 1 : 0, st);
 		if (conv && !out_direct) conv_forward(1, st);
 		hip_check(hipMemcpyAsync(h_err_.as<int>(), d_err_.as<int>(), nc * sizeof(int), hipMemcpyDeviceToHost, st), "err");
-		hip_check(hipMemcpyAsync(h_qcount_.as<int>(), d_qcount_.as<int>(), 2 * sizeof(int), hipMemcpyDeviceToHost, st),
+		hip_check(hipMemcpyAsync(h_qcount_.as<int>(), d_qcount_.as<int>(), kQueueCounters * sizeof(int), hipMemcpyDeviceToHost, st),
 		          "qcount");
 		hip_check(hipStreamSynchronize(st), "sync");
 		check_setup_errors();

@@ -14,8 +14,11 @@
 namespace profit_amd {
 
 // kernel launchers (kernels.hip)
-int launch_setup(const ProfileParams *params, const int *comp_inst, const InstDev *insts, const GalaxyDev *gals,
-                 Comp *comps, int *errors, int ncomp, hipStream_t st);
+// order: the components grouped by profile family (one thread each)
+int launch_setup(const ProfileParams *params, const int *order, const int *comp_inst, const InstDev *insts,
+                 const GalaxyDev *gals, Comp *comps, int *errors, int ncomp, hipStream_t st);
+// the QUADPACK luminosity gradients (King, core-Sersic, broken exponential)
+int launch_setup_grad(const int *order, Comp *comps, const int *errors, int ncomp, hipStream_t st);
 int launch_zero_d(double *p, long n, hipStream_t st);
 int launch_zero_i(int *p, long n, hipStream_t st);
 int launch_render(const GalaxyDev *gals, const InstDev *insts, const int *tile_off, int ninst, int nblocks,
@@ -244,7 +247,7 @@ class Engine {
 	void upload_galaxies();
 	void run(bool grad, bool use_graph);
 	void record(bool grad);
-	void ensure_queue(long need, long hneed);
+	void ensure_queue(const long *need, long hneed);
 	WorkQueue work_queue();
 	bool queue_ok() const;
 	void grow_queue();
@@ -302,11 +305,15 @@ class Engine {
 	int adj_grp_ = 1;  // canvas tiles per block of the adjoint render
 	int kx_max_ = 0;
 	DevBuf d_insts_, d_ct_off_, d_at_off_, d_ag_off_, d_comp_inst_;
+	// the set-up's order of the components (grouped by profile family) and
+	// the types it was formed from
+	DevBuf d_setup_order_;
+	std::vector<int> setup_types_;
 	// per instance: canvas, w, G, aux, FFT frame, direct-pixel mask, FP64 canvas
 	static constexpr int kInstBufs = 7;
 	std::vector<DevBuf> inst_bufs_;
 	DevBuf d_params_, d_comps_, d_err_, d_queue_, d_qcount_, d_partial_, d_ll_, d_gscale_, d_grad_acc_, d_grad_out_;
-	long qcap_ = 0, hcap_ = 0;
+	long lcap_[kFamilies] = {}, hcap_ = 0;
 	DevBuf d_hqueue_, d_qroot_, d_hroot_;
 	HostBuf h_params_, h_ll_, h_grad_, h_err_, h_qcount_;
 

@@ -346,7 +346,9 @@ __global__ __launch_bounds__(256, kGrad ? 3 : 4) void k_subsample(const GalaxyDe
 	// entries and component fields are read through the scalar cache
 	const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 	WalkLevel *L = levels[wv];
-	const int nl = min(wq.count[0], wq.lcap), nh = min(wq.count[1], wq.hcap);
+	static_assert(TYPE >= 0 && TYPE < kFamilies, "one walker launch per profile family");
+	const int lb = wq.lbase[TYPE];
+	const int nl = min(wq.count[light_count(TYPE)], wq.lbase[TYPE + 1] - lb), nh = min(wq.count[1], wq.hcap);
 	// slots this family can have: the geometric ones and its shape parameters
 	constexpr int kSlots = G_S0 + n_shape(TYPE);
 	double gs[kSlots];
@@ -408,8 +410,8 @@ __global__ __launch_bounds__(256, kGrad ? 3 : 4) void k_subsample(const GalaxyDe
 				if (l >= nl) break;
 				base = l;
 				take = min(kGrab, nl - l);
-				list = wq.light;
-				roots = wq.lroot;
+				list = wq.light + lb;
+				roots = wq.lroot + lb;
 			}
 		}
 		// the grab's entries (and in the adjoint their G(q) and root flags)

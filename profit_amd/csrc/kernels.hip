@@ -25,22 +25,39 @@ namespace profit_amd {
 
 // ------------------------------------------------------------------ set-up
 
-__global__ void k_setup(const ProfileParams *__restrict__ params, const int *__restrict__ comp_inst,
-                        const InstDev *__restrict__ insts, const GalaxyDev *__restrict__ gals, Comp *comps,
-                        int *errors, int ncomp)
+// One thread per component, in `order` (the components grouped by profile
+// family, so that a wavefront runs one family's set-up); the QUADPACK
+// luminosity gradients are left to k_setup_grad.
+__global__ void k_setup(const ProfileParams *__restrict__ params, const int *__restrict__ order,
+                        const int *__restrict__ comp_inst, const InstDev *__restrict__ insts,
+                        const GalaxyDev *__restrict__ gals, Comp *comps, int *errors, int ncomp)
 {
-	const int c = blockIdx.x * blockDim.x + threadIdx.x;
-	if (c >= ncomp) return;
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= ncomp) return;
+	const int c = order[t];
 	const int inst = comp_inst[c];
 	const GalaxyDev &g = gals[insts[inst].galaxy];
 	sf::QuadState qs;
 	Comp out;
-	int e = setup_component(params[c], g.magzero, g.sx, g.sy, g.img_sx, g.img_sy, g.padx, g.pady, &qs, &out);
+	int e = setup_component(params[c], g.magzero, g.sx, g.sy, g.img_sx, g.img_sy, g.padx, g.pady, &qs, &out, false);
 	if (e == SE_OK && params[c].type == POINTSRC && !g.psf_pt) e = SE_NO_PSF;
 	out.grad_box = g.grad_box;
 	out.instance = inst;
 	comps[c] = out;
 	errors[c] = e;
+}
+
+// d ln(flux_scale) / d shape of the King, core-Sersic and broken-exponential
+// components (lum_gradient): one thread per (component, slot), components in
+// `order`, after k_setup.
+__global__ void k_setup_grad(const int *__restrict__ order, Comp *comps, const int *__restrict__ errors, int ncomp)
+{
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= ncomp * 5) return;
+	const int c = order[t / 5], w = t % 5;
+	if (errors[c] != SE_OK || w >= n_lum_integrals(comps[c].type)) return;
+	sf::QuadState qs;
+	comps[c].dlnfs[G_S0 + w] = lum_gradient(comps[c], w, &qs);
 }
 
 __global__ void k_fill_zero_d(double *p, long n)
@@ -266,7 +283,9 @@ __global__ __launch_bounds__(256) void k_render(const GalaxyDev *__restrict__ ga
 			const unsigned long long dm = __ballot(cls == 2);
 			if (I.dmask && lane == 0 && j < g.DH)
 				I.dmask[((long)k * g.DH + j) * ((g.DW + 63) >> 6) + t.x] = dm;
-			append_entry(sub, lane, wq.light, wq.count, wq.lcap, QEntry{I.comp0 + k, pix});
+			const int fam = TYPE >= 0 ? TYPE : c.type;
+			append_entry(sub, lane, wq.light + wq.lbase[fam], wq.count + light_count(fam),
+			             wq.lbase[fam + 1] - wq.lbase[fam], QEntry{I.comp0 + k, pix});
 			append_entry(heavy, lane, wq.heavy, wq.count + 1, wq.hcap, QEntry{I.comp0 + k, pix});
 		}
 		if (inb) {
@@ -292,9 +311,22 @@ __global__ __launch_bounds__(256) void k_subsample_nodes(const GalaxyDev *__rest
 	const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 	WalkLevel *L = levels[wv];
 	const int gw = blockIdx.x * 4 + wv, nw = gridDim.x * 4;
-	const int nl = min(wq.count[0], wq.lcap), nh = min(wq.count[1], wq.hcap);
-	for (int e = gw; e < nl + nh; e += nw) {
-		const QEntry q = e < nh ? wq.heavy[e] : wq.light[e - nh];
+	const int nh = min(wq.count[1], wq.hcap);
+	int nlf[kFamilies], total = nh;
+	for (int f = 0; f < kFamilies; ++f) {
+		nlf[f] = min(wq.count[light_count(f)], wq.lbase[f + 1] - wq.lbase[f]);
+		total += nlf[f];
+	}
+	for (int e = gw; e < total; e += nw) {
+		QEntry q;
+		if (e < nh) {
+			q = wq.heavy[e];
+		}
+		else {
+			int r = e - nh, f = 0;
+			while (r >= nlf[f]) r -= nlf[f++];
+			q = wq.light[wq.lbase[f] + r];
+		}
 		const Comp &c = comps[q.comp];
 		const InstDev &I = insts[c.instance];
 		const GalaxyDev &g = gals[I.galaxy];
@@ -789,11 +821,18 @@ __global__ void k_dilate_cols(const uint8_t *__restrict__ in, uint8_t *out, int 
 		if (e_ != hipSuccess) return (int)e_;        \
 	} while (0)
 
-int launch_setup(const ProfileParams *params, const int *comp_inst, const InstDev *insts, const GalaxyDev *gals,
-                 Comp *comps, int *errors, int ncomp, hipStream_t st)
+int launch_setup(const ProfileParams *params, const int *order, const int *comp_inst, const InstDev *insts,
+                 const GalaxyDev *gals, Comp *comps, int *errors, int ncomp, hipStream_t st)
 {
 	if (ncomp <= 0) return 0;
-	k_setup<<<(ncomp + 63) / 64, 64, 0, st>>>(params, comp_inst, insts, gals, comps, errors, ncomp);
+	k_setup<<<(ncomp + 63) / 64, 64, 0, st>>>(params, order, comp_inst, insts, gals, comps, errors, ncomp);
+	return (int)hipGetLastError();
+}
+
+int launch_setup_grad(const int *order, Comp *comps, const int *errors, int ncomp, hipStream_t st)
+{
+	if (ncomp <= 0) return 0;
+	k_setup_grad<<<(ncomp * 5 + 63) / 64, 64, 0, st>>>(order, comps, errors, ncomp);
 	return (int)hipGetLastError();
 }
 

@@ -308,6 +308,16 @@ PA_HDI T log_t(T x)
 	return ::log(x);
 }
 
+// a / b: FP64 divides; FP32 multiplies by the hardware reciprocal (1 ulp),
+// for the derivative arithmetic, where a correctly rounded quotient buys
+// nothing (the FP32 values keep the division)
+template <class T>
+PA_HDI T div_t(T a, T b)
+{
+	if constexpr (sizeof(T) == 4) return a * __builtin_amdgcn_rcpf(b);
+	else return a / b;
+}
+
 template <class T>
 PA_HDI T boxy_r(const Comp &c, T x, T y)
 {
@@ -466,13 +476,15 @@ __device__ __forceinline__ float coresersic_f32(const Comp &c, float r, float *d
 	const float f = exp2_fast(e1 - bn * W * float(kLog2e));
 	if (ds) {
 		const float ln2 = float(kLn2), in = a * ina, lS = lS2 * ln2;
-		ds[0] = bn * W * in / re;
-		ds[1] = b * t / (rb * Q) - bn * W * rba * in / (rb * sum);
+		const float iQ = __builtin_amdgcn_rcpf(Q), isum = __builtin_amdgcn_rcpf(sum);
+		const float irb = __builtin_amdgcn_rcpf(rb), ir = __builtin_amdgcn_rcpf(r);
+		ds[0] = bn * W * in * __builtin_amdgcn_rcpf(re);
+		ds[1] = (b * t * iQ - bn * W * rba * in * isum) * irb;
 		ds[2] = -k.dbn * W + bn * W * lS * ina * in;
-		const float dlnS_da = (ra * lr + rba * lrb) * ln2 / sum - lre * ln2;
-		ds[3] = -(b * ia * ia) * lQ * ln2 - b * ia * t * (lr - lrb) * ln2 / Q - bn * W * (dlnS_da * ina - lS * ina * ia);
+		const float dlnS_da = (ra * lr + rba * lrb) * ln2 * isum - lre * ln2;
+		ds[3] = -(b * ia * ia) * lQ * ln2 - b * ia * t * (lr - lrb) * ln2 * iQ - bn * W * (dlnS_da * ina - lS * ina * ia);
 		ds[4] = lQ * ln2 * ia;
-		*dlog = -b * t / (r * Q) - bn * W * ra * in / (r * sum);
+		*dlog = (-b * t * iQ - bn * W * ra * in * isum) * ir;
 	}
 	return f;
 }
@@ -587,8 +599,15 @@ PA_HDI void dr_terms(const Comp &c, T x, T y, T r, T *drx, T *dry, T *drB)
 	const T B = pick<T>(c.box, c.f.box) + T(2);
 	const T ax = ::fabs(x), ay = ::fabs(y);
 	if (!c.boxy) {
-		*drx = x / r;
-		*dry = y / r;
+		if constexpr (sizeof(T) == 4) {
+			const float ir = __builtin_amdgcn_rcpf(r);
+			*drx = x * ir;
+			*dry = y * ir;
+		}
+		else {
+			*drx = x / r;
+			*dry = y / r;
+		}
 	}
 	else {
 		const T rp = pow_t<T>(r, T(1) - B);
@@ -599,7 +618,7 @@ PA_HDI void dr_terms(const Comp &c, T x, T y, T r, T *drx, T *dry, T *drB)
 	const T px = pow_t<T>(ax, B), py = pow_t<T>(ay, B);
 	const T S = px + py;
 	const T Sl = (ax > T(0) ? px * log_t<T>(ax) : T(0)) + (ay > T(0) ? py * log_t<T>(ay) : T(0));
-	*drB = r * (Sl / (B * S) - log_t<T>(S) / (B * B));
+	*drB = r * (div_t<T>(Sl, B * S) - div_t<T>(log_t<T>(S), B * B));
 }
 
 template <class T>
@@ -658,7 +677,7 @@ PA_HDI void sample_grad(const Comp &c, T x, T y, T *o)
 			f = exp_t<T>(-bn * (rf - T(1)));
 		}
 		o[0] = f;
-		const T k = -bn * f * rf * inB / base;  // df/dbase
+		const T k = div_t<T>(-bn * f * rf * inB, base);  // df/dbase
 		// d base / d x_prof = B pu / u / re (2 u / re without box)
 		const T bx = c.boxy ? (u > T(0) ? B * pu / u : T(0)) : T(2) * u;
 		const T by = c.boxy ? (v > T(0) ? B * pv / v : T(0)) : T(2) * v;
@@ -679,14 +698,14 @@ PA_HDI void sample_grad(const Comp &c, T x, T y, T *o)
 			// one logarithm for the value and the con derivative
 			const float lq = log2_fast(q), f = exp2_fast(-con * lq);
 			o[0] = f;
-			const float dfdr = -con * f / q * 2.f * t * c.f.aux[0];
+			const float dfdr = -con * f * __builtin_amdgcn_rcpf(q) * 2.f * t * c.f.aux[0];
 			float drx, dry, drB;
 			dr_terms<float>(c, x, y, r, &drx, &dry, &drB);
 			o[1] = dfdr * drx;
 			o[2] = dfdr * dry;
 			o[3] = dfdr * drB;
 			const float dfdrs = -dfdr * r * c.f.aux[0];
-			o[4] = dfdrs * rs / c.f.s[0];
+			o[4] = dfdrs * rs * __builtin_amdgcn_rcpf(c.f.s[0]);
 			o[5] = -f * lq * float(kLn2) + dfdrs * rs * c.f.gk;
 			return;
 		}
@@ -716,12 +735,12 @@ PA_HDI void sample_grad(const Comp &c, T x, T y, T *o)
 			o[0] = f;
 			float drx, dry, drB;
 			dr_terms<float>(c, x, y, r, &drx, &dry, &drB);
-			const float ga1 = f / g;
-			const float dfdr = t > 0.f ? a * ga1 * (-(2.f - b) * tb / t) / rout : 0.f;
+			const float ga1 = f * __builtin_amdgcn_rcpf(g), irout = __builtin_amdgcn_rcpf(rout);
+			const float dfdr = t > 0.f ? a * ga1 * (-(2.f - b) * tb * __builtin_amdgcn_rcpf(t)) * irout : 0.f;
 			o[1] = dfdr * drx;
 			o[2] = dfdr * dry;
 			o[3] = dfdr * drB;
-			o[4] = -dfdr * r / rout;
+			o[4] = -dfdr * r * irout;
 			o[5] = f * lg * float(kLn2);
 			o[6] = t > 0.f ? a * ga1 * tb * lt * float(kLn2) : 0.f;
 			return;
@@ -753,9 +772,10 @@ PA_HDI void sample_grad(const Comp &c, T x, T y, T *o)
 			if (!(D > 0.f)) return;
 			const float lD = log2_fast(D), f = exp2_fast(a * lD);
 			o[0] = f;
-			const float Da1 = a * f / D;  // a D^(a-1)
+			const float iD = __builtin_amdgcn_rcpf(D);
+			const float Da1 = a * f * iD;  // a D^(a-1)
 			const float w = rt * irc, qw = 1.f + w * w;
-			const float Aq = A / q, Cq = C / qw;
+			const float Aq = A * __builtin_amdgcn_rcpf(q), Cq = C * __builtin_amdgcn_rcpf(qw);
 			const float dfdr = Da1 * (-2.f * ia * irc * u) * Aq;
 			float drx, dry, drB;
 			dr_terms<float>(c, x, y, r, &drx, &dry, &drB);
@@ -764,7 +784,7 @@ PA_HDI void sample_grad(const Comp &c, T x, T y, T *o)
 			o[3] = dfdr * drB;
 			o[4] = Da1 * 2.f * ia * irc * (u * u * Aq - w * w * Cq);
 			o[5] = Da1 * 2.f * ia * irc * w * Cq;
-			o[6] = f * float(kLn2) * (lD + (A * lq - C * log2_fast(qw)) * ia / D);
+			o[6] = f * float(kLn2) * (lD + (A * lq - C * log2_fast(qw)) * ia * iD);
 			return;
 		}
 		o[0] = king_core<T>(r, rc, rt, a);
@@ -881,8 +901,41 @@ PA_HD bool wants_grad64(const ProfileParams &p)
 // initial_calculations() + the evaluate()-time centre shift, in FP64.
 // offx/offy: PSF padding (drawing-canvas shift); sx/sy: profile pixel scale
 // (image scale / finesampling); img_sx/img_sy: image pixel scale.
+PA_HD int n_lum_integrals(int type) { return type == KING ? 3 : type == CORESERSIC ? 5 : type == BROKENEXP ? 4 : 0; }
+
+// d ln(flux_scale) / d shape slot w of the families whose luminosity is a
+// QUADPACK integral (King 3 slots, core-Sersic 5, broken exponential 4):
+// -(2 pi int r df/dshape_w dr) / lumtot, from a component whose set-up has run.
+PA_HD double lum_gradient(const Comp &c, int w, sf::QuadState *qs)
+{
+	const double pi = 3.14159265358979323846;
+	const double *s = c.s;
+	double d = 0;
+	switch (c.type) {
+	case KING: {
+		KingIntegrand f{s[0], s[1], s[2], w};
+		d = 2 * pi * sf::integrate(f, 0.0, s[1], false, *qs);
+		break;
+	}
+	case CORESERSIC: {
+		CoreSersicIntegrand f{&c, w};
+		d = 2 * pi * sf::integrate(f, 0.0, 0.0, true, *qs);
+		break;
+	}
+	case BROKENEXP: {
+		BrokenExpIntegrand f{s[0], s[1], s[2], s[3], w};
+		d = 2 * pi * sf::integrate(f, 0.0, 0.0, true, *qs);
+		break;
+	}
+	default: return 0;
+	}
+	return -d / c.lumtot;
+}
+
+// grad_integrals = false leaves the slots of lum_gradient to a separate pass.
 PA_HD int setup_component(const ProfileParams &p, double magzero, double sx, double sy, double img_sx,
-                          double img_sy, unsigned offx, unsigned offy, sf::QuadState *qs, Comp *c)
+                          double img_sy, unsigned offx, unsigned offy, sf::QuadState *qs, Comp *c,
+                          bool grad_integrals = true)
 {
 	const int verr = validate_params(p);
 	if (verr != SE_OK) return verr;
@@ -1119,31 +1172,18 @@ PA_HD int setup_component(const ProfileParams &p, double magzero, double sx, dou
 		c->dlnfs[G_S2] = -((sf::digamma(cc) - sf::digamma(a + cc)) * 2 / ((2 - b) * (2 - b)));
 		break;
 	}
-	case KING: {
-		for (int w = 0; w < 3; ++w) {
-			KingIntegrand f{s[0], s[1], s[2], w};
-			const double d = 2 * pi * sf::integrate(f, 0.0, s[1], false, *qs);
-			c->dlnfs[G_S0 + w] = -d / lumtot;
-		}
+	case KING:
+	case BROKENEXP: {
+		if (grad_integrals)
+			for (int w = 0; w < n_lum_integrals(p.type); ++w) c->dlnfs[G_S0 + w] = lum_gradient(*c, w, qs);
 		break;
 	}
 	case CORESERSIC: {
 		const double n = s[2], bn = c->bn, a = 2 * n;
 		const double dens = ::exp((a - 1) * ::log(bn) - bn - sf::lgam(a));
 		c->dbn = -2 * sf::dpgamma_da(bn, a) / dens;
-		for (int w = 0; w < 5; ++w) {
-			CoreSersicIntegrand f{c, w};
-			const double d = 2 * pi * sf::integrate(f, 0.0, 0.0, true, *qs);
-			c->dlnfs[G_S0 + w] = -d / lumtot;
-		}
-		break;
-	}
-	case BROKENEXP: {
-		for (int w = 0; w < 4; ++w) {
-			BrokenExpIntegrand f{s[0], s[1], s[2], s[3], w};
-			const double d = 2 * pi * sf::integrate(f, 0.0, 0.0, true, *qs);
-			c->dlnfs[G_S0 + w] = -d / lumtot;
-		}
+		if (grad_integrals)
+			for (int w = 0; w < 5; ++w) c->dlnfs[G_S0 + w] = lum_gradient(*c, w, qs);
 		break;
 	}
 	default: break;

@@ -92,6 +92,7 @@ inline int __ffs(int v) { return __builtin_ffs(v); }
 inline float __builtin_amdgcn_logf(float x) { return std::log2(x); }
 inline float __builtin_amdgcn_exp2f(float x) { return std::exp2(x); }
 inline float __builtin_amdgcn_sqrtf(float x) { return std::sqrt(x); }
+inline float __builtin_amdgcn_rcpf(float x) { return 1.f / x; }
 inline double __shfl_xor(double v, int mask)
 {
 	const unsigned t = threadIdx.x;
