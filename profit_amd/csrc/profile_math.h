@@ -225,11 +225,11 @@ PA_HD int validate_params(const ProfileParams &p)
 
 // FP32 log2 and exp2 on the transcendental unit (v_log_f32, v_exp_f32: about
 // 1 ulp for normal arguments).  The FP32 profile values and derivatives need
-// a few parts in 1e7; subnormal logarithm arguments take the library path.
-__device__ __forceinline__ float log2_fast(float x)
-{
-	return x >= 1.17549435e-38f ? __builtin_amdgcn_logf(x) : ::log2f(x);
-}
+// a few parts in 1e7.  A subnormal argument counts as 0 (log2 -> -inf): the
+// profiles take logarithms of squared radii, sums of squares and profile
+// values, for which 1e-38 and 0 give the same result to FP32 precision but
+// at an exact zero, which this keeps exact.
+__device__ __forceinline__ float log2_fast(float x) { return __builtin_amdgcn_logf(x); }
 __device__ __forceinline__ float exp2_fast(float x) { return __builtin_amdgcn_exp2f(x); }
 constexpr double kLog2e = 1.4426950408889634, kLn2 = 0.6931471805599453;
 
