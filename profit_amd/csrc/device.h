@@ -122,7 +122,12 @@ struct WorkQueue {
 	// tree is the root alone), so the adjoint walk skips the refinement tests
 	uint8_t *lroot, *hroot;
 	int lbase[kFamilies + 1];
+	// per entry, kJacStride floats: the pixel's derivatives sum_s w_s df_s /
+	// d slot (slot order of GradSlot) and, last, its value sum_s w_s f_s,
+	// written by the forward walk of a gradient evaluation (null otherwise)
+	float *ljac, *hjac;
 };
+constexpr int kJacStride = 12;
 // count[1]: length of the heavy list; then a (heavy, light) pair of hand-out
 // counters per walker launch: the forward walk of each profile family, then
 // the adjoint walk of each family; then the length of each family's light

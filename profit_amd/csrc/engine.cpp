@@ -724,11 +724,13 @@ void Engine::ensure_queue(const long *need, long hneed)
 		if (total > INT32_MAX) throw std::runtime_error("sub-pixel work lists beyond 2^31 entries");
 		d_queue_.alloc((size_t)total * sizeof(QEntry));
 		d_qroot_.alloc((size_t)total);
+		d_qjac_.alloc((size_t)total * kJacStride * sizeof(float));
 	}
 	if (hneed > hcap_) {
 		hcap_ = hneed;
 		d_hqueue_.alloc((size_t)hcap_ * sizeof(QEntry));
 		d_hroot_.alloc((size_t)hcap_);
+		d_hjac_.alloc((size_t)hcap_ * kJacStride * sizeof(float));
 	}
 	invalidate_graphs(graph_exec_);
 }
@@ -736,7 +738,7 @@ void Engine::ensure_queue(const long *need, long hneed)
 WorkQueue Engine::work_queue()
 {
 	WorkQueue q{d_queue_.as<QEntry>(), d_hqueue_.as<QEntry>(), d_qcount_.as<int>(), (int)hcap_,
-	            d_qroot_.as<uint8_t>(), d_hroot_.as<uint8_t>(), {}};
+	            d_qroot_.as<uint8_t>(), d_hroot_.as<uint8_t>(), {}, d_qjac_.as<float>(), d_hjac_.as<float>()};
 	q.lbase[0] = 0;
 	for (int f = 0; f < kFamilies; ++f) q.lbase[f + 1] = q.lbase[f] + (int)lcap_[f];
 	return q;
@@ -999,7 +1001,7 @@ void Engine::record(bool grad)
 	                        work_queue(), -1, 0, nullptr, st, rtype),
 	          "render");
 	mark(2);
-	launch_walkers_fwd(0, st);
+	launch_walkers_fwd(0, grad, st);
 	mark(3);
 	conv_forward(0, st);
 	mark(4);
@@ -1021,10 +1023,12 @@ void Engine::record(bool grad)
 			hip_check(launch_adj_nonradial(gals, insts, n, d_comps_.as<Comp>(), d_grad_acc_.as<double>(), st),
 			          "adjoint point sources and sky");
 		mark(7);
+		// the sub-sampled pixels: their Jacobians from the forward walk,
+		// weighted by G(q)
 		for (int t = 0; t < 6; ++t)
 			if ((radial_types_ >> t) & 1u)
-				hip_check(launch_subsample_adj(t, subsample_blocks_, gals, insts, d_comps_.as<Comp>(), work_queue(),
-				                               d_grad_acc_.as<double>(), st),
+				hip_check(launch_adj_queue(t, subsample_blocks_, insts, d_comps_.as<Comp>(), work_queue(),
+				                           d_grad_acc_.as<double>(), st),
 				          "adjoint subsample");
 		mark(8);
 		hip_check(launch_grad_finalize(d_comps_.as<Comp>(), d_gscale_.as<double>(), d_grad_acc_.as<double>(),
@@ -1112,12 +1116,12 @@ bool Engine::sync_aux_flags(bool enable)
 }
 
 // the forward walk of every radial family present
-void Engine::launch_walkers_fwd(int split, hipStream_t st)
+void Engine::launch_walkers_fwd(int split, bool jac, hipStream_t st)
 {
 	for (int t = 0; t < 6; ++t)
 		if ((radial_types_ >> t) & 1u)
 			hip_check(launch_subsample_fwd(t, subsample_blocks_, d_gals_.as<GalaxyDev>(), d_insts_.as<InstDev>(),
-			                               d_comps_.as<Comp>(), work_queue(), split, st),
+			                               d_comps_.as<Comp>(), work_queue(), split, jac, st),
 			          "subsample");
 }
 
@@ -1282,7 +1286,7 @@ void Engine::model_images(const ProfileParams *params, float *out)
 		hip_check(launch_render(gals, insts, d_at_off_.as<int>(), n, n_at_blocks_, d_comps_.as<Comp>(),
 		                        work_queue(), -1, 0, nullptr, st),
 		          "render");
-		launch_walkers_fwd(0, st);
+		launch_walkers_fwd(0, false, st);
 		conv_forward(1, st);
 		hip_check(hipMemcpyAsync(h_err_.as<int>(), d_err_.as<int>(), nc * sizeof(int), hipMemcpyDeviceToHost, st), "err");
 		hip_check(hipMemcpyAsync(h_qcount_.as<int>(), d_qcount_.as<int>(), kQueueCounters * sizeof(int), hipMemcpyDeviceToHost, st),
@@ -1332,7 +1336,7 @@ void Engine::model_canvas(const ProfileParams *params, float *out, float *out_di
 			hip_check(launch_render(gals, insts, d_at_off_.as<int>(), 1, n_at_blocks_, d_comps_.as<Comp>(),
 			                        work_queue(), 0, 1, nullptr, st),
 			          "render");
-		launch_walkers_fwd(conv ? 1 : 0, st);
+		launch_walkers_fwd(conv ? 1 : 0, false, st);
 		if (conv && !out_direct) conv_forward(1, st);
 		hip_check(hipMemcpyAsync(h_err_.as<int>(), d_err_.as<int>(), nc * sizeof(int), hipMemcpyDeviceToHost, st), "err");
 		hip_check(hipMemcpyAsync(h_qcount_.as<int>(), d_qcount_.as<int>(), kQueueCounters * sizeof(int), hipMemcpyDeviceToHost, st),

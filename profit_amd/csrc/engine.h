@@ -25,11 +25,15 @@ int launch_render(const GalaxyDev *gals, const InstDev *insts, const int *tile_o
                   const Comp *comps, WorkQueue wq, int filter, int target, int *diag_cls, hipStream_t st, int type = -1);
 // the forward walker and the adjoint render, one launch per radial family
 // (and per sum precision in the adjoint render: 1 = FP64 per-thread sums,
-// steep cores only); the adjoint walker, one launch per family
+// steep cores only); the adjoint walker, one launch per family.  jac: the
+// forward walk also writes each entry's Jacobian (wq.ljac, wq.hjac), which
+// launch_adj_queue then weighs by G(q) in place of the adjoint walk.
 int launch_subsample_fwd(int type, int nblocks, const GalaxyDev *gals, const InstDev *insts, const Comp *comps,
-                         WorkQueue wq, int split, hipStream_t st);
+                         WorkQueue wq, int split, bool jac, hipStream_t st);
 int launch_subsample_adj(int type, int nblocks, const GalaxyDev *gals, const InstDev *insts, const Comp *comps,
                          WorkQueue wq, double *grad_acc, hipStream_t st);
+int launch_adj_queue(int type, int nblocks, const InstDev *insts, const Comp *comps, WorkQueue wq, double *grad_acc,
+                     hipStream_t st);
 int launch_subsample_nodes(int nblocks, const GalaxyDev *gals, const InstDev *insts, const Comp *comps,
                            WorkQueue wq, int *nodes, hipStream_t st);
 int launch_conv(int mode, int rows, int nblocks, int kx_max, int kc, const GalaxyDev *gals, const InstDev *insts,
@@ -298,7 +302,7 @@ class Engine {
 	// radial families present (bit t) and the adjoint's (family, precision)
 	// keys present (bit 2t + prec), which decide the walker launches
 	unsigned radial_types_ = 0, grad_keys_ = 0;
-	void launch_walkers_fwd(int split, hipStream_t st);
+	void launch_walkers_fwd(int split, bool jac, hipStream_t st);
 	bool sync_aux_flags(bool enable);
 	std::vector<int> ct_off_, at_off_, ag_off_;
 	int n_ct_blocks_ = 0, n_at_blocks_ = 0, n_ag_blocks_ = 0;
@@ -314,7 +318,7 @@ class Engine {
 	std::vector<DevBuf> inst_bufs_;
 	DevBuf d_params_, d_comps_, d_err_, d_queue_, d_qcount_, d_partial_, d_ll_, d_gscale_, d_grad_acc_, d_grad_out_;
 	long lcap_[kFamilies] = {}, hcap_ = 0;
-	DevBuf d_hqueue_, d_qroot_, d_hroot_;
+	DevBuf d_hqueue_, d_qroot_, d_hroot_, d_qjac_, d_hjac_;
 	HostBuf h_params_, h_ll_, h_grad_, h_err_, h_qcount_;
 
 	hipGraphExec_t graph_exec_[2] = {nullptr, nullptr};

@@ -190,10 +190,12 @@ __device__ __forceinline__ float pixel_accumulate_f32(const Comp &c, float xp, f
 // derivative sums: FP32 per-sample derivatives, summed in FP64 (gs, sum) for
 // steep cores and in FP32 (gsf, sumf) otherwise.  `scale` multiplies the
 // weights (G(q) * flux_scale in the adjoint, 1 otherwise).
-template <bool kGrad, int TYPE, int PREC>
+// kJac (forward walk of a gradient evaluation): every leaf also adds its
+// weighted FP32 derivatives to gj (per lane), for the pixel's Jacobian.
+template <bool kGrad, int TYPE, int PREC, bool kJac = false>
 __device__ __forceinline__ void walk_pixel(const Comp &c, WalkLevel *L, int lane, double x0, double x1, double y0, double y1, int res,
                            int maxrec, double scale, bool fp64, double &sum, double *gs, float &sumf, float *gsf,
-                           bool root_only, uint8_t *root_flag)
+                           bool root_only, uint8_t *root_flag, float *gj = nullptr)
 {
 	const int ncell = res * res;
 	const int nchunk = (ncell + 63) >> 6;
@@ -266,6 +268,7 @@ __device__ __forceinline__ void walk_pixel(const Comp &c, WalkLevel *L, int lane
 							}
 						}
 						sum += v64 * w;
+						if (kJac) pixel_accumulate_f32<TYPE>(c, xp, yp, (float)w, gj);
 					}
 				}
 			}
@@ -335,8 +338,10 @@ __device__ __forceinline__ void walk_pixel(const Comp &c, WalkLevel *L, int lane
 // flushed with one set of atomics per run.  `counter`: the pair of hand-out
 // counters of this launch (heavy, light).  Held to the registers of three
 // waves per SIMD in the adjoint and four in the forward walk.
-template <bool kGrad, int TYPE, int PREC>
-__global__ __launch_bounds__(256, kGrad ? 3 : 4) void k_subsample(const GalaxyDev *__restrict__ gals,
+// kJac (forward only): each entry's Jacobian and value go to wq.ljac / hjac
+// for k_adj_queue, which replaces the adjoint re-walk.
+template <bool kGrad, int TYPE, int PREC, bool kJac = false>
+__global__ __launch_bounds__(256, (kGrad || kJac) ? 3 : 4) void k_subsample(const GalaxyDev *__restrict__ gals,
                                                    const InstDev *__restrict__ insts,
                                                    const Comp *__restrict__ comps, WorkQueue wq, int counter,
                                                    int split, double *grad_acc)
@@ -360,6 +365,9 @@ __global__ __launch_bounds__(256, kGrad ? 3 : 4) void k_subsample(const GalaxyDe
 	double vsum = 0;
 	float vsumf = 0.f;
 	int cur = -1;
+	float gj[NGRAD];
+#pragma unroll
+	for (int s = 0; s < NGRAD; ++s) gj[s] = 0.f;
 	// the FP32 sums into the FP64 ones
 	auto fold = [&]() {
 #pragma unroll
@@ -457,9 +465,9 @@ __global__ __launch_bounds__(256, kGrad ? 3 : 4) void k_subsample(const GalaxyDe
 				maxrec = 10;
 			}
 			double s_local = 0;
-			walk_pixel<kGrad, TYPE, PREC>(c, L, lane, x - hx, x + hx, y - hy, y + hy, res, maxrec, scale,
-			                              g.render_fp64 != 0, s_local, gs, vsumf, gsf,
-			                              kGrad && __builtin_amdgcn_readlane(qroot, u) != 0, roots + e);
+			walk_pixel<kGrad, TYPE, PREC, kJac>(c, L, lane, x - hx, x + hx, y - hy, y + hy, res, maxrec, scale,
+			                                    g.render_fp64 != 0, s_local, gs, vsumf, gsf,
+			                                    kGrad && __builtin_amdgcn_readlane(qroot, u) != 0, roots + e, gj);
 			if (kGrad) {
 				vsum += s_local;
 			}
@@ -470,11 +478,76 @@ __global__ __launch_bounds__(256, kGrad ? 3 : 4) void k_subsample(const GalaxyDe
 					if (g.canvas64 && !to_aux) atomicAdd(&I.canvas64[q.pix], c.flux_scale * tot);
 					else atomicAdd(&(to_aux ? I.aux : I.canvas)[q.pix], (float)(c.flux_scale * tot));
 				}
+				if (kJac) {
+					// the pixel's Jacobian (cross-lane sums in FP64) and value:
+					// lane s writes element s
+					float jv = lane == kJacStride - 1 ? (float)tot : 0.f;
+#pragma unroll
+					for (int s = 0; s < G_S0 + n_shape(TYPE); ++s) {
+						const double t = wave_sum((double)gj[s]);
+						gj[s] = 0.f;
+						if (lane == s) jv = (float)t;
+					}
+					float *J = list == wq.heavy ? wq.hjac : wq.ljac + (long)lb * kJacStride;
+					if (lane < kJacStride) J[(long)e * kJacStride + lane] = jv;
+				}
 			}
 		}
 		if (kGrad) fold();
 	}
 	if (kGrad && cur >= 0) flush();
+}
+
+// The adjoint of the sub-sampled pixels from their Jacobians (written by the
+// forward walk, k_subsample<false, TYPE, 0, true>): per entry (component c,
+// pixel q), G(q) fs (J_s + dlnfs_s V) for every slot s.  Lanes take
+// consecutive entries of the family's lists (the heavy list first, Sersic
+// only); the lanes of one component are summed across the wave, one
+// component at a time, in FP64, and added with one atomic per slot.
+template <int TYPE>
+__global__ __launch_bounds__(256) void k_adj_queue(const InstDev *__restrict__ insts, const Comp *__restrict__ comps,
+                                                   WorkQueue wq, double *grad_acc)
+{
+	constexpr int kSlots = G_S0 + n_shape(TYPE);
+	const int lane = threadIdx.x & 63;
+	const int lb = wq.lbase[TYPE];
+	const long nl = min(wq.count[light_count(TYPE)], wq.lbase[TYPE + 1] - lb);
+	const long nh = TYPE == SERSIC ? min(wq.count[1], wq.hcap) : 0;
+	const long total = nl + nh;
+	const long stride = (long)gridDim.x * blockDim.x;
+	for (long base = (long)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < total; base += stride) {
+		const long e = base + lane;
+		int comp = -1;
+		double cs[kSlots];
+#pragma unroll
+		for (int s = 0; s < kSlots; ++s) cs[s] = 0;
+		if (e < total) {
+			const bool heavy = e < nh;
+			const long ei = heavy ? e : e - nh;
+			const QEntry q = heavy ? wq.heavy[ei] : wq.light[lb + ei];
+			const float *J = heavy ? wq.hjac + ei * kJacStride : wq.ljac + (lb + ei) * kJacStride;
+			const Comp &c = comps[q.comp];
+			const float G = insts[c.instance].G[q.pix];
+			if (c.type == TYPE && G != 0.f) {
+				comp = q.comp;
+				const double gw = (double)G * c.flux_scale, V = (double)J[kJacStride - 1];
+#pragma unroll
+				for (int s = 0; s < kSlots; ++s) cs[s] = gw * ((double)J[s] + c.dlnfs[s] * V);
+			}
+		}
+		unsigned long long left = __ballot(comp >= 0);
+		while (left) {
+			const int leader = __ffsll((long long)left) - 1;
+			const int lc = __builtin_amdgcn_readlane(comp, leader);
+			const bool mine = comp == lc;
+			left &= ~__ballot(mine);
+#pragma unroll
+			for (int s = 0; s < kSlots; ++s) {
+				const double t = wave_sum(mine ? cs[s] : 0.0);
+				if (lane == leader && t != 0) atomicAdd(&grad_acc[(long)lc * NGRAD + s], t);
+			}
+		}
+	}
 }
 
 // ------------------------------------------------------------------ adjoint render

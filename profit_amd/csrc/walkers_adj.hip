@@ -56,6 +56,29 @@ int launch_subsample_adj(int type, int nblocks, const GalaxyDev *gals, const Ins
 	}
 }
 
+template <int TYPE>
+static int adj_queue_launch(int nblocks, const InstDev *insts, const Comp *comps, WorkQueue wq, double *grad_acc,
+                            hipStream_t st)
+{
+	k_adj_queue<TYPE><<<nblocks, 256, 0, st>>>(insts, comps, wq, grad_acc);
+	return (int)hipGetLastError();
+}
+
+int launch_adj_queue(int type, int nblocks, const InstDev *insts, const Comp *comps, WorkQueue wq, double *grad_acc,
+                     hipStream_t st)
+{
+	if (nblocks <= 0) return 0;
+	switch (type) {
+	case SERSIC: return adj_queue_launch<SERSIC>(nblocks, insts, comps, wq, grad_acc, st);
+	case MOFFAT: return adj_queue_launch<MOFFAT>(nblocks, insts, comps, wq, grad_acc, st);
+	case FERRER: return adj_queue_launch<FERRER>(nblocks, insts, comps, wq, grad_acc, st);
+	case KING: return adj_queue_launch<KING>(nblocks, insts, comps, wq, grad_acc, st);
+	case CORESERSIC: return adj_queue_launch<CORESERSIC>(nblocks, insts, comps, wq, grad_acc, st);
+	case BROKENEXP: return adj_queue_launch<BROKENEXP>(nblocks, insts, comps, wq, grad_acc, st);
+	default: return (int)hipErrorInvalidValue;
+	}
+}
+
 int launch_adj_render(int type, int prec, const GalaxyDev *gals, const InstDev *insts, const int *grp_off, int ninst,
                       int nblocks, int grp, const Comp *comps, double *grad_acc, hipStream_t st)
 {

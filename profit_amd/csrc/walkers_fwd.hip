@@ -6,23 +6,27 @@ namespace profit_amd {
 
 template <int TYPE>
 static int fwd_launch(int nblocks, const GalaxyDev *gals, const InstDev *insts, const Comp *comps, WorkQueue wq,
-                      int split, hipStream_t st)
+                      int split, bool jac, hipStream_t st)
 {
-	k_subsample<false, TYPE, 0><<<nblocks, 256, 0, st>>>(gals, insts, comps, wq, fwd_counter(TYPE), split, nullptr);
+	if (jac)
+		k_subsample<false, TYPE, 0, true><<<nblocks, 256, 0, st>>>(gals, insts, comps, wq, fwd_counter(TYPE), split,
+		                                                           nullptr);
+	else
+		k_subsample<false, TYPE, 0><<<nblocks, 256, 0, st>>>(gals, insts, comps, wq, fwd_counter(TYPE), split, nullptr);
 	return (int)hipGetLastError();
 }
 
 int launch_subsample_fwd(int type, int nblocks, const GalaxyDev *gals, const InstDev *insts, const Comp *comps,
-                         WorkQueue wq, int split, hipStream_t st)
+                         WorkQueue wq, int split, bool jac, hipStream_t st)
 {
 	if (nblocks <= 0) return 0;
 	switch (type) {
-	case SERSIC: return fwd_launch<SERSIC>(nblocks, gals, insts, comps, wq, split, st);
-	case MOFFAT: return fwd_launch<MOFFAT>(nblocks, gals, insts, comps, wq, split, st);
-	case FERRER: return fwd_launch<FERRER>(nblocks, gals, insts, comps, wq, split, st);
-	case KING: return fwd_launch<KING>(nblocks, gals, insts, comps, wq, split, st);
-	case CORESERSIC: return fwd_launch<CORESERSIC>(nblocks, gals, insts, comps, wq, split, st);
-	case BROKENEXP: return fwd_launch<BROKENEXP>(nblocks, gals, insts, comps, wq, split, st);
+	case SERSIC: return fwd_launch<SERSIC>(nblocks, gals, insts, comps, wq, split, jac, st);
+	case MOFFAT: return fwd_launch<MOFFAT>(nblocks, gals, insts, comps, wq, split, jac, st);
+	case FERRER: return fwd_launch<FERRER>(nblocks, gals, insts, comps, wq, split, jac, st);
+	case KING: return fwd_launch<KING>(nblocks, gals, insts, comps, wq, split, jac, st);
+	case CORESERSIC: return fwd_launch<CORESERSIC>(nblocks, gals, insts, comps, wq, split, jac, st);
+	case BROKENEXP: return fwd_launch<BROKENEXP>(nblocks, gals, insts, comps, wq, split, jac, st);
 	default: return (int)hipErrorInvalidValue;
 	}
 }
