@@ -159,8 +159,9 @@ def calibrate_copy():
 
 def sample_counts(eng1, params1, geometry, comps):
     """Profile evaluations per instance at these parameters: the directly
-    evaluated pixels (k_render) and the cell values of the sub-pixel trees
-    (res^2 per tree node; resolution 8 at the Sersic centre pixels)."""
+    evaluated pixels (k_render), the cell values of the sub-pixel trees
+    (res^2 per tree node; resolution 8 at the Sersic centre pixels), and the
+    number of queued (sub-sampled) pixels."""
     cls, nodes = eng1.diagnostics(params1)
     adj = eng1.adjusted_settings(params1)
     _, _, px, py = geometry
@@ -174,7 +175,7 @@ def sample_counts(eng1, params1, geometry, comps):
             near = (np.abs(ii + 0.5 - (c["xcen"] + px)) < 1) & (np.abs(jj + 0.5 - (c["ycen"] + py)) < 1)
             res[jj[near], ii[near]] = 8
         cells += int((nodes[k][sub].astype(np.int64) * res[sub] ** 2).sum())
-    return direct, cells
+    return direct, cells, int((cls == 3).sum())
 
 
 def fft_work(dw, dh, w, h, n1, n2):
@@ -440,7 +441,7 @@ def main():
     n1 = max(8, 1 << int(np.ceil(np.log2(dw + K))))
     n2 = max(8, 1 << int(np.ceil(np.log2(dh + K))))
     render_bytes = B * dw * dh * 5.0
-    direct, cells = sample_counts(eng1, gpu_params(P, [wl["components"]]), geom, wl["components"])
+    direct, cells, queued = sample_counts(eng1, gpu_params(P, [wl["components"]]), geom, wl["components"])
     traffic = {}
     tfile = os.path.join(ROOT, "profiles", f"traffic_config2_batch{B}.json")
     if os.path.exists(tfile) and conv_used == "fft" and not args.render_fp64:
@@ -464,14 +465,18 @@ def main():
     add_kernel("render", "k_render<0>", stage_ms["render"], "hbm", render_bytes, PEAK_HBM_GBS, "GB/s",
                {"profile_evals_per_s": round(B * direct / (stage_ms["render"] * 1e-3), 0),
                 "direct_pixels_per_instance": direct})
-    add_kernel("subsample", "k_subsample<false, 0, 0>", stage_ms["subsample"], "hbm", render_bytes, PEAK_HBM_GBS,
-               "GB/s", {"cell_evals_per_s": round(B * cells / (stage_ms["subsample"] * 1e-3), 0),
+    # the forward walk of a gradient evaluation: cell values, refinement tests
+    # and each cell's derivatives into the pixel's Jacobian
+    add_kernel("subsample", "k_subsample<0, true>", stage_ms["subsample"], "hbm", render_bytes, PEAK_HBM_GBS,
+               "GB/s", {"cell_value_and_derivative_evals_per_s": round(B * cells / (stage_ms["subsample"] * 1e-3), 0),
                         "cells_per_instance": cells})
     add_kernel("adjoint_render", "k_adj_render<0, *>", stage_ms["adjoint_render"], "hbm", render_bytes, PEAK_HBM_GBS,
                "GB/s", {"derivative_evals_per_s": round(B * direct / (stage_ms["adjoint_render"] * 1e-3), 0)})
-    add_kernel("adjoint_subsample", "k_subsample<true, 0, 2>", stage_ms["adjoint_subsample"], "hbm", render_bytes,
-               PEAK_HBM_GBS, "GB/s",
-               {"cell_derivative_evals_per_s": round(B * cells / (stage_ms["adjoint_subsample"] * 1e-3), 0)})
+    # the adjoint of the sub-sampled pixels: a weighted reduction of the
+    # Jacobians, 48 B per queued entry read, against HBM
+    add_kernel("adjoint_subsample", "k_adj_queue<0>", stage_ms["adjoint_subsample"], "hbm", B * queued * 60.0,
+               PEAK_HBM_GBS, "GB/s", {"queued_entries_per_instance": queued,
+                                      "note": "work: 60 B per queued pixel (48 B Jacobian and value, 8 B entry, 4 B G)"})
     if conv_used == "fft":
         flops, stage_bytes = fft_work(dw, dh, 400, 400, n1, n2)
         names = {"like_rows_fwd": "k_fft_rows_fwd", "like_cols": "k_fft_cols", "like_rows_inv": "k_fft_rows_inv<0>",

@@ -118,9 +118,6 @@ struct WorkQueue {
 	QEntry *light, *heavy;
 	int *count;
 	int hcap;
-	// per entry: 1 when the forward walk refined no cell of the root (the
-	// tree is the root alone), so the adjoint walk skips the refinement tests
-	uint8_t *lroot, *hroot;
 	int lbase[kFamilies + 1];
 	// per entry, kJacStride floats: the pixel's derivatives sum_s w_s df_s /
 	// d slot (slot order of GradSlot) and, last, its value sum_s w_s f_s,
@@ -129,13 +126,11 @@ struct WorkQueue {
 };
 constexpr int kJacStride = 12;
 // count[1]: length of the heavy list; then a (heavy, light) pair of hand-out
-// counters per walker launch: the forward walk of each profile family, then
-// the adjoint walk of each family; then the length of each family's light
-// list (count[0] is not used)
-constexpr int kQueueCounters = 32;
+// counters per walker launch (one per profile family); then the length of
+// each family's light list (count[0] is not used)
+constexpr int kQueueCounters = 20;
 __host__ __device__ constexpr int fwd_counter(int type) { return 2 + 2 * type; }
-__host__ __device__ constexpr int adj_counter(int type) { return 14 + 2 * type; }
-__host__ __device__ constexpr int light_count(int type) { return 26 + type; }
+__host__ __device__ constexpr int light_count(int type) { return 14 + type; }
 constexpr int kGrab = 16;  // ordinary entries handed to a wavefront at a time
 
 

@@ -723,13 +723,11 @@ void Engine::ensure_queue(const long *need, long hneed)
 		}
 		if (total > INT32_MAX) throw std::runtime_error("sub-pixel work lists beyond 2^31 entries");
 		d_queue_.alloc((size_t)total * sizeof(QEntry));
-		d_qroot_.alloc((size_t)total);
 		d_qjac_.alloc((size_t)total * kJacStride * sizeof(float));
 	}
 	if (hneed > hcap_) {
 		hcap_ = hneed;
 		d_hqueue_.alloc((size_t)hcap_ * sizeof(QEntry));
-		d_hroot_.alloc((size_t)hcap_);
 		d_hjac_.alloc((size_t)hcap_ * kJacStride * sizeof(float));
 	}
 	invalidate_graphs(graph_exec_);
@@ -737,8 +735,8 @@ void Engine::ensure_queue(const long *need, long hneed)
 
 WorkQueue Engine::work_queue()
 {
-	WorkQueue q{d_queue_.as<QEntry>(), d_hqueue_.as<QEntry>(), d_qcount_.as<int>(), (int)hcap_,
-	            d_qroot_.as<uint8_t>(), d_hroot_.as<uint8_t>(), {}, d_qjac_.as<float>(), d_hjac_.as<float>()};
+	WorkQueue q{d_queue_.as<QEntry>(), d_hqueue_.as<QEntry>(), d_qcount_.as<int>(), (int)hcap_, {},
+	            d_qjac_.as<float>(), d_hjac_.as<float>()};
 	q.lbase[0] = 0;
 	for (int f = 0; f < kFamilies; ++f) q.lbase[f + 1] = q.lbase[f] + (int)lcap_[f];
 	return q;

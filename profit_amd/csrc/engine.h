@@ -25,13 +25,10 @@ int launch_render(const GalaxyDev *gals, const InstDev *insts, const int *tile_o
                   const Comp *comps, WorkQueue wq, int filter, int target, int *diag_cls, hipStream_t st, int type = -1);
 // the forward walker and the adjoint render, one launch per radial family
 // (and per sum precision in the adjoint render: 1 = FP64 per-thread sums,
-// steep cores only); the adjoint walker, one launch per family.  jac: the
-// forward walk also writes each entry's Jacobian (wq.ljac, wq.hjac), which
-// launch_adj_queue then weighs by G(q) in place of the adjoint walk.
+// steep cores only).  jac: the forward walk also writes each entry's Jacobian
+// (wq.ljac, wq.hjac), which launch_adj_queue weighs by G(q).
 int launch_subsample_fwd(int type, int nblocks, const GalaxyDev *gals, const InstDev *insts, const Comp *comps,
                          WorkQueue wq, int split, bool jac, hipStream_t st);
-int launch_subsample_adj(int type, int nblocks, const GalaxyDev *gals, const InstDev *insts, const Comp *comps,
-                         WorkQueue wq, double *grad_acc, hipStream_t st);
 int launch_adj_queue(int type, int nblocks, const InstDev *insts, const Comp *comps, WorkQueue wq, double *grad_acc,
                      hipStream_t st);
 int launch_subsample_nodes(int nblocks, const GalaxyDev *gals, const InstDev *insts, const Comp *comps,
@@ -318,7 +315,7 @@ class Engine {
 	std::vector<DevBuf> inst_bufs_;
 	DevBuf d_params_, d_comps_, d_err_, d_queue_, d_qcount_, d_partial_, d_ll_, d_gscale_, d_grad_acc_, d_grad_out_;
 	long lcap_[kFamilies] = {}, hcap_ = 0;
-	DevBuf d_hqueue_, d_qroot_, d_hroot_, d_qjac_, d_hjac_;
+	DevBuf d_hqueue_, d_qjac_, d_hjac_;
 	HostBuf h_params_, h_ll_, h_grad_, h_err_, h_qcount_;
 
 	hipGraphExec_t graph_exec_[2] = {nullptr, nullptr};

@@ -181,21 +181,17 @@ __device__ __forceinline__ float pixel_accumulate_f32(const Comp &c, float xp, f
 	return o[0];
 }
 
-// Walks the recursion tree of one pixel of a TYPE profile (-1: any family,
-// dispatched at run time).  Cell positions, values and refinement tests are
-// FP32; libprofit's FP64 cell position is formed only for an undecided test
-// and for FP64 leaf values (`fp64`).  The descent to a refined cell follows
-// libprofit's FP64 node bounds, so the trees are its own.  Every lane returns
-// its share of sum_s weight_s * f(s) in `sum` or, when kGrad, of the
-// derivative sums: FP32 per-sample derivatives, summed in FP64 (gs, sum) for
-// steep cores and in FP32 (gsf, sumf) otherwise.  `scale` multiplies the
-// weights (G(q) * flux_scale in the adjoint, 1 otherwise).
-// kJac (forward walk of a gradient evaluation): every leaf also adds its
-// weighted FP32 derivatives to gj (per lane), for the pixel's Jacobian.
-template <bool kGrad, int TYPE, int PREC, bool kJac = false>
-__device__ __forceinline__ void walk_pixel(const Comp &c, WalkLevel *L, int lane, double x0, double x1, double y0, double y1, int res,
-                           int maxrec, double scale, bool fp64, double &sum, double *gs, float &sumf, float *gsf,
-                           bool root_only, uint8_t *root_flag, float *gj = nullptr)
+// Walks the recursion tree of one pixel of a TYPE profile.  Cell positions,
+// values and refinement tests are FP32; libprofit's FP64 cell position is
+// formed only for an undecided test and for FP64 leaf values (`fp64`).  The
+// descent to a refined cell follows libprofit's FP64 node bounds, so the
+// trees are its own.  Every lane returns its share of sum_s weight_s f(s) in
+// `sum`; with kJac (the forward walk of a gradient evaluation) every leaf
+// also adds its weighted FP32 derivatives to gj (per lane), for the pixel's
+// Jacobian.
+template <int TYPE, bool kJac>
+__device__ __forceinline__ void walk_pixel(const Comp &c, WalkLevel *L, int lane, double x0, double x1, double y0,
+                                           double y1, int res, int maxrec, bool fp64, double &sum, float *gj)
 {
 	const int ncell = res * res;
 	const int nchunk = (ncell + 63) >> 6;
@@ -204,21 +200,23 @@ __device__ __forceinline__ void walk_pixel(const Comp &c, WalkLevel *L, int lane
 	// 0.5 / res from an integer, far beyond the product's rounding, so the
 	// truncation is the exact quotient
 	const float inv_res = 1.f / (float)res;
-	const bool sums64 = !kGrad || c.grad64;
 	int level = 0;
 	double nx0 = x0, nx1 = x1, ny0 = y0, ny1 = y1;
 	double weight = inv_cells;
 	for (;;) {
-		const double xbin = (nx1 - nx0) / res;
-		const double ybin = (ny1 - ny0) / res;
-		const double hx = xbin / 2., hy = ybin / 2.;
-		// root_only (adjoint): the forward walk found no cell of the root to
-		// refine, so every cell is a leaf and no test is repeated
-		const bool recurse = res > 1 && level < maxrec && !root_only;
-		const double dyp = -xbin * c.sin_over + ybin * c.cos_over;
+		const bool recurse = res > 1 && level < maxrec;
+		// the node's cell size and the test offset in FP32; libprofit's FP64
+		// ones (node64) only for the cells that need an FP64 position
+		const float fxbin = (float)(nx1 - nx0) * inv_res, fybin = (float)(ny1 - ny0) * inv_res;
+		const float fdyp = -fxbin * c.f.sin_over + fybin * c.f.cos_over;
 		const float ox = (float)(nx0 - c.xc), oy = (float)(ny0 - c.yc);
-		const float fxbin = (float)xbin, fybin = (float)ybin, fdyp = (float)dyp;
-		const double w = weight * scale;
+		auto node64 = [&](double &hx, double &hy, double &dyp) {
+			const double xbin = (nx1 - nx0) / res;
+			const double ybin = (ny1 - ny0) / res;
+			hx = xbin / 2.;
+			hy = ybin / 2.;
+			dyp = -xbin * c.sin_over + ybin * c.cos_over;
+		};
 		// the cells in chunks of 64 (one per lane); lane 0 stores each chunk's
 		// refine mask with the node, and `any` gathers them.  A node with no
 		// cell to refine is not stored: the search below starts under it.
@@ -231,52 +229,39 @@ __device__ __forceinline__ void walk_pixel(const Comp &c, WalkLevel *L, int lane
 				const int ci = (int)(((float)cell + 0.5f) * inv_res), cj = cell - ci * res;
 				float xp, yp;
 				cell_pos32(c, ox, oy, fxbin, fybin, ci, cj, &xp, &yp);
-				// the adjoint needs the FP32 value only for the refinement test
-				const float v32 = kGrad ? (recurse ? eval_fast<TYPE>(c, xp, yp) : 0.f) : eval_at<float, TYPE>(c, xp, yp);
+				const float v32 = eval_at<float, TYPE>(c, xp, yp);
 				int dec = recurse ? refine_cell32<TYPE>(c, xp, yp, fdyp, v32) : 0;
 				double v64 = 0;
 				bool have64 = false;
 				if (dec == 2) {
-					double xp64, yp64;
+					double hx, hy, dyp, xp64, yp64;
+					node64(hx, hy, dyp);
 					cell_pos64(c, nx0, ny0, hx, hy, ci, cj, &xp64, &yp64);
 					dec = refine_cell64<TYPE>(c, xp64, yp64, dyp, &v64) ? 1 : 0;
 					have64 = true;
 				}
 				fail = dec == 1;
 				if (!fail) {
-					if (kGrad) {
-						// value (for the flux-scale derivatives) and derivatives
-						if (sums64) {
-							const double f = sample_accumulate<float, TYPE>(c, (double)xp, (double)yp, w, gs);
-							sum += f * w;
+					// leaf values in FP64 or the FP32 value already at hand
+					if (!have64) {
+						if (fp64) {
+							double hx, hy, dyp, xp64, yp64;
+							node64(hx, hy, dyp);
+							cell_pos64(c, nx0, ny0, hx, hy, ci, cj, &xp64, &yp64);
+							v64 = eval_at<double, TYPE>(c, xp64, yp64);
 						}
 						else {
-							const float fw = (float)w;
-							sumf += fw * pixel_accumulate_f32<TYPE>(c, xp, yp, fw, gsf);
+							v64 = (double)v32;
 						}
 					}
-					else {
-						// leaf values in FP64 or the FP32 value already at hand
-						if (!have64) {
-							if (fp64) {
-								double xp64, yp64;
-								cell_pos64(c, nx0, ny0, hx, hy, ci, cj, &xp64, &yp64);
-								v64 = eval_at<double, TYPE>(c, xp64, yp64);
-							}
-							else {
-								v64 = (double)v32;
-							}
-						}
-						sum += v64 * w;
-						if (kJac) pixel_accumulate_f32<TYPE>(c, xp, yp, (float)w, gj);
-					}
+					sum += v64 * weight;
+					if (kJac) pixel_accumulate_f32<TYPE>(c, xp, yp, (float)weight, gj);
 				}
 			}
 			const unsigned long long mk = __ballot(fail);
 			any |= mk;
 			if (lane == 0) L[level].mask[k] = mk;
 		}
-		if (!kGrad && level == 0 && lane == 0) *root_flag = any == 0ull;
 		if (any != 0ull) {
 			if (lane == 0) {
 				L[level].x0 = nx0;
@@ -329,22 +314,16 @@ __device__ __forceinline__ void walk_pixel(const Comp &c, WalkLevel *L, int lane
 	}
 }
 
-// One wavefront per queued pixel of a TYPE component (entries of other
-// families, and in the adjoint those whose steep-core flag differs from PREC,
-// are left to the launch made for them).  kGrad=false: adds flux_scale *
-// value to the canvas.  kGrad=true: accumulates G(q) * d(flux_scale *
-// value)/d slot; per-lane sums are carried across consecutive entries of the
-// same component (the FP32 ones folded into FP64 after every grab) and
-// flushed with one set of atomics per run.  `counter`: the pair of hand-out
-// counters of this launch (heavy, light).  Held to the registers of three
-// waves per SIMD in the adjoint and four in the forward walk.
-// kJac (forward only): each entry's Jacobian and value go to wq.ljac / hjac
-// for k_adj_queue, which replaces the adjoint re-walk.
-template <bool kGrad, int TYPE, int PREC, bool kJac = false>
-__global__ __launch_bounds__(256, (kGrad || kJac) ? 3 : 4) void k_subsample(const GalaxyDev *__restrict__ gals,
-                                                   const InstDev *__restrict__ insts,
-                                                   const Comp *__restrict__ comps, WorkQueue wq, int counter,
-                                                   int split, double *grad_acc)
+// One wavefront per queued pixel of a TYPE component: adds flux_scale * value
+// to the canvas (or the aux canvas, or the FP64 canvas) and, with kJac, writes
+// the entry's Jacobian and value to wq.ljac / hjac for k_adj_queue.
+// `counter`: the pair of hand-out counters of this launch (heavy, light).
+// Held to the registers of four waves per SIMD (three with kJac).
+template <int TYPE, bool kJac>
+__global__ __launch_bounds__(256, kJac ? 3 : 4) void k_subsample(const GalaxyDev *__restrict__ gals,
+                                                                 const InstDev *__restrict__ insts,
+                                                                 const Comp *__restrict__ comps, WorkQueue wq,
+                                                                 int counter, int split)
 {
 	__shared__ WalkLevel levels[4][kMaxLevels];
 	// the wavefront index is uniform: make that visible to the compiler so queue
@@ -354,50 +333,15 @@ __global__ __launch_bounds__(256, (kGrad || kJac) ? 3 : 4) void k_subsample(cons
 	static_assert(TYPE >= 0 && TYPE < kFamilies, "one walker launch per profile family");
 	const int lb = wq.lbase[TYPE];
 	const int nl = min(wq.count[light_count(TYPE)], wq.lbase[TYPE + 1] - lb), nh = min(wq.count[1], wq.hcap);
-	// slots this family can have: the geometric ones and its shape parameters
-	constexpr int kSlots = G_S0 + n_shape(TYPE);
-	double gs[kSlots];
-	float gsf[NGRAD];
-#pragma unroll
-	for (int s = 0; s < kSlots; ++s) gs[s] = 0;
-#pragma unroll
-	for (int s = 0; s < NGRAD; ++s) gsf[s] = 0.f;
-	double vsum = 0;
-	float vsumf = 0.f;
-	int cur = -1;
 	float gj[NGRAD];
 #pragma unroll
 	for (int s = 0; s < NGRAD; ++s) gj[s] = 0.f;
-	// the FP32 sums into the FP64 ones
-	auto fold = [&]() {
-#pragma unroll
-		for (int s = 0; s < kSlots; ++s) {
-			gs[s] += (double)gsf[s];
-			gsf[s] = 0.f;
-		}
-		vsum += (double)vsumf;
-		vsumf = 0.f;
-	};
-	// the current component's sums to its gradient accumulators
-	auto flush = [&]() {
-		fold();
-		const Comp &pc = comps[cur];
-		const double tv = wave_sum(vsum);
-#pragma unroll
-		for (int s = 0; s < kSlots; ++s) {
-			const double t = wave_sum(gs[s]) + pc.dlnfs[s] * tv;
-			if (lane == 0 && t != 0) atomicAdd(&grad_acc[(long)cur * NGRAD + s], t);
-			gs[s] = 0;
-		}
-		vsum = 0;
-	};
 	// dynamic hand-out: the deep centre trees one at a time first, then the
 	// other entries in grabs of kGrab, so no wave is left with a long tail
 	bool heavy_left = nh > 0;
 	for (;;) {
 		int base = 0, take = 0;
-		const QEntry *list;
-		uint8_t *roots;
+		bool heavy = false;
 		{
 			int h = nh;
 			if (heavy_left) {
@@ -407,8 +351,7 @@ __global__ __launch_bounds__(256, (kGrad || kJac) ? 3 : 4) void k_subsample(cons
 			if (h < nh) {
 				base = h;
 				take = 1;
-				list = wq.heavy;
-				roots = wq.hroot;
+				heavy = true;
 			}
 			else {
 				heavy_left = false;
@@ -418,23 +361,16 @@ __global__ __launch_bounds__(256, (kGrad || kJac) ? 3 : 4) void k_subsample(cons
 				if (l >= nl) break;
 				base = l;
 				take = min(kGrab, nl - l);
-				list = wq.light + lb;
-				roots = wq.lroot + lb;
 			}
 		}
-		// the grab's entries (and in the adjoint their G(q) and root flags)
-		// fetched at once, lane u holding entry base + u, so the per-entry
-		// chain of dependent loads is paid once per grab
-		int qc = 0, qp = 0, qroot = 0;
-		float qg = 0.f;
+		const QEntry *list = heavy ? wq.heavy : wq.light + lb;
+		// the grab's entries fetched at once, lane u holding entry base + u, so
+		// the per-entry chain of dependent loads is paid once per grab
+		int qc = 0, qp = 0;
 		if (lane < take) {
 			const QEntry qe = list[base + lane];
 			qc = qe.comp;
 			qp = qe.pix;
-			if (kGrad) {
-				qg = insts[comps[qc].instance].G[qp];
-				qroot = roots[base + lane];
-			}
 		}
 		for (int e = base; e < base + take; ++e) {
 			const int u = e - base;
@@ -442,19 +378,9 @@ __global__ __launch_bounds__(256, (kGrad || kJac) ? 3 : 4) void k_subsample(cons
 			q.comp = __builtin_amdgcn_readlane(qc, u);
 			q.pix = __builtin_amdgcn_readlane(qp, u);
 			const Comp &c = comps[q.comp];
-			if ((TYPE >= 0 && c.type != TYPE) || (kGrad && PREC != 2 && c.grad64 != PREC)) continue;
-			if (kGrad && q.comp != cur) {
-				if (cur >= 0) flush();
-				cur = q.comp;
-			}
+			if (c.type != TYPE) continue;
 			const InstDev &I = insts[c.instance];
 			const GalaxyDev &g = gals[I.galaxy];
-			double scale = 1.0;
-			if (kGrad) {
-				const float Gq = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, qg), u));
-				if (Gq == 0.f) continue;
-				scale = (double)Gq * c.flux_scale;
-			}
 			const int i = q.pix % g.DW, j = q.pix / g.DW;
 			const double hx = g.sx / 2., hy = g.sy / 2.;
 			const double y = hy + j * g.sx;
@@ -465,41 +391,33 @@ __global__ __launch_bounds__(256, (kGrad || kJac) ? 3 : 4) void k_subsample(cons
 				maxrec = 10;
 			}
 			double s_local = 0;
-			walk_pixel<kGrad, TYPE, PREC, kJac>(c, L, lane, x - hx, x + hx, y - hy, y + hy, res, maxrec, scale,
-			                                    g.render_fp64 != 0, s_local, gs, vsumf, gsf,
-			                                    kGrad && __builtin_amdgcn_readlane(qroot, u) != 0, roots + e, gj);
-			if (kGrad) {
-				vsum += s_local;
+			walk_pixel<TYPE, kJac>(c, L, lane, x - hx, x + hx, y - hy, y + hy, res, maxrec, g.render_fp64 != 0, s_local,
+			                       gj);
+			const double tot = wave_sum(s_local);
+			const bool to_aux = split && !c.convolve;
+			if (lane == 0) {
+				if (g.canvas64 && !to_aux) atomicAdd(&I.canvas64[q.pix], c.flux_scale * tot);
+				else atomicAdd(&(to_aux ? I.aux : I.canvas)[q.pix], (float)(c.flux_scale * tot));
 			}
-			else {
-				const double tot = wave_sum(s_local);
-				const bool to_aux = split && !c.convolve;
-				if (lane == 0) {
-					if (g.canvas64 && !to_aux) atomicAdd(&I.canvas64[q.pix], c.flux_scale * tot);
-					else atomicAdd(&(to_aux ? I.aux : I.canvas)[q.pix], (float)(c.flux_scale * tot));
-				}
-				if (kJac) {
-					// the pixel's Jacobian (cross-lane sums in FP64) and value:
-					// lane s writes element s
-					float jv = lane == kJacStride - 1 ? (float)tot : 0.f;
+			if (kJac) {
+				// the pixel's Jacobian (cross-lane sums in FP64) and value:
+				// lane s writes element s
+				float jv = lane == kJacStride - 1 ? (float)tot : 0.f;
 #pragma unroll
-					for (int s = 0; s < G_S0 + n_shape(TYPE); ++s) {
-						const double t = wave_sum((double)gj[s]);
-						gj[s] = 0.f;
-						if (lane == s) jv = (float)t;
-					}
-					float *J = list == wq.heavy ? wq.hjac : wq.ljac + (long)lb * kJacStride;
-					if (lane < kJacStride) J[(long)e * kJacStride + lane] = jv;
+				for (int s = 0; s < G_S0 + n_shape(TYPE); ++s) {
+					const double t = wave_sum((double)gj[s]);
+					gj[s] = 0.f;
+					if (lane == s) jv = (float)t;
 				}
+				float *J = heavy ? wq.hjac : wq.ljac + (long)lb * kJacStride;
+				if (lane < kJacStride) J[(long)e * kJacStride + lane] = jv;
 			}
 		}
-		if (kGrad) fold();
 	}
-	if (kGrad && cur >= 0) flush();
 }
 
 // The adjoint of the sub-sampled pixels from their Jacobians (written by the
-// forward walk, k_subsample<false, TYPE, 0, true>): per entry (component c,
+// forward walk, k_subsample<TYPE, true>): per entry (component c,
 // pixel q), G(q) fs (J_s + dlnfs_s V) for every slot s.  Lanes take
 // consecutive entries of the family's lists (the heavy list first, Sersic
 // only); the lanes of one component are summed across the wave, one

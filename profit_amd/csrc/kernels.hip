@@ -5,14 +5,16 @@
 //                    direct FP32 evaluation (radial.cpp:291-344); pixels that need
 //                    sub-pixel integration are appended to a work queue
 //   k_subsample      one wavefront per queued pixel walks libprofit's adaptive
-//                    recursion (radial.cpp:54-134) depth-first in FP64; the walk
-//                    keeps one node and a 256-bit "cells to refine" mask per level
+//                    recursion (radial.cpp:54-134) depth-first, FP32 cell positions
+//                    and tests with FP64 re-tests in a guard band; the walk keeps
+//                    one node and a 256-bit "cells to refine" mask per level
 //   k_conv           LDS-tiled brute-force convolution (convolve.cpp:158-231) with
 //                    three epilogues: fused masked likelihood, model image, and
 //                    the adjoint correlation
 //   k_like_finalize  compensated FP64 reduction of the likelihood partials
-//   k_adj_render / k_subsample<true>
+//   k_adj_render / k_adj_queue
 //                    fixed-tree adjoint of the rendering: sum_q G(q) d pixel_q / d theta
+//                    (the sub-sampled pixels' derivatives come from the forward walk)
 //   k_dilate_rows/cols  Mask::expand_by (image.cpp:113-162) as a separable dilation
 //
 // Compiled with -ffp-contract=off: every FP64 operation of the classification

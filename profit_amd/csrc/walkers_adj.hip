@@ -1,10 +1,8 @@
-// Adjoint of the rendering: the re-walk of the recursion trees
-// (k_subsample<true>, one kernel per profile family, as the forward walk: the
-// Sersic one fits three waves per SIMD, 1.43 against 2.05 ms for one kernel
-// for every family on the 400x400 bulge+disk batch) and the adjoint of the
-// directly evaluated pixels (k_adj_render,
-// one kernel per radial family and sum precision, FP64 sums only for the
-// steep-core families Sersic and core-Sersic); see kernel_common.h.
+// Adjoint of the rendering: the sub-sampled pixels from the Jacobians of the
+// forward walk (k_adj_queue, one kernel per profile family) and the directly
+// evaluated pixels (k_adj_render, one kernel per radial family and sum
+// precision, FP64 sums only for the steep-core families Sersic and
+// core-Sersic); see kernel_common.h.
 #include "kernel_common.h"
 
 namespace profit_amd {
@@ -32,29 +30,6 @@ static int adj_render_launch(const GalaxyDev *gals, const InstDev *insts, const 
 	if (type == SERSIC) return CALL(SERSIC, 1);                       \
 	if (type == CORESERSIC) return CALL(CORESERSIC, 1);               \
 	return (int)hipErrorInvalidValue;
-
-template <int TYPE>
-static int adj_walk_launch(int nblocks, const GalaxyDev *gals, const InstDev *insts, const Comp *comps, WorkQueue wq,
-                           double *grad_acc, hipStream_t st)
-{
-	k_subsample<true, TYPE, 2><<<nblocks, 256, 0, st>>>(gals, insts, comps, wq, adj_counter(TYPE), 0, grad_acc);
-	return (int)hipGetLastError();
-}
-
-int launch_subsample_adj(int type, int nblocks, const GalaxyDev *gals, const InstDev *insts, const Comp *comps,
-                         WorkQueue wq, double *grad_acc, hipStream_t st)
-{
-	if (nblocks <= 0) return 0;
-	switch (type) {
-	case SERSIC: return adj_walk_launch<SERSIC>(nblocks, gals, insts, comps, wq, grad_acc, st);
-	case MOFFAT: return adj_walk_launch<MOFFAT>(nblocks, gals, insts, comps, wq, grad_acc, st);
-	case FERRER: return adj_walk_launch<FERRER>(nblocks, gals, insts, comps, wq, grad_acc, st);
-	case KING: return adj_walk_launch<KING>(nblocks, gals, insts, comps, wq, grad_acc, st);
-	case CORESERSIC: return adj_walk_launch<CORESERSIC>(nblocks, gals, insts, comps, wq, grad_acc, st);
-	case BROKENEXP: return adj_walk_launch<BROKENEXP>(nblocks, gals, insts, comps, wq, grad_acc, st);
-	default: return (int)hipErrorInvalidValue;
-	}
-}
 
 template <int TYPE>
 static int adj_queue_launch(int nblocks, const InstDev *insts, const Comp *comps, WorkQueue wq, double *grad_acc,

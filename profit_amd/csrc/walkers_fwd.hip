@@ -8,11 +8,8 @@ template <int TYPE>
 static int fwd_launch(int nblocks, const GalaxyDev *gals, const InstDev *insts, const Comp *comps, WorkQueue wq,
                       int split, bool jac, hipStream_t st)
 {
-	if (jac)
-		k_subsample<false, TYPE, 0, true><<<nblocks, 256, 0, st>>>(gals, insts, comps, wq, fwd_counter(TYPE), split,
-		                                                           nullptr);
-	else
-		k_subsample<false, TYPE, 0><<<nblocks, 256, 0, st>>>(gals, insts, comps, wq, fwd_counter(TYPE), split, nullptr);
+	if (jac) k_subsample<TYPE, true><<<nblocks, 256, 0, st>>>(gals, insts, comps, wq, fwd_counter(TYPE), split);
+	else k_subsample<TYPE, false><<<nblocks, 256, 0, st>>>(gals, insts, comps, wq, fwd_counter(TYPE), split);
 	return (int)hipGetLastError();
 }
 
