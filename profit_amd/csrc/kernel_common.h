@@ -181,6 +181,27 @@ __device__ __forceinline__ float pixel_accumulate_f32(const Comp &c, float xp, f
 	return o[0];
 }
 
+// The same for a Sersic sample without box from the intermediates of its
+// value (sersic_parts_f32): f, base = (r / re)^2, l2 = log2(base), rf =
+// base^(1/(2n)), so that a leaf pays no second logarithm or exponential.
+__device__ __forceinline__ void sersic_jac_f32(const Comp &c, float xp, float yp, float f, float base, float l2,
+                                               float rf, float wgt, float *gs)
+{
+	if (!(base > 0.f)) return;  // the centre: every derivative is 0
+	const float bn = c.f.bn, in = c.f.inv_n;
+	const float bfr = bn * f * rf * in;
+	// df/dxp = k xp, df/dyp = k yp with k = -bn f rf / (n re^2 base)
+	const float k = -bfr * c.f.inv_re2 * __builtin_amdgcn_rcpf(base);
+	const float fx = k * xp, fy = k * yp;
+	const float q = c.f.axrat, iq = c.f.inv_axrat;
+	gs[G_XCEN] += wgt * (fx * -c.f.cos_ang + fy * c.f.sin_over);
+	gs[G_YCEN] += wgt * (fx * -c.f.sin_ang + fy * -c.f.cos_over);
+	gs[G_ANG] += wgt * ((fx * (q * yp) + fy * (-xp * iq)) * (3.14159265358979323846f / 180.f));
+	gs[G_AXRAT] += wgt * (fy * (-yp * iq));
+	gs[G_S0] += wgt * (bfr * c.f.inv_re);
+	gs[G_S1] += wgt * (f * (-c.f.dbn * (rf - 1.f) + bn * rf * (l2 * float(kLn2)) * in * c.f.inv_2n));
+}
+
 // Walks the recursion tree of one pixel of a TYPE profile.  Cell positions,
 // values and refinement tests are FP32; libprofit's FP64 cell position is
 // formed only for an undecided test and for FP64 leaf values (`fp64`).  The
@@ -193,6 +214,9 @@ template <int TYPE, bool kJac>
 __device__ __forceinline__ void walk_pixel(const Comp &c, WalkLevel *L, int lane, double x0, double x1, double y0,
                                            double y1, int res, int maxrec, bool fp64, double &sum, float *gj)
 {
+	// a Sersic Jacobian without box (or box derivative) from the value's
+	// own logarithm and exponentials
+	const bool fuse = kJac && TYPE == SERSIC && !c.boxy && !c.grad_box;
 	const int ncell = res * res;
 	const int nchunk = (ncell + 63) >> 6;
 	const double inv_cells = 1.0 / (double)ncell;
@@ -229,7 +253,9 @@ __device__ __forceinline__ void walk_pixel(const Comp &c, WalkLevel *L, int lane
 				const int ci = (int)(((float)cell + 0.5f) * inv_res), cj = cell - ci * res;
 				float xp, yp;
 				cell_pos32(c, ox, oy, fxbin, fybin, ci, cj, &xp, &yp);
-				const float v32 = eval_at<float, TYPE>(c, xp, yp);
+				float v32, s_base = 0.f, s_l2 = 0.f, s_rf = 0.f;
+				if (fuse) v32 = sersic_parts_f32(c, xp, yp, s_base, s_l2, s_rf);
+				else v32 = eval_at<float, TYPE>(c, xp, yp);
 				int dec = recurse ? refine_cell32<TYPE>(c, xp, yp, fdyp, v32) : 0;
 				double v64 = 0;
 				bool have64 = false;
@@ -255,7 +281,8 @@ __device__ __forceinline__ void walk_pixel(const Comp &c, WalkLevel *L, int lane
 						}
 					}
 					sum += v64 * weight;
-					if (kJac) pixel_accumulate_f32<TYPE>(c, xp, yp, (float)weight, gj);
+					if (fuse) sersic_jac_f32(c, xp, yp, v32, s_base, s_l2, s_rf, (float)weight, gj);
+					else if (kJac) pixel_accumulate_f32<TYPE>(c, xp, yp, (float)weight, gj);
 				}
 			}
 			const unsigned long long mk = __ballot(fail);
@@ -326,13 +353,20 @@ __global__ __launch_bounds__(256, kJac ? 3 : 4) void k_subsample(const GalaxyDev
                                                                  int counter, int split)
 {
 	__shared__ WalkLevel levels[4][kMaxLevels];
+	// the Jacobian's slots of a pixel: per wavefront, one row of 64 lane sums
+	// per slot (rows kJacPitch floats apart, so that the 16 lanes reading one
+	// quarter of each row fall on distinct banks)
+	constexpr int kSlots = G_S0 + n_shape(TYPE), kJacPitch = 68;
+	__shared__ __align__(16) float jred[kJac ? 4 : 1][kJac ? kSlots * kJacPitch : 1];
 	// the wavefront index is uniform: make that visible to the compiler so queue
 	// entries and component fields are read through the scalar cache
 	const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 	WalkLevel *L = levels[wv];
 	static_assert(TYPE >= 0 && TYPE < kFamilies, "one walker launch per profile family");
+	static_assert(kSlots < kJacStride && kSlots <= 16, "Jacobian slots");
 	const int lb = wq.lbase[TYPE];
 	const int nl = min(wq.count[light_count(TYPE)], wq.lbase[TYPE + 1] - lb), nh = min(wq.count[1], wq.hcap);
+	// per-lane Jacobian sums
 	float gj[NGRAD];
 #pragma unroll
 	for (int s = 0; s < NGRAD; ++s) gj[s] = 0.f;
@@ -400,17 +434,32 @@ __global__ __launch_bounds__(256, kJac ? 3 : 4) void k_subsample(const GalaxyDev
 				else atomicAdd(&(to_aux ? I.aux : I.canvas)[q.pix], (float)(c.flux_scale * tot));
 			}
 			if (kJac) {
-				// the pixel's Jacobian (cross-lane sums in FP64) and value:
-				// lane s writes element s
-				float jv = lane == kJacStride - 1 ? (float)tot : 0.f;
+				// the pixel's Jacobian: the lane sums go to the wavefront's
+				// rows; lane l then adds quarter l >> 4 of row l & 15 in FP64
+				// and two exchanges add the quarters
+				float *R = jred[kJac ? wv : 0];
 #pragma unroll
-				for (int s = 0; s < G_S0 + n_shape(TYPE); ++s) {
-					const double t = wave_sum((double)gj[s]);
+				for (int s = 0; s < kSlots; ++s) {
+					R[s * kJacPitch + lane] = gj[s];
 					gj[s] = 0.f;
-					if (lane == s) jv = (float)t;
 				}
-				float *J = heavy ? wq.hjac : wq.ljac + (long)lb * kJacStride;
-				if (lane < kJacStride) J[(long)e * kJacStride + lane] = jv;
+				__builtin_amdgcn_wave_barrier();
+				const int slot = lane & 15, qt = lane >> 4;
+				double t = 0.0;
+				if (slot < kSlots) {
+					const float4 *row = reinterpret_cast<const float4 *>(R + slot * kJacPitch + qt * 16);
+#pragma unroll
+					for (int k = 0; k < 4; ++k) {
+						const float4 v = row[k];
+						t += ((double)v.x + (double)v.y) + ((double)v.z + (double)v.w);
+					}
+				}
+				__builtin_amdgcn_wave_barrier();
+				t += __shfl_xor(t, 16);
+				t += __shfl_xor(t, 32);
+				float *J = (heavy ? wq.hjac : wq.ljac + (long)lb * kJacStride) + (long)e * kJacStride;
+				if (lane < kSlots) J[lane] = (float)t;
+				if (lane == 0) J[kJacStride - 1] = (float)tot;
 			}
 		}
 	}

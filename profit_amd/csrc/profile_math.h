@@ -334,6 +334,17 @@ PA_HDI T boxy_r(const Comp &c, T x, T y)
 	}
 }
 
+// FP32 Sersic value without box at profile position (x, y), with the
+// intermediates its derivatives share: base = (r / re)^2, l2 = log2(base)
+// and rf = base^(1/(2n)).
+PA_HDI float sersic_parts_f32(const Comp &c, float x, float y, float &base, float &l2, float &rf)
+{
+	base = (x * x + y * y) * c.f.inv_re2;
+	l2 = log2_fast(base);
+	rf = exp2_fast(l2 * c.f.inv_2n);
+	return exp2_fast(-c.f.bn * (rf - 1.f) * float(kLog2e));
+}
+
 template <class T>
 PA_HDI T sersic_at(const Comp &c, T x, T y)
 {
@@ -341,16 +352,14 @@ PA_HDI T sersic_at(const Comp &c, T x, T y)
 		// base^(1/(n B)) = 2^(log2(base) / (n B)) and the exponential on the
 		// transcendental unit, for every n (the FP64 path keeps libprofit's
 		// special cases of n)
-		float rf;
 		if (c.boxy) {
 			const float B = c.f.box + 2.f;
 			const float base = pow_t<float>(::fabsf(x * c.f.inv_re), B) + pow_t<float>(::fabsf(y * c.f.inv_re), B);
-			rf = exp2_fast(log2_fast(base) * c.f.inv_nB);
+			const float rf = exp2_fast(log2_fast(base) * c.f.inv_nB);
+			return exp2_fast(-c.f.bn * (rf - 1.f) * float(kLog2e));
 		}
-		else {
-			rf = exp2_fast(log2_fast((x * x + y * y) * c.f.inv_re2) * c.f.inv_2n);
-		}
-		return exp2_fast(-c.f.bn * (rf - 1.f) * float(kLog2e));
+		float base, l2, rf;
+		return sersic_parts_f32(c, x, y, base, l2, rf);
 	}
 	else {
 		const T re = pick<T>(c.s[0], c.f.s[0]), nser = pick<T>(c.s[1], c.f.s[1]), bn = pick<T>(c.bn, c.f.bn);
