@@ -408,7 +408,7 @@ void Engine::conv_forward(int mode, hipStream_t st)
 	          "fft cols");
 	if (timed) mark_fft(0, 2, st);
 	if (mode == 0)
-		hip_check(launch_fft_rows_inv(ri0_off_[n], fft_nmax1_, gals, insts, d_ri0_blk_.as<int2>(), 0,
+		hip_check(launch_fft_rows_inv(ri0_off_[n], fft_nmax1_, gals, insts, d_ri0_blk_.as<int2>(), all_gauss_ ? 3 : 0,
 		                              d_tw_.as<double2>(), twn_, lk, partial, st),
 		          "fft rows inv");
 	else
@@ -790,6 +790,7 @@ void Engine::set_batch(const std::vector<int> &inst_galaxy, const std::vector<in
 	at_off_.assign(n + 1, 0);
 	ncomp_total_ = 0;
 	any_t_ = false;
+	all_gauss_ = true;
 	kx_max_ = 0;
 	long qneed = 1024, hneed = 64;
 	inst_bufs_.clear();
@@ -801,6 +802,7 @@ void Engine::set_batch(const std::vector<int> &inst_galaxy, const std::vector<in
 		if (inst_ncomp[i] < 0) throw std::invalid_argument("negative component count");
 		const GalaxyDev &g = gal_host_[gi];
 		any_t_ |= g.likefunc == LIKE_T;
+		all_gauss_ &= g.likefunc == LIKE_CHISQ || g.likefunc == LIKE_NORM;
 		inst_comp0_[i] = ncomp_total_;
 		ncomp_total_ += inst_ncomp[i];
 		for (int k = 0; k < inst_ncomp[i]; ++k) comp_inst.push_back(i);
@@ -995,8 +997,10 @@ void Engine::record(bool grad)
 	int rtype = -1;
 	for (int t = 0; t < 6; ++t)
 		if (!any_nonradial_ && radial_types_ == (1u << t)) rtype = t;
+	// the render's FP64 value path only where a pixel can take it
+	const bool fp64 = render_fp64_ || canvas64_ || (radial_types_ & ((1u << KING) | (1u << FERRER)));
 	hip_check(launch_render(gals, insts, d_at_off_.as<int>(), n, n_at_blocks_, d_comps_.as<Comp>(),
-	                        work_queue(), -1, 0, nullptr, st, rtype),
+	                        work_queue(), -1, 0, nullptr, st, rtype, fp64),
 	          "render");
 	mark(2);
 	launch_walkers_fwd(0, grad, st);

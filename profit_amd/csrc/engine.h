@@ -21,8 +21,11 @@ int launch_setup(const ProfileParams *params, const int *order, const int *comp_
 int launch_setup_grad(const int *order, Comp *comps, const int *errors, int ncomp, hipStream_t st);
 int launch_zero_d(double *p, long n, hipStream_t st);
 int launch_zero_i(int *p, long n, hipStream_t st);
+// fp64: some pixel may take an FP64 value (the FP64 model modes, King or
+// Ferrer components in the batch); false only when none can
 int launch_render(const GalaxyDev *gals, const InstDev *insts, const int *tile_off, int ninst, int nblocks,
-                  const Comp *comps, WorkQueue wq, int filter, int target, int *diag_cls, hipStream_t st, int type = -1);
+                  const Comp *comps, WorkQueue wq, int filter, int target, int *diag_cls, hipStream_t st, int type = -1,
+                  bool fp64 = true);
 // the forward walker and the adjoint render, one launch per radial family
 // (and per sum precision in the adjoint render: 1 = FP64 per-thread sums,
 // steep cores only).  jac: the forward walk also writes each entry's Jacobian
@@ -58,6 +61,8 @@ int launch_fft_psf_rows(int n1, int n2, const GalaxyDev *gal, const double2 *tw,
 // of that instance) per launched block.
 int launch_fft_cols(int nblocks, int n2max, int cb, const GalaxyDev *gals, const InstDev *insts, const int2 *blocks,
                     int op, int in_mode, int out_mode, const double2 *tw, int twn, double2 *dst64, hipStream_t st);
+// mode 0 likelihood, 1 model image, 2 adjoint image, 3 likelihood of a batch
+// whose galaxies all take a Gaussian likelihood (chi^2 or normal)
 int launch_fft_rows_inv(int nblocks, int nmax, const GalaxyDev *gals, const InstDev *insts, const int2 *blocks,
                         int mode, const double2 *tw, int twn, const int *lk_base, double *partial, hipStream_t st);
 int fft_cols_per_block(int n2);
@@ -292,6 +297,7 @@ class Engine {
 	std::vector<int> inst_galaxy_, inst_ncomp_, inst_comp0_;
 	int ncomp_total_ = 0;
 	bool any_t_ = false;  // an instance of the batch uses the t likelihood
+	bool all_gauss_ = true;  // every instance of the batch takes chi^2 or the normal likelihood
 	// point sources and sky of the current parameters: any at all (adjoint
 	// pass), and per instance whether one is drawn without convolution
 	bool any_nonradial_ = false;

@@ -46,12 +46,18 @@ __device__ inline unsigned bitrev(unsigned i, int logn) { return __brev(i) >> (3
 
 __device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-__device__ inline double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+// complex products with explicit FMAs (the library is built without
+// contraction, which the classification's libprofit rounding needs): two
+// multiplications and two FMAs instead of four and two additions
+__device__ inline double2 cmul(double2 a, double2 b)
+{
+	return make_double2(fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x));
+}
 __device__ inline double2 conjg(double2 a) { return make_double2(a.x, -a.y); }
 
 __device__ inline double2 cmulc(double2 a, double2 b)  // a * conj(b)
 {
-	return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+	return make_double2(fma(a.x, b.x, a.y * b.y), fma(a.y, b.x, -(a.x * b.y)));
 }
 
 // LDS position of element e of a sequence: bits 0-2 ^= bits 3-5 and bit 3 ^=
@@ -106,6 +112,71 @@ __device__ __forceinline__ void dit_regs(double2 *x, int s, int lo, const double
 	}
 }
 
+// Stages 1-3 of a forward DIT on elements e0 .. e0 + 7 (s = 0): their
+// twiddles are 1, -i and (+-1 - i) / sqrt(2), applied as constants.
+__device__ __forceinline__ void dit8_first(double2 *x)
+{
+	constexpr double c = 0.70710678118654752440;
+#pragma unroll
+	for (int j = 0; j < 8; j += 2) {
+		const double2 u = x[j], v = x[j + 1];
+		x[j] = cadd(u, v);
+		x[j + 1] = csub(u, v);
+	}
+#pragma unroll
+	for (int j = 0; j < 8; j += 4) {
+		double2 u = x[j], v = x[j + 2];
+		x[j] = cadd(u, v);
+		x[j + 2] = csub(u, v);
+		u = x[j + 1];
+		v = make_double2(x[j + 3].y, -x[j + 3].x);  // * -i
+		x[j + 1] = cadd(u, v);
+		x[j + 3] = csub(u, v);
+	}
+	const double2 v1 = make_double2(c * (x[5].x + x[5].y), c * (x[5].y - x[5].x));   // * (1 - i) / sqrt 2
+	const double2 v2 = make_double2(x[6].y, -x[6].x);                                 // * -i
+	const double2 v3 = make_double2(c * (x[7].y - x[7].x), -c * (x[7].x + x[7].y));  // * (-1 - i) / sqrt 2
+	const double2 u0 = x[0], u1 = x[1], u2 = x[2], u3 = x[3];
+	x[0] = cadd(u0, x[4]);
+	x[4] = csub(u0, x[4]);
+	x[1] = cadd(u1, v1);
+	x[5] = csub(u1, v1);
+	x[2] = cadd(u2, v2);
+	x[6] = csub(u2, v2);
+	x[3] = cadd(u3, v3);
+	x[7] = csub(u3, v3);
+}
+
+// Stages 3-1 of an inverse DIF on elements e0 .. e0 + 7 (s = 0), with the
+// conjugate twiddles 1, i and (+-1 + i) / sqrt(2) as constants.
+__device__ __forceinline__ void dif8_last(double2 *x)
+{
+	constexpr double c = 0.70710678118654752440;
+	{
+		const double2 d0 = csub(x[0], x[4]), d1 = csub(x[1], x[5]), d2 = csub(x[2], x[6]), d3 = csub(x[3], x[7]);
+#pragma unroll
+		for (int j = 0; j < 4; ++j) x[j] = cadd(x[j], x[j + 4]);
+		x[4] = d0;
+		x[5] = make_double2(c * (d1.x - d1.y), c * (d1.x + d1.y));    // * (1 + i) / sqrt 2
+		x[6] = make_double2(-d2.y, d2.x);                             // * i
+		x[7] = make_double2(-c * (d3.x + d3.y), c * (d3.x - d3.y));   // * (-1 + i) / sqrt 2
+	}
+#pragma unroll
+	for (int j = 0; j < 8; j += 4) {
+		const double2 d0 = csub(x[j], x[j + 2]), d1 = csub(x[j + 1], x[j + 3]);
+		x[j] = cadd(x[j], x[j + 2]);
+		x[j + 1] = cadd(x[j + 1], x[j + 3]);
+		x[j + 2] = d0;
+		x[j + 3] = make_double2(-d1.y, d1.x);  // * i
+	}
+#pragma unroll
+	for (int j = 0; j < 8; j += 2) {
+		const double2 u = x[j], v = x[j + 1];
+		x[j] = cadd(u, v);
+		x[j + 1] = csub(u, v);
+	}
+}
+
 // Radix-2 stages s+K down to s+1 of an inverse DIF (conjugate twiddles,
 // unscaled) in registers: stage S maps (u, v) at i and i + 2^(S-1) to
 // (u + v, (u - v) conj w).
@@ -146,7 +217,8 @@ __device__ __forceinline__ void dit_pass(double2 *data, int logn, int ncol, int 
 		double2 x[R];
 #pragma unroll
 		for (int j = 0; j < R; ++j) x[j] = col[swz(e0 + (j << s))];
-		dit_regs<K>(x, s, lo, wt);
+		if (K == 3 && s == 0) dit8_first(x);
+		else dit_regs<K>(x, s, lo, wt);
 #pragma unroll
 		for (int j = 0; j < R; ++j) col[swz(e0 + (j << s))] = x[j];
 	}
@@ -181,7 +253,7 @@ __device__ __forceinline__ void fft_dit_gather(double2 *data, int logn, int nseq
 		double2 x[8];
 #pragma unroll
 		for (int j = 0; j < 8; ++j) x[j] = load(c, (rev3(j) << lg) + bg);
-		dit_regs<3>(x, 0, 0, wt);
+		dit8_first(x);
 		double2 *col = data + (c << logn);
 #pragma unroll
 		for (int j = 0; j < 8; ++j) col[swz(8 * g + j)] = x[j];
@@ -207,7 +279,8 @@ __device__ __forceinline__ void dif_pass(double2 *data, int logn, int ncol, int 
 		double2 x[R];
 #pragma unroll
 		for (int j = 0; j < R; ++j) x[j] = col[swz(e0 + (j << s))];
-		dif_regs<K>(x, s, lo, wt);
+		if (K == 3 && s == 0) dif8_last(x);
+		else dif_regs<K>(x, s, lo, wt);
 #pragma unroll
 		for (int j = 0; j < R; ++j) col[swz(e0 + (j << s))] = x[j];
 	}
@@ -248,7 +321,7 @@ __device__ __forceinline__ void ifft_dif_scatter(double2 *data, int logn, int ns
 		double2 x[8];
 #pragma unroll
 		for (int j = 0; j < 8; ++j) x[j] = col[swz(8 * g + j)];
-		dif_regs<3>(x, 0, 0, wt);
+		dif8_last(x);
 		emit(c, g, lg ? (int)bitrev(g, lg) : 0, x);
 	}
 }
@@ -458,6 +531,8 @@ __global__ __launch_bounds__(kColThreads, 4) void k_fft_cols(const GalaxyDev *__
 //   mode 0 (likelihood): image rows (frame rows pady + r); model = value /
 //     (N1 N2) at column padx + x; likelihood terms into one partial per
 //     block, w written
+//   mode 3: mode 0 for batches whose galaxies all take a Gaussian
+//     likelihood (chi^2 or normal), without the Poisson and t code
 //   mode 1 (model image): canvas rows; G = model masked by calc
 //   mode 2 (adjoint): canvas rows; G = value / (N1 N2)
 template <int MODE>
@@ -466,7 +541,7 @@ __global__ __launch_bounds__(256) void k_fft_rows_inv(const GalaxyDev *__restric
                                                       const double2 *__restrict__ tw, int twn,
                                                       const int *__restrict__ lk_base, double *partial)
 {
-	constexpr int mode = MODE;
+	constexpr int mode = MODE == 3 ? 0 : MODE;
 	extern __shared__ double2 sh[];
 	const int2 bi = blocks[blockIdx.x];
 	const int inst = bi.x;
@@ -511,7 +586,7 @@ __global__ __launch_bounds__(256) void k_fft_rows_inv(const GalaxyDev *__restric
 	const int lg = L - 3;
 	if constexpr (mode == 0) {
 		double part = 0;
-		const bool pois = g.likefunc == LIKE_POIS, tl = g.likefunc == LIKE_T;
+		const bool pois = MODE == 0 && g.likefunc == LIKE_POIS, tl = MODE == 0 && g.likefunc == LIKE_T;
 		const double s = g.pois_scale;
 		// x[j] is frame column rev3(j) * N/8 + bg of rows 2p (real part) and
 		// 2p + 1 (imaginary part); image column = frame column - padx
@@ -605,7 +680,7 @@ static int ensure_lds(const void *fn, size_t lds, size_t *done)
 	done[dev] = lds;
 	return 0;
 }
-static size_t lds_rows_fwd[64], lds_rows_fwd64[64], lds_psf_rows[64], lds_cols[64], lds_rows_inv[64], lds_rows_inv1[64], lds_rows_inv2[64];
+static size_t lds_rows_fwd[64], lds_rows_fwd64[64], lds_psf_rows[64], lds_cols[64], lds_rows_inv[64], lds_rows_inv1[64], lds_rows_inv2[64], lds_rows_inv3[64];
 
 static int cols_per_block(int n2) { return n2 >= 4096 ? 1 : (n2 >= 2048 ? 2 : (4096 / n2 > 8 ? 8 : 4096 / n2)); }
 
@@ -662,7 +737,7 @@ int launch_fft_rows_inv(int nblocks, int nmax, const GalaxyDev *gals, const Inst
                         int mode, const double2 *tw, int twn, const int *lk_base, double *partial, hipStream_t st)
 {
 	if (nblocks <= 0) return 0;
-	if (mode < 0 || mode > 2) return (int)hipErrorInvalidValue;
+	if (mode < 0 || mode > 3) return (int)hipErrorInvalidValue;
 	const size_t lds = row_lds(nmax);
 	if (lds > 160 * 1024) return (int)hipErrorInvalidValue;
 	if (mode == 0) {
@@ -673,9 +748,13 @@ int launch_fft_rows_inv(int nblocks, int nmax, const GalaxyDev *gals, const Inst
 		if (int e = ensure_lds((const void *)k_fft_rows_inv<1>, lds, lds_rows_inv1)) return e;
 		k_fft_rows_inv<1><<<nblocks, 256, lds, st>>>(gals, insts, blocks, tw, twn, lk_base, partial);
 	}
-	else {
+	else if (mode == 2) {
 		if (int e = ensure_lds((const void *)k_fft_rows_inv<2>, lds, lds_rows_inv2)) return e;
 		k_fft_rows_inv<2><<<nblocks, 256, lds, st>>>(gals, insts, blocks, tw, twn, lk_base, partial);
+	}
+	else {
+		if (int e = ensure_lds((const void *)k_fft_rows_inv<3>, lds, lds_rows_inv3)) return e;
+		k_fft_rows_inv<3><<<nblocks, 256, lds, st>>>(gals, insts, blocks, tw, twn, lk_base, partial);
 	}
 	return (int)hipGetLastError();
 }

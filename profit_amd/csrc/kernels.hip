@@ -201,8 +201,11 @@ __device__ double point_value(const GalaxyDev &g, const PointFoot &f, int px, in
 // convolve flag equals it.  target: 0 canvas, 1 the aux canvas.  With filter
 // -1, point sources and sky drawn without convolution go to the aux canvas
 // when the instance has one in use (InstDev::has_aux); radial profiles are
-// always convolved there.
-template <int TYPE>
+// always convolved there.  F64: the kernel holds the FP64 value path (the
+// FP64 model modes, GalaxyDev::render_fp64 / canvas64, and King or Ferrer
+// pixels near their cut-off); without it a Sersic render needs 48 VGPRs
+// instead of 109, eight waves per SIMD instead of four.
+template <int TYPE, bool F64>
 __global__ __launch_bounds__(256) void k_render(const GalaxyDev *__restrict__ gals, const InstDev *__restrict__ insts,
                                                 const int *__restrict__ tile_off, int ninst,
                                                 const Comp *__restrict__ comps, WorkQueue wq, int filter, int target,
@@ -246,7 +249,7 @@ __global__ __launch_bounds__(256) void k_render(const GalaxyDev *__restrict__ ga
 					}
 				}
 				if (filter < 0 && !c.convolve && I.has_aux) aux_acc += v;
-				else if (g.canvas64 && !target) acc64 += (double)v;
+				else if (F64 && g.canvas64 && !target) acc64 += (double)v;
 				else acc += v;
 				if (diag_cls && inb) diag_cls[(long)(I.comp0 + k) * g.DW * g.DH + pix] = 0;
 				if (I.dmask && lane == 0 && j < g.DH) I.dmask[((long)k * g.DH + j) * ((g.DW + 63) >> 6) + t.x] = 0ull;
@@ -263,8 +266,8 @@ __global__ __launch_bounds__(256) void k_render(const GalaxyDev *__restrict__ ga
 				cls = classify32(c, fxp, fyp);
 				if (cls < 0) cls = classify(c, x, y).cls;
 				if (cls == 2) {
-					const bool c64 = g.canvas64 && !target;
-					if (c64 || g.render_fp64 || near_cut_fp64<TYPE>(c, fxp, fyp)) {
+					const bool c64 = F64 && g.canvas64 && !target;
+					if (F64 && (c64 || g.render_fp64 || near_cut_fp64<TYPE>(c, fxp, fyp))) {
 						const PixelClass pc = classify(c, x, y);
 						const double v = c.flux_scale * eval_at<double, TYPE>(c, pc.xp, pc.yp);
 						if (c64) acc64 += v;
@@ -291,7 +294,7 @@ __global__ __launch_bounds__(256) void k_render(const GalaxyDev *__restrict__ ga
 			append_entry(heavy, lane, wq.heavy, wq.count + 1, wq.hcap, QEntry{I.comp0 + k, pix});
 		}
 		if (inb) {
-			if (g.canvas64 && !target) I.canvas64[pix] = acc64;
+			if (F64 && g.canvas64 && !target) I.canvas64[pix] = acc64;
 			else out[pix] = acc;
 		}
 		if (inb && filter < 0 && I.has_aux) I.aux[pix] = aux_acc;
@@ -853,20 +856,34 @@ int launch_zero_i(int *p, long n, hipStream_t st)
 }
 
 int launch_render(const GalaxyDev *gals, const InstDev *insts, const int *tile_off, int ninst, int nblocks,
-                  const Comp *comps, WorkQueue wq, int filter, int target, int *diag_cls, hipStream_t st, int type)
+                  const Comp *comps, WorkQueue wq, int filter, int target, int *diag_cls, hipStream_t st, int type,
+                  bool fp64)
 {
 	if (nblocks <= 0) return 0;
 	// type >= 0: every component of the batch is of that radial family (the
-	// kernel then holds that family's code alone); -1 any mix
-#define PA_RENDER(T) k_render<T><<<nblocks, 256, 0, st>>>(gals, insts, tile_off, ninst, comps, wq, filter, target, diag_cls)
-	switch (type) {
-	case SERSIC: PA_RENDER(SERSIC); break;
-	case MOFFAT: PA_RENDER(MOFFAT); break;
-	case FERRER: PA_RENDER(FERRER); break;
-	case KING: PA_RENDER(KING); break;
-	case CORESERSIC: PA_RENDER(CORESERSIC); break;
-	case BROKENEXP: PA_RENDER(BROKENEXP); break;
-	default: PA_RENDER(-1); break;
+	// kernel then holds that family's code alone); -1 any mix.  King and
+	// Ferrer always take the kernel with the FP64 value path.
+	if (type == KING || type == FERRER) fp64 = true;
+#define PA_RENDER(T, F) k_render<T, F><<<nblocks, 256, 0, st>>>(gals, insts, tile_off, ninst, comps, wq, filter, target, diag_cls)
+	if (fp64) {
+		switch (type) {
+		case SERSIC: PA_RENDER(SERSIC, true); break;
+		case MOFFAT: PA_RENDER(MOFFAT, true); break;
+		case FERRER: PA_RENDER(FERRER, true); break;
+		case KING: PA_RENDER(KING, true); break;
+		case CORESERSIC: PA_RENDER(CORESERSIC, true); break;
+		case BROKENEXP: PA_RENDER(BROKENEXP, true); break;
+		default: PA_RENDER(-1, true); break;
+		}
+	}
+	else {
+		switch (type) {
+		case SERSIC: PA_RENDER(SERSIC, false); break;
+		case MOFFAT: PA_RENDER(MOFFAT, false); break;
+		case CORESERSIC: PA_RENDER(CORESERSIC, false); break;
+		case BROKENEXP: PA_RENDER(BROKENEXP, false); break;
+		default: PA_RENDER(-1, false); break;
+		}
 	}
 #undef PA_RENDER
 	return (int)hipGetLastError();

@@ -126,7 +126,8 @@ static void run_case(int W, int H, int K, int pad)
 		hip_cpu::launch(nblocks, 256, [&] {
 			if (mode == 0) k_fft_rows_inv<0>(gals, insts, blk, tw, twn, lk, partial);
 			else if (mode == 1) k_fft_rows_inv<1>(gals, insts, blk, tw, twn, lk, partial);
-			else k_fft_rows_inv<2>(gals, insts, blk, tw, twn, lk, partial);
+			else if (mode == 2) k_fft_rows_inv<2>(gals, insts, blk, tw, twn, lk, partial);
+			else k_fft_rows_inv<3>(gals, insts, blk, tw, twn, lk, partial);
 		});
 	};
 	set_lds(row_lds);
@@ -146,12 +147,17 @@ static void run_case(int W, int H, int K, int pad)
 	cols(1, 0, 1, nullptr);
 	rows_inv(1, row_blocks(g.DH));
 	for (int i = 0; i < g.DW * g.DH; ++i) err = std::fmax(err, std::fabs(insts[0].G[i] - insts[0].canvas[i]));
-	rows_fwd(0, row_blocks(g.DH));
-	cols(1, 0, 0, nullptr);
-	rows_inv(0, row_blocks(H));
-	for (int y = 0; y < H; ++y)
-		for (int x = 0; x < W; ++x)
-			err = std::fmax(err, std::fabs(insts[0].w[y * W + x] + insts[0].canvas[(y + pad) * g.DW + x + pad]));
+	// the likelihood epilogue (chi^2 with data 0 and unit weights: w = -model),
+	// generic (mode 0) and Gaussian-only (mode 3)
+	for (int mode : {0, 3}) {
+		std::memset(insts[0].w, 0, sizeof(float) * (size_t)W * H);
+		rows_fwd(0, row_blocks(g.DH));
+		cols(1, 0, 0, nullptr);
+		rows_inv(mode, row_blocks(H));
+		for (int y = 0; y < H; ++y)
+			for (int x = 0; x < W; ++x)
+				err = std::fmax(err, std::fabs(insts[0].w[y * W + x] + insts[0].canvas[(y + pad) * g.DW + x + pad]));
+	}
 	for (int i = 0; i < W * H; ++i) insts[0].w[i] = rnd();
 	rows_fwd(1, row_blocks(H));
 	cols(2, 1, 1, nullptr);
@@ -162,6 +168,46 @@ static void run_case(int W, int H, int K, int pad)
 			const float ref = in ? insts[0].w[(y - pad) * W + x - pad] : 0.f;
 			err = std::fmax(err, std::fabs(insts[0].G[y * g.DW + x] - ref));
 		}
+	// a three-tap PSF, one tap at a negative (wrapped) offset, so that the
+	// forward and inverse transforms are checked separately: model(x, y) =
+	// c(x, y) + 0.5 c(x - 1, y - 2) + 0.25 c(x + 3, y + 1), and the adjoint
+	// correlation G(x, y) = v(x, y) + 0.5 v(x + 1, y + 2) + 0.25 v(x - 3,
+	// y - 1) of w placed at (padx, pady)
+	if (n1 >= 8 && n2 >= 8) {
+		std::memset(src, 0, sizeof(double) * (size_t)n1 * n2);
+		src[0] = 1;
+		src[2 * n1 + 1] = 0.5;
+		src[(size_t)(n2 - 1) * n1 + (n1 - 3)] = 0.25;
+		set_lds(row_lds);
+		hip_cpu::launch((n2 / 2 + np - 1) / np, 256, [&] { k_fft_psf_rows(gals, tw, twn, src, spec); });
+		cols(0, 2, 0, spec);
+		for (int i = 0; i < g.DW * g.DH; ++i) insts[0].canvas[i] = rnd();
+		auto cv = [&](int x, int y) {
+			return x >= 0 && x < g.DW && y >= 0 && y < g.DH ? (double)insts[0].canvas[y * g.DW + x] : 0.0;
+		};
+		rows_fwd(0, row_blocks(g.DH));
+		cols(1, 0, 1, nullptr);
+		rows_inv(1, row_blocks(g.DH));
+		for (int y = 0; y < g.DH; ++y)
+			for (int x = 0; x < g.DW; ++x) {
+				const double ref = cv(x, y) + 0.5 * cv(x - 1, y - 2) + 0.25 * cv(x + 3, y + 1);
+				err = std::fmax(err, std::fabs(insts[0].G[y * g.DW + x] - ref));
+			}
+		for (int i = 0; i < W * H; ++i) insts[0].w[i] = rnd();
+		auto wv = [&](int x, int y) {
+			x -= pad;
+			y -= pad;
+			return x >= 0 && x < W && y >= 0 && y < H ? (double)insts[0].w[y * W + x] : 0.0;
+		};
+		rows_fwd(1, row_blocks(H));
+		cols(2, 1, 1, nullptr);
+		rows_inv(2, row_blocks(g.DH));
+		for (int y = 0; y < g.DH; ++y)
+			for (int x = 0; x < g.DW; ++x) {
+				const double ref = wv(x, y) + 0.5 * wv(x + 1, y + 2) + 0.25 * wv(x - 3, y - 1);
+				err = std::fmax(err, std::fabs(insts[0].G[y * g.DW + x] - ref));
+			}
+	}
 	std::printf("%s W=%d H=%d K=%d pad=%d frame=%dx%d max err %.3e\n", err < 1e-5 ? "ok" : "BAD", W, H, K, pad, n1,
 	            n2, err);
 }

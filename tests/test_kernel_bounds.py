@@ -2,7 +2,9 @@
 host stand-in of the HIP thread model (one thread per GPU thread, a barrier
 per block) under AddressSanitizer, with every device buffer at its exact size,
 for several geometries.  An access outside a buffer fails the test; with a
-unit-impulse PSF the three convolution modes must reproduce their input."""
+unit-impulse PSF the convolution modes must reproduce their input, and with a
+three-tap PSF (one tap at a wrapped negative offset) the model image and the
+adjoint correlation must equal their direct sums."""
 import os
 import shutil
 import subprocess
