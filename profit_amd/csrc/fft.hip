@@ -339,7 +339,8 @@ __device__ __forceinline__ void rows_fwd_pairs(double2 *sh, const double2 *wt, i
                                                long ld, int x0, int nx, int nrows, int pmax, double2 *out, int P)
 {
 	const int M = N / 2 + 1;
-	__syncthreads();  // the twiddle tables are staged
+	// the gather's radix-8 pass takes constant twiddles; its barrier orders
+	// the staged tables before the passes that read them
 	fft_dit_gather(sh, L, np, wt, [&](int p, int k) {
 		const int xi = k - x0;
 		const bool ok = p < pmax && xi >= 0 && xi < nx;
@@ -454,39 +455,74 @@ __global__ __launch_bounds__(kColThreads, 4) void k_fft_cols(const GalaxyDev *__
 	const int in_hi = in_mode == 0 ? g.DH : (in_mode == 1 ? g.pady + g.H : N);
 	// cb is a power of two: entry t of the tile is row t >> lcb, column t & (cb - 1)
 	const int lcb = __ffs(cb) - 1, total = cb << L;
-	for (int base = threadIdx.x; base < total; base += kBatch * blockDim.x) {
-		double2 v[kBatch];
+	// from 64 points on (and for the convolutions), stages 1-3 run on the
+	// frame's rows as they are loaded and stages 3-1 of the inverse before
+	// the rows are stored, as radix-8 groups: lane l takes column l mod cb,
+	// so that each row's cb columns are one coalesced run; group g combines
+	// rows rev3(j) * N/8 + bitrev(g)
+	const bool edge = op != 0 && L >= 6;
+	const int lg3 = L - 3;
+	if (edge) {
+		for (int gi = threadIdx.x; gi < (cb << lg3); gi += blockDim.x) {
+			const int c = gi & (cb - 1), gg = gi >> lcb;
+			const int bg = (int)bitrev(gg, lg3);
+			const bool okc = c0 + c < M;
+			double2 x[8];
 #pragma unroll
-		for (int u = 0; u < kBatch; ++u) {
-			const int t = base + u * blockDim.x;
-			const int y = t >> lcb, c = t & (cb - 1);
-			const bool ok = t < total && y >= in_lo && y < in_hi && c0 + c < M;
-			const double2 x = frame[ok ? (long)y * P + c0 + c : 0];
-			v[u] = ok ? x : make_double2(0.0, 0.0);
-		}
+			for (int j = 0; j < 8; ++j) {
+				const int y = (rev3(j) << lg3) + bg;
+				const bool ok = okc && y >= in_lo && y < in_hi;
+				const double2 v = frame[ok ? (long)y * P + c0 + c : 0];
+				x[j] = ok ? v : make_double2(0.0, 0.0);
+			}
+			dit8_first(x);
+			double2 *col = sh + c * S;
 #pragma unroll
-		for (int u = 0; u < kBatch; ++u) {
-			const int t = base + u * blockDim.x;
-			if (t < total) sh[(t & (cb - 1)) * S + swz(bitrev(t >> lcb, L))] = v[u];
+			for (int j = 0; j < 8; ++j) col[swz(8 * gg + j)] = x[j];
 		}
+		__syncthreads();
+		// stages 4 .. L-3 (the last three run around the product below)
+		int s = 3;
+		for (; s + 3 <= lg3; s += 3) dit_pass<3>(sh, L, cb, S, s, wt);
+		if (lg3 - s == 2) dit_pass<2>(sh, L, cb, S, s, wt);
+		else if (lg3 - s == 1) dit_pass<1>(sh, L, cb, S, s, wt);
 	}
-	__syncthreads();
-	if (op == 0) {
-		fft_dit(sh, L, cb, S, wt);
-		for (int t = threadIdx.x; t < total; t += blockDim.x) {
-			const int y = t >> lcb, c = t & (cb - 1);
-			if (c0 + c < M) frame[(long)y * P + c0 + c] = sh[c * S + swz(y)];
+	else {
+		for (int base = threadIdx.x; base < total; base += kBatch * blockDim.x) {
+			double2 v[kBatch];
+#pragma unroll
+			for (int u = 0; u < kBatch; ++u) {
+				const int t = base + u * blockDim.x;
+				const int y = t >> lcb, c = t & (cb - 1);
+				const bool ok = t < total && y >= in_lo && y < in_hi && c0 + c < M;
+				const double2 x = frame[ok ? (long)y * P + c0 + c : 0];
+				v[u] = ok ? x : make_double2(0.0, 0.0);
+			}
+#pragma unroll
+			for (int u = 0; u < kBatch; ++u) {
+				const int t = base + u * blockDim.x;
+				if (t < total) sh[(t & (cb - 1)) * S + swz(bitrev(t >> lcb, L))] = v[u];
+			}
 		}
-		return;
+		__syncthreads();
+		if (op == 0) {
+			fft_dit(sh, L, cb, S, wt);
+			for (int t = threadIdx.x; t < total; t += blockDim.x) {
+				const int y = t >> lcb, c = t & (cb - 1);
+				if (c0 + c < M) frame[(long)y * P + c0 + c] = sh[c * S + swz(y)];
+			}
+			return;
+		}
+		// forward DIT with the one or two odd stages first, then passes of
+		// three up to stage L-3
+		int s = L % 3;
+		if (s == 2) dit_pass<2>(sh, L, cb, S, 0, wt);
+		else if (s == 1) dit_pass<1>(sh, L, cb, S, 0, wt);
+		for (; s + 3 < L; s += 3) dit_pass<3>(sh, L, cb, S, s, wt);
 	}
-	// forward DIT with the one or two odd stages first, then passes of three;
-	// its last pass (stages L-2 .. L) and the first inverse pass (stages L ..
-	// L-2) combine the same eight elements, so they run in one step around the
-	// product with the PSF spectrum
-	int s = L % 3;
-	if (s == 2) dit_pass<2>(sh, L, cb, S, 0, wt);
-	else if (s == 1) dit_pass<1>(sh, L, cb, S, 0, wt);
-	for (; s + 3 < L; s += 3) dit_pass<3>(sh, L, cb, S, s, wt);
+	// the forward DIT's last three stages (L-2 .. L) and the inverse's first
+	// three (L .. L-2) combine the same eight elements, so they run in one
+	// step around the product with the PSF spectrum
 	const int lg = L - 3;
 	const double2 *spec = g.fft_spec;
 	for (int gi = threadIdx.x; gi < (cb << lg); gi += blockDim.x) {
@@ -511,13 +547,38 @@ __global__ __launch_bounds__(kColThreads, 4) void k_fft_cols(const GalaxyDev *__
 		for (int j = 0; j < 8; ++j) col[swz(lo + (j << lg))] = x[j];
 	}
 	__syncthreads();
+	// rows written back: 0 image rows [pady, pady + H); 1 canvas rows [0, DH)
+	const int out_lo = out_mode == 0 ? g.pady : 0;
+	const int out_hi = out_mode == 0 ? g.pady + g.H : g.DH;
+	if (edge) {
+		// inverse stages L-3 .. 4, then 3 .. 1 on the way out
+		int top = lg;
+		for (; top - 3 >= 3; top -= 3) dif_pass<3>(sh, L, cb, S, top, wt);
+		if (top - 3 == 2) dif_pass<2>(sh, L, cb, S, top, wt);
+		else if (top - 3 == 1) dif_pass<1>(sh, L, cb, S, top, wt);
+		for (int gi = threadIdx.x; gi < (cb << lg3); gi += blockDim.x) {
+			const int c = gi & (cb - 1), gg = gi >> lcb;
+			const int bg = (int)bitrev(gg, lg3);
+			const double2 *col = sh + c * S;
+			double2 x[8];
+#pragma unroll
+			for (int j = 0; j < 8; ++j) x[j] = col[swz(8 * gg + j)];
+			dif8_last(x);
+			// x[j] is output row rev3(j) * N/8 + bg
+			if (c0 + c < M) {
+#pragma unroll
+				for (int j = 0; j < 8; ++j) {
+					const int y = (rev3(j) << lg3) + bg;
+					if (y >= out_lo && y < out_hi) frame[(long)y * P + c0 + c] = x[j];
+				}
+			}
+		}
+		return;
+	}
 	int top = lg;
 	for (; top - 3 >= 0; top -= 3) dif_pass<3>(sh, L, cb, S, top, wt);
 	if (top == 2) dif_pass<2>(sh, L, cb, S, top, wt);
 	else if (top == 1) dif_pass<1>(sh, L, cb, S, top, wt);
-	// rows written back: 0 image rows [pady, pady + H); 1 canvas rows [0, DH)
-	const int out_lo = out_mode == 0 ? g.pady : 0;
-	const int out_hi = out_mode == 0 ? g.pady + g.H : g.DH;
 	for (int t = threadIdx.x; t < total; t += blockDim.x) {
 		const int y = t >> lcb, c = t & (cb - 1);
 		if (y >= out_lo && y < out_hi && c0 + c < M) frame[(long)y * P + c0 + c] = sh[c * S + swz(bitrev(y, L))];

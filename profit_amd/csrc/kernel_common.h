@@ -164,6 +164,8 @@ __device__ __forceinline__ void cell_pos64(const Comp &c, double nx0, double ny0
 template <int TYPE>
 __device__ __forceinline__ float pixel_accumulate_f32(const Comp &c, float xp, float yp, float wgt, float *gs)
 {
+	// FP32 derivatives only: products and sums may fuse into FMAs
+#pragma clang fp contract(fast)
 	float o[9];
 	sample_grad<float, TYPE>(c, xp, yp, o);
 	// the rotation's chain rule with the per-component sin/q, cos/q and 1/q
@@ -187,6 +189,7 @@ __device__ __forceinline__ float pixel_accumulate_f32(const Comp &c, float xp, f
 __device__ __forceinline__ void sersic_jac_f32(const Comp &c, float xp, float yp, float f, float base, float l2,
                                                float rf, float wgt, float *gs)
 {
+#pragma clang fp contract(fast)
 	if (!(base > 0.f)) return;  // the centre: every derivative is 0
 	const float bn = c.f.bn, in = c.f.inv_n;
 	const float bfr = bn * f * rf * in;
@@ -580,6 +583,9 @@ __global__ __launch_bounds__(256) void k_adj_render(const GalaxyDev *__restrict_
 			for (int s = 0; s < NGRAD; ++s) gf[s] = 0.f;
 			float vf = 0.f;
 			const float fs = (float)c.flux_scale;
+			// a Sersic without box (or box derivative): the derivatives from
+			// the value's own logarithm and exponentials
+			const bool fuse = TYPE == SERSIC && !c.boxy && !c.grad_box;
 			for (int ti = t0; ti < t1; ++ti) {
 				const int2 t = g.atiles[ti];
 				const int i = t.x * kTileW + lane;
@@ -598,7 +604,15 @@ __global__ __launch_bounds__(256) void k_adj_render(const GalaxyDev *__restrict_
 					const float xp = xr * c.f.cos_ang + yr * c.f.sin_ang;
 					const float yp = -xr * c.f.sin_over + yr * c.f.cos_over;
 					const float wq = Gq * fs;
-					vf += wq * pixel_accumulate_f32<TYPE>(c, xp, yp, wq, gf);
+					if (fuse) {
+						float base, l2, rf;
+						const float v = sersic_parts_f32(c, xp, yp, base, l2, rf);
+						sersic_jac_f32(c, xp, yp, v, base, l2, rf, wq, gf);
+						vf += wq * v;
+					}
+					else {
+						vf += wq * pixel_accumulate_f32<TYPE>(c, xp, yp, wq, gf);
+					}
 				}
 			}
 #pragma unroll

@@ -224,6 +224,11 @@ int main()
 	// frames of 2048 and 4096 columns: two pairs, one pair of rows per block
 	run_case(1100, 11, 5, 0);
 	run_case(2100, 5, 3, 1);
+	// column transforms of 64, 512 and 1024 points (radix-8 stages on the
+	// frame's loads and stores, with 0, 3 and 4 stages between them)
+	run_case(10, 40, 3, 0);
+	run_case(20, 300, 5, 2);
+	run_case(9, 700, 3, 0);
 	set_lds(profit_amd::kLdsMax);
 	return 0;
 }

@@ -38,4 +38,4 @@ def test_fft_kernels_stay_in_bounds(tmp_path):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=600,
                        env=dict(os.environ, ASAN_OPTIONS="detect_leaks=0"))
     assert r.returncode == 0, r.stderr[-4000:]
-    assert r.stdout.count("ok ") == 8, r.stdout
+    assert r.stdout.count("ok ") == 11, r.stdout
