@@ -116,8 +116,9 @@ struct Comp {
 	double inv_axrat;
 	// FP32 copies of the constants the FP32 evaluations read, so that they
 	// enter FP32 arithmetic as scalar operands without conversions; aux per
-	// family: King 1/rc, 1/a and C = (1 + (rt/rc)^2)^(-1/a); Moffat and Ferrer
-	// 1/rscale; core-Sersic log2 rb, rb^a, re^-a, 1/(n a) and log2 re
+	// family: King 1/rc, 1/a, C = (1 + (rt/rc)^2)^(-1/a), log2(1 + (rt/rc)^2)
+	// and C / (1 + (rt/rc)^2); Moffat and Ferrer 1/rscale; core-Sersic
+	// log2 rb, rb^a, re^-a, 1/(n a) and log2 re
 	struct F32 {
 		float cos_ang, sin_ang, cos_over, sin_over, axrat, inv_axrat;
 		float inv_re, inv_re2, inv_n, inv_nB, inv_2n, bn, dbn;
@@ -783,8 +784,8 @@ PA_HDI void sample_grad(const Comp &c, T x, T y, T *o)
 			o[0] = f;
 			const float iD = __builtin_amdgcn_rcpf(D);
 			const float Da1 = a * f * iD;  // a D^(a-1)
-			const float w = rt * irc, qw = 1.f + w * w;
-			const float Aq = A * __builtin_amdgcn_rcpf(q), Cq = C * __builtin_amdgcn_rcpf(qw);
+			const float w = rt * irc;
+			const float Aq = A * __builtin_amdgcn_rcpf(q), Cq = c.f.aux[4];
 			const float dfdr = Da1 * (-2.f * ia * irc * u) * Aq;
 			float drx, dry, drB;
 			dr_terms<float>(c, x, y, r, &drx, &dry, &drB);
@@ -793,7 +794,7 @@ PA_HDI void sample_grad(const Comp &c, T x, T y, T *o)
 			o[3] = dfdr * drB;
 			o[4] = Da1 * 2.f * ia * irc * (u * u * Aq - w * w * Cq);
 			o[5] = Da1 * 2.f * ia * irc * w * Cq;
-			o[6] = f * float(kLn2) * (lD + (A * lq - C * log2_fast(qw)) * ia * iD);
+			o[6] = f * float(kLn2) * (lD + (A * lq - C * c.f.aux[3]) * ia * iD);
 			return;
 		}
 		o[0] = king_core<T>(r, rc, rt, a);
@@ -1230,6 +1231,9 @@ PA_HD int setup_component(const ProfileParams &p, double magzero, double sx, dou
 		f.aux[0] = (float)(1 / s[0]);
 		f.aux[1] = (float)(1 / s[2]);
 		f.aux[2] = (float)::pow(1 + (s[1] / s[0]) * (s[1] / s[0]), -1 / s[2]);
+		// the truncation terms of the derivatives: log2(1 + (rt/rc)^2), C / (1 + (rt/rc)^2)
+		f.aux[3] = (float)::log2(1 + (s[1] / s[0]) * (s[1] / s[0]));
+		f.aux[4] = (float)(::pow(1 + (s[1] / s[0]) * (s[1] / s[0]), -1 / s[2]) / (1 + (s[1] / s[0]) * (s[1] / s[0])));
 	}
 	else if (p.type == MOFFAT || p.type == FERRER) {
 		f.aux[0] = (float)(1 / c->rscale);
