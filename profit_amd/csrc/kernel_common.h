@@ -348,25 +348,27 @@ __device__ __forceinline__ void walk_pixel(const Comp &c, WalkLevel *L, int lane
 // to the canvas (or the aux canvas, or the FP64 canvas) and, with kJac, writes
 // the entry's Jacobian and value to wq.ljac / hjac for k_adj_queue.
 // `counter`: the pair of hand-out counters of this launch (heavy, light).
-// Held to the registers of four waves per SIMD (three with kJac).
+// Held to the registers of four waves per SIMD (three for the Sersic, Moffat
+// and broken-exponential walks with kJac, whose code needs more).
 template <int TYPE, bool kJac>
-__global__ __launch_bounds__(256, kJac ? 3 : 4) void k_subsample(const GalaxyDev *__restrict__ gals,
+__global__ __launch_bounds__(256, kJac && (TYPE == SERSIC || TYPE == MOFFAT || TYPE == BROKENEXP) ? 3 : 4) void k_subsample(const GalaxyDev *__restrict__ gals,
                                                                  const InstDev *__restrict__ insts,
                                                                  const Comp *__restrict__ comps, WorkQueue wq,
                                                                  int counter, int split)
 {
 	__shared__ WalkLevel levels[4][kMaxLevels];
-	// the Jacobian's slots of a pixel: per wavefront, one row of 64 lane sums
-	// per slot (rows kJacPitch floats apart, so that the 16 lanes reading one
-	// quarter of each row fall on distinct banks)
+	// the Jacobian's slots of a pixel (with kJac) and its value: per
+	// wavefront, one row of 64 lane sums per slot (rows kJacPitch floats
+	// apart, so that the 16 lanes reading one quarter of each row fall on
+	// distinct banks)
 	constexpr int kSlots = G_S0 + n_shape(TYPE), kJacPitch = 68;
-	__shared__ __align__(16) float jred[kJac ? 4 : 1][kJac ? kSlots * kJacPitch : 1];
+	__shared__ __align__(16) float jred[4][(kJac ? kSlots + 1 : 1) * kJacPitch];
 	// the wavefront index is uniform: make that visible to the compiler so queue
 	// entries and component fields are read through the scalar cache
 	const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 	WalkLevel *L = levels[wv];
 	static_assert(TYPE >= 0 && TYPE < kFamilies, "one walker launch per profile family");
-	static_assert(kSlots < kJacStride && kSlots <= 16, "Jacobian slots");
+	static_assert(kSlots < kJacStride && kSlots + 1 <= 16, "Jacobian slots and the value");
 	const int lb = wq.lbase[TYPE];
 	const int nl = min(wq.count[light_count(TYPE)], wq.lbase[TYPE + 1] - lb), nh = min(wq.count[1], wq.hcap);
 	// per-lane Jacobian sums
@@ -430,26 +432,37 @@ __global__ __launch_bounds__(256, kJac ? 3 : 4) void k_subsample(const GalaxyDev
 			double s_local = 0;
 			walk_pixel<TYPE, kJac>(c, L, lane, x - hx, x + hx, y - hy, y + hy, res, maxrec, g.render_fp64 != 0, s_local,
 			                       gj);
-			const double tot = wave_sum(s_local);
 			const bool to_aux = split && !c.convolve;
-			if (lane == 0) {
-				if (g.canvas64 && !to_aux) atomicAdd(&I.canvas64[q.pix], c.flux_scale * tot);
-				else atomicAdd(&(to_aux ? I.aux : I.canvas)[q.pix], (float)(c.flux_scale * tot));
+			// on the FP32 canvas the value is summed through the wavefront's
+			// LDS rows (as one more row of the Jacobian's), the same way with
+			// and without the Jacobian, so that a pixel's value does not depend
+			// on whether gradients are asked for; the FP64 canvas sums it in
+			// FP64 across the wave
+			const bool fold = !g.canvas64;
+			double tot = 0.0;
+			if (!fold) {
+				tot = wave_sum(s_local);
+				if (lane == 0) {
+					if (!to_aux) atomicAdd(&I.canvas64[q.pix], c.flux_scale * tot);
+					else atomicAdd(&I.aux[q.pix], (float)(c.flux_scale * tot));
+				}
 			}
-			if (kJac) {
-				// the pixel's Jacobian: the lane sums go to the wavefront's
-				// rows; lane l then adds quarter l >> 4 of row l & 15 in FP64
-				// and two exchanges add the quarters
-				float *R = jred[kJac ? wv : 0];
+			if (kJac || fold) {
+				// the lane sums go to the wavefront's rows (the Jacobian's
+				// slots, then the value); lane l adds quarter l >> 4 of row
+				// l & 15 in FP64 and two exchanges add the quarters
+				constexpr int kRows = kJac ? kSlots : 0;  // the value's row
+				float *R = jred[wv];
 #pragma unroll
-				for (int s = 0; s < kSlots; ++s) {
+				for (int s = 0; s < kRows; ++s) {
 					R[s * kJacPitch + lane] = gj[s];
 					gj[s] = 0.f;
 				}
+				if (fold) R[kRows * kJacPitch + lane] = (float)s_local;
 				__builtin_amdgcn_wave_barrier();
 				const int slot = lane & 15, qt = lane >> 4;
 				double t = 0.0;
-				if (slot < kSlots) {
+				if (slot < kRows + (fold ? 1 : 0)) {
 					const float4 *row = reinterpret_cast<const float4 *>(R + slot * kJacPitch + qt * 16);
 #pragma unroll
 					for (int k = 0; k < 4; ++k) {
@@ -460,9 +473,16 @@ __global__ __launch_bounds__(256, kJac ? 3 : 4) void k_subsample(const GalaxyDev
 				__builtin_amdgcn_wave_barrier();
 				t += __shfl_xor(t, 16);
 				t += __shfl_xor(t, 32);
-				float *J = (heavy ? wq.hjac : wq.ljac + (long)lb * kJacStride) + (long)e * kJacStride;
-				if (lane < kSlots) J[lane] = (float)t;
-				if (lane == 0) J[kJacStride - 1] = (float)tot;
+				float *J = nullptr;
+				if (kJac) {
+					J = (heavy ? wq.hjac : wq.ljac + (long)lb * kJacStride) + (long)e * kJacStride;
+					if (lane < kSlots) J[lane] = (float)t;
+					if (!fold && lane == 0) J[kJacStride - 1] = (float)tot;
+				}
+				if (fold && lane == kRows) {
+					if (kJac) J[kJacStride - 1] = (float)t;
+					atomicAdd(&(to_aux ? I.aux : I.canvas)[q.pix], (float)(c.flux_scale * t));
+				}
 			}
 		}
 	}
