@@ -15,11 +15,11 @@
 //                   likelihood, masked model image, or the adjoint image G
 // Every transform runs in LDS: decimation in time from bit-reversed input for
 // the forward transforms, decimation in frequency to bit-reversed output for
-// the inverse ones, three radix-2 stages per pass in registers.  Two layouts
-// keep the LDS banks free of conflicts: element e of a sequence sits at
-// swz(e), an XOR of its low bits with the next ones, so the strided accesses
-// of every pass fall on distinct banks; and the twiddles are staged one table
-// per stage, in the order a stage reads them.  The row passes run one pair of
+// the inverse ones, three radix-2 stages per pass in registers.  Element e
+// of a sequence sits at swz(e) in LDS, an XOR of its low bits with the next
+// ones, so the strided accesses of every pass fall on distinct banks.  A
+// pass's twiddles come from one entry per thread of the transform's table
+// in LDS (Tw).  The row passes run one pair of
 // rows per wavefront, several pairs per block.  FP64 keeps the error relative
 // to the image maximum near 1e-15, so faint pixels keep their relative
 // accuracy.
@@ -75,37 +75,68 @@ constexpr int kColThreads = 512;
 
 __device__ __forceinline__ int swz(int e) { return e ^ ((e >> 3) & 7) ^ (((e >> 6) & 1) << 3); }
 
-// Twiddles of an n-point transform in LDS, one table per radix-2 stage S =
-// 1..log2(n): wt[2^(S-1) - 1 + k] = exp(-2 pi i k / 2^S), k < 2^(S-1) (n - 1
-// entries), taken from the global table of the largest transform (twn points).
-__device__ inline void stage_twiddles(double2 *wt, int n, int logn, const double2 *__restrict__ tw, int twn)
+// The twiddle table of an n = 2^lt point transform, tab[t] = exp(-2 pi i t /
+// n) for t < n/2, staged in LDS from the global table of the largest
+// transform: a radix-8 pass reads one entry per thread and derives the
+// other twiddles from it (csq, rot8).
+struct Tw {
+	const double2 *tab;
+	int lt;
+	// exp(-2 pi i k / 2^S), k < 2^(S-1)
+	__device__ __forceinline__ double2 at(int S, int k) const { return tab[k << (lt - S)]; }
+};
+
+// Stages the n/2 entries of an n = 2^logn point table (the caller's next
+// barrier orders them before their first use).
+__device__ __forceinline__ Tw stage_table(double2 *wl, int logn, const double2 *__restrict__ tw, int twn)
 {
-	const int step = twn / n;
-	for (int i = threadIdx.x; i < n - 1; i += blockDim.x) {
-		const int S = 32 - __clz(i + 1);  // floor(log2(i + 1)) + 1
-		const int k = i + 1 - (1 << (S - 1));
-		wt[i] = tw[(k << (logn - S)) * step];
+	const int sh = __ffs(twn) - 1 - logn;
+	for (int t = threadIdx.x; t < (1 << (logn - 1)); t += blockDim.x) wl[t] = tw[t << sh];
+	return Tw{wl, logn};
+}
+
+__device__ __forceinline__ double2 csq(double2 a) { return make_double2(fma(a.x, a.x, -(a.y * a.y)), 2.0 * a.x * a.y); }
+
+// a * exp(-2 pi i m / 8), m = 0..3 (a compile-time constant once unrolled)
+__device__ __forceinline__ double2 rot8(double2 a, int m)
+{
+	constexpr double c = 0.70710678118654752440;
+	switch (m) {
+	case 0: return a;
+	case 1: return make_double2(c * (a.x + a.y), c * (a.y - a.x));
+	case 2: return make_double2(a.y, -a.x);
+	default: return make_double2(c * (a.y - a.x), -c * (a.x + a.y));
 	}
 }
 
-__device__ __forceinline__ double2 twiddle(const double2 *wt, int S, int k) { return wt[(1 << (S - 1)) - 1 + k]; }
+// The twiddles of stages s+1 .. s+K at offset lo < 2^s: w[t-1] =
+// exp(-2 pi i lo / 2^(s+t)), from one table entry and K-1 squarings.  Stage
+// t's butterfly j (j mod 2^(t-1) = m) takes w[t-1] exp(-2 pi i m / 2^t).
+template <int K>
+__device__ __forceinline__ void pass_twiddles(const Tw &T, int s, int lo, double2 *w)
+{
+	w[K - 1] = T.at(s + K, lo);
+#pragma unroll
+	for (int t = K - 2; t >= 0; --t) w[t] = csq(w[t + 1]);
+}
 
 // Radix-2 stages s+1 .. s+K of a forward DIT on the 2^K elements a thread
 // holds in registers: x[j] is element e0 + (j << s), lo = e0 mod 2^s.  Stage
 // S combines i and i + 2^(S-1) with twiddle exp(-2 pi i (i mod 2^(S-1)) / 2^S).
 template <int K>
-__device__ __forceinline__ void dit_regs(double2 *x, int s, int lo, const double2 *wt)
+__device__ __forceinline__ void dit_regs(double2 *x, int s, int lo, const Tw &T)
 {
 	constexpr int R = 1 << K;
+	double2 w[K];
+	pass_twiddles<K>(T, s, lo, w);
 #pragma unroll
 	for (int t = 1; t <= K; ++t) {
 		const int h = 1 << (t - 1);
-		const int S = s + t;
 #pragma unroll
 		for (int j = 0; j < R; ++j) {
 			if (j & h) continue;
-			const int kS = ((j & (h - 1)) << s) + lo;  // index within the stage's half block
-			const double2 u = x[j], v = cmul(x[j + h], twiddle(wt, S, kS));
+			const double2 tw = rot8(w[t - 1], (j & (h - 1)) << (3 - t));
+			const double2 u = x[j], v = cmul(x[j + h], tw);
 			x[j] = cadd(u, v);
 			x[j + h] = csub(u, v);
 		}
@@ -181,20 +212,21 @@ __device__ __forceinline__ void dif8_last(double2 *x)
 // unscaled) in registers: stage S maps (u, v) at i and i + 2^(S-1) to
 // (u + v, (u - v) conj w).
 template <int K>
-__device__ __forceinline__ void dif_regs(double2 *x, int s, int lo, const double2 *wt)
+__device__ __forceinline__ void dif_regs(double2 *x, int s, int lo, const Tw &T)
 {
 	constexpr int R = 1 << K;
+	double2 w[K];
+	pass_twiddles<K>(T, s, lo, w);
 #pragma unroll
 	for (int t = K; t >= 1; --t) {
 		const int h = 1 << (t - 1);
-		const int S = s + t;
 #pragma unroll
 		for (int j = 0; j < R; ++j) {
 			if (j & h) continue;
-			const int kS = ((j & (h - 1)) << s) + lo;
+			const double2 tw = rot8(w[t - 1], (j & (h - 1)) << (3 - t));
 			const double2 u = x[j], v = x[j + h];
 			x[j] = cadd(u, v);
-			x[j + h] = cmul(csub(u, v), conjg(twiddle(wt, S, kS)));
+			x[j + h] = cmul(csub(u, v), conjg(tw));
 		}
 	}
 }
@@ -204,7 +236,7 @@ __device__ __forceinline__ void dif_regs(double2 *x, int s, int lo, const double
 // loads the 2^K elements a group combines, runs the stages in registers and
 // stores them back, so LDS is read and written once per pass.
 template <int K>
-__device__ __forceinline__ void dit_pass(double2 *data, int logn, int ncol, int stride, int s, const double2 *wt)
+__device__ __forceinline__ void dit_pass(double2 *data, int logn, int ncol, int stride, int s, const Tw &T)
 {
 	constexpr int R = 1 << K;
 	const int sub = 1 << s;                     // 2^s: element spacing inside a group
@@ -218,7 +250,7 @@ __device__ __forceinline__ void dit_pass(double2 *data, int logn, int ncol, int 
 #pragma unroll
 		for (int j = 0; j < R; ++j) x[j] = col[swz(e0 + (j << s))];
 		if (K == 3 && s == 0) dit8_first(x);
-		else dit_regs<K>(x, s, lo, wt);
+		else dit_regs<K>(x, s, lo, T);
 #pragma unroll
 		for (int j = 0; j < R; ++j) col[swz(e0 + (j << s))] = x[j];
 	}
@@ -227,11 +259,11 @@ __device__ __forceinline__ void dit_pass(double2 *data, int logn, int ncol, int 
 
 // In-place forward DIT, bit-reversed input, natural-order output: passes of
 // three stages from stage 1 up, the remaining one or two stages last.
-__device__ inline void fft_dit(double2 *data, int logn, int ncol, int stride, const double2 *wt, int s = 0)
+__device__ inline void fft_dit(double2 *data, int logn, int ncol, int stride, const Tw &T, int s = 0)
 {
-	for (; s + 3 <= logn; s += 3) dit_pass<3>(data, logn, ncol, stride, s, wt);
-	if (logn - s == 2) dit_pass<2>(data, logn, ncol, stride, s, wt);
-	else if (logn - s == 1) dit_pass<1>(data, logn, ncol, stride, s, wt);
+	for (; s + 3 <= logn; s += 3) dit_pass<3>(data, logn, ncol, stride, s, T);
+	if (logn - s == 2) dit_pass<2>(data, logn, ncol, stride, s, T);
+	else if (logn - s == 1) dit_pass<1>(data, logn, ncol, stride, s, T);
 }
 
 // Position j of a three-bit reversal.
@@ -244,7 +276,7 @@ __device__ __forceinline__ int rev3(int j) { return ((j & 1) << 2) | (j & 2) | (
 // rev3(j) * n/8 + bitrev(g), so no bit-reversed scatter goes through LDS.
 // Sequence c ends at data + (c << logn) in natural order.
 template <class F>
-__device__ __forceinline__ void fft_dit_gather(double2 *data, int logn, int nseq, const double2 *wt, F load)
+__device__ __forceinline__ void fft_dit_gather(double2 *data, int logn, int nseq, const Tw &T, F load)
 {
 	const int lg = logn - 3;
 	for (int gi = threadIdx.x; gi < (nseq << lg); gi += blockDim.x) {
@@ -259,13 +291,13 @@ __device__ __forceinline__ void fft_dit_gather(double2 *data, int logn, int nseq
 		for (int j = 0; j < 8; ++j) col[swz(8 * g + j)] = x[j];
 	}
 	__syncthreads();
-	fft_dit(data, logn, nseq, 1 << logn, wt, 3);
+	fft_dit(data, logn, nseq, 1 << logn, T, 3);
 }
 
 // One inverse DIF pass of K stages, top down to top-K+1 (indices differing in
 // bits top-1 .. top-K, spacing 2^(top-K)).
 template <int K>
-__device__ __forceinline__ void dif_pass(double2 *data, int logn, int ncol, int stride, int top, const double2 *wt)
+__device__ __forceinline__ void dif_pass(double2 *data, int logn, int ncol, int stride, int top, const Tw &T)
 {
 	constexpr int R = 1 << K;
 	const int s = top - K;
@@ -280,7 +312,7 @@ __device__ __forceinline__ void dif_pass(double2 *data, int logn, int ncol, int 
 #pragma unroll
 		for (int j = 0; j < R; ++j) x[j] = col[swz(e0 + (j << s))];
 		if (K == 3 && s == 0) dif8_last(x);
-		else dif_regs<K>(x, s, lo, wt);
+		else dif_regs<K>(x, s, lo, T);
 #pragma unroll
 		for (int j = 0; j < R; ++j) col[swz(e0 + (j << s))] = x[j];
 	}
@@ -288,12 +320,12 @@ __device__ __forceinline__ void dif_pass(double2 *data, int logn, int ncol, int 
 }
 
 // In-place inverse DIF, natural-order input, bit-reversed output.
-__device__ inline void ifft_dif(double2 *data, int logn, int ncol, int stride, const double2 *wt)
+__device__ inline void ifft_dif(double2 *data, int logn, int ncol, int stride, const Tw &T)
 {
 	int top = logn;
-	for (; top - 3 >= 0; top -= 3) dif_pass<3>(data, logn, ncol, stride, top, wt);
-	if (top == 2) dif_pass<2>(data, logn, ncol, stride, top, wt);
-	else if (top == 1) dif_pass<1>(data, logn, ncol, stride, top, wt);
+	for (; top - 3 >= 0; top -= 3) dif_pass<3>(data, logn, ncol, stride, top, T);
+	if (top == 2) dif_pass<2>(data, logn, ncol, stride, top, T);
+	else if (top == 1) dif_pass<1>(data, logn, ncol, stride, top, T);
 }
 
 // Inverse DIF of `nseq` sequences of length n = 2^logn >= 8 (sequence c at
@@ -301,19 +333,19 @@ __device__ inline void ifft_dif(double2 *data, int logn, int ncol, int stride, c
 // `emit(c, g, bg, x)` instead of LDS: x[j] is output rev3(j) * n/8 + bg,
 // bg = bitrev(g).  The remaining one or two stages run first.
 template <class F>
-__device__ __forceinline__ void ifft_dif_scatter(double2 *data, int logn, int nseq, const double2 *wt, F emit)
+__device__ __forceinline__ void ifft_dif_scatter(double2 *data, int logn, int nseq, const Tw &T, F emit)
 {
 	const int stride = 1 << logn;
 	int top = logn;
 	if (logn % 3 == 2) {
-		dif_pass<2>(data, logn, nseq, stride, top, wt);
+		dif_pass<2>(data, logn, nseq, stride, top, T);
 		top -= 2;
 	}
 	else if (logn % 3 == 1) {
-		dif_pass<1>(data, logn, nseq, stride, top, wt);
+		dif_pass<1>(data, logn, nseq, stride, top, T);
 		top -= 1;
 	}
-	for (; top > 3; top -= 3) dif_pass<3>(data, logn, nseq, stride, top, wt);
+	for (; top > 3; top -= 3) dif_pass<3>(data, logn, nseq, stride, top, T);
 	const int lg = logn - 3;
 	for (int gi = threadIdx.x; gi < (nseq << lg); gi += blockDim.x) {
 		const int c = gi >> lg, g = gi - (c << lg);
@@ -334,14 +366,12 @@ __device__ __forceinline__ void ifft_dif_scatter(double2 *data, int logn, int ns
 // half-spectrum rows 2p, 2p + 1 of `out` (pitch P):
 // A = (Z[k] + conj Z[N-k]) / 2 and B = (Z[k] - conj Z[N-k]) / 2i, k = 0..N/2,
 // for z = a + i b.  Sequence p sits at sh + p * N.
-template <class T>
-__device__ __forceinline__ void rows_fwd_pairs(double2 *sh, const double2 *wt, int N, int L, int np, const T *src,
+template <class Src>
+__device__ __forceinline__ void rows_fwd_pairs(double2 *sh, const Tw &T, int N, int L, int np, const Src *src,
                                                long ld, int x0, int nx, int nrows, int pmax, double2 *out, int P)
 {
 	const int M = N / 2 + 1;
-	// the gather's radix-8 pass takes constant twiddles; its barrier orders
-	// the staged tables before the passes that read them
-	fft_dit_gather(sh, L, np, wt, [&](int p, int k) {
+	fft_dit_gather(sh, L, np, T, [&](int p, int k) {
 		const int xi = k - x0;
 		const bool ok = p < pmax && xi >= 0 && xi < nx;
 		const bool okb = ok && 2 * p + 1 < nrows;
@@ -375,8 +405,7 @@ __global__ __launch_bounds__(256) void k_fft_rows_fwd(const GalaxyDev *__restric
 	const InstDev &I = insts[inst];
 	const GalaxyDev &g = gals[I.galaxy];
 	const int N = g.fft_n1, L = g.fft_l1, P = fft_pitch(N), np = fft_pairs_per_block(N);
-	double2 *wt = sh + np * N;
-	stage_twiddles(wt, N, L, tw, twn);
+	const Tw T = stage_table(sh + np * N, L, tw, twn);
 	const int p0 = (b - row_off[inst]) * np;  // first pair of the block
 	// straight-line selects of the mode's geometry
 	const bool adj = mode == 1;
@@ -387,7 +416,7 @@ __global__ __launch_bounds__(256) void k_fft_rows_fwd(const GalaxyDev *__restric
 	const int r0 = 2 * p0;
 	const float *src = (adj ? I.w : I.canvas) + (long)r0 * width;
 	const int pmax = (nrows - r0 + 1) / 2;
-	rows_fwd_pairs<float>(sh, wt, N, L, np, src, width, x0, width, nrows - r0, pmax, I.fbuf + (long)(y0 + r0) * P, P);
+	rows_fwd_pairs<float>(sh, T, N, L, np, src, width, x0, width, nrows - r0, pmax, I.fbuf + (long)(y0 + r0) * P, P);
 }
 
 // Forward rows of the FP64 canvas (mode 0 of k_fft_rows_fwd for galaxies with
@@ -404,12 +433,11 @@ __global__ __launch_bounds__(256) void k_fft_rows_fwd64(const GalaxyDev *__restr
 	const InstDev &I = insts[inst];
 	const GalaxyDev &g = gals[I.galaxy];
 	const int N = g.fft_n1, L = g.fft_l1, P = fft_pitch(N), np = fft_pairs_per_block(N);
-	double2 *wt = sh + np * N;
-	stage_twiddles(wt, N, L, tw, twn);
+	const Tw T = stage_table(sh + np * N, L, tw, twn);
 	const int r0 = 2 * (b - row_off[inst]) * np;
 	const double *src = I.canvas64 + (long)r0 * g.DW;
 	const int pmax = (g.DH - r0 + 1) / 2;
-	rows_fwd_pairs<double>(sh, wt, N, L, np, src, g.DW, 0, g.DW, g.DH - r0, pmax, I.fbuf + (long)r0 * P, P);
+	rows_fwd_pairs<double>(sh, T, N, L, np, src, g.DW, 0, g.DW, g.DH - r0, pmax, I.fbuf + (long)r0 * P, P);
 }
 
 // Forward rows of a galaxy's real FP64 PSF frame (N1 x N2, `src64`) into the
@@ -419,11 +447,10 @@ __global__ __launch_bounds__(256) void k_fft_psf_rows(const GalaxyDev *__restric
 {
 	extern __shared__ double2 sh[];
 	const int N = g->fft_n1, L = g->fft_l1, P = fft_pitch(N), np = fft_pairs_per_block(N);
-	double2 *wt = sh + np * N;
-	stage_twiddles(wt, N, L, tw, twn);
+	const Tw T = stage_table(sh + np * N, L, tw, twn);
 	const int r0 = 2 * np * blockIdx.x;
 	const int nrows = g->fft_n2 - r0;
-	rows_fwd_pairs<double>(sh, wt, N, L, np, src64 + (long)r0 * N, N, 0, N, nrows, (nrows + 1) / 2,
+	rows_fwd_pairs<double>(sh, T, N, L, np, src64 + (long)r0 * N, N, 0, N, nrows, (nrows + 1) / 2,
 	                       dst64 + (long)r0 * P, P);
 }
 
@@ -446,8 +473,7 @@ __global__ __launch_bounds__(kColThreads, 4) void k_fft_cols(const GalaxyDev *__
 	// column stride: 2 mod 16 entries keeps the columns-fastest pass on
 	// distinct LDS banks
 	const int S = N + 2;
-	double2 *wt = sh + cb * S;
-	stage_twiddles(wt, N, L, tw, twn);
+	const Tw T = stage_table(sh + cb * S, L, tw, twn);
 	const int c0 = blk.y * cb;
 	double2 *frame = (op == 0) ? dst64 : I.fbuf;
 	// rows holding data: 0 canvas rows [0, DH); 1 image rows [pady, pady + H); 2 all rows
@@ -483,9 +509,9 @@ __global__ __launch_bounds__(kColThreads, 4) void k_fft_cols(const GalaxyDev *__
 		__syncthreads();
 		// stages 4 .. L-3 (the last three run around the product below)
 		int s = 3;
-		for (; s + 3 <= lg3; s += 3) dit_pass<3>(sh, L, cb, S, s, wt);
-		if (lg3 - s == 2) dit_pass<2>(sh, L, cb, S, s, wt);
-		else if (lg3 - s == 1) dit_pass<1>(sh, L, cb, S, s, wt);
+		for (; s + 3 <= lg3; s += 3) dit_pass<3>(sh, L, cb, S, s, T);
+		if (lg3 - s == 2) dit_pass<2>(sh, L, cb, S, s, T);
+		else if (lg3 - s == 1) dit_pass<1>(sh, L, cb, S, s, T);
 	}
 	else {
 		for (int base = threadIdx.x; base < total; base += kBatch * blockDim.x) {
@@ -506,7 +532,7 @@ __global__ __launch_bounds__(kColThreads, 4) void k_fft_cols(const GalaxyDev *__
 		}
 		__syncthreads();
 		if (op == 0) {
-			fft_dit(sh, L, cb, S, wt);
+			fft_dit(sh, L, cb, S, T);
 			for (int t = threadIdx.x; t < total; t += blockDim.x) {
 				const int y = t >> lcb, c = t & (cb - 1);
 				if (c0 + c < M) frame[(long)y * P + c0 + c] = sh[c * S + swz(y)];
@@ -516,9 +542,9 @@ __global__ __launch_bounds__(kColThreads, 4) void k_fft_cols(const GalaxyDev *__
 		// forward DIT with the one or two odd stages first, then passes of
 		// three up to stage L-3
 		int s = L % 3;
-		if (s == 2) dit_pass<2>(sh, L, cb, S, 0, wt);
-		else if (s == 1) dit_pass<1>(sh, L, cb, S, 0, wt);
-		for (; s + 3 < L; s += 3) dit_pass<3>(sh, L, cb, S, s, wt);
+		if (s == 2) dit_pass<2>(sh, L, cb, S, 0, T);
+		else if (s == 1) dit_pass<1>(sh, L, cb, S, 0, T);
+		for (; s + 3 < L; s += 3) dit_pass<3>(sh, L, cb, S, s, T);
 	}
 	// the forward DIT's last three stages (L-2 .. L) and the inverse's first
 	// three (L .. L-2) combine the same eight elements, so they run in one
@@ -533,7 +559,7 @@ __global__ __launch_bounds__(kColThreads, 4) void k_fft_cols(const GalaxyDev *__
 		double2 x[8], sp[8];
 #pragma unroll
 		for (int j = 0; j < 8; ++j) x[j] = col[swz(lo + (j << lg))];
-		dit_regs<3>(x, lg, lo, wt);
+		dit_regs<3>(x, lg, lo, T);
 		// spectrum entries four at a time (the registers of four waves per SIMD)
 #pragma unroll
 		for (int h = 0; h < 8; h += 4) {
@@ -542,7 +568,7 @@ __global__ __launch_bounds__(kColThreads, 4) void k_fft_cols(const GalaxyDev *__
 #pragma unroll
 			for (int j = h; j < h + 4; ++j) x[j] = (op == 1) ? cmul(x[j], sp[j]) : cmulc(x[j], sp[j]);
 		}
-		dif_regs<3>(x, lg, lo, wt);
+		dif_regs<3>(x, lg, lo, T);
 #pragma unroll
 		for (int j = 0; j < 8; ++j) col[swz(lo + (j << lg))] = x[j];
 	}
@@ -553,9 +579,9 @@ __global__ __launch_bounds__(kColThreads, 4) void k_fft_cols(const GalaxyDev *__
 	if (edge) {
 		// inverse stages L-3 .. 4, then 3 .. 1 on the way out
 		int top = lg;
-		for (; top - 3 >= 3; top -= 3) dif_pass<3>(sh, L, cb, S, top, wt);
-		if (top - 3 == 2) dif_pass<2>(sh, L, cb, S, top, wt);
-		else if (top - 3 == 1) dif_pass<1>(sh, L, cb, S, top, wt);
+		for (; top - 3 >= 3; top -= 3) dif_pass<3>(sh, L, cb, S, top, T);
+		if (top - 3 == 2) dif_pass<2>(sh, L, cb, S, top, T);
+		else if (top - 3 == 1) dif_pass<1>(sh, L, cb, S, top, T);
 		for (int gi = threadIdx.x; gi < (cb << lg3); gi += blockDim.x) {
 			const int c = gi & (cb - 1), gg = gi >> lcb;
 			const int bg = (int)bitrev(gg, lg3);
@@ -576,9 +602,9 @@ __global__ __launch_bounds__(kColThreads, 4) void k_fft_cols(const GalaxyDev *__
 		return;
 	}
 	int top = lg;
-	for (; top - 3 >= 0; top -= 3) dif_pass<3>(sh, L, cb, S, top, wt);
-	if (top == 2) dif_pass<2>(sh, L, cb, S, top, wt);
-	else if (top == 1) dif_pass<1>(sh, L, cb, S, top, wt);
+	for (; top - 3 >= 0; top -= 3) dif_pass<3>(sh, L, cb, S, top, T);
+	if (top == 2) dif_pass<2>(sh, L, cb, S, top, T);
+	else if (top == 1) dif_pass<1>(sh, L, cb, S, top, T);
 	for (int t = threadIdx.x; t < total; t += blockDim.x) {
 		const int y = t >> lcb, c = t & (cb - 1);
 		if (y >= out_lo && y < out_hi && c0 + c < M) frame[(long)y * P + c0 + c] = sh[c * S + swz(bitrev(y, L))];
@@ -609,8 +635,7 @@ __global__ __launch_bounds__(256) void k_fft_rows_inv(const GalaxyDev *__restric
 	const InstDev &I = insts[inst];
 	const GalaxyDev &g = gals[I.galaxy];
 	const int N = g.fft_n1, L = g.fft_l1, M = N / 2 + 1, P = fft_pitch(N), np = fft_pairs_per_block(N);
-	double2 *wt = sh + np * N;
-	stage_twiddles(wt, N, L, tw, twn);
+	const Tw T = stage_table(sh + np * N, L, tw, twn);
 	const int blk = bi.y;
 	const int r0 = 2 * np * blk;  // first output row of the block (image rows in mode 0)
 	const int nrows = mode == 0 ? g.H : g.DH;
@@ -651,7 +676,7 @@ __global__ __launch_bounds__(256) void k_fft_rows_inv(const GalaxyDev *__restric
 		const double s = g.pois_scale;
 		// x[j] is frame column rev3(j) * N/8 + bg of rows 2p (real part) and
 		// 2p + 1 (imaginary part); image column = frame column - padx
-		ifft_dif_scatter(sh, L, np, wt, [&](int p, int, int bg, const double2 *x) {
+		ifft_dif_scatter(sh, L, np, T, [&](int p, int, int bg, const double2 *x) {
 #pragma unroll
 			for (int half = 0; half < 4; ++half) {
 				const int row = half >> 1, j0 = (half & 1) * 4;
@@ -706,7 +731,7 @@ __global__ __launch_bounds__(256) void k_fft_rows_inv(const GalaxyDev *__restric
 		}
 		return;
 	}
-	ifft_dif_scatter(sh, L, np, wt, [&](int p, int, int bg, const double2 *x) {
+	ifft_dif_scatter(sh, L, np, T, [&](int p, int, int bg, const double2 *x) {
 #pragma unroll
 		for (int row = 0; row < 2; ++row) {
 			const int r = r0 + 2 * p + row;
@@ -745,9 +770,9 @@ static size_t lds_rows_fwd[64], lds_rows_fwd64[64], lds_psf_rows[64], lds_cols[6
 
 static int cols_per_block(int n2) { return n2 >= 4096 ? 1 : (n2 >= 2048 ? 2 : (4096 / n2 > 8 ? 8 : 4096 / n2)); }
 
-// LDS of a row pass: the block's pairs and the per-stage twiddles (their
-// total grows with N, so the largest frame sets it)
-static size_t row_lds(int nmax) { return sizeof(double2) * (size_t)(fft_pairs_per_block(nmax) + 1) * nmax; }
+// LDS of a row pass: the block's pairs and the twiddle table (the largest
+// frame sets it)
+static size_t row_lds(int nmax) { return sizeof(double2) * ((size_t)fft_pairs_per_block(nmax) * nmax + nmax / 2); }
 
 int launch_fft_rows_fwd(int nblocks, int nmax, const GalaxyDev *gals, const InstDev *insts, const int *row_off,
                         int ninst, int mode, const double2 *tw, int twn, hipStream_t st)
@@ -787,7 +812,7 @@ int launch_fft_cols(int nblocks, int n2max, int cb, const GalaxyDev *gals, const
                     int op, int in_mode, int out_mode, const double2 *tw, int twn, double2 *dst64, hipStream_t st)
 {
 	if (nblocks <= 0) return 0;
-	const size_t lds = sizeof(double2) * ((size_t)cb * (n2max + 2) + n2max);
+	const size_t lds = sizeof(double2) * ((size_t)cb * (n2max + 2) + n2max / 2);
 	if (lds > 160 * 1024) return (int)hipErrorInvalidValue;
 	if (int e = ensure_lds((const void *)k_fft_cols, lds, lds_cols)) return e;
 	k_fft_cols<<<nblocks, kColThreads, lds, st>>>(gals, insts, blocks, cb, op, in_mode, out_mode, tw, twn, dst64);

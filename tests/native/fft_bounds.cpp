@@ -99,7 +99,7 @@ static void run_case(int W, int H, int K, int pad)
 	int *off = dev<int>(2);
 	const int np = fft_pairs_per_block(n1), rpb = 2 * np;
 	auto row_blocks = [rpb](int rows) { return (rows + rpb - 1) / rpb; };
-	const size_t row_lds = (size_t)(np + 1) * n1, col_lds = (size_t)cb * (n2 + 2) + n2;
+	const size_t row_lds = (size_t)np * n1 + n1 / 2, col_lds = (size_t)cb * (n2 + 2) + n2 / 2;
 	int *lk = dev<int>(2);
 	double *partial = dev<double>(row_blocks(H));
 	lk[1] = row_blocks(H);
