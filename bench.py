@@ -326,8 +326,9 @@ def main():
     ap.add_argument("--no-other-configs", action="store_true",
                     help="skip the likelihood-only / per-call figures of configurations 1, 2 and 5")
     ap.add_argument("--tile-rows", type=int, default=8)
-    ap.add_argument("--render-fp64", action="store_true",
-                    help="profile values (direct pixels, recursion leaves) in FP64 instead of FP32")
+    ap.add_argument("--render-fp64", type=int, nargs="?", const=1, default=0, choices=(0, 1, 2),
+                    help="model precision: 1 (the flag alone) profile values (direct pixels, recursion leaves) in "
+                         "FP64 instead of FP32; 2 also an FP64 canvas and the FP64 FFT for every PSF")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy 256 MiB device-to-device three times with the library's copy kernel (the "
                          "reference kernel for scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -462,7 +463,7 @@ def main():
             e.update(extra)
         kernels[key] = e
 
-    add_kernel("render", "k_render<0>", stage_ms["render"], "hbm", render_bytes, PEAK_HBM_GBS, "GB/s",
+    add_kernel("render", "k_render<0, false>", stage_ms["render"], "hbm", render_bytes, PEAK_HBM_GBS, "GB/s",
                {"profile_evals_per_s": round(B * direct / (stage_ms["render"] * 1e-3), 0),
                 "direct_pixels_per_instance": direct})
     # the forward walk of a gradient evaluation: cell values, refinement tests
@@ -479,7 +480,7 @@ def main():
                                       "note": "work: 60 B per queued pixel (48 B Jacobian and value, 8 B entry, 4 B G)"})
     if conv_used == "fft":
         flops, stage_bytes = fft_work(dw, dh, 400, 400, n1, n2)
-        names = {"like_rows_fwd": "k_fft_rows_fwd", "like_cols": "k_fft_cols", "like_rows_inv": "k_fft_rows_inv<0>",
+        names = {"like_rows_fwd": "k_fft_rows_fwd", "like_cols": "k_fft_cols", "like_rows_inv": "k_fft_rows_inv<3>",
                  "adj_rows_fwd": "k_fft_rows_fwd", "adj_cols": "k_fft_cols", "adj_rows_inv": "k_fft_rows_inv<2>"}
         for key, kname in names.items():
             ms = fft_kernel_ms[key]
@@ -519,7 +520,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": (("f64 (FFT convolution" if conv_used == "fft" else "f32 (convolution") +
-                  (", profile values" if args.render_fp64 else "") + ", cell positions, likelihood sums) + f32 (" +
+                  (", profile values" if args.render_fp64 else "") + (", canvas" if args.render_fp64 == 2 else "") +
+                  ", cell positions, likelihood sums) + f32 (" +
                   ("" if args.render_fp64 else "profile values, ") + "refinement tests, adjoint derivatives)"),
         "data": "synthetic (noiseless model from this GPU path plus seeded SplitMix64/Box-Muller noise)",
         "config": {
@@ -532,6 +534,7 @@ def main():
             "drawing_canvas": f"{dw}x{dh}",
             "tile_rows": args.tile_rows,
             "profile_values": "fp64" if args.render_fp64 else "fp32",
+            "canvas": "fp64" if args.render_fp64 == 2 else "fp32",
             "convolution": conv_used,
             "autotune_ms": tune["ms"] if tune else None,
         },
