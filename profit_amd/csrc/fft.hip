@@ -245,14 +245,16 @@ __device__ __forceinline__ void dit_pass(double2 *data, int logn, int ncol, int 
 		const int c = gi >> (logn - K), g = gi - (c << (logn - K));
 		const int lo = g & (sub - 1), hi = g >> s;
 		double2 *col = data + c * stride;
-		const int e0 = (hi << (s + K)) + lo;
+		// swz is linear over XOR and e0, j << s occupy disjoint bits: the
+		// addresses are swz(e0) ^ swz(j << s), the second term uniform
+		const int b0 = swz((hi << (s + K)) + lo);
 		double2 x[R];
 #pragma unroll
-		for (int j = 0; j < R; ++j) x[j] = col[swz(e0 + (j << s))];
+		for (int j = 0; j < R; ++j) x[j] = col[b0 ^ swz(j << s)];
 		if (K == 3 && s == 0) dit8_first(x);
 		else dit_regs<K>(x, s, lo, T);
 #pragma unroll
-		for (int j = 0; j < R; ++j) col[swz(e0 + (j << s))] = x[j];
+		for (int j = 0; j < R; ++j) col[b0 ^ swz(j << s)] = x[j];
 	}
 	__syncthreads();
 }
@@ -307,14 +309,14 @@ __device__ __forceinline__ void dif_pass(double2 *data, int logn, int ncol, int 
 		const int c = gi >> (logn - K), g = gi - (c << (logn - K));
 		const int lo = g & (sub - 1), hi = g >> s;
 		double2 *col = data + c * stride;
-		const int e0 = (hi << top) + lo;
+		const int b0 = swz((hi << top) + lo);  // as in dit_pass
 		double2 x[R];
 #pragma unroll
-		for (int j = 0; j < R; ++j) x[j] = col[swz(e0 + (j << s))];
+		for (int j = 0; j < R; ++j) x[j] = col[b0 ^ swz(j << s)];
 		if (K == 3 && s == 0) dif8_last(x);
 		else dif_regs<K>(x, s, lo, T);
 #pragma unroll
-		for (int j = 0; j < R; ++j) col[swz(e0 + (j << s))] = x[j];
+		for (int j = 0; j < R; ++j) col[b0 ^ swz(j << s)] = x[j];
 	}
 	__syncthreads();
 }
@@ -558,7 +560,7 @@ __global__ __launch_bounds__(kColThreads, 4) void k_fft_cols(const GalaxyDev *__
 		double2 *col = sh + c * S;
 		double2 x[8], sp[8];
 #pragma unroll
-		for (int j = 0; j < 8; ++j) x[j] = col[swz(lo + (j << lg))];
+		for (int j = 0; j < 8; ++j) x[j] = col[swz(lo) ^ swz(j << lg)];  // lo < 2^lg: disjoint bits
 		dit_regs<3>(x, lg, lo, T);
 		// spectrum entries four at a time (the registers of four waves per SIMD)
 #pragma unroll
@@ -570,7 +572,7 @@ __global__ __launch_bounds__(kColThreads, 4) void k_fft_cols(const GalaxyDev *__
 		}
 		dif_regs<3>(x, lg, lo, T);
 #pragma unroll
-		for (int j = 0; j < 8; ++j) col[swz(lo + (j << lg))] = x[j];
+		for (int j = 0; j < 8; ++j) col[swz(lo) ^ swz(j << lg)] = x[j];
 	}
 	__syncthreads();
 	// rows written back: 0 image rows [pady, pady + H); 1 canvas rows [0, DH)
