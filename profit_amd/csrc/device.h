@@ -123,15 +123,21 @@ struct WorkQueue {
 	// d slot (slot order of GradSlot) and, last, its value sum_s w_s f_s,
 	// written by the forward walk of a gradient evaluation (null otherwise)
 	float *ljac, *hjac;
+	// the light entries whose root node has a cell to refine (indices into
+	// the family's light list, in the same segments): the root pass, one lane
+	// per entry, hands these to the wavefront walker
+	int *deep;
 };
 constexpr int kJacStride = 12;
-// count[1]: length of the heavy list; then a (heavy, light) pair of hand-out
-// counters per walker launch (one per profile family); then the length of
-// each family's light list (count[0] is not used)
-constexpr int kQueueCounters = 20;
+// count[1]: length of the heavy list; then a (heavy, deep) pair of hand-out
+// counters per walker launch (one per profile family); the length of each
+// family's light list; the root pass's hand-out counter per family; the
+// length of each family's deep list (count[0] is not used)
+constexpr int kQueueCounters = 32;
 __host__ __device__ constexpr int fwd_counter(int type) { return 2 + 2 * type; }
 __host__ __device__ constexpr int light_count(int type) { return 14 + type; }
-constexpr int kGrab = 16;  // ordinary entries handed to a wavefront at a time
+__host__ __device__ constexpr int root_counter(int type) { return 20 + type; }
+__host__ __device__ constexpr int deep_count(int type) { return 26 + type; }
 
 
 }  // namespace profit_amd

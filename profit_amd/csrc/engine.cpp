@@ -724,6 +724,7 @@ void Engine::ensure_queue(const long *need, long hneed)
 		if (total > INT32_MAX) throw std::runtime_error("sub-pixel work lists beyond 2^31 entries");
 		d_queue_.alloc((size_t)total * sizeof(QEntry));
 		d_qjac_.alloc((size_t)total * kJacStride * sizeof(float));
+		d_qdeep_.alloc((size_t)total * sizeof(int));
 	}
 	if (hneed > hcap_) {
 		hcap_ = hneed;
@@ -736,7 +737,7 @@ void Engine::ensure_queue(const long *need, long hneed)
 WorkQueue Engine::work_queue()
 {
 	WorkQueue q{d_queue_.as<QEntry>(), d_hqueue_.as<QEntry>(), d_qcount_.as<int>(), (int)hcap_, {},
-	            d_qjac_.as<float>(), d_hjac_.as<float>()};
+	            d_qjac_.as<float>(), d_hjac_.as<float>(), d_qdeep_.as<int>()};
 	q.lbase[0] = 0;
 	for (int f = 0; f < kFamilies; ++f) q.lbase[f + 1] = q.lbase[f] + (int)lcap_[f];
 	return q;

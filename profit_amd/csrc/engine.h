@@ -321,7 +321,7 @@ class Engine {
 	std::vector<DevBuf> inst_bufs_;
 	DevBuf d_params_, d_comps_, d_err_, d_queue_, d_qcount_, d_partial_, d_ll_, d_gscale_, d_grad_acc_, d_grad_out_;
 	long lcap_[kFamilies] = {}, hcap_ = 0;
-	DevBuf d_hqueue_, d_qjac_, d_hjac_;
+	DevBuf d_hqueue_, d_qjac_, d_hjac_, d_qdeep_;
 	HostBuf h_params_, h_ll_, h_grad_, h_err_, h_qcount_;
 
 	hipGraphExec_t graph_exec_[2] = {nullptr, nullptr};

@@ -15,8 +15,17 @@
 
 namespace profit_amd {
 
+// The instance whose block range [off[i], off[i+1]) holds block b (off: the
+// n + 1 prefix sums of the instances' block counts).
 __device__ inline int owner_of(const int *__restrict__ off, int n, int b)
 {
+	// a first guess from the mean count per instance (exact, up to the
+	// rounding that the neighbour checks absorb, when every instance has the
+	// same count), then the binary search if it misses
+	int gs = min(n - 1, (int)((float)b * ((float)n / (float)off[n])));
+	if (gs > 0 && off[gs] > b) --gs;
+	else if (gs + 1 < n && off[gs + 1] <= b) ++gs;
+	if (off[gs] <= b && off[gs + 1] > b) return gs;
 	int lo = 0, hi = n - 1;
 	while (lo < hi) {
 		const int mid = (lo + hi + 1) >> 1;
@@ -32,6 +41,15 @@ __device__ inline double wave_sum(double v)
 	for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
 	return v;
 }
+
+// 1 / res^2 and 1 / res for the recursion's resolutions 1 .. 16 (setup
+// rejects higher ones), correctly rounded like the divisions they replace
+static __constant__ double kInvCells[17] = {0.0,        1.0 / 1,   1.0 / 4,   1.0 / 9,   1.0 / 16,  1.0 / 25,
+                                            1.0 / 36,   1.0 / 49,  1.0 / 64,  1.0 / 81,  1.0 / 100, 1.0 / 121,
+                                            1.0 / 144,  1.0 / 169, 1.0 / 196, 1.0 / 225, 1.0 / 256};
+static __constant__ float kInvRes[17] = {0.f,       1.f / 1,  1.f / 2,  1.f / 3,  1.f / 4,  1.f / 5,
+                                         1.f / 6,   1.f / 7,  1.f / 8,  1.f / 9,  1.f / 10, 1.f / 11,
+                                         1.f / 12,  1.f / 13, 1.f / 14, 1.f / 15, 1.f / 16};
 
 // sersic.cpp:331-347: pixels within one image pixel of the centre of an
 // nser > 1 Sersic profile are sub-sampled at resolution 8, depth 10
@@ -205,6 +223,88 @@ __device__ __forceinline__ void sersic_jac_f32(const Comp &c, float xp, float yp
 	gs[G_S1] += wgt * (f * (-c.f.dbn * (rf - 1.f) + bn * rf * (l2 * float(kLn2)) * in * c.f.inv_2n));
 }
 
+// A node of the recursion: its FP64 bounds (libprofit's), the resolution,
+// and the FP32 cell size, test offset and lower corner relative to the
+// component centre that the FP32 cell positions use.
+struct Node {
+	double x0, x1, y0, y1;
+	int res;
+	float inv_res, xbin, ybin, dyp, ox, oy;
+};
+
+__device__ __forceinline__ Node make_node(const Comp &c, double x0, double x1, double y0, double y1, int res)
+{
+	Node n;
+	n.x0 = x0;
+	n.x1 = x1;
+	n.y0 = y0;
+	n.y1 = y1;
+	n.res = res;
+	n.inv_res = kInvRes[res];
+	n.xbin = (float)(x1 - x0) * n.inv_res;
+	n.ybin = (float)(y1 - y0) * n.inv_res;
+	n.dyp = -n.xbin * c.f.sin_over + n.ybin * c.f.cos_over;
+	n.ox = (float)(x0 - c.xc);
+	n.oy = (float)(y0 - c.yc);
+	return n;
+}
+
+// One cell of a node: its FP32 position, value and refinement test (in FP64
+// at libprofit's own cell centre when the FP32 test is undecided); a leaf
+// adds weight * value to sum and, with kJac, its weighted FP32 derivatives
+// to gj.  `fuse`: a Sersic without box, whose derivatives reuse the value's
+// logarithm and exponentials.  Returns whether the cell is to be refined.
+template <int TYPE, bool kJac>
+__device__ __forceinline__ bool cell_step(const Comp &c, const Node &n, int cell, bool recurse, bool fp64, bool fuse,
+                                          double weight, double &sum, float *gj)
+{
+	// libprofit's FP64 cell size and test offset, for the cells that need an
+	// FP64 position
+	auto node64 = [&](double &hx, double &hy, double &dyp) {
+		const double xbin = (n.x1 - n.x0) / n.res;
+		const double ybin = (n.y1 - n.y0) / n.res;
+		hx = xbin / 2.;
+		hy = ybin / 2.;
+		dyp = -xbin * c.sin_over + ybin * c.cos_over;
+	};
+	// cell -> (ci, cj) by an FP32 reciprocal: (cell + 0.5) / res lies at least
+	// 0.5 / res from an integer, far beyond the product's rounding, so the
+	// truncation is the exact quotient
+	const int ci = (int)(((float)cell + 0.5f) * n.inv_res), cj = cell - ci * n.res;
+	float xp, yp;
+	cell_pos32(c, n.ox, n.oy, n.xbin, n.ybin, ci, cj, &xp, &yp);
+	float v32, s_base = 0.f, s_l2 = 0.f, s_rf = 0.f;
+	if (fuse) v32 = sersic_parts_f32(c, xp, yp, s_base, s_l2, s_rf);
+	else v32 = eval_at<float, TYPE>(c, xp, yp);
+	int dec = recurse ? refine_cell32<TYPE>(c, xp, yp, n.dyp, v32) : 0;
+	double v64 = 0;
+	bool have64 = false;
+	if (dec == 2) {
+		double hx, hy, dyp, xp64, yp64;
+		node64(hx, hy, dyp);
+		cell_pos64(c, n.x0, n.y0, hx, hy, ci, cj, &xp64, &yp64);
+		dec = refine_cell64<TYPE>(c, xp64, yp64, dyp, &v64) ? 1 : 0;
+		have64 = true;
+	}
+	if (dec == 1) return true;
+	// leaf values in FP64 or the FP32 value already at hand
+	if (!have64) {
+		if (fp64) {
+			double hx, hy, dyp, xp64, yp64;
+			node64(hx, hy, dyp);
+			cell_pos64(c, n.x0, n.y0, hx, hy, ci, cj, &xp64, &yp64);
+			v64 = eval_at<double, TYPE>(c, xp64, yp64);
+		}
+		else {
+			v64 = (double)v32;
+		}
+	}
+	sum += v64 * weight;
+	if (fuse) sersic_jac_f32(c, xp, yp, v32, s_base, s_l2, s_rf, (float)weight, gj);
+	else if (kJac) pixel_accumulate_f32<TYPE>(c, xp, yp, (float)weight, gj);
+	return false;
+}
+
 // Walks the recursion tree of one pixel of a TYPE profile.  Cell positions,
 // values and refinement tests are FP32; libprofit's FP64 cell position is
 // formed only for an undecided test and for FP64 leaf values (`fp64`).  The
@@ -222,28 +322,14 @@ __device__ __forceinline__ void walk_pixel(const Comp &c, WalkLevel *L, int lane
 	const bool fuse = kJac && TYPE == SERSIC && !c.boxy && !c.grad_box;
 	const int ncell = res * res;
 	const int nchunk = (ncell + 63) >> 6;
-	const double inv_cells = 1.0 / (double)ncell;
-	// cell -> (ci, cj) by an FP32 reciprocal: (cell + 0.5) / res lies at least
-	// 0.5 / res from an integer, far beyond the product's rounding, so the
-	// truncation is the exact quotient
-	const float inv_res = 1.f / (float)res;
+	// 1 / res^2 from a table (the quotient a division rounds to)
+	const double inv_cells = kInvCells[res];
 	int level = 0;
 	double nx0 = x0, nx1 = x1, ny0 = y0, ny1 = y1;
 	double weight = inv_cells;
 	for (;;) {
 		const bool recurse = res > 1 && level < maxrec;
-		// the node's cell size and the test offset in FP32; libprofit's FP64
-		// ones (node64) only for the cells that need an FP64 position
-		const float fxbin = (float)(nx1 - nx0) * inv_res, fybin = (float)(ny1 - ny0) * inv_res;
-		const float fdyp = -fxbin * c.f.sin_over + fybin * c.f.cos_over;
-		const float ox = (float)(nx0 - c.xc), oy = (float)(ny0 - c.yc);
-		auto node64 = [&](double &hx, double &hy, double &dyp) {
-			const double xbin = (nx1 - nx0) / res;
-			const double ybin = (ny1 - ny0) / res;
-			hx = xbin / 2.;
-			hy = ybin / 2.;
-			dyp = -xbin * c.sin_over + ybin * c.cos_over;
-		};
+		const Node n = make_node(c, nx0, nx1, ny0, ny1, res);
 		// the cells in chunks of 64 (one per lane); lane 0 stores each chunk's
 		// refine mask with the node, and `any` gathers them.  A node with no
 		// cell to refine is not stored: the search below starts under it.
@@ -251,43 +337,7 @@ __device__ __forceinline__ void walk_pixel(const Comp &c, WalkLevel *L, int lane
 #pragma unroll 1
 		for (int k = 0; k < nchunk; ++k) {
 			const int cell = k * 64 + lane;
-			bool fail = false;
-			if (cell < ncell) {
-				const int ci = (int)(((float)cell + 0.5f) * inv_res), cj = cell - ci * res;
-				float xp, yp;
-				cell_pos32(c, ox, oy, fxbin, fybin, ci, cj, &xp, &yp);
-				float v32, s_base = 0.f, s_l2 = 0.f, s_rf = 0.f;
-				if (fuse) v32 = sersic_parts_f32(c, xp, yp, s_base, s_l2, s_rf);
-				else v32 = eval_at<float, TYPE>(c, xp, yp);
-				int dec = recurse ? refine_cell32<TYPE>(c, xp, yp, fdyp, v32) : 0;
-				double v64 = 0;
-				bool have64 = false;
-				if (dec == 2) {
-					double hx, hy, dyp, xp64, yp64;
-					node64(hx, hy, dyp);
-					cell_pos64(c, nx0, ny0, hx, hy, ci, cj, &xp64, &yp64);
-					dec = refine_cell64<TYPE>(c, xp64, yp64, dyp, &v64) ? 1 : 0;
-					have64 = true;
-				}
-				fail = dec == 1;
-				if (!fail) {
-					// leaf values in FP64 or the FP32 value already at hand
-					if (!have64) {
-						if (fp64) {
-							double hx, hy, dyp, xp64, yp64;
-							node64(hx, hy, dyp);
-							cell_pos64(c, nx0, ny0, hx, hy, ci, cj, &xp64, &yp64);
-							v64 = eval_at<double, TYPE>(c, xp64, yp64);
-						}
-						else {
-							v64 = (double)v32;
-						}
-					}
-					sum += v64 * weight;
-					if (fuse) sersic_jac_f32(c, xp, yp, v32, s_base, s_l2, s_rf, (float)weight, gj);
-					else if (kJac) pixel_accumulate_f32<TYPE>(c, xp, yp, (float)weight, gj);
-				}
-			}
+			const bool fail = cell < ncell && cell_step<TYPE, kJac>(c, n, cell, recurse, fp64, fuse, weight, sum, gj);
 			const unsigned long long mk = __ballot(fail);
 			any |= mk;
 			if (lane == 0) L[level].mask[k] = mk;
@@ -370,52 +420,38 @@ __global__ __launch_bounds__(256, kJac && (TYPE == SERSIC || TYPE == MOFFAT || T
 	static_assert(TYPE >= 0 && TYPE < kFamilies, "one walker launch per profile family");
 	static_assert(kSlots < kJacStride && kSlots + 1 <= 16, "Jacobian slots and the value");
 	const int lb = wq.lbase[TYPE];
-	const int nl = min(wq.count[light_count(TYPE)], wq.lbase[TYPE + 1] - lb), nh = min(wq.count[1], wq.hcap);
+	// the light entries the root pass left to this walker (its deep list)
+	const int nl = min(wq.count[deep_count(TYPE)], wq.lbase[TYPE + 1] - lb), nh = min(wq.count[1], wq.hcap);
 	// per-lane Jacobian sums
 	float gj[NGRAD];
 #pragma unroll
 	for (int s = 0; s < NGRAD; ++s) gj[s] = 0.f;
-	// dynamic hand-out: the deep centre trees one at a time first, then the
-	// other entries in grabs of kGrab, so no wave is left with a long tail
+	// dynamic hand-out, one entry at a time: the centre-pixel trees first,
+	// then the light entries the root pass left (each a tree of several
+	// nodes), so that no wavefront is left with a long tail
 	bool heavy_left = nh > 0;
 	for (;;) {
-		int base = 0, take = 0;
 		bool heavy = false;
+		int h = nh;
+		if (heavy_left) {
+			if (lane == 0) h = atomicAdd(&wq.count[counter], 1);
+			h = __builtin_amdgcn_readfirstlane(h);
+		}
+		int e;  // the entry's index in its list (heavy, or the family's light list)
+		if (h < nh) {
+			e = h;
+			heavy = true;
+		}
+		else {
+			heavy_left = false;
+			int l = 0;
+			if (lane == 0) l = atomicAdd(&wq.count[counter + 1], 1);
+			l = __builtin_amdgcn_readfirstlane(l);
+			if (l >= nl) break;
+			e = wq.deep[lb + l];
+		}
 		{
-			int h = nh;
-			if (heavy_left) {
-				if (lane == 0) h = atomicAdd(&wq.count[counter], 1);
-				h = __shfl(h, 0);
-			}
-			if (h < nh) {
-				base = h;
-				take = 1;
-				heavy = true;
-			}
-			else {
-				heavy_left = false;
-				int l = 0;
-				if (lane == 0) l = atomicAdd(&wq.count[counter + 1], kGrab);
-				l = __shfl(l, 0);
-				if (l >= nl) break;
-				base = l;
-				take = min(kGrab, nl - l);
-			}
-		}
-		const QEntry *list = heavy ? wq.heavy : wq.light + lb;
-		// the grab's entries fetched at once, lane u holding entry base + u, so
-		// the per-entry chain of dependent loads is paid once per grab
-		int qc = 0, qp = 0;
-		if (lane < take) {
-			const QEntry qe = list[base + lane];
-			qc = qe.comp;
-			qp = qe.pix;
-		}
-		for (int e = base; e < base + take; ++e) {
-			const int u = e - base;
-			QEntry q;
-			q.comp = __builtin_amdgcn_readlane(qc, u);
-			q.pix = __builtin_amdgcn_readlane(qp, u);
+			const QEntry q = heavy ? wq.heavy[e] : wq.light[lb + e];
 			const Comp &c = comps[q.comp];
 			if (c.type != TYPE) continue;
 			const InstDev &I = insts[c.instance];
@@ -484,6 +520,78 @@ __global__ __launch_bounds__(256, kJac && (TYPE == SERSIC || TYPE == MOFFAT || T
 					atomicAdd(&(to_aux ? I.aux : I.canvas)[q.pix], (float)(c.flux_scale * t));
 				}
 			}
+		}
+	}
+}
+
+// The root pass of the sub-sampled pixels, one lane per light entry of a
+// TYPE family: the lane evaluates its pixel's root cells one after another
+// with the walker's cell step and, when none is to be refined (most pixels:
+// 98% on a 400x400 bulge+disk), adds the value to the canvas and, with kJac,
+// writes the entry's Jacobian and value itself, with no exchange between
+// lanes.  An entry with a cell to refine goes to the family's deep list,
+// whose trees the wavefront walker (k_subsample) walks from the root.
+template <int TYPE, bool kJac>
+__global__ __launch_bounds__(256) void k_subsample_root(const GalaxyDev *__restrict__ gals,
+                                                         const InstDev *__restrict__ insts,
+                                                         const Comp *__restrict__ comps, WorkQueue wq, int split)
+{
+	const int lane = threadIdx.x & 63;
+	const int lb = wq.lbase[TYPE];
+	const int nl = min(wq.count[light_count(TYPE)], wq.lbase[TYPE + 1] - lb);
+	for (;;) {
+		int base = 0;
+		if (lane == 0) base = atomicAdd(&wq.count[root_counter(TYPE)], 64);
+		base = __shfl(base, 0);
+		if (base >= nl) break;
+		const int e = base + lane;
+		bool deep = false;
+		if (e < nl) {
+			const QEntry q = wq.light[lb + e];
+			const Comp &c = comps[q.comp];
+			const InstDev &I = insts[c.instance];
+			const GalaxyDev &g = gals[I.galaxy];
+			const int i = q.pix % g.DW, j = q.pix / g.DW;
+			const double hx = g.sx / 2., hy = g.sy / 2.;
+			const double y = hy + j * g.sx;
+			const double x = hx + i * g.sy;
+			// centre-rule pixels belong to the heavy list: any found here go
+			// to the walker, which applies the rule
+			deep = centre_pixel(c, x, y);
+			if (c.type == TYPE && !deep) {
+				const int res = c.res, ncell = res * res;
+				const bool recurse = res > 1 && c.maxrec > 0;
+				const bool fuse = kJac && TYPE == SERSIC && !c.boxy && !c.grad_box;
+				const Node n = make_node(c, x - hx, x + hx, y - hy, y + hy, res);
+				const double weight = kInvCells[res];
+				const bool fp64 = g.render_fp64 != 0;
+				double sum = 0;
+				float gj[NGRAD];
+#pragma unroll
+				for (int s = 0; s < NGRAD; ++s) gj[s] = 0.f;
+				for (int cell = 0; cell < ncell && !deep; ++cell)
+					deep = cell_step<TYPE, kJac>(c, n, cell, recurse, fp64, fuse, weight, sum, gj);
+				if (!deep) {
+					const bool to_aux = split && !c.convolve;
+					if (g.canvas64 && !to_aux) atomicAdd(&I.canvas64[q.pix], c.flux_scale * sum);
+					else atomicAdd(&(to_aux ? I.aux : I.canvas)[q.pix], (float)(c.flux_scale * sum));
+					if (kJac) {
+						float *J = wq.ljac + (long)(lb + e) * kJacStride;
+#pragma unroll
+						for (int s = 0; s < G_S0 + n_shape(TYPE); ++s) J[s] = gj[s];
+						J[kJacStride - 1] = (float)sum;
+					}
+				}
+			}
+		}
+		// the deep entries appended, one atomic per wave
+		const unsigned long long m = __ballot(deep);
+		if (m) {
+			const int leader = __ffsll((long long)m) - 1;
+			int at = 0;
+			if (lane == leader) at = atomicAdd(&wq.count[deep_count(TYPE)], __popcll(m));
+			at = __shfl(at, leader);
+			if (deep) wq.deep[lb + at + __popcll(m & ((1ull << lane) - 1ull))] = e;
 		}
 	}
 }
