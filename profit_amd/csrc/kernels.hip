@@ -27,39 +27,49 @@ namespace profit_amd {
 
 // ------------------------------------------------------------------ set-up
 
-// One thread per component, in `order` (the components grouped by profile
-// family, so that a wavefront runs one family's set-up); the QUADPACK
-// luminosity gradients are left to k_setup_grad.
-__global__ void k_setup(const ProfileParams *__restrict__ params, const int *__restrict__ order,
-                        const int *__restrict__ comp_inst, const InstDev *__restrict__ insts,
-                        const GalaxyDev *__restrict__ gals, Comp *comps, int *errors, int ncomp)
+// The components' set-up in `order` (grouped by profile family), one
+// wavefront per component: the set-up runs alike on every lane and the
+// Kronrod points of its luminosity integrals spread over the lanes; the
+// QUADPACK luminosity gradients are left to k_setup_grad.
+__global__ __launch_bounds__(256, 2) void k_setup(const ProfileParams *__restrict__ params,
+                                                  const int *__restrict__ order, const int *__restrict__ comp_inst,
+                                                  const InstDev *__restrict__ insts,
+                                                  const GalaxyDev *__restrict__ gals, Comp *comps, int *errors,
+                                                  int ncomp)
 {
-	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	const int t = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
 	if (t >= ncomp) return;
 	const int c = order[t];
 	const int inst = comp_inst[c];
 	const GalaxyDev &g = gals[insts[inst].galaxy];
 	sf::QuadState qs;
 	Comp out;
-	int e = setup_component(params[c], g.magzero, g.sx, g.sy, g.img_sx, g.img_sy, g.padx, g.pady, &qs, &out, false);
+	int e = setup_component<true>(params[c], g.magzero, g.sx, g.sy, g.img_sx, g.img_sy, g.padx, g.pady, &qs, &out,
+	                              false);
 	if (e == SE_OK && params[c].type == POINTSRC && !g.psf_pt) e = SE_NO_PSF;
 	out.grad_box = g.grad_box;
 	out.instance = inst;
-	comps[c] = out;
-	errors[c] = e;
+	// every lane holds the same result; lane 0 stores it
+	if ((threadIdx.x & 63) == 0) {
+		comps[c] = out;
+		errors[c] = e;
+	}
 }
 
 // d ln(flux_scale) / d shape of the King, core-Sersic and broken-exponential
-// components (lum_gradient): one thread per (component, slot), components in
-// `order`, after k_setup.
-__global__ void k_setup_grad(const int *__restrict__ order, Comp *comps, const int *__restrict__ errors, int ncomp)
+// components (lum_gradient): one wavefront per (component, slot), the
+// integral's Kronrod points spread over its lanes; components in `order`,
+// after k_setup.
+__global__ __launch_bounds__(256, 4) void k_setup_grad(const int *__restrict__ order, Comp *comps,
+                                                    const int *__restrict__ errors, int ncomp)
 {
-	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	const int t = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
 	if (t >= ncomp * 5) return;
 	const int c = order[t / 5], w = t % 5;
 	if (errors[c] != SE_OK || w >= n_lum_integrals(comps[c].type)) return;
 	sf::QuadState qs;
-	comps[c].dlnfs[G_S0 + w] = lum_gradient(comps[c], w, &qs);
+	const double v = lum_gradient<true>(comps[c], w, &qs);
+	if ((threadIdx.x & 63) == 0) comps[c].dlnfs[G_S0 + w] = v;
 }
 
 __global__ void k_fill_zero_d(double *p, long n)
@@ -830,14 +840,15 @@ int launch_setup(const ProfileParams *params, const int *order, const int *comp_
                  const GalaxyDev *gals, Comp *comps, int *errors, int ncomp, hipStream_t st)
 {
 	if (ncomp <= 0) return 0;
-	k_setup<<<(ncomp + 63) / 64, 64, 0, st>>>(params, order, comp_inst, insts, gals, comps, errors, ncomp);
+	k_setup<<<(unsigned)(((long)ncomp * 64 + 255) / 256), 256, 0, st>>>(params, order, comp_inst, insts, gals, comps,
+	                                                                   errors, ncomp);
 	return (int)hipGetLastError();
 }
 
 int launch_setup_grad(const int *order, Comp *comps, const int *errors, int ncomp, hipStream_t st)
 {
 	if (ncomp <= 0) return 0;
-	k_setup_grad<<<(ncomp * 5 + 63) / 64, 64, 0, st>>>(order, comps, errors, ncomp);
+	k_setup_grad<<<(unsigned)(((long)ncomp * 5 * 64 + 255) / 256), 256, 0, st>>>(order, comps, errors, ncomp);
 	return (int)hipGetLastError();
 }
 

@@ -916,6 +916,7 @@ PA_HD int n_lum_integrals(int type) { return type == KING ? 3 : type == CORESERS
 // d ln(flux_scale) / d shape slot w of the families whose luminosity is a
 // QUADPACK integral (King 3 slots, core-Sersic 5, broken exponential 4):
 // -(2 pi int r df/dshape_w dr) / lumtot, from a component whose set-up has run.
+template <bool kWave = false>
 PA_HD double lum_gradient(const Comp &c, int w, sf::QuadState *qs)
 {
 	const double pi = 3.14159265358979323846;
@@ -924,17 +925,17 @@ PA_HD double lum_gradient(const Comp &c, int w, sf::QuadState *qs)
 	switch (c.type) {
 	case KING: {
 		KingIntegrand f{s[0], s[1], s[2], w};
-		d = 2 * pi * sf::integrate(f, 0.0, s[1], false, *qs);
+		d = 2 * pi * sf::integrate<kWave>(f, 0.0, s[1], false, *qs);
 		break;
 	}
 	case CORESERSIC: {
 		CoreSersicIntegrand f{&c, w};
-		d = 2 * pi * sf::integrate(f, 0.0, 0.0, true, *qs);
+		d = 2 * pi * sf::integrate<kWave>(f, 0.0, 0.0, true, *qs);
 		break;
 	}
 	case BROKENEXP: {
 		BrokenExpIntegrand f{s[0], s[1], s[2], s[3], w};
-		d = 2 * pi * sf::integrate(f, 0.0, 0.0, true, *qs);
+		d = 2 * pi * sf::integrate<kWave>(f, 0.0, 0.0, true, *qs);
 		break;
 	}
 	default: return 0;
@@ -943,6 +944,10 @@ PA_HD double lum_gradient(const Comp &c, int w, sf::QuadState *qs)
 }
 
 // grad_integrals = false leaves the slots of lum_gradient to a separate pass.
+// kWave (device code): a whole wavefront runs the set-up with the same
+// arguments, and the luminosity integrals spread their Kronrod points over
+// its lanes (sf::kronrod).
+template <bool kWave = false>
 PA_HD int setup_component(const ProfileParams &p, double magzero, double sx, double sy, double img_sx,
                           double img_sy, unsigned offx, unsigned offy, sf::QuadState *qs, Comp *c,
                           bool grad_integrals = true)
@@ -1028,17 +1033,17 @@ PA_HD int setup_component(const ProfileParams &p, double magzero, double sx, dou
 	case FERRER: lumtot = ::pow(s[0], 2.0) * pi * (s[1] * sf::beta(s[1], 1 + 2 / (2 - s[2]))); break;
 	case KING: {
 		KingIntegrand f{s[0], s[1], s[2], -2};
-		lumtot = 2 * pi * sf::integrate(f, 0.0, s[1], false, *qs);
+		lumtot = 2 * pi * sf::integrate<kWave>(f, 0.0, s[1], false, *qs);
 		break;
 	}
 	case CORESERSIC: {
 		CoreSersicIntegrand f{c};
-		lumtot = 2 * pi * sf::integrate(f, 0.0, 0.0, true, *qs);
+		lumtot = 2 * pi * sf::integrate<kWave>(f, 0.0, 0.0, true, *qs);
 		break;
 	}
 	default: {
 		BrokenExpIntegrand f{s[0], s[1], s[2], s[3]};
-		lumtot = 2 * pi * sf::integrate(f, 0.0, 0.0, true, *qs);
+		lumtot = 2 * pi * sf::integrate<kWave>(f, 0.0, 0.0, true, *qs);
 		break;
 	}
 	}
@@ -1185,7 +1190,7 @@ PA_HD int setup_component(const ProfileParams &p, double magzero, double sx, dou
 	case KING:
 	case BROKENEXP: {
 		if (grad_integrals)
-			for (int w = 0; w < n_lum_integrals(p.type); ++w) c->dlnfs[G_S0 + w] = lum_gradient(*c, w, qs);
+			for (int w = 0; w < n_lum_integrals(p.type); ++w) c->dlnfs[G_S0 + w] = lum_gradient<kWave>(*c, w, qs);
 		break;
 	}
 	case CORESERSIC: {
@@ -1193,7 +1198,7 @@ PA_HD int setup_component(const ProfileParams &p, double magzero, double sx, dou
 		const double dens = ::exp((a - 1) * ::log(bn) - bn - sf::lgam(a));
 		c->dbn = -2 * sf::dpgamma_da(bn, a) / dens;
 		if (grad_integrals)
-			for (int w = 0; w < 5; ++w) c->dlnfs[G_S0 + w] = lum_gradient(*c, w, qs);
+			for (int w = 0; w < 5; ++w) c->dlnfs[G_S0 + w] = lum_gradient<kWave>(*c, w, qs);
 		break;
 	}
 	default: break;

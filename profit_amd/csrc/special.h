@@ -244,21 +244,57 @@ PA_HD void rule_error(double resk, double resg, double half, double resabs, doub
 	*err = e;
 }
 
-// Kronrod rule on [a,b] (finite) or on (a,b] of the map t -> bound + (1-t)/t
-template <class F>
+// Kronrod rule on [a,b] (finite) or on (a,b] of the map t -> bound + (1-t)/t.
+// kWave (device code, a whole wavefront with the same arguments): the 21 or
+// 15 integrand values are computed on as many lanes at once and broadcast;
+// every lane then forms the sums below in the same order, so the results are
+// those of the serial rule.
+template <bool kWave = false, class F>
 PA_HD void kronrod(const F &f, bool infinite, double bound, double a, double b, double *result,
                    double *err, double *resabs, double *resasc)
 {
 	const double centre = 0.5 * (a + b), half = 0.5 * (b - a);
 	double fa[10], fb[10];
+	// kWave: lane k holds the value at abscissa k of the serial order, k < 10
+	// (finite; 7 infinite) centre - half x_k, then the centre, then centre +
+	// half x_k; the node of lane k selected without a table lookup
+	double fw = 0;
+	if constexpr (kWave) {
+		const int lane = __lane_id(), nside = infinite ? 7 : 10;
+		const int k = lane < nside ? lane : lane - nside - 1;
+		double xk = 0;
+#pragma unroll
+		for (int j = 0; j < 10; ++j)
+			if (j == k) xk = infinite ? (j < 7 ? QuadNodes::xk15(j) : 0.0) : QuadNodes::xk21(j);
+		if (lane <= 2 * nside) {
+			const double d = half * xk;
+			if (!infinite) {
+				fw = f(lane == nside ? centre : (lane < nside ? centre - d : centre + d));
+			}
+			else {
+				const double u = lane == nside ? centre : (lane < nside ? centre - d : centre + d);
+				fw = lane == nside ? (f(bound + (1 - centre) / centre) / centre) / centre : (f(bound + (1 - u) / u) / u) / u;
+			}
+		}
+	}
 	if (!infinite) {
-		const double fc = f(centre);
+		double fc;
+		if constexpr (kWave) fc = __shfl(fw, 10);
+		else fc = f(centre);
 		double resg = 0, resk = QuadNodes::wk21(10) * fc;
 		double sabs = ::fabs(resk);
 		for (int j = 0; j < 5; ++j) {  // Gauss abscissae first (odd Kronrod indices)
 			const int k = 2 * j + 1;
 			const double d = half * QuadNodes::xk21(k);
-			const double f1 = f(centre - d), f2 = f(centre + d);
+			double f1, f2;
+			if constexpr (kWave) {
+				f1 = __shfl(fw, k);
+				f2 = __shfl(fw, 11 + k);
+			}
+			else {
+				f1 = f(centre - d);
+				f2 = f(centre + d);
+			}
 			fa[k] = f1;
 			fb[k] = f2;
 			resg += QuadNodes::wg10(j) * (f1 + f2);
@@ -268,7 +304,15 @@ PA_HD void kronrod(const F &f, bool infinite, double bound, double a, double b, 
 		for (int j = 0; j < 5; ++j) {
 			const int k = 2 * j;
 			const double d = half * QuadNodes::xk21(k);
-			const double f1 = f(centre - d), f2 = f(centre + d);
+			double f1, f2;
+			if constexpr (kWave) {
+				f1 = __shfl(fw, k);
+				f2 = __shfl(fw, 11 + k);
+			}
+			else {
+				f1 = f(centre - d);
+				f2 = f(centre + d);
+			}
 			fa[k] = f1;
 			fb[k] = f2;
 			resk += QuadNodes::wk21(k) * (f1 + f2);
@@ -284,14 +328,23 @@ PA_HD void kronrod(const F &f, bool infinite, double bound, double a, double b, 
 		return;
 	}
 	const double tc = bound + (1 - centre) / centre;
-	const double fc = (f(tc) / centre) / centre;
+	double fc;
+	if constexpr (kWave) fc = __shfl(fw, 7);
+	else fc = (f(tc) / centre) / centre;
 	double resg = QuadNodes::wg7(7) * fc, resk = QuadNodes::wk15(7) * fc;
 	double sabs = ::fabs(resk);
 	for (int j = 0; j < 7; ++j) {
 		const double d = half * QuadNodes::xk15(j);
 		const double u1 = centre - d, u2 = centre + d;
-		const double f1 = (f(bound + (1 - u1) / u1) / u1) / u1;
-		const double f2 = (f(bound + (1 - u2) / u2) / u2) / u2;
+		double f1, f2;
+		if constexpr (kWave) {
+			f1 = __shfl(fw, j);
+			f2 = __shfl(fw, 8 + j);
+		}
+		else {
+			f1 = (f(bound + (1 - u1) / u1) / u1) / u1;
+			f2 = (f(bound + (1 - u2) / u2) / u2) / u2;
+		}
 		fa[j] = f1;
 		fb[j] = f2;
 		resg += QuadNodes::wg7(j) * (f1 + f2);
@@ -454,7 +507,7 @@ PA_HD void extrapolate(double *tab, int &n, double &result, double &abserr, doub
 }
 
 // dqagse (infinite = false, range [a,b]) and dqagie (infinite = true, [a, inf))
-template <class F>
+template <bool kWave = false, class F>
 PA_HD double integrate(const F &f, double a, double b, bool infinite, QuadState &s, double epsabs = 1e-4,
                        double epsrel = 1e-4, int limit = kQuadLimit)
 {
@@ -463,7 +516,7 @@ PA_HD double integrate(const F &f, double a, double b, bool infinite, QuadState 
 	double result = 0, abserr = 0, defabs = 0, resabs = 0;
 	s.lo[1] = infinite ? 0.0 : a;
 	s.hi[1] = infinite ? 1.0 : b;
-	kronrod(f, infinite, a, s.lo[1], s.hi[1], &result, &abserr, &defabs, &resabs);
+	kronrod<kWave>(f, infinite, a, s.lo[1], s.hi[1], &result, &abserr, &defabs, &resabs);
 	const double dres = ::fabs(result);
 	double errbnd = ::fmax(epsabs, epsrel * dres);
 	int last = 1;
@@ -490,8 +543,8 @@ PA_HD double integrate(const F &f, double a, double b, bool infinite, QuadState 
 		const double a2 = b1, b2 = s.hi[maxerr];
 		const double erlast = errmax;
 		double area1, area2, error1, error2, r1, r2, defab1, defab2;
-		kronrod(f, infinite, a, a1, b1, &area1, &error1, &r1, &defab1);
-		kronrod(f, infinite, a, a2, b2, &area2, &error2, &r2, &defab2);
+		kronrod<kWave>(f, infinite, a, a1, b1, &area1, &error1, &r1, &defab1);
+		kronrod<kWave>(f, infinite, a, a2, b2, &area2, &error2, &r2, &defab2);
 		const double area12 = area1 + area2, erro12 = error1 + error2;
 		errsum = errsum + erro12 - errmax;
 		area = area + area12 - s.val[maxerr];

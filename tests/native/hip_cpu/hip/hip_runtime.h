@@ -102,3 +102,14 @@ inline double __shfl_xor(double v, int mask)
 	__syncthreads();
 	return r;
 }
+// lane of the wavefront and a read of another lane's value
+inline int __lane_id() { return (int)(threadIdx.x & 63u); }
+inline double __shfl(double v, int src)
+{
+	const unsigned t = threadIdx.x;
+	hip_cpu::xchg_d[t] = v;
+	__syncthreads();
+	const double r = hip_cpu::xchg_d[(t & ~63u) + ((unsigned)src & 63u)];
+	__syncthreads();
+	return r;
+}
