@@ -191,7 +191,13 @@ def fft_work(dw, dh, w, h, n1, n2):
     flops = {"like_rows_fwd": pairs(dh) * row, "like_cols": col, "like_rows_inv": pairs(h) * row,
              "adj_rows_fwd": pairs(h) * row, "adj_cols": col, "adj_rows_inv": pairs(dh) * row}
     stage_bytes = {"conv_like": 4.0 * dw * dh + 13.0 * w * h, "adjoint_conv": 4.0 * w * h + 4.0 * dw * dh}
-    return flops, stage_bytes
+    # each kernel's algorithmic HBM bytes: the half-spectrum rows it reads and
+    # writes (16 B per entry, N1/2 + 1 entries a row) and its image data
+    m = 16.0 * (n1 // 2 + 1)
+    kernel_bytes = {"like_rows_fwd": 4.0 * dw * dh + m * dh, "like_cols": m * (dh + h),
+                    "like_rows_inv": m * h + 20.0 * w * h, "adj_rows_fwd": 4.0 * w * h + m * h,
+                    "adj_cols": m * (h + dh), "adj_rows_inv": m * dh + 4.0 * dw * dh}
+    return flops, stage_bytes, kernel_bytes
 
 
 def time_calls(fn, reps):
@@ -468,7 +474,8 @@ def main():
                 "direct_pixels_per_instance": direct})
     # the forward walk of a gradient evaluation: cell values, refinement tests
     # and each cell's derivatives into the pixel's Jacobian
-    add_kernel("subsample", "k_subsample<0, true>", stage_ms["subsample"], "hbm", render_bytes, PEAK_HBM_GBS,
+    add_kernel("subsample", "k_subsample_root<0, true> + k_subsample<0, true>", stage_ms["subsample"], "hbm",
+               render_bytes, PEAK_HBM_GBS,
                "GB/s", {"cell_value_and_derivative_evals_per_s": round(B * cells / (stage_ms["subsample"] * 1e-3), 0),
                         "cells_per_instance": cells})
     add_kernel("adjoint_render", "k_adj_render<0, *>", stage_ms["adjoint_render"], "hbm", render_bytes, PEAK_HBM_GBS,
@@ -479,15 +486,18 @@ def main():
                PEAK_HBM_GBS, "GB/s", {"queued_entries_per_instance": queued,
                                       "note": "work: 60 B per queued pixel (48 B Jacobian and value, 8 B entry, 4 B G)"})
     if conv_used == "fft":
-        flops, stage_bytes = fft_work(dw, dh, 400, 400, n1, n2)
+        flops, stage_bytes, kbytes = fft_work(dw, dh, 400, 400, n1, n2)
         names = {"like_rows_fwd": "k_fft_rows_fwd", "like_cols": "k_fft_cols", "like_rows_inv": "k_fft_rows_inv<3>",
                  "adj_rows_fwd": "k_fft_rows_fwd", "adj_cols": "k_fft_cols", "adj_rows_inv": "k_fft_rows_inv<2>"}
         for key, kname in names.items():
             ms = fft_kernel_ms[key]
-            # each kernel's limit is the FP64 frame it streams through HBM (its
-            # counter traffic beside it); its executed FP64 FLOPs for the compute view
-            add_kernel(key, kname, ms, "hbm", B * flops[key], PEAK_FP64_TFLOPS, "TFLOP/s")
-            kernels[key]["note"] = "work: executed FP64 FLOPs; bound by the frame's HBM traffic (traffic_frac)"
+            # against HBM: the half-spectrum rows each kernel streams (and its
+            # image data); its executed FP64 FLOPs beside it
+            add_kernel(key, kname, ms, "hbm", B * kbytes[key], PEAK_HBM_GBS, "GB/s",
+                       {"fp64_tflops": round(B * flops[key] / (ms * 1e-3) / 1e12, 3),
+                        "fp64_frac": round(B * flops[key] / (ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, 5)})
+            kernels[key]["note"] = ("work: the half-spectrum rows read and written (16 B an entry) and the image "
+                                    "data; fp64_tflops: executed FP64 FLOPs (5 N log2 N per transform)")
         for stage in ("conv_like", "adjoint_conv"):
             add_kernel(stage, "FFT stage (3 kernels)", stage_ms[stage], "hbm", B * stage_bytes[stage], PEAK_HBM_GBS,
                        "GB/s")
@@ -503,7 +513,8 @@ def main():
             kernels[k]["note"] = "work: 2 K^2 FLOP per output pixel; FP32 VALU" if bound == "hbm" else \
                 "work: 2 K^2 FLOP per output pixel (three split-BF16 products issued per MAC)"
     # the dominant single kernel of the step (stages of several kernels excluded)
-    single = [k for k in kernels if not kernels[k]["kernel"].startswith("FFT stage") and "*" not in kernels[k]["kernel"]]
+    single = [k for k in kernels if not kernels[k]["kernel"].startswith("FFT stage") and "*" not in kernels[k]["kernel"]
+              and "+" not in kernels[k]["kernel"]]
     dom = max(single, key=lambda k: kernels[k]["ms"])
     d = kernels[dom]
     stage_per_step = {k: v / args.steps for k, v in sm.items()}
