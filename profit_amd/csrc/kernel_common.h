@@ -531,6 +531,10 @@ __global__ __launch_bounds__(256, kJac && (TYPE == SERSIC || TYPE == MOFFAT || T
 // writes the entry's Jacobian and value itself, with no exchange between
 // lanes.  An entry with a cell to refine goes to the family's deep list,
 // whose trees the wavefront walker (k_subsample) walks from the root.
+// entries from which the root pass works a lane per entry: 2^17, two
+// wavefronts of 64 entries for each SIMD of the chip
+constexpr int kRootMinEntries = 1 << 17;
+
 template <int TYPE, bool kJac>
 __global__ __launch_bounds__(256) void k_subsample_root(const GalaxyDev *__restrict__ gals,
                                                          const InstDev *__restrict__ insts,
@@ -539,6 +543,10 @@ __global__ __launch_bounds__(256) void k_subsample_root(const GalaxyDev *__restr
 	const int lane = threadIdx.x & 63;
 	const int lb = wq.lbase[TYPE];
 	const int nl = min(wq.count[light_count(TYPE)], wq.lbase[TYPE + 1] - lb);
+	// a lane per entry pays off when the entries fill the GPU; a shorter list
+	// (a single image, say) goes whole to the walker, whose wavefronts share
+	// each pixel's cells and so finish sooner
+	const bool lanes = nl >= kRootMinEntries;
 	for (;;) {
 		int base = 0;
 		if (lane == 0) base = atomicAdd(&wq.count[root_counter(TYPE)], 64);
@@ -557,7 +565,7 @@ __global__ __launch_bounds__(256) void k_subsample_root(const GalaxyDev *__restr
 			const double x = hx + i * g.sy;
 			// centre-rule pixels belong to the heavy list: any found here go
 			// to the walker, which applies the rule
-			deep = centre_pixel(c, x, y);
+			deep = !lanes || centre_pixel(c, x, y);
 			if (c.type == TYPE && !deep) {
 				const int res = c.res, ncell = res * res;
 				const bool recurse = res > 1 && c.maxrec > 0;
