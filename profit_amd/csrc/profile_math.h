@@ -1006,7 +1006,27 @@ PA_HD int setup_component(const ProfileParams &p, double magzero, double sx, dou
 		c->gk = 0.5 * tt * 0.69314718055994530942 / (s[1] * s[1] * (tt - 1));
 	}
 
-	if (p.type == SERSIC || p.type == CORESERSIC) c->bn = sf::qgamma(0.5, 2 * (p.type == SERSIC ? s[1] : s[2]));
+	// b_n = qgamma(0.5, 2n); a Sersic component's adjust rule needs two more
+	// quantiles of the same gamma law (rscale_switch, and rscale_max when it is
+	// 0).  In a wavefront each of the three runs on a lane of its own.
+	double q_sw = 0, q_max = 0;
+	if (p.type == SERSIC || p.type == CORESERSIC) {
+		const double a2n = 2 * (p.type == SERSIC ? s[1] : s[2]);
+		if constexpr (kWave) {
+			const int l = __lane_id();
+			const bool more = p.type == SERSIC && p.adjust;
+			double pq = 0.5;
+			if (more && l == 1) pq = 1. - s[1] * s[1] / 2e3;
+			if (more && l == 2 && p.rscale_max == 0) pq = 0.9999;
+			const double q = l < 3 ? sf::qgamma(pq, a2n) : 0.0;
+			c->bn = __shfl(q, 0);
+			q_sw = __shfl(q, 1);
+			q_max = __shfl(q, 2);
+		}
+		else {
+			c->bn = sf::qgamma(0.5, a2n);
+		}
+	}
 	c->inv_re = c->inv_re2 = c->inv_n = c->inv_nB = c->inv_2n = 0;
 	if (p.type == SERSIC) {
 		c->inv_re = 1 / s[0];
@@ -1060,7 +1080,7 @@ PA_HD int setup_component(const ProfileParams &p, double magzero, double sx, dou
 		switch (p.type) {
 		case SERSIC: {
 			const double n = s[1];
-			const double ff = s[0] * ::pow(sf::qgamma(1. - n * n / 2e3, 2 * n) / c->bn, n);
+			const double ff = s[0] * ::pow((kWave ? q_sw : sf::qgamma(1. - n * n / 2e3, 2 * n)) / c->bn, n);
 			double v = ::ceil(ff);
 			v = ::fmax(::fmin(v, 20.), 2.);
 			rsw = v / s[0];
@@ -1083,7 +1103,7 @@ PA_HD int setup_component(const ProfileParams &p, double magzero, double sx, dou
 			switch (p.type) {
 			case SERSIC: {
 				const double n = s[1];
-				const double ff = s[0] * ::pow(sf::qgamma(0.9999, 2 * n) / c->bn, n);
+				const double ff = s[0] * ::pow((kWave ? q_max : sf::qgamma(0.9999, 2 * n)) / c->bn, n);
 				rmax = ::ceil(::fmax(ff, 2.) / s[0]);
 				break;
 			}

@@ -1,6 +1,7 @@
 // The set-up with its QUADPACK integrals spread over a wavefront's lanes
 // (setup_component<true>, lum_gradient<true>: each lane evaluates one Kronrod
-// point, every lane forms the sums) against the serial set-up, on the host
+// point, every lane forms the sums; a Sersic component's three gamma quantiles
+// on three lanes) against the serial set-up, on the host
 // stand-in of the HIP thread model: one block of 64 threads.  Prints one line
 // per case: <profile> <case> <bitwise-equal set-up> <bitwise-equal gradients>
 #include <cstdio>
@@ -45,6 +46,19 @@ static void check(const char *name, int k, const ProfileParams &p)
 
 int main()
 {
+	// Sersic: b_n and the adjust rule's two quantiles, one lane each (the
+	// fixed rscale_max and the unadjusted component leave lanes 1-2 idle)
+	for (int k = 0; k < 4; ++k) {
+		ProfileParams p;
+		default_params(SERSIC, &p);
+		p.xcen = 40; p.ycen = 40; p.mag = 16; p.axrat = 0.7; p.ang = 30;
+		const double nser[4] = {0.5, 4, 8, 2.5};
+		p.shape[0] = 3 + k;          // re
+		p.shape[1] = nser[k];        // nser
+		p.adjust = k != 3;
+		p.rscale_max = k == 2 ? 5 : 0;
+		check("sersic", k, p);
+	}
 	for (int k = 0; k < 3; ++k) {
 		ProfileParams p;
 		default_params(KING, &p);
